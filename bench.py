@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Ray-cast benchmark: incoherent closest-hit Mrays/s on a 1M-triangle BVH4.
+
+Workload (one "step" = one trace of the whole ray set, inputs resident in
+HBM): a tetrahedron subdivided 9 times (1,048,576 triangles) displaced by
+Perlin noise (amplitude 0.1, seed 1), BVH4 from the reference's builder
+(<= 3 triangles per leaf), 16,777,216 rays with origins uniform in the mesh
+bounds, directions uniform on the sphere and time uniform in [0,1).
+
+Prints one JSON line.  `roofline` models the traversal kernel's algorithmic
+bytes: per ray, internal-node visits x 128 B node record + triangle tests x
+48 B triangle record + 58 B of ray and hit I/O, with visit counts from an
+instrumented run of the same kernel on the same rays.  `cpu_baseline` times
+the oracle (a C++ restatement of the reference's ray-stream tracer, batches
+of 4096 rays, one tracer per thread) on a sample of the same rays.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "incoherent Mrays/s on 1M-tri BVH4; samples/s + p50/p99 latency served vs plain"
+HBM_PEAK_GBS = 8000.0
+RAY_IN_BYTES = 8 * 4 + 1     # ox oy oz dx dy dz time max_t + flags
+HIT_OUT_BYTES = 4 * 4 + 2 * 4 + 1  # t b0 b1 b2 prim inst + kind
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--levels", type=int, default=9)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(rays and requests shard with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from psychopath_amd import scenes
+    from psychopath_amd.tracer import Assembly, DeviceHits, DeviceRays, Mesh, RayBatch, Tracer
+
+    t0 = time.time()
+    verts, tris = scenes.displaced_sphere(args.levels)
+    lo, hi = verts.min(axis=0), verts.max(axis=0)
+    asm = Assembly()
+    asm.add_mesh(Mesh(verts, tris))
+    asm.add_instance(0)
+    scene = asm.build()
+    info = scene.info()
+    log(f"scene: {len(tris)} triangles, {info['node_count']} nodes, depth {info['max_depth']}, "
+        f"build {info['build_seconds']:.2f}s")
+    r = scenes.incoherent_rays(lo, hi, args.rays)
+    batch = RayBatch(r["ox"], r["oy"], r["oz"], r["dx"], r["dy"], r["dz"], r["time"], r["max_t"])
+    tracer = Tracer(scene, max_batch=1 << 20)
+    d_rays = DeviceRays(batch)
+    d_hits = DeviceHits(args.rays)
+    log(f"setup {time.time() - t0:.1f}s; warmup {args.warmup}")
+
+    for _ in range(args.warmup):
+        tracer.trace_device(d_rays, d_hits, sync=False)
+        tracer.sync()
+
+    kernel_ms = []
+    tracer.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        tracer.trace_device(d_rays, d_hits, sync=False)
+        tracer.sync()
+        kernel_ms.append(tracer.last_kernel_ms())
+    tracer.sync()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1e3 / args.steps
+    mrays = args.rays * args.steps / wall / 1e6
+    avg_kernel_ms = float(np.mean(kernel_ms))
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, kernel {avg_kernel_ms:.2f} ms, {mrays:.1f} Mrays/s")
+
+    # Algorithmic bytes per launch from the instrumented kernel.
+    counts = tracer.trace_counted(d_rays, d_hits)
+    h = d_hits.to_host()
+    hit_frac = float((h.kind == 2).mean())
+    node_b = info["node_bytes"]
+    tri_b = info["tri_bytes"]
+    io_b = RAY_IN_BYTES + HIT_OUT_BYTES
+    alg_bytes = counts["node_visits"] * node_b + counts["tri_tests"] * tri_b + args.rays * io_b
+    achieved = alg_bytes / (avg_kernel_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "trace_incoherent_1m_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # Any-hit and coherent rays, a few launches each.
+    extra = {}
+    for name, rr in (("anyhit_shadow", scenes.shadow_rays(lo, hi, args.rays)),
+                     ("coherent_primary", scenes.coherent_rays(lo, hi, 4096, max(1, args.rays // 4096)))):
+        b = RayBatch(rr["ox"], rr["oy"], rr["oz"], rr["dx"], rr["dy"], rr["dz"], rr["time"], rr["max_t"],
+                     rr.get("occlusion"))
+        dr = DeviceRays(b)
+        dh = DeviceHits(len(b))
+        tracer.trace_device(dr, dh)
+        ks = []
+        for _ in range(3):
+            tracer.trace_device(dr, dh, sync=False)
+            tracer.sync()
+            ks.append(tracer.last_kernel_ms())
+        extra[name + "_mrays_per_s_kernel"] = round(len(b) / (np.mean(ks) * 1e-3) / 1e6, 1)
+        del dr, dh
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.psyref import OracleScene
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n = min(args.cpu_sample, args.rays)
+        o = OracleScene()
+        o.add_mesh(verts, tris)
+        o.add_instance(0)
+        o.build()
+        sample = {k: v[:n] for k, v in r.items()}
+        res = o.trace(sample, batch=4096, threads=threads)
+        cpu = {
+            "value": round(n / res["seconds"] / 1e6, 3),
+            "unit": "Mrays/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {n:,} of the {args.rays:,} incoherent rays, psyref ray-stream tracer "
+                      f"(C++ restatement of tracer.rs/bvh4.rs), 4096-ray batches, trace calls timed",
+        }
+        log(f"cpu baseline {cpu['value']} Mrays/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "raycast incoherent closest-hit: 1,048,576-tri displaced tetra-sphere BVH4, "
+                        f"{args.rays:,} rays (origins in bounds, uniform directions, time in [0,1))",
+            "triangles": int(len(tris)),
+            "rays": args.rays,
+            "bvh_nodes": info["node_count"],
+            "bvh_depth": info["max_depth"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "trace_kernel<false>",
+            "kernel_ms": round(avg_kernel_ms, 3),
+            "algorithmic_bytes_per_launch": int(alg_bytes),
+            "bytes_model": f"node_visits*{node_b} + tri_tests*{tri_b} + rays*{io_b}",
+            "node_visits_per_ray": round(counts["node_visits"] / args.rays, 3),
+            "tri_tests_per_ray": round(counts["tri_tests"] / args.rays, 3),
+        },
+        "cpu_baseline": cpu,
+        "hit_fraction": round(hit_frac, 4),
+        "kernel_mrays_per_s": round(args.rays / (avg_kernel_ms * 1e-3) / 1e6, 1),
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+/*
+ * psy_rt.h — C ABI of the MI355X (gfx950) ray-tracing path for psychopath.
+ *
+ * This is the drop-in boundary under the renderer.  It replaces
+ *   Tracer::from_assembly(&Assembly) -> Tracer      (src/tracer.rs:22-32)
+ *   Tracer::trace(&mut RayBatch) -> &[SurfaceIntersection]  (src/tracer.rs:34-37)
+ *   Tracer::rays_traced() -> u64                    (src/tracer.rs:39-41)
+ * together with the scene data those calls read: triangle meshes
+ * (TriangleMesh::from_verts_and_indices, src/surface/triangle_mesh.rs:29-116)
+ * and instances with time-sampled transforms (AssemblyBuilder::add_instance /
+ * build, src/scene/assembly.rs:223-350).  The BVH4s are built on the host by
+ * the reference's builder algorithm (src/accel/bvh_base.rs,
+ * src/accel/objects_split.rs, src/accel/bvh4.rs:58-95,186-358) and then
+ * repacked for the GPU.
+ *
+ * Conventions: every function returns PSY_OK (0) or a negative PSY_E* code;
+ * psy_last_error() gives a message for the calling thread.  No exception
+ * crosses the ABI.  Buffers are caller-owned.  A tracer owns one HIP stream
+ * and is used by one host thread at a time (the reference keeps one Tracer
+ * per render thread, src/renderer.rs:210).
+ */
+#ifndef PSY_RT_H
+#define PSY_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSY_OK 0
+#define PSY_E_INVALID (-1)      /* bad argument or handle */
+#define PSY_E_RANGE (-2)        /* size beyond a documented limit */
+#define PSY_E_DEVICE (-3)       /* HIP runtime error */
+#define PSY_E_NOMEM (-4)        /* allocation failure */
+#define PSY_E_STATE (-5)        /* call out of order */
+
+/* Hit kinds, the variants of SurfaceIntersection (src/surface/mod.rs:72-80). */
+#define PSY_HIT_MISS 0u
+#define PSY_HIT_OCCLUDE 1u
+#define PSY_HIT_SURFACE 2u
+
+/* Ray flags (src/ray.rs:9-10). */
+#define PSY_RAY_OCCLUSION 1u
+
+/* Memory-space flags for psy_trace(). */
+#define PSY_MEM_HOST 0u
+#define PSY_MEM_DEVICE 1u
+
+#define PSY_NONE 0xffffffffu
+
+typedef struct psy_scene psy_scene;
+typedef struct psy_tracer psy_tracer;
+
+/* One triangle mesh (TriangleMesh::from_verts_and_indices,
+ * src/surface/triangle_mesh.rs:29-116).  `verts` holds `time_samples`
+ * consecutive arrays of `vert_count` xyz points (deformation motion blur);
+ * `normals` is NULL or has the same layout; `indices` holds `tri_count`
+ * triples in the file's winding order. */
+typedef struct {
+    uint32_t vert_count;
+    uint32_t time_samples;
+    const float* verts;
+    const float* normals;
+    uint32_t tri_count;
+    const uint32_t* indices;
+} psy_mesh_desc;
+
+/* One instance of a mesh (Instance, src/scene/assembly.rs:404-411).
+ * `xform_count` transforms starting at `xform_start` in psy_scene_desc.xforms
+ * are its world-to-object time samples (0 = no transform). */
+typedef struct {
+    uint32_t mesh;
+    uint32_t shader;      /* surface shader index, PSY_NONE = unassigned */
+    uint32_t xform_start;
+    uint32_t xform_count;
+} psy_instance_desc;
+
+/* A single-level assembly: meshes and the instances that place them.
+ * Transforms are 12 floats each, the arguments a..l of
+ * Transform::new_from_values (sub_crates/math3d/src/transform.rs:29-47). */
+typedef struct {
+    uint32_t mesh_count;
+    const psy_mesh_desc* meshes;
+    uint32_t instance_count;
+    const psy_instance_desc* instances;
+    uint32_t xform_count;
+    const float* xforms;
+} psy_scene_desc;
+
+/* Rays as structure-of-arrays (RayBatch, src/ray.rs:24-44).  `flags` may be
+ * NULL (all closest-hit).  `time` in [0,1]. */
+typedef struct {
+    const float *ox, *oy, *oz;
+    const float *dx, *dy, *dz;
+    const float* time;
+    const float* max_t;
+    const uint8_t* flags;
+} psy_ray_soa;
+
+/* Hit records as structure-of-arrays.  Barycentrics refer to the stored
+ * (winding-swapped) vertex order (src/surface/triangle_mesh.rs:80): b0 to the
+ * first vertex, b1 to the file's third, b2 to the file's second.  prim_id is
+ * the triangle's index in the mesh description, inst_id the instance's index
+ * in psy_scene_desc.instances.  Fields other than `kind` are written for
+ * PSY_HIT_SURFACE only. */
+typedef struct {
+    uint8_t* kind;
+    float *t, *b0, *b1, *b2;
+    uint32_t *prim_id, *inst_id;
+} psy_hit_soa;
+
+/* Per-scene figures, including the record sizes used for the algorithmic-byte
+ * roofline model. */
+typedef struct {
+    uint64_t node_count;        /* BVH4 internal nodes, all trees */
+    uint64_t leaf_count;        /* BVH4 leaves, all trees */
+    uint64_t motion_bounds;     /* extra time-sample BBox4 records */
+    uint64_t tri_count;
+    uint64_t device_bytes;      /* scene bytes resident in HBM */
+    uint32_t node_bytes;        /* bytes of one node record */
+    uint32_t motion_bounds_bytes; /* bytes of one time-sample BBox4 record */
+    uint32_t tri_bytes;         /* bytes of one static triangle record */
+    uint32_t max_depth;         /* deepest BVH4 (src/accel/bvh4.rs:87) */
+    double build_seconds;       /* host BVH build time */
+} psy_scene_info;
+
+/* Traversal counters of one psy_trace_counted() call. */
+typedef struct {
+    uint64_t node_visits;       /* internal-node box tests, summed over rays */
+    uint64_t motion_node_visits;
+    uint64_t tri_tests;         /* ray-triangle tests, summed over rays */
+    uint64_t motion_tri_tests;  /* of which with deformation time samples */
+    uint64_t instance_visits;   /* instance entries */
+    uint64_t rays;
+} psy_trace_counters;
+
+/* ---- errors ---------------------------------------------------------- */
+const char* psy_last_error(void);
+const char* psy_version(void);
+
+/* ---- scene: AssemblyBuilder::build (src/scene/assembly.rs:288-350) ------
+ * Builds every mesh BVH4 (<= 3 triangles per leaf, triangle_mesh.rs:17) and
+ * the instance BVH4 (1 instance per leaf, assembly.rs:293) on the host.  The
+ * repacked scene is uploaded to the current HIP device when the first tracer
+ * is created, so scenes can be built and inspected without a GPU. */
+int psy_scene_create(const psy_scene_desc* desc, psy_scene** out);
+int psy_scene_destroy(psy_scene* scene);
+int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
+
+/* Canonical pre-order dump of one host-built BVH4 for parity checks against
+ * the reference builder: mesh index, or -1 for the instance BVH.  Writes at
+ * most `cap` u32 to `out` and the object order of the leaves (triangle or
+ * instance indices) to `order` (up to `ocap`), and returns the dump length
+ * in u32 (or a negative error). */
+int64_t psy_scene_dump_bvh(const psy_scene* scene, int32_t mesh, uint32_t* out, uint64_t cap, uint32_t* order,
+                           uint64_t ocap);
+
+/* ---- tracer: Tracer::from_assembly (src/tracer.rs:22-32) ------------------
+ * `max_batch` bounds the rays of one psy_trace() call when the rays are in
+ * host memory (device staging is sized by it). */
+int psy_tracer_create(const psy_scene* scene, uint64_t max_batch, psy_tracer** out);
+int psy_tracer_destroy(psy_tracer* tracer);
+
+/* Tracer::trace (src/tracer.rs:34-37): closest hit for ordinary rays, any hit
+ * for rays flagged PSY_RAY_OCCLUSION, one record per ray.  `mem` says whether
+ * `rays` and `hits` are host (PSY_MEM_HOST) or device (PSY_MEM_DEVICE)
+ * pointers.  Synchronous: returns when the hits are written. */
+int psy_trace(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits, uint32_t mem);
+
+/* As psy_trace with device pointers, but only enqueued on the tracer's
+ * stream; psy_tracer_sync() waits.  The kernel time of the call is recorded
+ * with HIP events on that stream. */
+int psy_trace_async(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits);
+int psy_tracer_sync(psy_tracer* tracer);
+/* Milliseconds of the last completed psy_trace / psy_trace_async kernel,
+ * measured with HIP events on the tracer's stream. */
+int psy_tracer_last_kernel_ms(psy_tracer* tracer, float* ms);
+
+/* psy_trace with device pointers plus traversal counters (an instrumented
+ * kernel: slower; the counts feed the roofline's algorithmic bytes and match
+ * the meaning of ACCEL_NODE_RAY_TESTS, src/accel/mod.rs:22-25). */
+int psy_trace_counted(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
+                      psy_trace_counters* counters);
+
+/* Tracer::rays_traced (src/tracer.rs:39-41). */
+uint64_t psy_tracer_rays_traced(const psy_tracer* tracer);
+
+/* ---- device memory helpers for callers without a GPU runtime of their own */
+int psy_device_alloc(uint64_t bytes, void** ptr);
+int psy_device_free(void* ptr);
+int psy_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int psy_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int psy_device_count(int* count);
+/* A plain device-to-device copy kernel of `bytes` (multiple of 16), used to
+ * calibrate HBM byte counters; returns its kernel time in ms. */
+int psy_device_copy_kernel(void* dst, const void* src, uint64_t bytes, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSY_RT_H */

@@ -1,0 +1,162 @@
+"""ctypes wrapper of oracle/libpsyref.so.
+
+TEST INFRASTRUCTURE ONLY: the parity oracle and the CPU baseline.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsyref.so")
+
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+_SIGS = {
+    "psyref_lerp_f32": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+    "psyref_lerp_slice_f32": (C.c_float, [_fp, C.c_uint32, C.c_float]),
+    "psyref_xform_lerp": (None, [_fp, _fp, C.c_float, _fp]),
+    "psyref_xform_mul": (None, [_fp, _fp, _fp]),
+    "psyref_xform_inverse": (None, [_fp, _fp]),
+    "psyref_point_xform": (None, [_fp, _fp, _fp]),
+    "psyref_vector_xform": (None, [_fp, _fp, _fp]),
+    "psyref_normal_xform": (None, [_fp, _fp, _fp]),
+    "psyref_vec_lerp": (None, [_fp, _fp, C.c_float, _fp]),
+    "psyref_vec_normalize": (None, [_fp, _fp]),
+    "psyref_vec_length": (C.c_float, [_fp]),
+    "psyref_vec_dot": (C.c_float, [_fp, _fp]),
+    "psyref_vec_cross": (None, [_fp, _fp, _fp]),
+    "psyref_increment_ulp": (C.c_float, [C.c_float]),
+    "psyref_decrement_ulp": (C.c_float, [C.c_float]),
+    "psyref_fp_gamma": (C.c_float, [C.c_uint32]),
+    "psyref_log2_64": (C.c_uint64, [C.c_uint64]),
+    "psyref_hash_u32": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "psyref_hash_u64": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "psyref_quick_select_i32": (None, [C.POINTER(C.c_int32), C.c_uint32, C.c_uint32]),
+    "psyref_traversal_table": (None, [_u8p]),
+    "psyref_triangle_intersect": (C.c_int, [_fp, _fp, C.c_float, _fp, _fp]),
+    "psyref_bbox4_intersect": (C.c_uint32, [_fp, _fp, _fp, C.c_float]),
+    "psyref_scene_new": (_vp, []),
+    "psyref_scene_free": (None, [_vp]),
+    "psyref_scene_add_mesh": (C.c_int, [_vp, _fp, C.c_uint32, C.c_uint32, _fp, _u32p, C.c_uint32]),
+    "psyref_scene_add_instance": (C.c_int, [_vp, C.c_uint32, C.c_int32, _fp, C.c_uint32]),
+    "psyref_scene_build": (C.c_int, [_vp]),
+    "psyref_scene_dump_bvh": (C.c_uint64, [_vp, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_uint64]),
+    "psyref_trace": (C.c_int, [_vp] + [_fp] * 8 + [_u8p, C.c_uint64, C.c_uint32, C.c_uint32, _u8p] + [_fp] * 4
+                     + [_u32p, _u32p, _fp, _fp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleScene:
+    """A single-level assembly in the oracle (meshes + instances)."""
+
+    def __init__(self):
+        self.h = lib().psyref_scene_new()
+        self.tri_counts = []
+        self.instance_count = 0
+
+    def add_mesh(self, verts, tris, normals=None) -> int:
+        v = f32(verts)
+        if v.ndim == 2:
+            v = v[None]
+        t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        n = None if normals is None else f32(normals).reshape(v.shape)
+        r = lib().psyref_scene_add_mesh(self.h, fptr(v), v.shape[1], v.shape[0],
+                                        None if n is None else fptr(n), t.ctypes.data_as(_u32p), t.shape[0])
+        self.tri_counts.append(t.shape[0])
+        return r
+
+    def add_instance(self, mesh: int, xforms=None, shader: int = -1) -> int:
+        if xforms is None:
+            r = lib().psyref_scene_add_instance(self.h, mesh, shader, None, 0)
+        else:
+            x = f32(xforms).reshape(-1, 12)
+            r = lib().psyref_scene_add_instance(self.h, mesh, shader, fptr(x), x.shape[0])
+        self.instance_count += 1
+        return r
+
+    def build(self) -> "OracleScene":
+        lib().psyref_scene_build(self.h)
+        return self
+
+    def dump_bvh(self, mesh: int):
+        n = lib().psyref_scene_dump_bvh(self.h, mesh, None, 0, None, 0)
+        out = np.zeros(n, np.uint32)
+        order = np.zeros(self.instance_count if mesh < 0 else self.tri_counts[mesh], np.uint32)
+        lib().psyref_scene_dump_bvh(self.h, mesh, out.ctypes.data_as(_u32p), n, order.ctypes.data_as(_u32p),
+                                    len(order))
+        return out, order
+
+    def trace(self, rays: dict, batch: int = 4096, threads: int = 0, want_geometry: bool = False) -> dict:
+        """Ray-stream tracing of SoA rays; returns hits plus the ray-node test
+        count and the summed trace time per thread."""
+        n = len(rays["ox"])
+        fields = [f32(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "time", "max_t")]
+        occl = rays.get("occlusion")
+        occl = None if occl is None else np.ascontiguousarray(occl, dtype=np.uint8)
+        out = dict(kind=np.zeros(n, np.uint8), t=np.zeros(n, np.float32), b0=np.zeros(n, np.float32),
+                   b1=np.zeros(n, np.float32), b2=np.zeros(n, np.float32),
+                   prim_id=np.zeros(n, np.uint32), inst_id=np.zeros(n, np.uint32))
+        nor_g = np.zeros((n, 3), np.float32) if want_geometry else None
+        pos = np.zeros((n, 4), np.float32) if want_geometry else None
+        tests = C.c_uint64()
+        secs = C.c_double()
+        if threads <= 0:
+            threads = os.cpu_count() or 1
+        rc = lib().psyref_trace(self.h, *[fptr(a) for a in fields],
+                                None if occl is None else occl.ctypes.data_as(_u8p), n, batch, threads,
+                                out["kind"].ctypes.data_as(_u8p), fptr(out["t"]), fptr(out["b0"]),
+                                fptr(out["b1"]), fptr(out["b2"]), out["prim_id"].ctypes.data_as(_u32p),
+                                out["inst_id"].ctypes.data_as(_u32p),
+                                None if nor_g is None else fptr(nor_g), None if pos is None else fptr(pos),
+                                C.byref(tests), C.byref(secs))
+        if rc != 0:
+            raise RuntimeError(f"psyref_trace failed ({rc})")
+        out["node_tests"] = tests.value
+        out["seconds"] = secs.value
+        out["threads"] = threads
+        if want_geometry:
+            out["nor_g"] = nor_g
+            out["pos"] = pos
+        return out
+
+    def __del__(self):
+        try:
+            lib().psyref_scene_free(self.h)
+        except Exception:
+            pass

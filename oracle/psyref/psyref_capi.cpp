@@ -1,0 +1,221 @@
+// psyref — TEST INFRASTRUCTURE ONLY (parity oracle / CPU baseline).
+//
+// Flat C interface over the restatement, loaded by the test suite (ctypes)
+// and by the benchmark's CPU-baseline leg.  Never linked by the product.
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+#include "psyref_scene.h"
+
+using namespace psyref;
+
+namespace {
+struct OracleScene {
+    Assembly root;
+    bool built = false;
+};
+Transform xf_from(const float* v) { return Transform::from_values(v); }
+void xf_to(const Transform& m, float* v) { m.to_values(v); }
+
+// Canonical pre-order dump of a BVH4 (shared format with the product's dump):
+//   internal: 0, child_count, traversal_code, bounds_len, bounds_len*24 f32 bits, children...
+//   leaf:     1, obj_start, obj_end
+void dump_node(const BVH4& bvh, size_t ni, std::vector<uint32_t>& out) {
+    const BVH4Node& n = bvh.nodes[ni];
+    if (n.leaf) {
+        out.push_back(1);
+        out.push_back((uint32_t)n.obj_start);
+        out.push_back((uint32_t)n.obj_end);
+        return;
+    }
+    out.push_back(0);
+    out.push_back((uint32_t)n.children_len);
+    out.push_back(n.traversal_code);
+    out.push_back((uint32_t)n.bounds.size());
+    for (const BBox4& b : n.bounds) {
+        const float* arrs[6] = {b.xmin, b.xmax, b.ymin, b.ymax, b.zmin, b.zmax};
+        for (int a = 0; a < 6; a++)
+            for (int i = 0; i < 4; i++) out.push_back(f2u(arrs[a][i]));
+    }
+    for (size_t c = 0; c < n.children_len; c++) dump_node(bvh, n.children_start + c, out);
+}
+}  // namespace
+
+extern "C" {
+
+// ---- math hooks: the reference's unit tests are restated against these ----
+float psyref_lerp_f32(float a, float b, float alpha) { return lerp(a, b, alpha); }
+float psyref_lerp_slice_f32(const float* s, uint32_t n, float alpha) { return lerp_slice(s, n, alpha); }
+void psyref_xform_lerp(const float* a, const float* b, float alpha, float* out) {
+    xf_to(lerp(xf_from(a), xf_from(b), alpha), out);
+}
+void psyref_xform_mul(const float* a, const float* b, float* out) { xf_to(xform_compose(xf_from(a), xf_from(b)), out); }
+void psyref_xform_inverse(const float* a, float* out) { xf_to(xform_inverse(xf_from(a)), out); }
+void psyref_point_xform(const float* p, const float* m, float* out) {
+    Vec3 r = xform_point(Vec3(p[0], p[1], p[2]), xf_from(m));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void psyref_vector_xform(const float* p, const float* m, float* out) {
+    Vec3 r = xform_vector(Vec3(p[0], p[1], p[2]), xf_from(m));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void psyref_normal_xform(const float* p, const float* m, float* out) {
+    Vec3 r = xform_normal(Vec3(p[0], p[1], p[2]), xf_from(m));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void psyref_vec_lerp(const float* a, const float* b, float alpha, float* out) {
+    Vec3 r = lerp(Vec3(a[0], a[1], a[2]), Vec3(b[0], b[1], b[2]), alpha);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void psyref_vec_normalize(const float* a, float* out) {
+    Vec3 r = normalized(Vec3(a[0], a[1], a[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float psyref_vec_length(const float* a) { return length(Vec3(a[0], a[1], a[2])); }
+float psyref_vec_dot(const float* a, const float* b) { return dot(Vec3(a[0], a[1], a[2]), Vec3(b[0], b[1], b[2])); }
+void psyref_vec_cross(const float* a, const float* b, float* out) {
+    Vec3 r = cross(Vec3(a[0], a[1], a[2]), Vec3(b[0], b[1], b[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float psyref_increment_ulp(float v) { return increment_ulp(v); }
+float psyref_decrement_ulp(float v) { return decrement_ulp(v); }
+float psyref_fp_gamma(uint32_t n) { return fp_gamma(n); }
+uint64_t psyref_log2_64(uint64_t n) { return log2_64(n); }
+uint32_t psyref_hash_u32(uint32_t n, uint32_t seed) { return hash_u32(n, seed); }
+uint64_t psyref_hash_u64(uint64_t n, uint64_t seed) { return hash_u64(n, seed); }
+void psyref_quick_select_i32(int32_t* s, uint32_t n, uint32_t nth) { quick_select_i32(s, n, nth); }
+void psyref_traversal_table(uint8_t* out) { std::memcpy(out, traversal_table(), 8 * 48); }
+
+// Triangle test (triangle.rs:58).  out = (t, b0, b1, b2).  Returns 1 on hit.
+int psyref_triangle_intersect(const float* orig, const float* dir, float max_t, const float* tri, float* out) {
+    RayTriPrecompute pre(Vec3(dir[0], dir[1], dir[2]));
+    TriHit h;
+    bool hit = intersect_triangle(Vec3(orig[0], orig[1], orig[2]), pre, max_t, Vec3(tri[0], tri[1], tri[2]),
+                                  Vec3(tri[3], tri[4], tri[5]), Vec3(tri[6], tri[7], tri[8]), h);
+    if (hit) { out[0] = h.t; out[1] = h.b0; out[2] = h.b1; out[3] = h.b2; }
+    return hit ? 1 : 0;
+}
+
+// BBox4 test (bbox4.rs:63).  b = xmin[4], xmax[4], ymin[4], ymax[4], zmin[4], zmax[4].
+uint32_t psyref_bbox4_intersect(const float* b, const float* orig, const float* dir_inv, float max_t) {
+    BBox4 bb;
+    std::memcpy(bb.xmin, b + 0, 16); std::memcpy(bb.xmax, b + 4, 16);
+    std::memcpy(bb.ymin, b + 8, 16); std::memcpy(bb.ymax, b + 12, 16);
+    std::memcpy(bb.zmin, b + 16, 16); std::memcpy(bb.zmax, b + 20, 16);
+    return bb.intersect_ray(Vec3(orig[0], orig[1], orig[2]), Vec3(dir_inv[0], dir_inv[1], dir_inv[2]), max_t);
+}
+
+// ---- scene -----------------------------------------------------------------
+void* psyref_scene_new() { return new OracleScene(); }
+void psyref_scene_free(void* s) { delete static_cast<OracleScene*>(s); }
+
+// verts: [time_samples][vert_count][3]; normals: NULL or same layout.
+int psyref_scene_add_mesh(void* s, const float* verts, uint32_t vert_count, uint32_t time_samples,
+                          const float* normals, const uint32_t* tris, uint32_t tri_count) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    sc->root.objects.push_back(
+        TriangleMesh::from_verts_and_indices(verts, vert_count, time_samples, normals, tris, tri_count));
+    return (int)sc->root.objects.size() - 1;
+}
+
+// xforms: [n][12] values a..l of Transform::new_from_values (world -> object).
+int psyref_scene_add_instance(void* s, uint32_t mesh, int32_t shader, const float* xforms, uint32_t n_xf) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    std::vector<Transform> xf;
+    for (uint32_t i = 0; i < n_xf; i++) xf.push_back(xf_from(xforms + 12 * i));
+    sc->root.add_instance(false, mesh, shader, xf.data(), n_xf);
+    return (int)sc->root.instances.size() - 1;
+}
+
+int psyref_scene_build(void* s) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    sc->root.build();
+    sc->built = true;
+    return 0;
+}
+
+// Canonical dump of a mesh's BVH4 (or the instance BVH when mesh < 0) and of
+// the object order in its leaves.  Returns the number of u32 written, or the
+// required size when `cap` is too small.
+uint64_t psyref_scene_dump_bvh(void* s, int32_t mesh, uint32_t* out, uint64_t cap, uint32_t* order, uint64_t ocap) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    const BVH4& bvh = mesh < 0 ? sc->root.object_accel : sc->root.objects[mesh]->accel;
+    std::vector<uint32_t> v;
+    v.push_back((uint32_t)bvh.has_root);
+    v.push_back((uint32_t)bvh.depth);
+    v.push_back((uint32_t)bvh.node_count);
+    if (bvh.has_root) dump_node(bvh, 0, v);
+    if (v.size() <= cap) std::memcpy(out, v.data(), v.size() * 4);
+    if (order) {
+        if (mesh < 0) {
+            for (size_t i = 0; i < sc->root.instances.size() && i < ocap; i++)
+                order[i] = (uint32_t)sc->root.instances[i].id;
+        } else {
+            const auto& idx = sc->root.objects[mesh]->indices;
+            for (size_t i = 0; i < idx.size() && i < ocap; i++) order[i] = idx[i].orig;
+        }
+    }
+    return v.size();
+}
+
+// Traces `n` rays (SoA) in batches of `batch` rays (<= 65536) on `threads`
+// worker threads, one Tracer per thread, as Renderer::render_job does
+// (renderer.rs:194-310).  Only the trace calls are timed; `seconds` receives
+// the summed per-thread trace time divided by the thread count.
+int psyref_trace(void* s, const float* ox, const float* oy, const float* oz, const float* dx, const float* dy,
+                 const float* dz, const float* time, const float* max_t, const uint8_t* occl, uint64_t n,
+                 uint32_t batch, uint32_t threads, uint8_t* kind, float* t, float* b0, float* b1, float* b2,
+                 uint32_t* prim_id, uint32_t* inst_id, float* nor_g /*n*3 or NULL*/, float* pos /*n*4 or NULL*/,
+                 uint64_t* node_tests_out, double* seconds) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    if (!sc->built || batch == 0 || batch > 65536) return -1;
+    if (threads == 0) threads = 1;
+    std::atomic<uint64_t> next{0};
+    std::atomic<uint64_t> total_tests{0};
+    std::vector<double> tsec(threads, 0.0);
+    auto worker = [&](uint32_t w) {
+        Tracer tracer(&sc->root);
+        RayBatch rays;
+        for (;;) {
+            uint64_t start = next.fetch_add(batch);
+            if (start >= n) break;
+            uint64_t end = std::min<uint64_t>(n, start + batch);
+            rays.clear();
+            for (uint64_t i = start; i < end; i++) {
+                Ray r;
+                r.orig = Vec3(ox[i], oy[i], oz[i]);
+                r.dir = Vec3(dx[i], dy[i], dz[i]);
+                r.time = time[i];
+                r.wavelength = 0.0f;
+                r.max_t = max_t[i];
+                rays.push(r, occl && occl[i] != 0);
+            }
+            auto t0 = std::chrono::steady_clock::now();
+            const std::vector<Isect>& is = tracer.trace(rays);
+            auto t1 = std::chrono::steady_clock::now();
+            tsec[w] += std::chrono::duration<double>(t1 - t0).count();
+            for (uint64_t i = start; i < end; i++) {
+                const Isect& h = is[i - start];
+                kind[i] = h.kind;
+                t[i] = h.t; b0[i] = h.b0; b1[i] = h.b1; b2[i] = h.b2;
+                prim_id[i] = h.prim_id;
+                inst_id[i] = h.inst_id;
+                if (nor_g) { nor_g[3 * i] = h.nor_g.x; nor_g[3 * i + 1] = h.nor_g.y; nor_g[3 * i + 2] = h.nor_g.z; }
+                if (pos) { pos[4 * i] = h.pos.x; pos[4 * i + 1] = h.pos.y; pos[4 * i + 2] = h.pos.z; pos[4 * i + 3] = h.pos_err; }
+            }
+        }
+        total_tests += tracer.node_tests;
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t w = 0; w < threads; w++) pool.emplace_back(worker, w);
+    for (auto& th : pool) th.join();
+    double sum = 0;
+    for (double v : tsec) sum += v;
+    if (seconds) *seconds = sum / threads;
+    if (node_tests_out) *node_tests_out = total_tests.load();
+    return 0;
+}
+
+}  // extern "C"

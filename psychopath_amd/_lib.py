@@ -1,0 +1,164 @@
+"""ctypes binding of libpsy_rt.so (include/psy_rt.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or ``make -C
+psychopath_amd/csrc``).  Loading fails loudly when it is missing: there is no
+CPU fallback for the GPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsy_rt.so")
+
+PSY_OK = 0
+PSY_HIT_MISS = 0
+PSY_HIT_OCCLUDE = 1
+PSY_HIT_SURFACE = 2
+PSY_RAY_OCCLUSION = 1
+PSY_MEM_HOST = 0
+PSY_MEM_DEVICE = 1
+PSY_NONE = 0xFFFFFFFF
+
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("vert_count", C.c_uint32),
+        ("time_samples", C.c_uint32),
+        ("verts", _fp),
+        ("normals", _fp),
+        ("tri_count", C.c_uint32),
+        ("indices", _u32p),
+    ]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_uint32),
+        ("shader", C.c_uint32),
+        ("xform_start", C.c_uint32),
+        ("xform_count", C.c_uint32),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("mesh_count", C.c_uint32),
+        ("meshes", C.POINTER(MeshDesc)),
+        ("instance_count", C.c_uint32),
+        ("instances", C.POINTER(InstanceDesc)),
+        ("xform_count", C.c_uint32),
+        ("xforms", _fp),
+    ]
+
+
+class RaySoA(C.Structure):
+    _fields_ = [
+        ("ox", C.c_void_p), ("oy", C.c_void_p), ("oz", C.c_void_p),
+        ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p),
+        ("time", C.c_void_p), ("max_t", C.c_void_p), ("flags", C.c_void_p),
+    ]
+
+
+class HitSoA(C.Structure):
+    _fields_ = [
+        ("kind", C.c_void_p),
+        ("t", C.c_void_p), ("b0", C.c_void_p), ("b1", C.c_void_p), ("b2", C.c_void_p),
+        ("prim_id", C.c_void_p), ("inst_id", C.c_void_p),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("node_count", C.c_uint64),
+        ("leaf_count", C.c_uint64),
+        ("motion_bounds", C.c_uint64),
+        ("tri_count", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("node_bytes", C.c_uint32),
+        ("motion_bounds_bytes", C.c_uint32),
+        ("tri_bytes", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("build_seconds", C.c_double),
+    ]
+
+
+class TraceCounters(C.Structure):
+    _fields_ = [
+        ("node_visits", C.c_uint64),
+        ("motion_node_visits", C.c_uint64),
+        ("tri_tests", C.c_uint64),
+        ("motion_tri_tests", C.c_uint64),
+        ("instance_visits", C.c_uint64),
+        ("rays", C.c_uint64),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_SIGNATURES = {
+    "psy_last_error": (C.c_char_p, []),
+    "psy_version": (C.c_char_p, []),
+    "psy_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "psy_scene_destroy": (C.c_int, [C.c_void_p]),
+    "psy_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+    "psy_scene_dump_bvh": (C.c_int64, [C.c_void_p, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_uint64]),
+    "psy_tracer_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "psy_tracer_destroy": (C.c_int, [C.c_void_p]),
+    "psy_trace": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA), C.c_uint32]),
+    "psy_trace_async": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA)]),
+    "psy_tracer_sync": (C.c_int, [C.c_void_p]),
+    "psy_tracer_last_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "psy_trace_counted": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA),
+                                    C.POINTER(TraceCounters)]),
+    "psy_tracer_rays_traced": (C.c_uint64, [C.c_void_p]),
+    "psy_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "psy_device_free": (C.c_int, [C.c_void_p]),
+    "psy_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "psy_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "psy_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "psy_device_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float)]),
+}
+
+
+class PsyError(RuntimeError):
+    """A psy_rt call returned an error code."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Returns the loaded library, loading it on first use."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the GPU path has no fallback)")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(fn: str, rc: int) -> None:
+    if rc != PSY_OK:
+        msg = lib().psy_last_error().decode(errors="replace")
+        raise PsyError(fn, rc, msg)
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)

@@ -1,0 +1,400 @@
+// C ABI (include/psy_rt.h): scene upload, tracers and device helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/psy_rt.h"
+#include "scene.h"
+#include "trace.h"
+
+using namespace psy;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define PSY_HIP(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(PSY_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int upload(const std::vector<T>& v, T** out) {
+    *out = nullptr;
+    if (v.empty()) return PSY_OK;
+    PSY_HIP(hipMalloc((void**)out, v.size() * sizeof(T)));
+    PSY_HIP(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PSY_OK;
+}
+
+}  // namespace
+
+struct psy_scene {
+    PackedScene host;
+    GpuNode* nodes = nullptr;
+    GpuBox4* motion = nullptr;
+    uint8_t* tris = nullptr;
+    GpuMesh* meshes = nullptr;
+    GpuInstance* instances = nullptr;
+    GpuXform* xforms = nullptr;
+    GpuXform* inv_static = nullptr;
+    uint64_t device_bytes = 0;
+    bool uploaded = false;
+
+    ~psy_scene() {
+        (void)hipFree(nodes);
+        (void)hipFree(motion);
+        (void)hipFree(tris);
+        (void)hipFree(meshes);
+        (void)hipFree(instances);
+        (void)hipFree(xforms);
+        (void)hipFree(inv_static);
+    }
+};
+
+struct psy_tracer {
+    const psy_scene* scene = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t max_batch = 0;
+    uint64_t rays_traced = 0;
+    bool timed = false;
+    // device staging for host-memory calls: 8 ray floats + flags, hit fields
+    void* staging = nullptr;
+    unsigned long long* counters = nullptr;
+
+    ~psy_tracer() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(staging);
+        (void)hipFree(counters);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Uploads the packed scene on first use, so scenes can be built and
+// inspected on hosts without a GPU.
+int ensure_uploaded(psy_scene* s) {
+    if (s->uploaded) return PSY_OK;
+    const PackedScene& h = s->host;
+    int r = PSY_OK;
+    if (!r) r = upload(h.nodes, &s->nodes);
+    if (!r) r = upload(h.motion, &s->motion);
+    if (!r) r = upload(h.tris, &s->tris);
+    if (!r) r = upload(h.meshes, &s->meshes);
+    if (!r) r = upload(h.instances, &s->instances);
+    if (!r) r = upload(h.xforms, &s->xforms);
+    if (!r) r = upload(h.inv_static, &s->inv_static);
+    if (r == PSY_OK) s->uploaded = true;
+    return r;
+}
+
+TraceArgs scene_args(const psy_scene* s) {
+    TraceArgs a{};
+    a.nodes = s->nodes;
+    a.motion = s->motion;
+    a.tris = s->tris;
+    a.meshes = s->meshes;
+    a.instances = s->instances;
+    a.xforms = s->xforms;
+    a.inv_static = s->inv_static;
+    a.top_root = s->host.top_root;
+    return a;
+}
+
+void set_rays(TraceArgs& a, const psy_ray_soa* r) {
+    a.ox = r->ox; a.oy = r->oy; a.oz = r->oz;
+    a.dx = r->dx; a.dy = r->dy; a.dz = r->dz;
+    a.time = r->time; a.max_t = r->max_t; a.flags = r->flags;
+}
+void set_hits(TraceArgs& a, const psy_hit_soa* h) {
+    a.kind = h->kind;
+    a.t = h->t; a.b0 = h->b0; a.b1 = h->b1; a.b2 = h->b2;
+    a.prim_id = h->prim_id; a.inst_id = h->inst_id;
+}
+int check_soa(const psy_ray_soa* r, const psy_hit_soa* h) {
+    if (!r || !h) return fail(PSY_E_INVALID, "rays or hits is NULL");
+    if (!r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->time || !r->max_t)
+        return fail(PSY_E_INVALID, "a ray array is NULL");
+    if (!h->kind || !h->t || !h->b0 || !h->b1 || !h->b2 || !h->prim_id || !h->inst_id)
+        return fail(PSY_E_INVALID, "a hit array is NULL");
+    return PSY_OK;
+}
+
+int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits, bool counted) {
+    TraceArgs a = scene_args(tr->scene);
+    set_rays(a, rays);
+    set_hits(a, hits);
+    a.n = n;
+    a.counters = tr->counters;
+    PSY_HIP(hipEventRecord(tr->ev0, tr->stream));
+    hipError_t e = launch_trace(a, counted, tr->stream);
+    if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+    PSY_HIP(hipEventRecord(tr->ev1, tr->stream));
+    tr->timed = true;
+    tr->rays_traced += n;
+    return PSY_OK;
+}
+
+// Host-memory path: stage rays in, trace, copy hits out, in chunks of
+// max_batch rays.
+int trace_host(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits) {
+    const uint64_t B = tr->max_batch;
+    if (!tr->staging) {
+        // 8 f32 + 1 u8 per ray in, 1 u8 + 4 f32 + 2 u32 per ray out.
+        PSY_HIP(hipMalloc(&tr->staging, B * (8 * 4 + 1 + 1 + 6 * 4) + 256));
+    }
+    uint8_t* base = static_cast<uint8_t*>(tr->staging);
+    float* f = reinterpret_cast<float*>(base);
+    psy_ray_soa dr{f, f + B, f + 2 * B, f + 3 * B, f + 4 * B, f + 5 * B, f + 6 * B, f + 7 * B, nullptr};
+    float* hf = f + 8 * B;
+    uint32_t* hu = reinterpret_cast<uint32_t*>(hf + 4 * B);
+    uint8_t* flags = reinterpret_cast<uint8_t*>(hu + 2 * B);
+    uint8_t* kind = flags + B;
+    psy_hit_soa dh{kind, hf, hf + B, hf + 2 * B, hf + 3 * B, hu, hu + B};
+
+    for (uint64_t s = 0; s < n; s += B) {
+        uint64_t m = std::min(B, n - s);
+        const float* src[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->time, rays->max_t};
+        float* dst[8] = {(float*)dr.ox, (float*)dr.oy, (float*)dr.oz, (float*)dr.dx, (float*)dr.dy, (float*)dr.dz,
+                         (float*)dr.time, (float*)dr.max_t};
+        for (int k = 0; k < 8; k++)
+            PSY_HIP(hipMemcpyAsync(dst[k], src[k] + s, m * 4, hipMemcpyHostToDevice, tr->stream));
+        dr.flags = nullptr;
+        if (rays->flags) {
+            PSY_HIP(hipMemcpyAsync(flags, rays->flags + s, m, hipMemcpyHostToDevice, tr->stream));
+            dr.flags = flags;
+        }
+        int rc = enqueue(tr, &dr, m, &dh, false);
+        if (rc) return rc;
+        PSY_HIP(hipMemcpyAsync(hits->kind + s, dh.kind, m, hipMemcpyDeviceToHost, tr->stream));
+        float* hsrc[4] = {dh.t, dh.b0, dh.b1, dh.b2};
+        float* hdst[4] = {hits->t, hits->b0, hits->b1, hits->b2};
+        for (int k = 0; k < 4; k++)
+            PSY_HIP(hipMemcpyAsync(hdst[k] + s, hsrc[k], m * 4, hipMemcpyDeviceToHost, tr->stream));
+        PSY_HIP(hipMemcpyAsync(hits->prim_id + s, dh.prim_id, m * 4, hipMemcpyDeviceToHost, tr->stream));
+        PSY_HIP(hipMemcpyAsync(hits->inst_id + s, dh.inst_id, m * 4, hipMemcpyDeviceToHost, tr->stream));
+        PSY_HIP(hipStreamSynchronize(tr->stream));
+    }
+    return PSY_OK;
+}
+
+__global__ void copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* psy_last_error(void) { return g_error.c_str(); }
+const char* psy_version(void) { return "psy_rt 0.1 gfx950"; }
+
+int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
+    if (!out) return fail(PSY_E_INVALID, "out is NULL");
+    *out = nullptr;
+    psy_scene* s = new (std::nothrow) psy_scene();
+    if (!s) return fail(PSY_E_NOMEM, "out of host memory");
+    std::string err;
+    int rc;
+    try {
+        rc = pack_scene(desc, s->host, err);
+    } catch (const std::bad_alloc&) {
+        rc = PSY_E_NOMEM;
+        err = "out of host memory while building the scene";
+    }
+    if (rc != PSY_OK) {
+        delete s;
+        return fail(rc, err);
+    }
+    const PackedScene& h = s->host;
+    s->device_bytes = h.nodes.size() * sizeof(GpuNode) + h.motion.size() * sizeof(GpuBox4) + h.tris.size() +
+                      h.meshes.size() * sizeof(GpuMesh) + h.instances.size() * sizeof(GpuInstance) +
+                      (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform);
+    *out = s;
+    return PSY_OK;
+}
+
+int psy_scene_destroy(psy_scene* scene) {
+    delete scene;
+    return PSY_OK;
+}
+
+int psy_scene_get_info(const psy_scene* s, psy_scene_info* info) {
+    if (!s || !info) return fail(PSY_E_INVALID, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->node_count = s->host.nodes.size();
+    info->leaf_count = s->host.leaf_count;
+    info->motion_bounds = s->host.motion.size();
+    info->tri_count = s->host.tri_count;
+    info->device_bytes = s->device_bytes;
+    info->node_bytes = sizeof(GpuNode);
+    info->motion_bounds_bytes = sizeof(GpuBox4);
+    info->tri_bytes = sizeof(GpuTriStatic);
+    info->max_depth = s->host.max_depth;
+    info->build_seconds = s->host.build_seconds;
+    return PSY_OK;
+}
+
+int64_t psy_scene_dump_bvh(const psy_scene* s, int32_t mesh, uint32_t* out, uint64_t cap, uint32_t* order,
+                           uint64_t ocap) {
+    if (!s) return fail(PSY_E_INVALID, "scene is NULL");
+    const auto& bvhs = s->host.bvhs;
+    int64_t nmesh = (int64_t)bvhs.size() - 1;
+    if (mesh < -1 || mesh >= nmesh) return fail(PSY_E_INVALID, "mesh index out of range");
+    const HostBvh4& b = mesh < 0 ? bvhs.back() : bvhs[mesh];
+    std::vector<uint32_t> v;
+    dump_bvh4(b, v);
+    if (out && v.size() <= cap) std::memcpy(out, v.data(), v.size() * 4);
+    if (order) std::memcpy(order, b.order.data(), std::min<uint64_t>(ocap, b.order.size()) * 4);
+    return (int64_t)v.size();
+}
+
+int psy_tracer_create(const psy_scene* scene, uint64_t max_batch, psy_tracer** out) {
+    if (!scene || !out) return fail(PSY_E_INVALID, "NULL argument");
+    *out = nullptr;
+    if (max_batch == 0 || max_batch > (1ull << 31)) return fail(PSY_E_RANGE, "max_batch must be in [1, 2^31]");
+    if (scene->host.max_stack > (uint32_t)TRACE_STACK_CAPACITY)
+        return fail(PSY_E_RANGE, "scene BVHs are deeper than the traversal stack holds");
+    int up = ensure_uploaded(const_cast<psy_scene*>(scene));
+    if (up) return up;
+    psy_tracer* t = new (std::nothrow) psy_tracer();
+    if (!t) return fail(PSY_E_NOMEM, "out of host memory");
+    t->scene = scene;
+    t->max_batch = max_batch;
+    hipError_t e = hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&t->ev1);
+    if (e == hipSuccess) e = hipMalloc((void**)&t->counters, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        delete t;
+        return fail(PSY_E_DEVICE, std::string("tracer setup: ") + hipGetErrorString(e));
+    }
+    *out = t;
+    return PSY_OK;
+}
+
+int psy_tracer_destroy(psy_tracer* tracer) {
+    delete tracer;
+    return PSY_OK;
+}
+
+int psy_trace(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits, uint32_t mem) {
+    if (!tr) return fail(PSY_E_INVALID, "tracer is NULL");
+    int rc = check_soa(rays, hits);
+    if (rc) return rc;
+    if (n == 0) return PSY_OK;
+    if (mem == PSY_MEM_HOST) return trace_host(tr, rays, n, hits);
+    if (mem != PSY_MEM_DEVICE) return fail(PSY_E_INVALID, "mem must be PSY_MEM_HOST or PSY_MEM_DEVICE");
+    rc = enqueue(tr, rays, n, hits, false);
+    if (rc) return rc;
+    PSY_HIP(hipStreamSynchronize(tr->stream));
+    return PSY_OK;
+}
+
+int psy_trace_async(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits) {
+    if (!tr) return fail(PSY_E_INVALID, "tracer is NULL");
+    int rc = check_soa(rays, hits);
+    if (rc) return rc;
+    if (n == 0) return PSY_OK;
+    return enqueue(tr, rays, n, hits, false);
+}
+
+int psy_tracer_sync(psy_tracer* tr) {
+    if (!tr) return fail(PSY_E_INVALID, "tracer is NULL");
+    PSY_HIP(hipStreamSynchronize(tr->stream));
+    return PSY_OK;
+}
+
+int psy_tracer_last_kernel_ms(psy_tracer* tr, float* ms) {
+    if (!tr || !ms) return fail(PSY_E_INVALID, "NULL argument");
+    if (!tr->timed) return fail(PSY_E_STATE, "no kernel has run on this tracer");
+    PSY_HIP(hipEventSynchronize(tr->ev1));
+    PSY_HIP(hipEventElapsedTime(ms, tr->ev0, tr->ev1));
+    return PSY_OK;
+}
+
+int psy_trace_counted(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
+                      psy_trace_counters* c) {
+    if (!tr || !c) return fail(PSY_E_INVALID, "NULL argument");
+    int rc = check_soa(rays, hits);
+    if (rc) return rc;
+    std::memset(c, 0, sizeof(*c));
+    if (n == 0) return PSY_OK;
+    unsigned long long zero[8] = {};
+    PSY_HIP(hipMemcpyAsync(tr->counters, zero, sizeof(zero), hipMemcpyHostToDevice, tr->stream));
+    rc = enqueue(tr, rays, n, hits, true);
+    if (rc) return rc;
+    unsigned long long h[8];
+    PSY_HIP(hipMemcpyAsync(h, tr->counters, sizeof(h), hipMemcpyDeviceToHost, tr->stream));
+    PSY_HIP(hipStreamSynchronize(tr->stream));
+    c->node_visits = h[0];
+    c->motion_node_visits = h[1];
+    c->tri_tests = h[2];
+    c->motion_tri_tests = h[3];
+    c->instance_visits = h[4];
+    c->rays = n;
+    return PSY_OK;
+}
+
+uint64_t psy_tracer_rays_traced(const psy_tracer* tr) { return tr ? tr->rays_traced : 0; }
+
+int psy_device_alloc(uint64_t bytes, void** ptr) {
+    if (!ptr) return fail(PSY_E_INVALID, "ptr is NULL");
+    PSY_HIP(hipMalloc(ptr, bytes));
+    return PSY_OK;
+}
+int psy_device_free(void* ptr) {
+    PSY_HIP(hipFree(ptr));
+    return PSY_OK;
+}
+int psy_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+    PSY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PSY_OK;
+}
+int psy_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+    PSY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PSY_OK;
+}
+int psy_device_count(int* count) {
+    if (!count) return fail(PSY_E_INVALID, "count is NULL");
+    PSY_HIP(hipGetDeviceCount(count));
+    return PSY_OK;
+}
+
+int psy_device_copy_kernel(void* dst, const void* src, uint64_t bytes, float* ms) {
+    if (!dst || !src || bytes % 16 != 0) return fail(PSY_E_INVALID, "copy needs non-NULL pointers and bytes % 16 == 0");
+    hipEvent_t e0, e1;
+    PSY_HIP(hipEventCreate(&e0));
+    PSY_HIP(hipEventCreate(&e1));
+    uint64_t n = bytes / 16;
+    PSY_HIP(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, nullptr, static_cast<const float4*>(src),
+                       static_cast<float4*>(dst), n);
+    PSY_HIP(hipGetLastError());
+    PSY_HIP(hipEventRecord(e1, nullptr));
+    PSY_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    PSY_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return PSY_OK;
+}
+
+}  // extern "C"

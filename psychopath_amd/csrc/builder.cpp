@@ -1,0 +1,472 @@
+// Host BVH4 builder.  Follows the reference builder step for step so the GPU
+// traverses exactly the tree the reference would build:
+//   BVHBase::recursive_build        src/accel/bvh_base.rs:90-215
+//   acc_bounds / union rule         src/accel/bvh_base.rs:64-88, 116-132, 194-203
+//   sah_split / median_split        src/accel/objects_split.rs:153-238, 291-341
+//   partition / quick_select        src/algorithm.rs:44-81, 203-231
+//   merge_slices_append             src/algorithm.rs:233-262
+//   BVH4::construct_from_base       src/accel/bvh4.rs:186-358
+// Differences from the reference are in organisation only: per-object
+// mid-time boxes and centroids are computed once instead of at every level,
+// independent subtrees are built on worker threads, and BVH4 nodes are
+// emitted straight into 128-B GPU records.
+#include "builder.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <future>
+#include <memory>
+#include <thread>
+
+namespace psy {
+
+namespace {
+
+constexpr uint32_t SAH_BINS = 13;          // objects_split.rs:13
+constexpr uint32_t MAX_BASE_DEPTH = 42;    // bvh_base.rs:7
+constexpr float UNION_FACTOR = 1.4f;       // bvh_base.rs:12
+
+// Child visit order for one node code and ray octant, as the bvh_order table
+// defines it (sub_crates/bvh_order/build.rs:5-59): the children of a split
+// are visited low side first unless the ray runs toward negative coordinates
+// along that split's axis.
+uint8_t visit_order(int octant, int code) {
+    auto neg = [&](int axis) { return ((octant >> axis) & 1) != 0; };
+    auto pack = [](const int* p, int n) {
+        int v = 0;
+        for (int i = 0; i < n; i++) v |= p[i] << (2 * i);
+        return (uint8_t)v;
+    };
+    if (code < 27) {
+        int top = code % 3, left = (code / 3) % 3, right = code / 9;
+        int p[4] = {0, 1, 2, 3};
+        if (neg(left)) std::swap(p[0], p[1]);
+        if (neg(right)) std::swap(p[2], p[3]);
+        if (neg(top)) { std::swap(p[0], p[2]); std::swap(p[1], p[3]); }
+        return pack(p, 4);
+    }
+    if (code < 36) {  // left pair split, right child a leaf
+        int top = (code - 27) % 3, left = (code - 27) / 3;
+        int p[3] = {0, 1, 2};
+        if (neg(left)) std::swap(p[0], p[1]);
+        if (neg(top)) { int q[3] = {p[2], p[0], p[1]}; std::memcpy(p, q, sizeof(p)); }
+        return pack(p, 3);
+    }
+    if (code < 45) {  // right pair split, left child a leaf
+        int top = (code - 36) % 3, right = (code - 36) / 3;
+        int p[3] = {0, 1, 2};
+        if (neg(right)) std::swap(p[1], p[2]);
+        if (neg(top)) { int q[3] = {p[1], p[2], p[0]}; std::memcpy(p, q, sizeof(p)); }
+        return pack(p, 3);
+    }
+    int top = code - 45;
+    int p[2] = {0, 1};
+    if (neg(top)) std::swap(p[0], p[1]);
+    return pack(p, 2);
+}
+
+struct OrderTable {
+    uint8_t t[8 * 48];
+    OrderTable() {
+        for (int o = 0; o < 8; o++)
+            for (int c = 0; c < 48; c++) t[o * 48 + c] = visit_order(o, c);
+    }
+};
+
+// Per-object data derived once from the input bounds.
+struct ObjInfo {
+    Box mid;        // lerp_slice(bounds, 0.5)
+    V3 mid_center;  // mid.center()  (BBox::center, a lerp)
+    V3 centroid;    // (mid.min + mid.max) * 0.5
+};
+
+// BVH2 node of the intermediate tree (BVHBaseNode, bvh_base.rs:23-35).
+struct BaseNode {
+    bool leaf = false;
+    std::vector<Box> bounds;  // time samples
+    std::unique_ptr<BaseNode> kid[2];
+    uint8_t axis = 0;
+    uint32_t obj_start = 0, obj_end = 0;
+};
+
+Box lerp_slice_box(const Box* s, uint32_t n, float alpha) {
+    uint32_t i1;
+    float a2;
+    if (!lerp_slice_pos(n, alpha, i1, a2)) return s[n - 1];
+    return box_lerp(s[i1], s[i1 + 1], a2);
+}
+
+// The union rule: keep one union box unless it is more than 1.4x the mean
+// time-sample area (bvh_base.rs:116-132, 194-203).
+void union_or_samples(const std::vector<Box>& samples, std::vector<Box>& out) {
+    Box u = box_empty();
+    for (const Box& b : samples) u = box_union(u, b);
+    float area = 0.0f;
+    for (const Box& b : samples) area = area + box_area(b);
+    float avg = area / (float)samples.size();
+    out.clear();
+    if (box_area(u) <= (avg * UNION_FACTOR)) out.push_back(u);
+    else out = samples;
+}
+
+class Builder {
+   public:
+    Builder(const BuildInput& in, uint32_t per_leaf, unsigned threads)
+        : in_(in), per_leaf_(per_leaf), threads_(threads) {
+        info_.resize(in.count);
+        for (uint32_t k = 0; k < in.count; k++) {
+            const Box* s = in.bounds + in.offsets[k];
+            uint32_t n = in.offsets[k + 1] - in.offsets[k];
+            ObjInfo& o = info_[k];
+            o.mid = lerp_slice_box(s, n, 0.5f);
+            o.mid_center = box_center(o.mid);
+            o.centroid = mul(add(o.mid.lo, o.mid.hi), 0.5f);
+        }
+    }
+
+    std::unique_ptr<BaseNode> build(uint32_t* keys, uint32_t n, uint32_t offset, uint32_t depth, unsigned spawn_budget,
+                                    uint32_t& max_depth) {
+        auto node = std::make_unique<BaseNode>();
+        if (n <= per_leaf_) {
+            leaf_bounds(keys, n, node->bounds);
+            node->leaf = true;
+            node->obj_start = offset;
+            node->obj_end = offset + n;
+            max_depth = std::max(max_depth, depth);
+            return node;
+        }
+        uint32_t split;
+        uint8_t axis;
+        if ((uint32_t)(63 - __builtin_clzll((unsigned long long)n)) < (MAX_BASE_DEPTH - depth))
+            sah_split(keys, n, split, axis);
+        else
+            median_split(keys, n, split, axis);
+        node->axis = axis;
+
+        uint32_t d0 = 0, d1 = 0;
+        if (spawn_budget > 1 && n > 20000) {
+            unsigned half = spawn_budget / 2;
+            auto fut = std::async(std::launch::async, [&] {
+                return build(keys + split, n - split, offset + split, depth + 1, spawn_budget - half, d1);
+            });
+            node->kid[0] = build(keys, split, offset, depth + 1, half, d0);
+            node->kid[1] = fut.get();
+        } else {
+            node->kid[0] = build(keys, split, offset, depth + 1, 1, d0);
+            node->kid[1] = build(keys + split, n - split, offset + split, depth + 1, 1, d1);
+        }
+        max_depth = std::max(max_depth, std::max(d0, d1));
+
+        std::vector<Box> merged;
+        merge_union(node->kid[0]->bounds, node->kid[1]->bounds, merged);
+        union_or_samples(merged, node->bounds);
+        return node;
+    }
+
+   private:
+    const BuildInput& in_;
+    uint32_t per_leaf_;
+    unsigned threads_;
+    std::vector<ObjInfo> info_;
+
+    // acc_bounds (bvh_base.rs:64-88) followed by the union rule.
+    void leaf_bounds(const uint32_t* keys, uint32_t n, std::vector<Box>& out) {
+        uint32_t max_len = 0;
+        for (uint32_t i = 0; i < n; i++) max_len = std::max(max_len, in_.offsets[keys[i] + 1] - in_.offsets[keys[i]]);
+        std::vector<Box> acc(max_len, box_empty());
+        for (uint32_t i = 0; i < n; i++) {
+            const Box* s = in_.bounds + in_.offsets[keys[i]];
+            uint32_t len = in_.offsets[keys[i] + 1] - in_.offsets[keys[i]];
+            if (len == max_len) {
+                for (uint32_t j = 0; j < len; j++) acc[j] = box_union(acc[j], s[j]);
+            } else {
+                float sc = (float)(max_len - 1);
+                for (uint32_t j = 0; j < max_len; j++) acc[j] = box_union(acc[j], lerp_slice_box(s, len, (float)j / sc));
+            }
+        }
+        union_or_samples(acc, out);
+    }
+
+    // merge_slices_append with union (algorithm.rs:233-262).
+    static void merge_union(const std::vector<Box>& a, const std::vector<Box>& b, std::vector<Box>& out) {
+        out.clear();
+        uint32_t na = (uint32_t)a.size(), nb = (uint32_t)b.size();
+        if (na == 0 || nb == 0) return;
+        if (na == nb) {
+            for (uint32_t i = 0; i < na; i++) out.push_back(box_union(a[i], b[i]));
+        } else if (na > nb) {
+            float s = (float)(na - 1);
+            for (uint32_t i = 0; i < na; i++) out.push_back(box_union(a[i], lerp_slice_box(b.data(), nb, (float)i / s)));
+        } else {
+            float s = (float)(nb - 1);
+            for (uint32_t i = 0; i < nb; i++) out.push_back(box_union(lerp_slice_box(a.data(), na, (float)i / s), b[i]));
+        }
+    }
+
+    // Hoare-style partition of algorithm.rs:44-81.
+    template <typename Pred>
+    static uint32_t hoare_partition(uint32_t* s, uint32_t n, Pred&& pred) {
+        uint32_t a = 0, b = n;
+        for (;;) {
+            while (true) {
+                if (a == b) return a;
+                if (!pred(s[a])) break;
+                a++;
+            }
+            while (true) {
+                b--;
+                if (a == b) return a;
+                if (pred(s[b])) break;
+            }
+            std::swap(s[a], s[b]);
+            a++;
+        }
+    }
+
+    // sah_split (objects_split.rs:153-238)
+    void sah_split(uint32_t* keys, uint32_t n, uint32_t& split, uint8_t& axis) {
+        Box cb = box_empty();
+        for (uint32_t i = 0; i < n; i++) cb = box_union_pt(cb, info_[keys[i]].mid_center);
+
+        float divs[3][SAH_BINS - 1];
+        for (int d = 0; d < 3; d++) {
+            float lo = comp(cb.lo, d), extent = comp(cb.hi, d) - lo;
+            for (uint32_t k = 0; k < SAH_BINS - 1; k++) divs[d][k] = lo + extent * ((float)(k + 1) / (float)SAH_BINS);
+        }
+
+        Box left[3][SAH_BINS - 1], right[3][SAH_BINS - 1];
+        uint32_t nl[3][SAH_BINS - 1] = {}, nr[3][SAH_BINS - 1] = {};
+        for (int d = 0; d < 3; d++)
+            for (uint32_t k = 0; k < SAH_BINS - 1; k++) left[d][k] = right[d][k] = box_empty();
+        for (uint32_t i = 0; i < n; i++) {
+            const ObjInfo& o = info_[keys[i]];
+            for (int d = 0; d < 3; d++) {
+                float c = comp(o.centroid, d);
+                for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
+                    if (c <= divs[d][k]) {
+                        left[d][k] = box_union(left[d][k], o.mid);
+                        nl[d][k]++;
+                    } else {
+                        right[d][k] = box_union(right[d][k], o.mid);
+                        nr[d][k]++;
+                    }
+                }
+            }
+        }
+
+        int best_d = 0;
+        float best_div = 0.0f;
+        float best = __builtin_huge_valf();
+        for (int d = 0; d < 3; d++)
+            for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
+                float lc = box_area(left[d][k]) * (float)nl[d][k];
+                float rc = box_area(right[d][k]) * (float)nr[d][k];
+                float cost = (lc * box_diag(left[d][k])) + (rc * box_diag(right[d][k]));
+                if (cost < best) {
+                    best_d = d;
+                    best_div = divs[d][k];
+                    best = cost;
+                }
+            }
+
+        uint32_t s = hoare_partition(keys, n, [&](uint32_t k) { return comp(info_[k].centroid, best_d) < best_div; });
+        if (s < 1) s = 1;
+        else if (s >= n) s = n - 1;
+        split = s;
+        axis = (uint8_t)best_d;
+    }
+
+    static uint64_t hash64(uint64_t n, uint64_t seed) {  // src/hash.rs:12-21
+        uint64_t h = n;
+        for (int i = 0; i < 4; i++) {
+            h *= 0x8d879e7eb3c956e9ull;  // 32_416_190_071 * 314_604_959
+            h ^= h >> 32;
+            h ^= seed;
+        }
+        return h;
+    }
+
+    // median_split (objects_split.rs:291-341) with quick_select
+    // (algorithm.rs:203-231).
+    void median_split(uint32_t* keys, uint32_t n, uint32_t& split, uint8_t& axis_out) {
+        Box b = box_empty();
+        for (uint32_t i = 0; i < n; i++) b = box_union(b, info_[keys[i]].mid);
+        int axis = 0;
+        float largest = -__builtin_huge_valf();
+        for (int d = 0; d < 3; d++) {
+            float e = comp(b.hi, d) - comp(b.lo, d);
+            if (e > largest) { largest = e; axis = d; }
+        }
+        uint32_t place = n / 2;
+        if (place == 0) place = 1;
+
+        auto key_of = [&](uint32_t k) { return comp(info_[k].centroid, axis); };
+        uint32_t left = 0, right = n;
+        uint64_t seed = place;
+        for (;;) {
+            uint32_t i = left + (uint32_t)(hash64(right, seed) % (uint64_t)(right - left));
+            std::swap(keys[i], keys[right - 1]);
+            float pivot = key_of(keys[right - 1]);
+            uint32_t ii = left + hoare_partition(keys + left, right - 1 - left,
+                                                 [&](uint32_t k) { return key_of(k) < pivot; });
+            std::swap(keys[ii], keys[right - 1]);
+            if (ii == place) break;
+            if (ii > place) right = ii; else left = ii + 1;
+            seed += 1;
+        }
+        split = place;
+        axis_out = (uint8_t)axis;
+    }
+};
+
+// BVH4 collapse (bvh4.rs:186-358) into GPU records.
+class Collapser {
+   public:
+    explicit Collapser(HostBvh4& out) : out_(out), table_(traversal_order_table()) {}
+
+    // Returns the reference for `n`: a leaf reference or the node index.
+    uint32_t emit(const BaseNode* n, uint32_t slot) {
+        if (n->leaf) {
+            out_.node_count += 1;
+            out_.leaf_count += 1;
+            return make_leaf_ref(n->obj_start, n->obj_end - n->obj_start);
+        }
+        const BaseNode* l = n->kid[0].get();
+        const BaseNode* r = n->kid[1].get();
+        const BaseNode* ch[4] = {nullptr, nullptr, nullptr, nullptr};
+        uint32_t cnt;
+        uint32_t code;
+        if (!l->leaf && !r->leaf) {
+            ch[0] = l->kid[0].get(); ch[1] = l->kid[1].get(); ch[2] = r->kid[0].get(); ch[3] = r->kid[1].get();
+            cnt = 4;
+            code = n->axis + l->axis * 3u + r->axis * 9u;
+        } else if (!l->leaf) {
+            ch[0] = l->kid[0].get(); ch[1] = l->kid[1].get(); ch[2] = r;
+            cnt = 3;
+            code = n->axis + l->axis * 3u + 27u;
+        } else if (!r->leaf) {
+            ch[0] = l; ch[1] = r->kid[0].get(); ch[2] = r->kid[1].get();
+            cnt = 3;
+            code = n->axis + r->axis * 3u + 36u;
+        } else {
+            ch[0] = l; ch[1] = r;
+            cnt = 2;
+            code = n->axis + 45u;
+        }
+        out_.node_count += cnt;
+
+        uint32_t blen = 0;
+        for (int i = 0; i < 4; i++)
+            if (ch[i]) blen = std::max(blen, (uint32_t)ch[i]->bounds.size());
+        std::vector<GpuBox4> samples(blen);
+        for (uint32_t s = 0; s < blen; s++) {
+            float time = blen < 2 ? 0.0f : (float)s / (float)(blen - 1);
+            for (int i = 0; i < 4; i++) {
+                Box b = box_empty();
+                if (ch[i]) {
+                    if (blen < 2) b = ch[i]->bounds[0];
+                    else b = lerp_slice_box(ch[i]->bounds.data(), (uint32_t)ch[i]->bounds.size(), time);
+                }
+                samples[s].lo_x[i] = b.lo.x; samples[s].hi_x[i] = b.hi.x;
+                samples[s].lo_y[i] = b.lo.y; samples[s].hi_y[i] = b.hi.y;
+                samples[s].lo_z[i] = b.lo.z; samples[s].hi_z[i] = b.hi.z;
+            }
+        }
+
+        // Children get a contiguous block of slots, like the reference's
+        // per-node child array; leaves occupy no slot.
+        uint32_t internal = 0;
+        for (uint32_t i = 0; i < cnt; i++) internal += ch[i]->leaf ? 0 : 1;
+        uint32_t base = (uint32_t)out_.nodes.size();
+        out_.nodes.resize(out_.nodes.size() + internal);
+        uint32_t refs[4] = {REF_EMPTY, REF_EMPTY, REF_EMPTY, REF_EMPTY};
+        uint32_t next = base;
+        for (uint32_t i = 0; i < cnt; i++) {
+            if (ch[i]->leaf) refs[i] = emit(ch[i], 0);
+            else { refs[i] = emit(ch[i], next); next++; }
+        }
+
+        GpuNode& g = out_.nodes[slot];
+        std::memset(&g, 0, sizeof(g));
+        std::memcpy(g.lo_x, samples[0].lo_x, sizeof(GpuBox4));
+        for (int i = 0; i < 4; i++) g.child[i] = refs[i];
+        for (int o = 0; o < 8; o++) g.order[o] = table_[o * 48 + code];
+        g.meta = cnt | (blen << 8) | (code << 16);
+        g.motion = REF_EMPTY;
+        if (blen > 1) {
+            g.motion = (uint32_t)out_.motion.size();
+            out_.motion.insert(out_.motion.end(), samples.begin(), samples.end());
+        }
+        return slot;
+    }
+
+   private:
+    HostBvh4& out_;
+    const uint8_t* table_;
+};
+
+void dump_ref(const HostBvh4& bvh, uint32_t ref, std::vector<uint32_t>& out) {
+    if (ref & REF_LEAF) {
+        uint32_t first = (ref & ~REF_LEAF) >> 2, cnt = ref & 3u;
+        out.push_back(1);
+        out.push_back(first);
+        out.push_back(first + cnt);
+        return;
+    }
+    const GpuNode& g = bvh.nodes[ref];
+    uint32_t cnt = g.meta & 7u, blen = (g.meta >> 8) & 0xffu, code = (g.meta >> 16) & 0x3fu;
+    out.push_back(0);
+    out.push_back(cnt);
+    out.push_back(code);
+    out.push_back(blen);
+    for (uint32_t s = 0; s < blen; s++) {
+        const GpuBox4* b = blen > 1 ? &bvh.motion[g.motion + s] : reinterpret_cast<const GpuBox4*>(g.lo_x);
+        const float* f = b->lo_x;
+        for (int i = 0; i < 24; i++) {
+            uint32_t u;
+            std::memcpy(&u, f + i, 4);
+            out.push_back(u);
+        }
+    }
+    for (uint32_t i = 0; i < cnt; i++) dump_ref(bvh, g.child[i], out);
+}
+
+}  // namespace
+
+const uint8_t* traversal_order_table() {
+    static OrderTable t;
+    return t.t;
+}
+
+HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads) {
+    HostBvh4 out;
+    if (in.count == 0) return out;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    out.order.resize(in.count);
+    for (uint32_t i = 0; i < in.count; i++) out.order[i] = i;
+
+    Builder b(in, per_leaf, threads);
+    uint32_t base_depth = 0;
+    std::unique_ptr<BaseNode> root = b.build(out.order.data(), in.count, 0, 0, threads, base_depth);
+    out.depth = base_depth / 2 + 1;
+    out.root_bounds = root->bounds;
+
+    Collapser c(out);
+    if (root->leaf) {
+        out.root = c.emit(root.get(), 0);
+    } else {
+        out.nodes.resize(1);
+        out.root = c.emit(root.get(), 0);
+    }
+    return out;
+}
+
+void dump_bvh4(const HostBvh4& bvh, std::vector<uint32_t>& out) {
+    out.push_back(bvh.root != REF_EMPTY ? 1u : 0u);
+    out.push_back(bvh.depth);
+    out.push_back((uint32_t)bvh.node_count);
+    if (bvh.root != REF_EMPTY) dump_ref(bvh, bvh.root, out);
+}
+
+}  // namespace psy

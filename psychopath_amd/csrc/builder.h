@@ -1,0 +1,53 @@
+// Host BVH4 builder: the reference's accel builder (binned SAH BVH2 with the
+// union-vs-time-samples rule, median splits near the depth limit, collapse to
+// BVH4 with time-sampled bounds), emitting GPU node records directly.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "layout.h"
+#include "psymath.h"
+
+namespace psy {
+
+// Objects to build over.  Object k has time-sampled bounds
+// bounds[offsets[k] .. offsets[k+1]).
+struct BuildInput {
+    uint32_t count;
+    const Box* bounds;
+    const uint32_t* offsets;
+};
+
+// A built BVH4 in host form, ready to be appended to a scene's node arrays.
+struct HostBvh4 {
+    // Node records; child references of internal nodes are indices into
+    // `nodes`; leaf references carry positions in `order`.
+    std::vector<GpuNode> nodes;
+    // Time-sample records referenced by GpuNode::motion (index into this).
+    std::vector<GpuBox4> motion;
+    // Objects in leaf order (the permutation the builder applied).
+    std::vector<uint32_t> order;
+    // Root reference: index 0 of `nodes`, a leaf reference, or REF_EMPTY.
+    uint32_t root = REF_EMPTY;
+    uint32_t depth = 0;        // BVH4 depth (src/accel/bvh4.rs:87)
+    uint64_t node_count = 0;   // as counted by BVH4::construct_from_base
+    uint64_t leaf_count = 0;
+    std::vector<Box> root_bounds;  // time-sampled bounds of the whole tree
+};
+
+// `per_leaf` objects per leaf at most.  `threads` bounds the worker threads
+// used for independent subtrees (0 = hardware concurrency).
+HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads);
+
+// Canonical pre-order dump (shared with the parity tests):
+//   internal: 0, child_count, traversal_code, bounds_len, bounds_len*24 f32 bits, children...
+//   leaf:     1, obj_start, obj_end
+// preceded by (has_root, depth, node_count).
+void dump_bvh4(const HostBvh4& bvh, std::vector<uint32_t>& out);
+
+// TRAVERSAL_TABLE[octant][code] (sub_crates/bvh_order/build.rs).
+const uint8_t* traversal_order_table();
+
+}  // namespace psy

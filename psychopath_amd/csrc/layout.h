@@ -1,0 +1,92 @@
+// Device-resident scene layout shared by the host packer and the gfx950
+// kernels.  Everything here is plain data: it is written by the host builder
+// (builder.cpp / scene.cpp) and read by the traversal kernels (trace.hip).
+#pragma once
+
+#include <stdint.h>
+
+namespace psy {
+
+// Child references inside a BVH4 node.
+//   internal child: node index (bit 31 clear)
+//   leaf child:     bit 31 set, bits 2..30 first object, bits 0..1 count
+// Mesh leaves hold 1..3 triangles (src/surface/triangle_mesh.rs:17); instance
+// leaves hold exactly one instance (src/scene/assembly.rs:293).
+constexpr uint32_t REF_LEAF = 0x80000000u;
+constexpr uint32_t REF_EMPTY = 0xffffffffu;
+constexpr uint32_t MAX_LEAF_FIRST = (1u << 29) - 1;
+
+static inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) { return REF_LEAF | (first << 2) | count; }
+
+// One BVH4 node: a 64-B-aligned 128-B record.  The first 96 B are the four
+// child boxes as SoA (BBox4, src/bbox4.rs:16-21) for time sample 0; nodes with
+// time-sampled bounds keep all samples in the motion-bounds array instead.
+//   order[o]  visit order for ray octant o, 2 bits per slot, first child in
+//             the low bits (TRAVERSAL_TABLE[o][traversal_code],
+//             sub_crates/bvh_order, src/accel/bvh4.rs:152-163).
+//   meta      bits 0..2 child count, bits 8..15 bounds_len (time samples),
+//             bits 16..21 traversal code.
+//   motion    first BBox4 record of this node's time samples.
+struct alignas(64) GpuNode {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint8_t order[8];
+    uint32_t meta;
+    uint32_t motion;
+};
+static_assert(sizeof(GpuNode) == 128, "GpuNode must be 128 bytes");
+
+// One time sample of a node's four child boxes (BBox4).
+struct GpuBox4 {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+};
+static_assert(sizeof(GpuBox4) == 96, "GpuBox4 must be 96 bytes");
+
+// Triangle records: the three vertices in stored (winding-swapped) order
+// (src/surface/triangle_mesh.rs:80) and the original triangle index.
+// Static meshes use 48-B records; meshes with S deformation time samples use
+// records of S*36+4 bytes rounded up to 16: vertex v of sample s at float
+// offset s*9 + v*3, the index after all samples.
+struct GpuTriStatic {
+    float v[9];
+    uint32_t prim;
+    uint32_t pad[2];
+};
+static_assert(sizeof(GpuTriStatic) == 48, "GpuTriStatic must be 48 bytes");
+
+static inline uint32_t tri_record_bytes(uint32_t time_samples) {
+    return ((time_samples * 36u + 4u) + 15u) & ~15u;
+}
+
+// Per mesh: root reference of its BVH4, byte offset of its triangle records,
+// record stride and time-sample count.
+struct GpuMesh {
+    uint32_t root;
+    uint32_t time_samples;
+    uint32_t tri_stride;   // bytes
+    uint32_t pad0;
+    uint64_t tri_offset;   // bytes into the triangle buffer
+    uint64_t pad1;
+};
+
+// Per instance (src/scene/assembly.rs:404-411).  `xf_start/xf_count` index
+// the transform array (world -> object); `inv_static` indexes the inverse of
+// the single transform of a non-motion instance (precomputed on the host with
+// the same arithmetic the device uses).
+struct GpuInstance {
+    uint32_t mesh;
+    uint32_t id;
+    uint32_t shader;
+    uint32_t xf_start;
+    uint32_t xf_count;
+    uint32_t inv_static;
+    uint32_t pad[2];
+};
+
+// Transform: glam Affine3A columns c0, c1, c2 and translation t, stored as
+// three rows (x: c0.x c1.x c2.x t.x, ...) so each row is one float4.
+struct GpuXform {
+    float r[3][4];
+};
+
+}  // namespace psy

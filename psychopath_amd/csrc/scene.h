@@ -1,0 +1,40 @@
+// Host-side scene packing: meshes and instances (the reference's
+// TriangleMesh and AssemblyBuilder) flattened into the device layout.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/psy_rt.h"
+#include "builder.h"
+#include "layout.h"
+
+namespace psy {
+
+// The whole scene in device layout, still in host memory.
+struct PackedScene {
+    std::vector<GpuNode> nodes;
+    std::vector<GpuBox4> motion;
+    std::vector<uint8_t> tris;         // triangle records, per-mesh stride
+    std::vector<GpuMesh> meshes;
+    std::vector<GpuInstance> instances;  // in instance-BVH leaf order
+    std::vector<GpuXform> xforms;
+    std::vector<GpuXform> inv_static;
+    uint32_t top_root = REF_EMPTY;
+
+    // Host builds, kept for parity dumps (index = mesh; the instance BVH last).
+    std::vector<HostBvh4> bvhs;
+    uint32_t max_stack = 0;     // deepest pending-node count traversal can need
+    uint32_t max_depth = 0;
+    uint64_t leaf_count = 0;
+    uint64_t tri_count = 0;
+    double build_seconds = 0;
+};
+
+// Validates `desc` and builds every BVH.  Returns PSY_OK or an error code
+// with a message in `err`.
+int pack_scene(const psy_scene_desc* desc, PackedScene& out, std::string& err);
+
+}  // namespace psy

@@ -1,0 +1,40 @@
+// Launch interface of the traversal kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/psy_rt.h"
+#include "layout.h"
+
+namespace psy {
+
+constexpr int TRACE_BLOCK = 128;           // lanes per workgroup (two wave64s)
+constexpr int TRACE_LDS_DEPTH = 24;        // stack entries per lane held in LDS
+constexpr int TRACE_STACK_CAPACITY = 96;   // total stack entries per lane
+
+struct TraceArgs {
+    // scene
+    const GpuNode* nodes;
+    const GpuBox4* motion;
+    const uint8_t* tris;
+    const GpuMesh* meshes;
+    const GpuInstance* instances;
+    const GpuXform* xforms;
+    const GpuXform* inv_static;
+    uint32_t top_root;
+    // rays (device)
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
+    const uint8_t* flags;
+    uint64_t n;
+    // hits (device)
+    uint8_t* kind;
+    float *t, *b0, *b1, *b2;
+    uint32_t *prim_id, *inst_id;
+    // counters (device, 5 x u64) for the instrumented kernel
+    unsigned long long* counters;
+};
+
+hipError_t launch_trace(const TraceArgs& args, bool counted, hipStream_t stream);
+
+}  // namespace psy

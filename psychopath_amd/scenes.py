@@ -1,0 +1,199 @@
+"""Seeded, deterministic synthetic scenes and ray sets.
+
+Ray-cast workload (1M triangles, 16M rays):
+  * mesh: a regular tetrahedron subdivided `levels` times (4 * 4**levels
+    triangles; 9 levels give 1,048,576), vertices projected to the unit
+    sphere at every level, then displaced along the sphere normal by
+    `amplitude * perlin3(p)` (Perlin's 2002 improved noise, permutation
+    shuffled with seed 1);
+  * incoherent rays: origins uniform in the mesh bounds (seed 2), directions
+    `uniform_sample_sphere(u, v)` (src/sampling/monte_carlo.rs:49-56,
+    seed 3), time uniform in [0,1) (seed 4), max_t = +inf;
+  * shadow rays: origin and end point uniform in the bounds (seeds 5, 6),
+    dir = end - origin, max_t = 1.0, occlusion flag set (the shape of
+    src/renderer.rs:568-574);
+  * coherent rays: a W x H pinhole camera at pixel centres, built as
+    Camera::generate_ray does (src/camera.rs:74-102) with lens (0.5, 0.5).
+
+Random numbers come from SplitMix64: value i of stream `seed` is the i-th
+output of a SplitMix64 generator seeded with `seed`; a float in [0,1) is its
+top 24 bits times 2**-24.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """Outputs start..start+n-1 of SplitMix64 seeded with `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """n floats in [0,1) with 24 random bits each (exact in float32)."""
+    return ((splitmix64(seed, n, start) >> np.uint64(40)).astype(np.float32)
+            * np.float32(1.0 / 16777216.0)).astype(np.float32)
+
+
+# ---------------------------------------------------------------- geometry
+
+def tetra_sphere(levels: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit-sphere mesh from a subdivided regular tetrahedron (float64 verts,
+    uint32 triangles, outward counter-clockwise winding)."""
+    v = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], np.float64) / np.sqrt(3.0)
+    f = np.array([[0, 1, 2], [0, 3, 1], [0, 2, 3], [1, 3, 2]], np.int64)
+    # orient every face outward
+    n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    flip = np.einsum("ij,ij->i", n, v[f].mean(axis=1)) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    for _ in range(levels):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(3, -1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = inv[0] + base, inv[1] + base, inv[2] + base
+        f = np.concatenate([
+            np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1),
+            np.stack([c, ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    return v, f.astype(np.uint32)
+
+
+def perlin_permutation(seed: int) -> np.ndarray:
+    """0..255 shuffled by Fisher-Yates driven by SplitMix64(seed), doubled."""
+    p = np.arange(256, dtype=np.int64)
+    r = splitmix64(seed, 255)
+    for k, i in enumerate(range(255, 0, -1)):
+        j = int(r[k] % np.uint64(i + 1))
+        p[i], p[j] = p[j], p[i]
+    return np.concatenate([p, p])
+
+
+def perlin3(pts: np.ndarray, perm: np.ndarray) -> np.ndarray:
+    """Perlin's improved noise (2002) at points [N,3] (float64)."""
+    x, y, z = pts[:, 0], pts[:, 1], pts[:, 2]
+    fx, fy, fz = np.floor(x), np.floor(y), np.floor(z)
+    X = fx.astype(np.int64) & 255
+    Y = fy.astype(np.int64) & 255
+    Z = fz.astype(np.int64) & 255
+    x, y, z = x - fx, y - fy, z - fz
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(h, x, y, z):
+        h = h & 15
+        u = np.where(h < 8, x, y)
+        v = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -u, u) + np.where(h & 2, -v, v)
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    u, v, w = fade(x), fade(y), fade(z)
+    p = perm
+    A = p[X] + Y
+    AA, AB = p[A] + Z, p[A + 1] + Z
+    B = p[X + 1] + Y
+    BA, BB = p[B] + Z, p[B + 1] + Z
+    return lerp(w,
+                lerp(v, lerp(u, grad(p[AA], x, y, z), grad(p[BA], x - 1, y, z)),
+                     lerp(u, grad(p[AB], x, y - 1, z), grad(p[BB], x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(p[AA + 1], x, y, z - 1), grad(p[BA + 1], x - 1, y, z - 1)),
+                     lerp(u, grad(p[AB + 1], x, y - 1, z - 1), grad(p[BB + 1], x - 1, y - 1, z - 1))))
+
+
+def displaced_sphere(levels: int = 9, amplitude: float = 0.1, seed: int = 1,
+                     frequency: float = 1.0) -> tuple[np.ndarray, np.ndarray]:
+    """The ray-cast mesh: float32 verts [V,3] and uint32 triangles [T,3]."""
+    v, f = tetra_sphere(levels)
+    perm = perlin_permutation(seed)
+    d = perlin3(v * frequency, perm)
+    v = v * (1.0 + amplitude * d)[:, None]
+    v = v.astype(np.float32) + np.float32(0.0)  # no negative zeros
+    return v, f
+
+
+# -------------------------------------------------------------------- rays
+
+def _sphere_dirs(u: np.ndarray, v: np.ndarray):
+    """uniform_sample_sphere (monte_carlo.rs:49-56) in float32."""
+    z = np.float32(1.0) - np.float32(2.0) * u
+    r = np.sqrt(np.maximum(np.float32(1.0) - z * z, np.float32(0.0)))
+    phi = np.float32(2.0 * np.pi) * v
+    return (r * np.cos(phi)).astype(np.float32), (r * np.sin(phi)).astype(np.float32), z.astype(np.float32)
+
+
+def incoherent_rays(lo, hi, n: int, seeds=(2, 3, 4)) -> dict:
+    lo = np.asarray(lo, np.float32)
+    ext = (np.asarray(hi, np.float32) - lo).astype(np.float32)
+    uo = uniform01(seeds[0], 3 * n).reshape(n, 3)
+    o = (lo + ext * uo).astype(np.float32)
+    ud = uniform01(seeds[1], 2 * n).reshape(n, 2)
+    dx, dy, dz = _sphere_dirs(ud[:, 0], ud[:, 1])
+    t = uniform01(seeds[2], n)
+    return dict(ox=o[:, 0].copy(), oy=o[:, 1].copy(), oz=o[:, 2].copy(), dx=dx, dy=dy, dz=dz, time=t,
+                max_t=np.full(n, np.inf, np.float32))
+
+
+def shadow_rays(lo, hi, n: int, seeds=(5, 6)) -> dict:
+    lo = np.asarray(lo, np.float32)
+    ext = (np.asarray(hi, np.float32) - lo).astype(np.float32)
+    a = (lo + ext * uniform01(seeds[0], 3 * n).reshape(n, 3)).astype(np.float32)
+    b = (lo + ext * uniform01(seeds[1], 3 * n).reshape(n, 3)).astype(np.float32)
+    d = (b - a).astype(np.float32)
+    return dict(ox=a[:, 0].copy(), oy=a[:, 1].copy(), oz=a[:, 2].copy(), dx=d[:, 0].copy(), dy=d[:, 1].copy(),
+                dz=d[:, 2].copy(), time=np.zeros(n, np.float32), max_t=np.ones(n, np.float32),
+                occlusion=np.ones(n, np.uint8))
+
+
+def coherent_rays(lo, hi, width: int, height: int, fov_deg: float = 40.0) -> dict:
+    """Pinhole primary rays at pixel centres looking down +z at the bounds.
+
+    Camera space follows src/camera.rs:74-102 with lens (0.5, 0.5), so the
+    origin is the camera position and dir = normalize(x*tfov, y*tfov, 1); the
+    camera transform is a translation placing the camera in front of the
+    bounds.  Image-plane x spans [-1, 1] across the width and y keeps the
+    aspect ratio (src/renderer.rs:248-255 without the pixel filter).
+    """
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    c = (lo + hi) / 2
+    radius = float(np.linalg.norm(hi - lo) / 2)
+    half = np.radians(fov_deg) / 2
+    tfov = np.float32(np.sin(half) / np.cos(half))
+    dist = radius / np.sin(half)
+    cam = np.array([c[0], c[1], c[2] - dist], np.float32)
+    aspect = height / width
+    px = ((np.arange(width, dtype=np.float32) + np.float32(0.5)) / np.float32(width)) * np.float32(2) - np.float32(1)
+    py = (np.float32(1) - ((np.arange(height, dtype=np.float32) + np.float32(0.5)) / np.float32(height))
+          * np.float32(2)) * np.float32(aspect)
+    X, Y = np.meshgrid(px, py)
+    x = (X.ravel() * tfov).astype(np.float32)
+    y = (Y.ravel() * tfov).astype(np.float32)
+    z = np.ones_like(x)
+    inv_len = (np.float32(1.0) / np.sqrt((x * x + y * y) + z * z)).astype(np.float32)
+    n = width * height
+    return dict(ox=np.full(n, cam[0], np.float32), oy=np.full(n, cam[1], np.float32),
+                oz=np.full(n, cam[2], np.float32), dx=(x * inv_len).astype(np.float32),
+                dy=(y * inv_len).astype(np.float32), dz=(z * inv_len).astype(np.float32),
+                time=np.zeros(n, np.float32), max_t=np.full(n, np.inf, np.float32))
+
+
+def raycast_workload(levels: int = 9, n_rays: int = 1 << 24):
+    """The 1M-triangle ray-cast mesh and its incoherent ray set."""
+    verts, tris = displaced_sphere(levels)
+    lo, hi = verts.min(axis=0), verts.max(axis=0)
+    return verts, tris, incoherent_rays(lo, hi, n_rays)

@@ -1,0 +1,336 @@
+"""Host-side mirror of the reference's tracing interface over the C ABI.
+
+    Assembly       ~ AssemblyBuilder / Assembly   (src/scene/assembly.rs:150-350)
+    Tracer         ~ Tracer                       (src/tracer.rs:15-42)
+    RayBatch       ~ RayBatch                     (src/ray.rs:24-195), SoA numpy
+    Hits           ~ [SurfaceIntersection]        (src/surface/mod.rs:70-92)
+
+Rays and hits are numpy structure-of-arrays; `DeviceRays` / `DeviceHits` keep
+them resident in HBM for repeated traces.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "time", "max_t")
+HIT_FLOATS = ("t", "b0", "b1", "b2")
+HIT_U32 = ("prim_id", "inst_id")
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+@dataclass
+class Mesh:
+    """A triangle mesh: verts [time_samples, V, 3], indices [T, 3],
+    optional normals like verts (TriangleMesh, triangle_mesh.rs:29-116)."""
+    verts: np.ndarray
+    indices: np.ndarray
+    normals: np.ndarray | None = None
+
+    def __post_init__(self):
+        v = _f32(self.verts)
+        if v.ndim == 2:
+            v = v[None]
+        if v.ndim != 3 or v.shape[2] != 3:
+            raise ValueError("verts must be [V,3] or [S,V,3]")
+        self.verts = v
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 3)
+        if self.normals is not None:
+            n = _f32(self.normals)
+            if n.ndim == 2:
+                n = n[None]
+            if n.shape != v.shape:
+                raise ValueError("normals must match verts")
+            self.normals = n
+
+
+@dataclass
+class Instance:
+    """An instance of mesh `mesh` with world-to-object transforms, each the 12
+    values a..l of Transform::new_from_values (math3d transform.rs:29-47)."""
+    mesh: int
+    xforms: np.ndarray | None = None
+    shader: int = _lib.PSY_NONE
+
+
+@dataclass
+class Assembly:
+    """Single-level assembly of meshes and instances
+    (AssemblyBuilder::add_object / add_instance, assembly.rs:197-282)."""
+    meshes: list = field(default_factory=list)
+    instances: list = field(default_factory=list)
+
+    def add_mesh(self, mesh: Mesh) -> int:
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh: int, xforms=None, shader: int = _lib.PSY_NONE) -> int:
+        if xforms is not None:
+            xforms = _f32(xforms).reshape(-1, 12)
+            if len(xforms) == 0:
+                xforms = None
+        self.instances.append(Instance(mesh, xforms, shader))
+        return len(self.instances) - 1
+
+    def build(self) -> "Scene":
+        return Scene(self)
+
+
+class Scene:
+    """The assembly uploaded to the GPU (psy_scene_create)."""
+
+    def __init__(self, asm: Assembly):
+        self._keep = []
+        mds = (_lib.MeshDesc * max(1, len(asm.meshes)))()
+        for i, m in enumerate(asm.meshes):
+            v = m.verts
+            idx = m.indices
+            self._keep += [v, idx]
+            mds[i].vert_count = v.shape[1]
+            mds[i].time_samples = v.shape[0]
+            mds[i].verts = v.ctypes.data_as(_lib._fp)
+            mds[i].normals = None
+            if m.normals is not None:
+                self._keep.append(m.normals)
+                mds[i].normals = m.normals.ctypes.data_as(_lib._fp)
+            mds[i].tri_count = idx.shape[0]
+            mds[i].indices = idx.ctypes.data_as(_lib._u32p)
+        xf_list = []
+        ids = (_lib.InstanceDesc * max(1, len(asm.instances)))()
+        nxf = 0
+        for i, inst in enumerate(asm.instances):
+            ids[i].mesh = inst.mesh
+            ids[i].shader = inst.shader
+            ids[i].xform_start = nxf
+            cnt = 0 if inst.xforms is None else len(inst.xforms)
+            ids[i].xform_count = cnt
+            if cnt:
+                xf_list.append(inst.xforms)
+            nxf += cnt
+        xfa = _f32(np.concatenate(xf_list)) if xf_list else np.zeros((1, 12), np.float32)
+        self._keep.append(xfa)
+        desc = _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp))
+        h = C.c_void_p()
+        check("psy_scene_create", lib().psy_scene_create(C.byref(desc), C.byref(h)))
+        self.handle = h
+        self._keep = None
+        self.mesh_count = len(asm.meshes)
+        self.tri_counts = [m.indices.shape[0] for m in asm.meshes]
+        self.instance_count = len(asm.instances)
+
+    def info(self) -> dict:
+        i = _lib.SceneInfo()
+        check("psy_scene_get_info", lib().psy_scene_get_info(self.handle, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _lib.SceneInfo._fields_}
+
+    def dump_bvh(self, mesh: int) -> tuple[np.ndarray, np.ndarray]:
+        """Canonical dump of a mesh BVH4 (mesh >= 0) or the instance BVH
+        (mesh = -1), and the object order of its leaves."""
+        n = lib().psy_scene_dump_bvh(self.handle, mesh, None, 0, None, 0)
+        if n < 0:
+            check("psy_scene_dump_bvh", int(n))
+        out = np.zeros(n, np.uint32)
+        order = np.zeros(self.instance_count if mesh < 0 else self.tri_counts[mesh], np.uint32)
+        r = lib().psy_scene_dump_bvh(self.handle, mesh, out.ctypes.data_as(_lib._u32p), n,
+                                     order.ctypes.data_as(_lib._u32p), len(order))
+        if r < 0:
+            check("psy_scene_dump_bvh", int(r))
+        return out, order
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().psy_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RayBatch:
+    """Rays as SoA numpy arrays (RayBatch, ray.rs:24-195)."""
+
+    def __init__(self, ox, oy, oz, dx, dy, dz, time=None, max_t=None, occlusion=None):
+        n = len(ox)
+        self.ox, self.oy, self.oz = _f32(ox), _f32(oy), _f32(oz)
+        self.dx, self.dy, self.dz = _f32(dx), _f32(dy), _f32(dz)
+        self.time = _f32(np.zeros(n) if time is None else time)
+        self.max_t = _f32(np.full(n, np.inf) if max_t is None else max_t)
+        self.flags = None if occlusion is None else np.ascontiguousarray(occlusion, dtype=np.uint8)
+        for f in RAY_FIELDS:
+            if len(getattr(self, f)) != n:
+                raise ValueError(f"ray field {f} has the wrong length")
+
+    def __len__(self):
+        return len(self.ox)
+
+    def slice(self, a: int, b: int) -> "RayBatch":
+        return RayBatch(*(getattr(self, f)[a:b] for f in RAY_FIELDS),
+                        occlusion=None if self.flags is None else self.flags[a:b])
+
+    def soa(self) -> _lib.RaySoA:
+        r = _lib.RaySoA()
+        for f in RAY_FIELDS:
+            setattr(r, f, getattr(self, f).ctypes.data)
+        r.flags = None if self.flags is None else self.flags.ctypes.data
+        return r
+
+
+class Hits:
+    """Hit records as SoA numpy arrays."""
+
+    def __init__(self, n: int):
+        self.kind = np.zeros(n, np.uint8)
+        for f in HIT_FLOATS:
+            setattr(self, f, np.zeros(n, np.float32))
+        for f in HIT_U32:
+            setattr(self, f, np.full(n, _lib.PSY_NONE, np.uint32))
+
+    def soa(self) -> _lib.HitSoA:
+        h = _lib.HitSoA()
+        h.kind = self.kind.ctypes.data
+        for f in HIT_FLOATS + HIT_U32:
+            setattr(h, f, getattr(self, f).ctypes.data)
+        return h
+
+
+class DeviceBuffer:
+    """A raw HBM allocation (psy_device_alloc)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check("psy_device_alloc", lib().psy_device_alloc(max(16, self.nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        check("psy_memcpy_h2d", lib().psy_memcpy_h2d(self.ptr + offset, arr.ctypes.data, arr.nbytes))
+
+    def download(self, arr: np.ndarray, offset: int = 0):
+        check("psy_memcpy_d2h", lib().psy_memcpy_d2h(arr.ctypes.data, self.ptr + offset, arr.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().psy_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceRays:
+    """A RayBatch resident in HBM."""
+
+    def __init__(self, rays: RayBatch):
+        n = len(rays)
+        self.n = n
+        self.buf = DeviceBuffer(n * 4 * 8 + n + 64)
+        self.has_flags = rays.flags is not None
+        for k, f in enumerate(RAY_FIELDS):
+            self.buf.upload(getattr(rays, f), k * n * 4)
+        if self.has_flags:
+            self.buf.upload(rays.flags, 8 * n * 4)
+
+    def soa(self) -> _lib.RaySoA:
+        r = _lib.RaySoA()
+        for k, f in enumerate(RAY_FIELDS):
+            setattr(r, f, self.buf.ptr + k * self.n * 4)
+        r.flags = self.buf.ptr + 8 * self.n * 4 if self.has_flags else None
+        return r
+
+
+class DeviceHits:
+    """Hit records resident in HBM."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.buf = DeviceBuffer(n * (1 + 4 * 4 + 2 * 4) + 64)
+
+    def soa(self) -> _lib.HitSoA:
+        h = _lib.HitSoA()
+        base = self.buf.ptr
+        n = self.n
+        for k, f in enumerate(HIT_FLOATS):
+            setattr(h, f, base + k * n * 4)
+        for k, f in enumerate(HIT_U32):
+            setattr(h, f, base + (4 + k) * n * 4)
+        h.kind = base + 6 * n * 4
+        return h
+
+    def to_host(self) -> Hits:
+        out = Hits(self.n)
+        n = self.n
+        for k, f in enumerate(HIT_FLOATS + HIT_U32):
+            self.buf.download(getattr(out, f), k * n * 4)
+        self.buf.download(out.kind, 6 * n * 4)
+        return out
+
+
+class Tracer:
+    """Tracer::from_assembly / trace / rays_traced (tracer.rs:15-42)."""
+
+    def __init__(self, scene: Scene, max_batch: int = 1 << 22):
+        self.scene = scene
+        h = C.c_void_p()
+        check("psy_tracer_create", lib().psy_tracer_create(scene.handle, max_batch, C.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def from_assembly(cls, scene: Scene, max_batch: int = 1 << 22) -> "Tracer":
+        return cls(scene, max_batch)
+
+    def trace(self, rays: RayBatch) -> Hits:
+        hits = Hits(len(rays))
+        rs, hs = rays.soa(), hits.soa()
+        check("psy_trace", lib().psy_trace(self.handle, C.byref(rs), len(rays), C.byref(hs), _lib.PSY_MEM_HOST))
+        return hits
+
+    def trace_device(self, rays: DeviceRays, hits: DeviceHits, sync: bool = True) -> None:
+        rs, hs = rays.soa(), hits.soa()
+        if sync:
+            check("psy_trace", lib().psy_trace(self.handle, C.byref(rs), rays.n, C.byref(hs), _lib.PSY_MEM_DEVICE))
+        else:
+            check("psy_trace_async", lib().psy_trace_async(self.handle, C.byref(rs), rays.n, C.byref(hs)))
+
+    def trace_counted(self, rays: DeviceRays, hits: DeviceHits) -> dict:
+        c = _lib.TraceCounters()
+        rs, hs = rays.soa(), hits.soa()
+        check("psy_trace_counted", lib().psy_trace_counted(self.handle, C.byref(rs), rays.n, C.byref(hs), C.byref(c)))
+        return {f: getattr(c, f) for f, _ in _lib.TraceCounters._fields_}
+
+    def sync(self) -> None:
+        check("psy_tracer_sync", lib().psy_tracer_sync(self.handle))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        check("psy_tracer_last_kernel_ms", lib().psy_tracer_last_kernel_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def rays_traced(self) -> int:
+        return int(lib().psy_tracer_rays_traced(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().psy_tracer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,106 @@
+"""The C ABI library loads, exports every symbol include/psy_rt.h declares,
+and rejects bad input with error codes (no GPU needed)."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from psychopath_amd import _lib
+from psychopath_amd.tracer import Assembly, Mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "psy_rt.h")) as f:
+        src = f.read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(psy_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the binding covers all of them
+    assert sorted(_lib.exported_symbols()) == names
+
+
+def test_version_and_error_string():
+    L = _lib.lib()
+    assert b"gfx950" in L.psy_version()
+    rc = L.psy_scene_create(None, C.byref(C.c_void_p()))
+    assert rc == -1
+    assert b"NULL" in L.psy_last_error()
+
+
+def mesh_desc(verts, tris, samples=1):
+    md = _lib.MeshDesc()
+    md.vert_count = verts.shape[-2]
+    md.time_samples = samples
+    md.verts = verts.ctypes.data_as(_lib._fp)
+    md.normals = None
+    md.tri_count = tris.shape[0]
+    md.indices = tris.ctypes.data_as(_lib._u32p)
+    return md
+
+
+def create(meshes, insts, xforms=None):
+    L = _lib.lib()
+    mds = (_lib.MeshDesc * len(meshes))(*meshes)
+    ids = (_lib.InstanceDesc * max(1, len(insts)))(*insts)
+    xf = np.zeros((1, 12), np.float32) if xforms is None else xforms
+    desc = _lib.SceneDesc(len(meshes), mds, len(insts), ids, 0 if xforms is None else len(xforms),
+                          xf.ctypes.data_as(_lib._fp))
+    h = C.c_void_p()
+    rc = L.psy_scene_create(C.byref(desc), C.byref(h))
+    if rc == 0:
+        L.psy_scene_destroy(h)
+    return rc, L.psy_last_error().decode()
+
+
+def test_rejects_bad_scenes():
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    t = np.array([[0, 1, 2]], np.uint32)
+    bad = np.array([[0, 1, 3]], np.uint32)
+    ok_inst = _lib.InstanceDesc(0, _lib.PSY_NONE, 0, 0)
+    assert create([mesh_desc(v, t)], [ok_inst])[0] == 0
+    rc, msg = create([mesh_desc(v, bad)], [ok_inst])
+    assert rc == -1 and "out of range" in msg
+    rc, msg = create([mesh_desc(v, t, samples=0)], [ok_inst])
+    assert rc == -2 and "time_samples" in msg
+    rc, msg = create([mesh_desc(v, t)], [_lib.InstanceDesc(3, _lib.PSY_NONE, 0, 0)])
+    assert rc == -1 and "mesh out of range" in msg
+    rc, msg = create([mesh_desc(v, t)], [_lib.InstanceDesc(0, _lib.PSY_NONE, 0, 2)], np.zeros((1, 12), np.float32))
+    assert rc == -1 and "transform range" in msg
+
+
+def test_scene_info_and_dump_on_host():
+    from psychopath_amd import scenes
+    v, f = scenes.displaced_sphere(3)
+    a = Assembly()
+    a.add_mesh(Mesh(v, f))
+    a.add_instance(0)
+    s = a.build()
+    info = s.info()
+    assert info["tri_count"] == 256
+    assert info["node_bytes"] == 128 and info["tri_bytes"] == 48 and info["motion_bounds_bytes"] == 96
+    assert info["device_bytes"] == info["node_count"] * 128 + 256 * 48 + 32 + 32
+    d, order = s.dump_bvh(0)
+    assert sorted(order.tolist()) == list(range(256))
+    with pytest.raises(_lib.PsyError):
+        L = _lib.lib()
+        _lib.check("psy_scene_dump_bvh", int(L.psy_scene_dump_bvh(s.handle, 5, None, 0, None, 0)))
+
+
+def test_null_tracer_calls_fail_cleanly():
+    L = _lib.lib()
+    assert L.psy_trace(None, None, 0, None, 0) == -1
+    assert L.psy_tracer_sync(None) == -1
+    assert L.psy_tracer_rays_traced(None) == 0
+    assert L.psy_tracer_create(None, 16, C.byref(C.c_void_p())) == -1

@@ -1,0 +1,113 @@
+"""The product's host BVH4 builder must build the same trees as the oracle's
+restatement of the reference builder (src/accel/bvh_base.rs,
+objects_split.rs, bvh4.rs): identical node structure, traversal codes,
+time-sampled bounds (bit for bit) and leaf object order.  Runs on the CPU."""
+
+import numpy as np
+import pytest
+
+from psychopath_amd import scenes
+from psychopath_amd.tracer import Assembly, Mesh
+
+
+def both(meshes, instances):
+    """meshes: list of (verts [S,V,3], tris); instances: list of (mesh, xforms|None)."""
+    from oracle.psyref import OracleScene
+    a = Assembly()
+    o = OracleScene()
+    for v, t in meshes:
+        a.add_mesh(Mesh(v, t))
+        o.add_mesh(v, t)
+    for m, xf in instances:
+        a.add_instance(m, xf)
+        o.add_instance(m, xf)
+    return a.build(), o.build()
+
+
+def assert_same(s, o, n_meshes):
+    for m in list(range(n_meshes)) + [-1]:
+        d1, o1 = s.dump_bvh(m)
+        d2, o2 = o.dump_bvh(m)
+        assert d1[:3].tolist() == d2[:3].tolist(), (m, d1[:3], d2[:3])
+        np.testing.assert_array_equal(d1, d2, err_msg=f"tree {m}")
+        np.testing.assert_array_equal(o1, o2, err_msg=f"order {m}")
+
+
+@pytest.mark.parametrize("levels", [0, 1, 2, 5, 7])
+def test_sphere_meshes(levels):
+    v, f = scenes.displaced_sphere(levels)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 17, 1000])
+def test_random_soups(n):
+    rng = np.random.default_rng(n)
+    v = rng.uniform(-3, 3, size=(3 * n, 3)).astype(np.float32)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
+
+
+def test_identical_triangles_force_median_splits():
+    # 2000 copies of one triangle: every SAH cost is NaN, splits peel one
+    # triangle at a time and the depth guard switches to median splits with
+    # quick_select (bvh_base.rs:156-165, algorithm.rs:203-231).
+    n = 2000
+    v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (n, 1))
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
+    assert s.info()["max_depth"] >= 15
+
+
+@pytest.mark.parametrize("samples", [2, 3])
+def test_deformation_time_samples(samples):
+    rng = np.random.default_rng(samples)
+    v0, f = scenes.displaced_sphere(4)
+    vs = [v0]
+    for k in range(1, samples):
+        vs.append((v0 + rng.normal(0, 0.02 * k, size=v0.shape)).astype(np.float32))
+    v = np.stack(vs)
+    # A mesh that moves a lot in part keeps per-time bounds in some nodes.
+    v[-1, : len(v0) // 3] += np.float32(0.5)
+    s, o = both([(v, f)], [(0, None)])
+    assert_same(s, o, 1)
+    assert s.info()["motion_bounds"] > 0
+
+
+def rot_xf(angle, axis, t):
+    c, s_ = np.cos(angle), np.sin(angle)
+    if axis == 0:
+        m = [[1, 0, 0], [0, c, -s_], [0, s_, c]]
+    elif axis == 1:
+        m = [[c, 0, s_], [0, 1, 0], [-s_, 0, c]]
+    else:
+        m = [[c, -s_, 0], [s_, c, 0], [0, 0, 1]]
+    m = np.array(m, np.float64)
+    return np.array([m[0, 0], m[0, 1], m[0, 2], t[0], m[1, 0], m[1, 1], m[1, 2], t[1],
+                     m[2, 0], m[2, 1], m[2, 2], t[2]], np.float32)
+
+
+def test_instances_with_motion_transforms():
+    rng = np.random.default_rng(9)
+    v, f = scenes.displaced_sphere(3)
+    v2, f2 = scenes.displaced_sphere(2, seed=5)
+    inst = []
+    for i in range(60):
+        k = int(rng.integers(1, 4)) if i % 4 else 0
+        xf = None
+        if k:
+            xf = np.stack([rot_xf(rng.uniform(0, 0.5), int(rng.integers(3)), rng.uniform(-8, 8, 3)) for _ in range(k)])
+        inst.append((i % 2, xf))
+    s, o = both([(v[None], f), (v2[None], f2)], inst)
+    assert_same(s, o, 2)
+
+
+def test_single_instance_leaf_root():
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
+    d, _ = s.dump_bvh(0)
+    assert d.tolist() == [1, 1, 1, 1, 0, 1]  # has_root, depth, node_count, leaf(0..1)
