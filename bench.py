@@ -44,6 +44,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also run one device copy kernel of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -71,7 +73,33 @@ def main():
     tracer = Tracer(scene, max_batch=1 << 20)
     d_rays = DeviceRays(batch)
     d_hits = DeviceHits(args.rays)
-    log(f"setup {time.time() - t0:.1f}s; warmup {args.warmup}")
+    log(f"setup {time.time() - t0:.1f}s")
+
+    if args.copy_bytes:
+        import ctypes as C
+
+        from psychopath_amd._lib import check, lib
+        from psychopath_amd.tracer import DeviceBuffer
+        nb = (args.copy_bytes // 16) * 16
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        ms = C.c_float()
+        check("psy_device_copy_kernel", lib().psy_device_copy_kernel(dst.ptr, src.ptr, nb, C.byref(ms)))
+        log(f"copy kernel {nb} bytes: {ms.value:.3f} ms, {2 * nb / ms.value / 1e6:.0f} GB/s")
+        del src, dst
+
+    # Launch shape: the simple grid kernel for reference, then the autotuner
+    # over persistent shapes on a 2M-ray sample of the same workload.
+    tracer.set_tuning(kernel=1)
+    tracer.trace_device(d_rays, d_hits)
+    tracer.trace_device(d_rays, d_hits)
+    simple_ms = tracer.last_kernel_ms()
+    ns = min(args.rays, 1 << 21)
+    tune_rays = DeviceRays(RayBatch(*(r[k][:ns] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "time", "max_t"))))
+    tune_hits = DeviceHits(ns)
+    t1 = time.time()
+    tuning = tracer.autotune(tune_rays, tune_hits)
+    del tune_rays, tune_hits
+    log(f"simple kernel {simple_ms:.2f} ms; autotune {time.time() - t1:.1f}s -> {tuning}; warmup {args.warmup}")
 
     for _ in range(args.warmup):
         tracer.trace_device(d_rays, d_hits, sync=False)
@@ -173,10 +201,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "trace_kernel<false>",
+            "kernel": "trace_persistent_kernel" if tuning["kernel"] == 0 else "trace_kernel",
             "kernel_ms": round(avg_kernel_ms, 3),
             "algorithmic_bytes_per_launch": int(alg_bytes),
             "bytes_model": f"node_visits*{node_b} + tri_tests*{tri_b} + rays*{io_b}",
+            "launch": tuning,
+            "simple_kernel_ms": round(simple_ms, 3),
             "node_visits_per_ray": round(counts["node_visits"] / args.rays, 3),
             "tri_tests_per_ray": round(counts["tri_tests"] / args.rays, 3),
         },

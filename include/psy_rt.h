@@ -136,6 +136,21 @@ typedef struct {
     uint64_t rays;
 } psy_trace_counters;
 
+/* Launch shape of the traversal kernel.  The persistent kernel keeps
+ * `waves_per_cu` wave64 workgroups per compute unit resident, each taking
+ * rays from a global counter; lanes whose ray finished are refilled once
+ * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
+ * live in LDS (8, 16 or 24) and the rest in private memory.  The simple
+ * kernel runs one lane per ray over a launch-wide grid. */
+#define PSY_KERNEL_PERSISTENT 0u
+#define PSY_KERNEL_SIMPLE 1u
+typedef struct {
+    uint32_t kernel;
+    uint32_t waves_per_cu;
+    uint32_t refill_min;
+    uint32_t stack_lds;
+} psy_trace_tuning;
+
 /* ---- errors ---------------------------------------------------------- */
 const char* psy_last_error(void);
 const char* psy_version(void);
@@ -183,6 +198,18 @@ int psy_tracer_last_kernel_ms(psy_tracer* tracer, float* ms);
  * the meaning of ACCEL_NODE_RAY_TESTS, src/accel/mod.rs:22-25). */
 int psy_trace_counted(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
                       psy_trace_counters* counters);
+
+/* Current launch shape of a tracer. */
+int psy_tracer_get_tuning(const psy_tracer* tracer, psy_trace_tuning* tuning);
+/* Sets the launch shape; values outside the documented ranges are rejected. */
+int psy_tracer_set_tuning(psy_tracer* tracer, const psy_trace_tuning* tuning);
+/* Autotuner: times every candidate launch shape (kernel, waves per CU,
+ * refill threshold, LDS stack depth) on the given device-resident rays,
+ * keeps the fastest on the tracer and returns it in `best` (may be NULL).
+ * `hits` is device scratch for the candidate runs.  Results do not depend on
+ * the launch shape. */
+int psy_tracer_autotune(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
+                        psy_trace_tuning* best);
 
 /* Tracer::rays_traced (src/tracer.rs:39-41). */
 uint64_t psy_tracer_rays_traced(const psy_tracer* tracer);

@@ -100,6 +100,18 @@ class TraceCounters(C.Structure):
     ]
 
 
+class TraceTuning(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_uint32),
+        ("waves_per_cu", C.c_uint32),
+        ("refill_min", C.c_uint32),
+        ("stack_lds", C.c_uint32),
+    ]
+
+
+PSY_KERNEL_PERSISTENT = 0
+PSY_KERNEL_SIMPLE = 1
+
 # Every symbol the header declares, with its ctypes signature.
 _SIGNATURES = {
     "psy_last_error": (C.c_char_p, []),
@@ -117,6 +129,10 @@ _SIGNATURES = {
     "psy_trace_counted": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA),
                                     C.POINTER(TraceCounters)]),
     "psy_tracer_rays_traced": (C.c_uint64, [C.c_void_p]),
+    "psy_tracer_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
+    "psy_tracer_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
+    "psy_tracer_autotune": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA),
+                                      C.POINTER(TraceTuning)]),
     "psy_device_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "psy_device_free": (C.c_int, [C.c_void_p]),
     "psy_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),

@@ -72,11 +72,15 @@ struct psy_tracer {
     // device staging for host-memory calls: 8 ray floats + flags, hit fields
     void* staging = nullptr;
     unsigned long long* counters = nullptr;
+    unsigned int* work = nullptr;   // persistent-kernel ray counter
+    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16};
+    uint32_t cu_count = 256;
 
     ~psy_tracer() {
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(staging);
         (void)hipFree(counters);
+        (void)hipFree(work);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -141,7 +145,14 @@ int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hi
     a.n = n;
     a.counters = tr->counters;
     PSY_HIP(hipEventRecord(tr->ev0, tr->stream));
-    hipError_t e = launch_trace(a, counted, tr->stream);
+    hipError_t e;
+    if (tr->tuning.kernel == PSY_KERNEL_SIMPLE) {
+        e = launch_trace(a, counted, tr->stream);
+    } else {
+        TraceTuning tt{tr->cu_count * tr->tuning.waves_per_cu, tr->tuning.refill_min, tr->tuning.stack_lds};
+        e = launch_trace_persistent(a, counted, tr->scene->host.has_xf, tr->scene->host.has_motion, tr->work, tt,
+                                    tr->stream);
+    }
     if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
     PSY_HIP(hipEventRecord(tr->ev1, tr->stream));
     tr->timed = true;
@@ -281,6 +292,13 @@ int psy_tracer_create(const psy_scene* scene, uint64_t max_batch, psy_tracer** o
     if (e == hipSuccess) e = hipEventCreate(&t->ev0);
     if (e == hipSuccess) e = hipEventCreate(&t->ev1);
     if (e == hipSuccess) e = hipMalloc((void**)&t->counters, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc((void**)&t->work, 64);
+    if (e == hipSuccess) {
+        int dev = 0, cus = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess && cus > 0) t->cu_count = (uint32_t)cus;
+    }
     if (e != hipSuccess) {
         delete t;
         return fail(PSY_E_DEVICE, std::string("tracer setup: ") + hipGetErrorString(e));
@@ -313,6 +331,63 @@ int psy_trace_async(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit
     if (rc) return rc;
     if (n == 0) return PSY_OK;
     return enqueue(tr, rays, n, hits, false);
+}
+
+int psy_tracer_get_tuning(const psy_tracer* tr, psy_trace_tuning* t) {
+    if (!tr || !t) return fail(PSY_E_INVALID, "NULL argument");
+    *t = tr->tuning;
+    return PSY_OK;
+}
+
+int psy_tracer_set_tuning(psy_tracer* tr, const psy_trace_tuning* t) {
+    if (!tr || !t) return fail(PSY_E_INVALID, "NULL argument");
+    if (t->kernel != PSY_KERNEL_PERSISTENT && t->kernel != PSY_KERNEL_SIMPLE)
+        return fail(PSY_E_INVALID, "unknown kernel");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(PSY_E_RANGE, "waves_per_cu must be in [1, 32]");
+    if (t->refill_min < 1 || t->refill_min > 64) return fail(PSY_E_RANGE, "refill_min must be in [1, 64]");
+    if (t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 24)
+        return fail(PSY_E_RANGE, "stack_lds must be 8, 16 or 24");
+    tr->tuning = *t;
+    return PSY_OK;
+}
+
+int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
+                        psy_trace_tuning* best) {
+    if (!tr) return fail(PSY_E_INVALID, "tracer is NULL");
+    int rc = check_soa(rays, hits);
+    if (rc) return rc;
+    if (n == 0) return fail(PSY_E_INVALID, "autotune needs rays");
+    std::vector<psy_trace_tuning> cands;
+    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8});
+    const uint32_t wpc[] = {8, 12, 16, 24, 32};
+    const uint32_t refill[] = {1, 8, 16, 32};
+    const uint32_t depth[] = {8, 16, 24};
+    for (uint32_t w : wpc)
+        for (uint32_t r : refill)
+            for (uint32_t dd : depth) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd});
+    const psy_trace_tuning saved = tr->tuning;
+    float best_ms = 1e30f;
+    psy_trace_tuning best_t = saved;
+    for (const psy_trace_tuning& c : cands) {
+        tr->tuning = c;
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            rc = enqueue(tr, rays, n, hits, false);
+            if (rc) { tr->tuning = saved; return rc; }
+            float ms = 0;
+            PSY_HIP(hipEventSynchronize(tr->ev1));
+            PSY_HIP(hipEventElapsedTime(&ms, tr->ev0, tr->ev1));
+            if (rep > 0) ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best_t = c;
+        }
+    }
+    tr->rays_traced -= 3 * n * cands.size();
+    tr->tuning = best_t;
+    if (best) *best = best_t;
+    return PSY_OK;
 }
 
 int psy_tracer_sync(psy_tracer* tr) {

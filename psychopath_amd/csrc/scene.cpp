@@ -194,7 +194,11 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
             s.inv_static.push_back(to_gpu(xf_inverse(xf_at(d, id.xform_start))));
         }
         s.instances.push_back(gi);
+        if (id.xform_count > 0) s.has_xf = true;
     }
+    s.has_motion = !s.motion.empty();
+    for (const GpuMesh& gm : s.meshes)
+        if (gm.time_samples > 1) s.has_motion = true;
 
     // Pending-entry bound for the traversal stack: at most three siblings per
     // internal level along the current path, in both trees.

@@ -26,6 +26,8 @@ struct PackedScene {
 
     // Host builds, kept for parity dumps (index = mesh; the instance BVH last).
     std::vector<HostBvh4> bvhs;
+    bool has_xf = false;        // some instance has transforms
+    bool has_motion = false;    // time-sampled node bounds or vertices
     uint32_t max_stack = 0;     // deepest pending-node count traversal can need
     uint32_t max_depth = 0;
     uint64_t leaf_count = 0;

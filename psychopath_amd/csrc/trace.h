@@ -35,6 +35,19 @@ struct TraceArgs {
     unsigned long long* counters;
 };
 
+// Launch shape of the persistent kernel (chosen per scene by the autotuner).
+struct TraceTuning {
+    uint32_t waves;       // wave64 workgroups launched
+    uint32_t refill_min;  // idle lanes that trigger a refill (1..64)
+    uint32_t lds_depth;   // stack entries per lane kept in LDS (8, 16, 24)
+};
+
+// One lane per ray, one launch-wide grid (trace.hip).
 hipError_t launch_trace(const TraceArgs& args, bool counted, hipStream_t stream);
+
+// Persistent waves with lane refill (trace_persistent.hip).  `work` is a
+// device counter the launch resets.
+hipError_t launch_trace_persistent(const TraceArgs& args, bool counted, bool has_xf, bool has_motion,
+                                   unsigned int* work, const TraceTuning& tune, hipStream_t stream);
 
 }  // namespace psy

@@ -1,0 +1,327 @@
+// Persistent BVH4 traversal for gfx950 with per-lane ray refill.
+//
+// Same traversal as trace.hip (each ray walks the reference's depth-first
+// order, src/accel/bvh4.rs:101-184, with the same box and triangle tests),
+// reorganised for throughput on incoherent rays:
+//   * one wave64 per workgroup, launched only as many waves as the chip holds
+//     at once; each wave keeps taking rays from a global counter;
+//   * lanes whose ray finished are refilled in bulk once at least
+//     `refill_min` lanes of the wave are idle (one atomic per refill, the
+//     new rays spread over the idle lanes by a ballot prefix count);
+//   * the traversal is "while-while": lanes descend internal nodes until the
+//     whole wave stands on leaves, then run the leaf tests together;
+//   * kernels are specialised on whether the scene has instance transforms
+//     (XF) and time-sampled bounds or vertices (MOTION), so the common static
+//     case keeps no transform or lerp state in registers.
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+#include "psymath.h"
+#include "trace.h"
+#include "trace_device.h"
+
+namespace psy {
+
+namespace {
+
+constexpr int WAVE = 64;
+
+template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH>
+__global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min) {
+    constexpr int SPILL_DEPTH = TRACE_STACK_CAPACITY - LDS_DEPTH;
+    __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
+    const int lane = threadIdx.x;
+    uint32_t* const lds_col = lds_stack + lane;
+    uint32_t spill[SPILL_DEPTH];
+
+    const uint32_t n = (uint32_t)a.n;
+    const Xf ident = xf_identity();
+
+    // Per-lane ray state.
+    bool has = false;
+    uint32_t ray = 0;
+    V3 o{}, d{};
+    float time = 0.0f, tmax = 0.0f;
+    bool occl = false;
+    TriPre pre{};
+    V3 lo{}, di{};
+    uint32_t oct = 0;
+    uint32_t cur = REF_EMPTY;
+    int sp = 0;
+    bool in_inst = false;
+    int inst_base = 0;
+    bool has_space = false;
+    Xf space;
+    uint32_t inst_id = PSY_NONE;
+    uint64_t tri_base = 0;
+    uint32_t tri_stride = sizeof(GpuTriStatic), tri_samples = 1;
+    uint8_t kind = PSY_HIT_MISS;
+    float ht = 0.0f, hb0 = 0.0f, hb1 = 0.0f, hb2 = 0.0f;
+    uint32_t hprim = PSY_NONE, hinst = PSY_NONE;
+    uint64_t c_nodes = 0, c_mnodes = 0, c_tris = 0, c_mtris = 0, c_inst = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        // ---- refill idle lanes ------------------------------------------
+        const uint64_t idle = __ballot(!has);
+        if (!exhausted && (uint32_t)__popcll(idle) >= refill_min) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            const uint32_t want = (uint32_t)__popcll(idle);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(work, want);
+            base = __shfl(base, leader);
+            if (base + want >= n) exhausted = true;
+            if (!has) {
+                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (mine < n) {
+                    has = true;
+                    ray = mine;
+                    o = v3(a.ox[ray], a.oy[ray], a.oz[ray]);
+                    d = v3(a.dx[ray], a.dy[ray], a.dz[ray]);
+                    time = MOTION || XF ? a.time[ray] : 0.0f;
+                    tmax = a.max_t[ray];
+                    occl = a.flags != nullptr && (a.flags[ray] & 1u) != 0;
+                    // Top-level local ray: update_local with the identity
+                    // (tracer.rs:65-70).
+                    lo = xf_point(ident, o);
+                    V3 ld = xf_vector(ident, d);
+                    di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+                    oct = octant_of(di);
+                    pre = tri_precompute(d);
+                    cur = a.top_root;
+                    sp = 0;
+                    in_inst = false;
+                    kind = PSY_HIT_MISS;
+                }
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (!has) continue;
+
+        // ---- descend internal nodes until this lane stands on a leaf ------
+        while (cur != REF_EMPTY && !(cur & REF_LEAF)) {
+            const GpuNode* node = a.nodes + cur;
+            const float4* nf = reinterpret_cast<const float4*>(node);
+            const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
+            const uint4 tail = *reinterpret_cast<const uint4*>(nf + 7);
+            const uint32_t cnt = tail.z & 7u;
+            float4 lx, hx, ly, hy, lz, hz;
+            const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
+            if (!MOTION || blen <= 1) {
+                lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
+                if (COUNT) c_nodes++;
+            } else {
+                uint32_t i1;
+                float a2;
+                const float4* m0 = reinterpret_cast<const float4*>(a.motion + tail.w);
+                if (!lerp_slice_pos(blen, time, i1, a2)) {
+                    m0 += 6 * i1;
+                    lx = m0[0]; hx = m0[1]; ly = m0[2]; hy = m0[3]; lz = m0[4]; hz = m0[5];
+                } else {
+                    m0 += 6 * i1;
+                    const float4* m1 = m0 + 6;
+                    lx = lerp4(m0[0], m1[0], a2); hx = lerp4(m0[1], m1[1], a2);
+                    ly = lerp4(m0[2], m1[2], a2); hy = lerp4(m0[3], m1[3], a2);
+                    lz = lerp4(m0[4], m1[4], a2); hz = lerp4(m0[5], m1[5], a2);
+                }
+                if (COUNT) c_mnodes++;
+            }
+            uint32_t mask = 0;
+            mask |= slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, lo, di, tmax) ? 1u : 0u;
+            mask |= slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, lo, di, tmax) ? 2u : 0u;
+            mask |= slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, lo, di, tmax) ? 4u : 0u;
+            mask |= slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, lo, di, tmax) ? 8u : 0u;
+            mask &= (1u << cnt) - 1u;
+            const uint32_t ord = ((oct < 4 ? tail.x : tail.y) >> (8u * (oct & 3u))) & 0xffu;
+            uint32_t next = REF_EMPTY;
+            for (int k = 3; k >= 0; --k) {
+                if (k >= (int)cnt) continue;
+                const uint32_t c = (ord >> (2 * k)) & 3u;
+                if (mask & (1u << c)) {
+                    if (next != REF_EMPTY) {
+                        if (sp < LDS_DEPTH) lds_col[sp * WAVE] = next;
+                        else spill[sp - LDS_DEPTH] = next;
+                        sp++;
+                    }
+                    next = c == 0 ? refs.x : (c == 1 ? refs.y : (c == 2 ? refs.z : refs.w));
+                }
+            }
+            cur = next;
+        }
+
+        // ---- leaf --------------------------------------------------------
+        bool finished = false;
+        if (cur != REF_EMPTY) {
+            const uint32_t first = (cur & ~REF_LEAF) >> 2;
+            const uint32_t count = cur & 3u;
+            if (!in_inst) {
+                // Instance leaf (tracer.rs:95-128).
+                const GpuInstance inst = a.instances[first];
+                const GpuMesh mesh = a.meshes[inst.mesh];
+                if (COUNT) c_inst++;
+                if (XF) {
+                    if (inst.xf_count > 0) {
+                        uint32_t i1;
+                        float a2;
+                        Xf x;
+                        if (!lerp_slice_pos(inst.xf_count, time, i1, a2))
+                            x = load_xf(a.xforms + inst.xf_start + i1);
+                        else
+                            x = xf_lerp(load_xf(a.xforms + inst.xf_start + i1),
+                                        load_xf(a.xforms + inst.xf_start + i1 + 1), a2);
+                        lo = xf_point(x, o);
+                        V3 ld = xf_vector(x, d);
+                        di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+                        oct = octant_of(di);
+                        space = inst.xf_count == 1 ? load_xf(a.inv_static + inst.inv_static) : xf_inverse(x);
+                        has_space = true;
+                    } else {
+                        has_space = false;
+                    }
+                }
+                in_inst = true;
+                inst_base = sp;
+                inst_id = inst.id;
+                tri_base = mesh.tri_offset;
+                if (MOTION) {
+                    tri_stride = mesh.tri_stride;
+                    tri_samples = mesh.time_samples;
+                }
+                cur = mesh.root;
+            } else {
+                // Triangle leaf (triangle_mesh.rs:174-260).
+                for (uint32_t j = 0; j < count; j++) {
+                    const uint8_t* rec = a.tris + tri_base + (uint64_t)(first + j) * tri_stride;
+                    V3 v0, v1, v2;
+                    uint32_t prim;
+                    if (!MOTION || tri_samples == 1) {
+                        const float4* f = reinterpret_cast<const float4*>(rec);
+                        const float4 q0 = f[0], q1 = f[1], q2 = f[2];
+                        v0 = v3(q0.x, q0.y, q0.z);
+                        v1 = v3(q0.w, q1.x, q1.y);
+                        v2 = v3(q1.z, q1.w, q2.x);
+                        prim = __float_as_uint(q2.y);
+                        if (COUNT) c_tris++;
+                    } else {
+                        const float* f = reinterpret_cast<const float*>(rec);
+                        uint32_t i1;
+                        float a2;
+                        if (!lerp_slice_pos(tri_samples, time, i1, a2)) {
+                            const float* s0 = f + i1 * 9;
+                            v0 = v3(s0[0], s0[1], s0[2]);
+                            v1 = v3(s0[3], s0[4], s0[5]);
+                            v2 = v3(s0[6], s0[7], s0[8]);
+                        } else {
+                            const float* s0 = f + i1 * 9;
+                            const float* s1 = s0 + 9;
+                            v0 = lerp3(v3(s0[0], s0[1], s0[2]), v3(s1[0], s1[1], s1[2]), a2);
+                            v1 = lerp3(v3(s0[3], s0[4], s0[5]), v3(s1[3], s1[4], s1[5]), a2);
+                            v2 = lerp3(v3(s0[6], s0[7], s0[8]), v3(s1[6], s1[7], s1[8]), a2);
+                        }
+                        prim = *reinterpret_cast<const uint32_t*>(f + 9 * tri_samples);
+                        if (COUNT) c_mtris++;
+                    }
+                    if (XF && has_space) {
+                        v0 = xf_point(space, v0);
+                        v1 = xf_point(space, v1);
+                        v2 = xf_point(space, v2);
+                    }
+                    float t, b0, b1, b2;
+                    if (tri_test(o, pre, tmax, v0, v1, v2, t, b0, b1, b2)) {
+                        if (occl) {
+                            kind = PSY_HIT_OCCLUDE;
+                            finished = true;
+                            break;
+                        }
+                        tmax = t;
+                        kind = PSY_HIT_SURFACE;
+                        ht = t; hb0 = b0; hb1 = b1; hb2 = b2;
+                        hprim = prim;
+                        hinst = inst_id;
+                    }
+                }
+                cur = REF_EMPTY;
+            }
+        }
+
+        // ---- pop, leaving the instance when its subtree is done -----------
+        if (!finished && cur == REF_EMPTY) {
+            if (in_inst && sp == inst_base) {
+                in_inst = false;
+                if (XF) {
+                    lo = xf_point(ident, o);
+                    V3 ld = xf_vector(ident, d);
+                    di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+                    oct = octant_of(di);
+                }
+            }
+            if (sp == 0) {
+                finished = true;
+            } else {
+                sp--;
+                cur = sp < LDS_DEPTH ? lds_col[sp * WAVE] : spill[sp - LDS_DEPTH];
+            }
+        }
+
+        if (finished) {
+            a.kind[ray] = kind;
+            if (kind == PSY_HIT_SURFACE) {
+                a.t[ray] = ht;
+                a.b0[ray] = hb0;
+                a.b1[ray] = hb1;
+                a.b2[ray] = hb2;
+                a.prim_id[ray] = hprim;
+                a.inst_id[ray] = hinst;
+            }
+            has = false;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(a.counters + 0, (unsigned long long)c_nodes);
+        atomicAdd(a.counters + 1, (unsigned long long)c_mnodes);
+        atomicAdd(a.counters + 2, (unsigned long long)c_tris);
+        atomicAdd(a.counters + 3, (unsigned long long)c_mtris);
+        atomicAdd(a.counters + 4, (unsigned long long)c_inst);
+    }
+}
+
+template <bool COUNT, bool XF, bool MOTION, int D>
+hipError_t launch_one(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
+    hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D>), dim3(tune.waves), dim3(WAVE), 0, s, a, work,
+                       tune.refill_min);
+    return hipGetLastError();
+}
+
+template <bool COUNT, bool XF, bool MOTION>
+hipError_t launch_depth(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
+    if (COUNT) return launch_one<COUNT, XF, MOTION, 16>(a, work, tune, s);
+    switch (tune.lds_depth) {
+        case 8: return launch_one<COUNT, XF, MOTION, 8>(a, work, tune, s);
+        case 24: return launch_one<COUNT, XF, MOTION, 24>(a, work, tune, s);
+        default: return launch_one<COUNT, XF, MOTION, 16>(a, work, tune, s);
+    }
+}
+
+template <bool COUNT>
+hipError_t launch_feat(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, bool xf, bool motion,
+                       hipStream_t s) {
+    if (xf && motion) return launch_depth<COUNT, true, true>(a, work, tune, s);
+    if (xf) return launch_depth<COUNT, true, false>(a, work, tune, s);
+    if (motion) return launch_depth<COUNT, false, true>(a, work, tune, s);
+    return launch_depth<COUNT, false, false>(a, work, tune, s);
+}
+
+}  // namespace
+
+hipError_t launch_trace_persistent(const TraceArgs& args, bool counted, bool has_xf, bool has_motion,
+                                   unsigned int* work, const TraceTuning& tune, hipStream_t stream) {
+    if (args.n == 0) return hipSuccess;
+    if (args.n >= 0x80000000ull || tune.waves == 0 || tune.refill_min == 0 || tune.refill_min > 64)
+        return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(work, 0, sizeof(unsigned int), stream);
+    if (e != hipSuccess) return e;
+    return counted ? launch_feat<true>(args, work, tune, has_xf, has_motion, stream)
+                   : launch_feat<false>(args, work, tune, has_xf, has_motion, stream);
+}
+
+}  // namespace psy

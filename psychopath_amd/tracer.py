@@ -313,6 +313,28 @@ class Tracer:
         check("psy_trace_counted", lib().psy_trace_counted(self.handle, C.byref(rs), rays.n, C.byref(hs), C.byref(c)))
         return {f: getattr(c, f) for f, _ in _lib.TraceCounters._fields_}
 
+    def tuning(self) -> dict:
+        t = _lib.TraceTuning()
+        check("psy_tracer_get_tuning", lib().psy_tracer_get_tuning(self.handle, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
+
+    def set_tuning(self, kernel=None, waves_per_cu=None, refill_min=None, stack_lds=None) -> None:
+        cur = self.tuning()
+        for k, v in dict(kernel=kernel, waves_per_cu=waves_per_cu, refill_min=refill_min,
+                         stack_lds=stack_lds).items():
+            if v is not None:
+                cur[k] = v
+        t = _lib.TraceTuning(**cur)
+        check("psy_tracer_set_tuning", lib().psy_tracer_set_tuning(self.handle, C.byref(t)))
+
+    def autotune(self, rays: DeviceRays, hits: DeviceHits) -> dict:
+        """Times the candidate launch shapes on `rays`, keeps the fastest."""
+        t = _lib.TraceTuning()
+        rs, hs = rays.soa(), hits.soa()
+        check("psy_tracer_autotune", lib().psy_tracer_autotune(self.handle, C.byref(rs), rays.n, C.byref(hs),
+                                                               C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
+
     def sync(self) -> None:
         check("psy_tracer_sync", lib().psy_tracer_sync(self.handle))
 

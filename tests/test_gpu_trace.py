@@ -264,3 +264,38 @@ def test_instanced_motion_within_bar(instanced):
     sh = scenes.shadow_rays(lo, hi, 40_000, seeds=(134, 135))
     g, w = gpu_trace(tr, sh), o.trace(sh)
     assert (g["kind"] != w["kind"]).sum() <= 2
+
+
+def test_launch_shapes_give_identical_hits(small, instanced):
+    # The simple grid kernel and persistent kernels of every LDS depth and
+    # refill threshold must write the same records (static scene and the
+    # instanced/motion scene, which selects the other kernel specialisations).
+    v, f, s, o, tr = small
+    r = scenes.incoherent_rays(v.min(0), v.max(0), 30_000, seeds=(141, 142, 143))
+    si, oi, tri = instanced
+    lo, hi = np.array([-10, -10, -2], np.float32), np.array([10, 10, 2], np.float32)
+    ri = scenes.incoherent_rays(lo, hi, 30_000, seeds=(144, 145, 146))
+    for tracer, rays in ((tr, r), (tri, ri)):
+        base = None
+        for shape in ((1, 1, 1, 8), (0, 1, 1, 8), (0, 8, 16, 16), (0, 16, 64, 24), (0, 32, 32, 8)):
+            tracer.set_tuning(*shape)
+            g = gpu_trace(tracer, rays)
+            if base is None:
+                base = g
+            else:
+                assert_exact(g, base, f"shape {shape}")
+        tracer.set_tuning(0, 16, 16, 16)
+
+
+def test_autotune_picks_a_valid_shape(small):
+    v, f, s, o, tr = small
+    r = scenes.incoherent_rays(v.min(0), v.max(0), 200_000, seeds=(151, 152, 153))
+    rays = DeviceRays(ray_batch(r))
+    hits = DeviceHits(rays.n)
+    before = tr.rays_traced()
+    best = tr.autotune(rays, hits)
+    assert best == tr.tuning()
+    assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24)
+    assert tr.rays_traced() == before
+    assert_exact(gpu_trace(tr, r), o.trace(r), "after autotune")
+    tr.set_tuning(0, 16, 16, 16)
