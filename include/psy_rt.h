@@ -140,8 +140,10 @@ typedef struct {
  * `waves_per_cu` wave64 workgroups per compute unit resident, each taking
  * rays from a global counter; lanes whose ray finished are refilled once
  * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
- * live in LDS (8, 16 or 24) and the rest in private memory.  The simple
- * kernel runs one lane per ray over a launch-wide grid. */
+ * live in LDS (8, 16 or 24) and the rest in private memory; `simd_waves`
+ * is the register budget, as resident waves per SIMD (4: up to 128 VGPRs,
+ * 8: at most 64).  The simple kernel runs one lane per ray over a
+ * launch-wide grid. */
 #define PSY_KERNEL_PERSISTENT 0u
 #define PSY_KERNEL_SIMPLE 1u
 typedef struct {
@@ -149,6 +151,7 @@ typedef struct {
     uint32_t waves_per_cu;
     uint32_t refill_min;
     uint32_t stack_lds;
+    uint32_t simd_waves;
 } psy_trace_tuning;
 
 /* ---- errors ---------------------------------------------------------- */
@@ -204,7 +207,8 @@ int psy_tracer_get_tuning(const psy_tracer* tracer, psy_trace_tuning* tuning);
 /* Sets the launch shape; values outside the documented ranges are rejected. */
 int psy_tracer_set_tuning(psy_tracer* tracer, const psy_trace_tuning* tuning);
 /* Autotuner: times every candidate launch shape (kernel, waves per CU,
- * refill threshold, LDS stack depth) on the given device-resident rays,
+ * refill threshold, LDS stack depth, register budget) on the given
+ * device-resident rays,
  * keeps the fastest on the tracer and returns it in `best` (may be NULL).
  * `hits` is device scratch for the candidate runs.  Results do not depend on
  * the launch shape. */

@@ -106,6 +106,7 @@ class TraceTuning(C.Structure):
         ("waves_per_cu", C.c_uint32),
         ("refill_min", C.c_uint32),
         ("stack_lds", C.c_uint32),
+        ("simd_waves", C.c_uint32),
     ]
 
 

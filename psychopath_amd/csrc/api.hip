@@ -73,7 +73,7 @@ struct psy_tracer {
     void* staging = nullptr;
     unsigned long long* counters = nullptr;
     unsigned int* work = nullptr;   // persistent-kernel ray counter
-    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16};
+    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
     uint32_t cu_count = 256;
 
     ~psy_tracer() {
@@ -149,7 +149,8 @@ int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hi
     if (tr->tuning.kernel == PSY_KERNEL_SIMPLE) {
         e = launch_trace(a, counted, tr->stream);
     } else {
-        TraceTuning tt{tr->cu_count * tr->tuning.waves_per_cu, tr->tuning.refill_min, tr->tuning.stack_lds};
+        TraceTuning tt{tr->cu_count * tr->tuning.waves_per_cu, tr->tuning.refill_min, tr->tuning.stack_lds,
+                       tr->tuning.simd_waves};
         e = launch_trace_persistent(a, counted, tr->scene->host.has_xf, tr->scene->host.has_motion, tr->work, tt,
                                     tr->stream);
     }
@@ -347,6 +348,7 @@ int psy_tracer_set_tuning(psy_tracer* tr, const psy_trace_tuning* t) {
     if (t->refill_min < 1 || t->refill_min > 64) return fail(PSY_E_RANGE, "refill_min must be in [1, 64]");
     if (t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 24)
         return fail(PSY_E_RANGE, "stack_lds must be 8, 16 or 24");
+    if (t->simd_waves != 4 && t->simd_waves != 8) return fail(PSY_E_RANGE, "simd_waves must be 4 or 8");
     tr->tuning = *t;
     return PSY_OK;
 }
@@ -358,13 +360,14 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
     if (rc) return rc;
     if (n == 0) return fail(PSY_E_INVALID, "autotune needs rays");
     std::vector<psy_trace_tuning> cands;
-    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8});
+    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8, 4});
     const uint32_t wpc[] = {8, 12, 16, 24, 32};
     const uint32_t refill[] = {1, 8, 16, 32};
     const uint32_t depth[] = {8, 16, 24};
     for (uint32_t w : wpc)
         for (uint32_t r : refill)
-            for (uint32_t dd : depth) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd});
+            for (uint32_t dd : depth)
+                for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ});
     const psy_trace_tuning saved = tr->tuning;
     float best_ms = 1e30f;
     psy_trace_tuning best_t = saved;

@@ -40,6 +40,7 @@ struct TraceTuning {
     uint32_t waves;       // wave64 workgroups launched
     uint32_t refill_min;  // idle lanes that trigger a refill (1..64)
     uint32_t lds_depth;   // stack entries per lane kept in LDS (8, 16, 24)
+    uint32_t occupancy;   // register budget as waves per SIMD (4 or 8)
 };
 
 // One lane per ray, one launch-wide grid (trace.hip).

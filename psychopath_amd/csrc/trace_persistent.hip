@@ -26,13 +26,41 @@ namespace {
 
 constexpr int WAVE = 64;
 
-template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH>
-__global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min) {
-    constexpr int SPILL_DEPTH = TRACE_STACK_CAPACITY - LDS_DEPTH;
+// Short stack: the newest LDS_DEPTH entries of each lane live in an LDS ring
+// (one column per lane, stride WAVE); when the ring is full the oldest entry
+// moves to a private overflow array and comes back only when the stack
+// unwinds to it, so the pushes and pops of the hot path touch LDS only.
+// `sp` counts entries, entries [0, bot) are in the overflow array.  Macros
+// keep sp and bot in registers (a struct holding the array would live in
+// private memory as a whole).
+#define SSTACK_PUSH(v)                                                    \
+    do {                                                                  \
+        if (sp - bot == LDS_DEPTH) {                                      \
+            spill[bot] = lds_col[(bot % LDS_DEPTH) * WAVE];               \
+            bot++;                                                        \
+        }                                                                 \
+        lds_col[(sp % LDS_DEPTH) * WAVE] = (v);                           \
+        sp++;                                                             \
+    } while (0)
+#define SSTACK_POP(out)                                                   \
+    do {                                                                  \
+        sp--;                                                             \
+        if (sp < bot) {                                                   \
+            bot = sp;                                                     \
+            (out) = spill[sp];                                            \
+        } else {                                                          \
+            (out) = lds_col[(sp % LDS_DEPTH) * WAVE];                     \
+        }                                                                 \
+    } while (0)
+
+template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
+void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
     const int lane = threadIdx.x;
     uint32_t* const lds_col = lds_stack + lane;
-    uint32_t spill[SPILL_DEPTH];
+    uint32_t spill[TRACE_STACK_CAPACITY];
+    int sp = 0, bot = 0;
 
     const uint32_t n = (uint32_t)a.n;
     const Xf ident = xf_identity();
@@ -47,7 +75,6 @@ __global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, uns
     V3 lo{}, di{};
     uint32_t oct = 0;
     uint32_t cur = REF_EMPTY;
-    int sp = 0;
     bool in_inst = false;
     int inst_base = 0;
     bool has_space = false;
@@ -90,6 +117,7 @@ __global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, uns
                     pre = tri_precompute(d);
                     cur = a.top_root;
                     sp = 0;
+                    bot = 0;
                     in_inst = false;
                     kind = PSY_HIT_MISS;
                 }
@@ -138,11 +166,7 @@ __global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, uns
                 if (k >= (int)cnt) continue;
                 const uint32_t c = (ord >> (2 * k)) & 3u;
                 if (mask & (1u << c)) {
-                    if (next != REF_EMPTY) {
-                        if (sp < LDS_DEPTH) lds_col[sp * WAVE] = next;
-                        else spill[sp - LDS_DEPTH] = next;
-                        sp++;
-                    }
+                    if (next != REF_EMPTY) SSTACK_PUSH(next);
                     next = c == 0 ? refs.x : (c == 1 ? refs.y : (c == 2 ? refs.z : refs.w));
                 }
             }
@@ -255,12 +279,8 @@ __global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, uns
                     oct = octant_of(di);
                 }
             }
-            if (sp == 0) {
-                finished = true;
-            } else {
-                sp--;
-                cur = sp < LDS_DEPTH ? lds_col[sp * WAVE] : spill[sp - LDS_DEPTH];
-            }
+            if (sp == 0) finished = true;
+            else SSTACK_POP(cur);
         }
 
         if (finished) {
@@ -287,8 +307,14 @@ __global__ __launch_bounds__(WAVE) void trace_persistent_kernel(TraceArgs a, uns
 
 template <bool COUNT, bool XF, bool MOTION, int D>
 hipError_t launch_one(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
-    hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D>), dim3(tune.waves), dim3(WAVE), 0, s, a, work,
-                       tune.refill_min);
+    // Register budget: 8 waves per SIMD caps the kernel at 64 VGPRs (the
+    // compiler keeps the rest in private memory), 4 lets it use up to 128.
+    if (!COUNT && tune.occupancy >= 8)
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8>), dim3(tune.waves), dim3(WAVE), 0, s, a,
+                           work, tune.refill_min);
+    else
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4>), dim3(tune.waves), dim3(WAVE), 0, s, a,
+                           work, tune.refill_min);
     return hipGetLastError();
 }
 

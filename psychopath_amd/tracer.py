@@ -318,10 +318,11 @@ class Tracer:
         check("psy_tracer_get_tuning", lib().psy_tracer_get_tuning(self.handle, C.byref(t)))
         return {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
 
-    def set_tuning(self, kernel=None, waves_per_cu=None, refill_min=None, stack_lds=None) -> None:
+    def set_tuning(self, kernel=None, waves_per_cu=None, refill_min=None, stack_lds=None,
+                   simd_waves=None) -> None:
         cur = self.tuning()
         for k, v in dict(kernel=kernel, waves_per_cu=waves_per_cu, refill_min=refill_min,
-                         stack_lds=stack_lds).items():
+                         stack_lds=stack_lds, simd_waves=simd_waves).items():
             if v is not None:
                 cur[k] = v
         t = _lib.TraceTuning(**cur)

@@ -277,14 +277,35 @@ def test_launch_shapes_give_identical_hits(small, instanced):
     ri = scenes.incoherent_rays(lo, hi, 30_000, seeds=(144, 145, 146))
     for tracer, rays in ((tr, r), (tri, ri)):
         base = None
-        for shape in ((1, 1, 1, 8), (0, 1, 1, 8), (0, 8, 16, 16), (0, 16, 64, 24), (0, 32, 32, 8)):
+        for shape in ((1, 1, 1, 8, 4), (0, 1, 1, 8, 4), (0, 8, 16, 16, 8), (0, 16, 64, 24, 4), (0, 32, 32, 8, 8),
+                      (0, 4, 2, 8, 8)):
             tracer.set_tuning(*shape)
             g = gpu_trace(tracer, rays)
             if base is None:
                 base = g
             else:
                 assert_exact(g, base, f"shape {shape}")
-        tracer.set_tuning(0, 16, 16, 16)
+        tracer.set_tuning(0, 16, 16, 16, 4)
+
+
+def test_deep_stack_spills_to_private_memory():
+    # 4000 copies of one triangle build a deep tree (median splits): many
+    # pending entries overflow the 8-entry LDS ring into private memory and
+    # come back; hits must still match the oracle exactly.
+    n = 4000
+    v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (n, 1))
+    v = (v + np.repeat(np.arange(n, dtype=np.float32)[:, None] * np.float32(1e-3), 3, axis=0)
+         * np.array([0, 0, 1], np.float32)).astype(np.float32)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    s, o = build_pair([(v, f)], [(0, None)])
+    assert s.info()["max_depth"] >= 6
+    tr = Tracer(s, max_batch=1 << 16)
+    r = scenes.incoherent_rays(np.array([-0.5, -0.5, -1], np.float32), np.array([1.5, 1.5, 5], np.float32), 20_000,
+                               seeds=(161, 162, 163))
+    want = o.trace(r)
+    for shape in ((0, 8, 8, 8, 8), (0, 8, 8, 8, 4), (1, 1, 1, 8, 4)):
+        tr.set_tuning(*shape)
+        assert_exact(gpu_trace(tr, r), want, f"deep {shape}")
 
 
 def test_autotune_picks_a_valid_shape(small):
@@ -295,7 +316,7 @@ def test_autotune_picks_a_valid_shape(small):
     before = tr.rays_traced()
     best = tr.autotune(rays, hits)
     assert best == tr.tuning()
-    assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24)
+    assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24) and best["simd_waves"] in (4, 8)
     assert tr.rays_traced() == before
     assert_exact(gpu_trace(tr, r), o.trace(r), "after autotune")
-    tr.set_tuning(0, 16, 16, 16)
+    tr.set_tuning(0, 16, 16, 16, 4)
