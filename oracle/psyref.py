@@ -160,3 +160,84 @@ class OracleScene:
             lib().psyref_scene_free(self.h)
         except Exception:
             pass
+
+
+class _Color(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("a", C.c_float), ("b", C.c_float), ("c", C.c_float)]
+
+
+class _Shader(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("color", _Color), ("roughness", C.c_float), ("fresnel", C.c_float)]
+
+
+class _Distant(C.Structure):
+    _fields_ = [("radius", C.c_float), ("dir", C.c_float * 3), ("color", _Color)]
+
+
+_SIGS.update({
+    "psyref_scene_add_rect_light": (C.c_int, [_vp, C.c_float, C.c_float, C.POINTER(_Color)]),
+    "psyref_scene_add_sphere_light": (C.c_int, [_vp, C.c_float, C.POINTER(_Color)]),
+    "psyref_scene_add_light_instance": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _fp, C.c_uint32]),
+    "psyref_render": (C.c_int, [_vp, C.POINTER(_Shader), C.c_uint32, C.POINTER(_Color), C.POINTER(_Distant),
+                                C.c_uint32, _fp, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _fp,
+                                C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "psyref_halton": (C.c_float, [C.c_uint32, C.c_uint32]),
+    "psyref_sample_4d": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
+    "psyref_probit": (C.c_float, [C.c_float, C.c_float]),
+    "psyref_fast_exp": (C.c_float, [C.c_float]),
+    "psyref_xyz_to_spectrum": (None, [C.c_float, C.c_float, C.c_float, C.c_float, _fp]),
+    "psyref_spectrum_to_xyz": (None, [_fp, C.c_float, _fp]),
+    "psyref_planck": (C.c_float, [C.c_float, C.c_float]),
+    "psyref_meng_sha256": (C.c_char_p, []),
+})
+
+
+def _color(c):
+    return _Color(int(c[0]), float(c[1]), float(c[2]), float(c[3]))
+
+
+def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
+    """Renders a psychopath_amd.scenes.RenderScene with the oracle.
+    Returns the XYZ image [H, W, 3] and timing figures."""
+    o = OracleScene()
+    for v, t in scene.meshes:
+        o.add_mesh(v, t)
+    L = lib()
+    for (w, h, col) in scene.rect_lights:
+        L.psyref_scene_add_rect_light(o.h, w, h, C.byref(_color(col)))
+    for (r, col) in scene.sphere_lights:
+        L.psyref_scene_add_sphere_light(o.h, r, C.byref(_color(col)))
+    for inst in scene.instances:
+        xf = inst.get("xforms")
+        if inst["kind"] == "mesh":
+            o.add_instance(inst["index"], xf, -1 if inst["shader"] is None else inst["shader"])
+        else:
+            x = f32(xf).reshape(-1, 12)
+            kind = 1 if inst["kind"] == "rect" else 2
+            if L.psyref_scene_add_light_instance(o.h, kind, inst["index"], fptr(x), x.shape[0]) < 0:
+                raise ValueError("light instances need transforms")
+            o.instance_count += 1
+    o.build()
+    sh = (_Shader * max(1, len(scene.shaders)))()
+    for i, (k, col, rough, fres) in enumerate(scene.shaders):
+        sh[i] = _Shader(k, _color(col), rough, fres)
+    dl = (_Distant * max(1, len(scene.distant_lights)))()
+    for i, (r, d, col) in enumerate(scene.distant_lights):
+        dl[i] = _Distant(r, (C.c_float * 3)(*d), _color(col))
+    cam = scene.camera
+    cx = f32(cam["xforms"]).reshape(-1, 12)
+    st = scene.settings
+    img = np.zeros((st["height"], st["width"], 3), np.float32)
+    ts = C.c_double()
+    nr = C.c_uint64()
+    threads = threads or min(16, os.cpu_count() or 1)
+    t0 = __import__("time").time()
+    rc = L.psyref_render(o.h, sh, len(scene.shaders), C.byref(_color(scene.background)), dl,
+                         len(scene.distant_lights), fptr(cx), cx.shape[0], cam["fov"], cam["aperture"], cam["focus"],
+                         st["width"], st["height"], st["spp"], st["max_bounces"], st["seed"], threads, fptr(img),
+                         C.byref(ts), C.byref(nr))
+    wall = __import__("time").time() - t0
+    if rc != 0:
+        raise RuntimeError(f"psyref_render failed ({rc})")
+    return img, dict(trace_seconds=ts.value, rays=nr.value, wall_seconds=wall, threads=threads)

@@ -219,3 +219,107 @@ int psyref_trace(void* s, const float* ox, const float* oy, const float* oz, con
 }
 
 }  // extern "C"
+
+// ---- rendering ---------------------------------------------------------------
+#include "psyref_render.h"
+
+namespace {
+struct PsyrefColor { uint32_t kind; float a, b, c; };
+struct PsyrefShader { uint32_t kind; PsyrefColor color; float roughness, fresnel; };
+struct PsyrefDistant { float radius; float dir[3]; PsyrefColor color; };
+Color to_color(const PsyrefColor& c) {
+    Color r;
+    r.kind = c.kind;
+    r.a = c.a; r.b = c.b; r.c = c.c;
+    return r;
+}
+}  // namespace
+
+extern "C" {
+
+int psyref_scene_add_rect_light(void* s, float w, float h, const PsyrefColor* color) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    RectangleLight l;
+    l.w = w; l.h = h; l.color = to_color(*color);
+    sc->root.rect_lights.push_back(l);
+    return (int)sc->root.rect_lights.size() - 1;
+}
+int psyref_scene_add_sphere_light(void* s, float radius, const PsyrefColor* color) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    SphereLight l;
+    l.radius = radius; l.color = to_color(*color);
+    sc->root.sphere_lights.push_back(l);
+    return (int)sc->root.sphere_lights.size() - 1;
+}
+// kind: 1 rectangle light, 2 sphere light.  Lights need transforms
+// (the reference lerps the transform list, rectangle_light.rs:270).
+int psyref_scene_add_light_instance(void* s, uint32_t kind, uint32_t light, const float* xforms, uint32_t n_xf) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    if (n_xf == 0 || (kind != OBJ_RECT_LIGHT && kind != OBJ_SPHERE_LIGHT)) return -1;
+    std::vector<Transform> xf;
+    for (uint32_t i = 0; i < n_xf; i++) xf.push_back(xf_from(xforms + 12 * i));
+    sc->root.add_instance(false, light, -1, xf.data(), n_xf, (uint8_t)kind);
+    return (int)sc->root.instances.size() - 1;
+}
+
+// Renders the built scene.  camera_xforms: n_cam x 12 (camera to world).
+int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, const PsyrefColor* background,
+                  const PsyrefDistant* distant, uint32_t n_distant, const float* camera_xforms, uint32_t n_cam,
+                  float fov, float aperture, float focus, uint32_t width, uint32_t height, uint32_t spp,
+                  uint32_t max_bounces, uint32_t seed, uint32_t threads, float* xyz, double* trace_seconds,
+                  uint64_t* rays) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    if (!sc->built || n_cam == 0 || width == 0 || height == 0 || spp == 0) return -1;
+    RenderScene rsc;
+    rsc.root = &sc->root;
+    for (uint32_t i = 0; i < n_shaders; i++) {
+        Closure c;
+        c.kind = shaders[i].kind;
+        c.color = to_color(shaders[i].color);
+        c.roughness = shaders[i].roughness;
+        c.fresnel = shaders[i].fresnel;
+        rsc.shaders.push_back(c);
+    }
+    for (const Instance& inst : sc->root.instances)
+        if (inst.obj_kind == OBJ_MESH && (inst.shader < 0 || (uint32_t)inst.shader >= n_shaders)) return -2;
+    rsc.world.background = to_color(*background);
+    for (uint32_t i = 0; i < n_distant; i++) {
+        DistantDiskLight l;
+        l.radius = distant[i].radius;
+        l.direction = Vec3(distant[i].dir[0], distant[i].dir[1], distant[i].dir[2]);
+        l.color = to_color(distant[i].color);
+        rsc.world.lights.push_back(l);
+    }
+    for (uint32_t i = 0; i < n_cam; i++) rsc.camera.transforms.push_back(xf_from(camera_xforms + 12 * i));
+    rsc.camera.fov = fov;
+    rsc.camera.aperture_radius = aperture;
+    rsc.camera.focus_distance = focus;
+    rsc.build_lights();
+    RenderSettings rs;
+    rs.width = width; rs.height = height; rs.spp = spp; rs.max_bounces = max_bounces; rs.seed = seed;
+    double ts = 0;
+    uint64_t nr = 0;
+    render(rsc, rs, xyz, threads ? threads : 1, ts, nr);
+    if (trace_seconds) *trace_seconds = ts;
+    if (rays) *rays = nr;
+    return 0;
+}
+
+float psyref_halton(uint32_t dim, uint32_t index) { return halton().sample(dim, index); }
+void psyref_sample_4d(uint32_t i, uint32_t set, uint32_t px, uint32_t py, uint32_t seed, float* out) {
+    get_sample_4d(i, set, px, py, seed, out);
+}
+float psyref_probit(float n, float width) { return probit(n, width); }
+float psyref_fast_exp(float p) { return fast_exp(p); }
+void psyref_xyz_to_spectrum(float x, float y, float z, float hero, float* out) {
+    Vec4 r = xyz_to_spectrum_4(x, y, z, wavelengths(hero));
+    for (int k = 0; k < 4; k++) out[k] = r[k];
+}
+void psyref_spectrum_to_xyz(const float* e, float hero, float* out) {
+    XYZ r = xyz_from_spectral_sample(Vec4(e[0], e[1], e[2], e[3]), hero);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+float psyref_planck(float temperature, float wavelength) { return plancks_law(temperature, wavelength); }
+const char* psyref_meng_sha256() { return PSY_MENG_TABLES_SHA256; }
+
+}  // extern "C"

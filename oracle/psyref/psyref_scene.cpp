@@ -206,8 +206,10 @@ void TransformStack::pop() {
 }
 
 // ---- scene/assembly.rs -----------------------------------------------------
-void Assembly::add_instance(bool is_assembly, size_t data_index, int32_t shader, const Transform* xf, size_t n_xf) {
+void Assembly::add_instance(bool is_assembly, size_t data_index, int32_t shader, const Transform* xf, size_t n_xf,
+                            uint8_t obj_kind) {
     Instance inst;
+    inst.obj_kind = obj_kind;
     inst.is_assembly = is_assembly;
     inst.data_index = data_index;
     inst.shader = shader;
@@ -226,8 +228,7 @@ void Assembly::build() {
     std::vector<size_t> bis{0};
     std::vector<BBox> bbs;
     for (const Instance& inst : instances) {
-        const std::vector<BBox>& src =
-            inst.is_assembly ? assemblies[inst.data_index]->bounds() : objects[inst.data_index]->bounds();
+        std::vector<BBox> src = object_bounds(inst);
         std::vector<BBox> out;
         if (inst.has_xforms)
             transform_bbox_slice_from(src.data(), src.size(), xforms.data() + inst.xf_start,
@@ -249,6 +250,73 @@ void Assembly::build() {
     std::vector<Instance> permuted(instances.size());
     for (size_t i = 0; i < keys.size(); i++) permuted[i] = instances[keys[i]];
     instances.swap(permuted);
+    inst_bound_start = bis;
+    inst_bounds = bbs;
+}
+
+std::vector<BBox> Assembly::object_bounds(const Instance& inst) const {
+    if (inst.is_assembly) return assemblies[inst.data_index]->bounds();
+    switch (inst.obj_kind) {
+        case OBJ_RECT_LIGHT: return {rect_lights[inst.data_index].bounds()};
+        case OBJ_SPHERE_LIGHT: return {sphere_lights[inst.data_index].bounds()};
+        default: return objects[inst.data_index]->bounds();
+    }
+}
+
+float Assembly::light_energy(const Instance& inst) const {
+    switch (inst.obj_kind) {
+        case OBJ_RECT_LIGHT: return rect_lights[inst.data_index].approximate_energy();
+        case OBJ_SPHERE_LIGHT: return sphere_lights[inst.data_index].approximate_energy();
+        default: return 0.0f;
+    }
+}
+
+// Light surfaces as trace leaves: RectangleLight::intersect_rays
+// (rectangle_light.rs:257-338) and SphereLight::intersect_rays
+// (sphere_light.rs:206-339).
+static void intersect_light_rays(const Assembly* a, const Instance& inst, RayBatch& rays, RayStack& ray_stack,
+                                 std::vector<Isect>& isects, const Transform* space, size_t space_len) {
+    ray_stack.pop_do_next_task([&](size_t ri) {
+        float time = rays.time(ri);
+        Transform xform = lerp_slice(space, space_len, time);
+        float t, pos_err, pdf;
+        Vec3 pos, nrm;
+        bool hit;
+        Color emit;
+        if (inst.obj_kind == OBJ_RECT_LIGHT) {
+            const RectangleLight& l = a->rect_lights[inst.data_index];
+            hit = l.intersect(xform, rays.orig(ri), rays.dir(ri), rays.max_t(ri), t, pos, pos_err, nrm, pdf);
+            emit = l.emit_color();
+        } else {
+            const SphereLight& l = a->sphere_lights[inst.data_index];
+            hit = l.intersect(xform, rays.orig(ri), rays.orig_local(ri), rays.dir(ri), rays.max_t(ri), t, pos,
+                              pos_err, nrm, pdf);
+            emit = l.emit_color();
+        }
+        if (!hit) return;
+        Isect& is = isects[ri];
+        if (rays.is_occlusion(ri)) {
+            is = Isect();
+            is.kind = ISECT_OCCLUDE;
+            is.inst_id = (uint32_t)inst.id;
+            rays.mark_done(ri);
+            return;
+        }
+        is = Isect();
+        is.kind = ISECT_HIT;
+        is.t = t;
+        is.inst_id = (uint32_t)inst.id;
+        is.incoming = rays.dir(ri);
+        is.pos = pos;
+        is.pos_err = pos_err;
+        is.nor = nrm;
+        is.nor_g = nrm;
+        is.local_space = xform;
+        is.sample_pdf = pdf;
+        is.light = true;
+        is.emit = emit;
+        rays.set_max_t(ri, t);
+    });
 }
 
 // ---- tracer.rs -------------------------------------------------------------
@@ -278,7 +346,11 @@ void Tracer::trace_assembly(const Assembly* assembly, RayBatch& rays) {
                 ray_stack.do_next_task([&](size_t ri) { rays.update_local(ri, lerp_slice(xf, n, rays.time(ri))); });
                 ray_stack.duplicate_next_task();
             }
-            if (!inst.is_assembly) {
+            if (!inst.is_assembly && inst.obj_kind != OBJ_MESH) {
+                size_t n;
+                const Transform* xf = xform_stack.top(n);
+                intersect_light_rays(assembly, inst, rays, ray_stack, isects, xf, n);
+            } else if (!inst.is_assembly) {
                 size_t n;
                 const Transform* xf = xform_stack.top(n);
                 assembly->objects[inst.data_index]->intersect_rays(rays, ray_stack, isects, inst.shader,

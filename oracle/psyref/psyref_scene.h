@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "psyref_bvh.h"
+#include "psyref_lights.h"
 #include "psyref_tri.h"
 
 namespace psyref {
@@ -28,6 +29,8 @@ struct Isect {
     float pos_err = 0;
     Transform local_space;
     float sample_pdf = 0;
+    bool light = false;      // hit a light surface: Emit(emit) closure
+    Color emit;
 };
 
 // triangle_mesh.rs:19-116
@@ -65,8 +68,11 @@ struct TransformStack {
     }
 };
 
-// scene/assembly.rs:398-417
+// scene/assembly.rs:398-417.  Objects are meshes (Object::Surface) or
+// light surfaces (Object::SurfaceLight).
+enum ObjectKind : uint8_t { OBJ_MESH = 0, OBJ_RECT_LIGHT = 1, OBJ_SPHERE_LIGHT = 2 };
 struct Instance {
+    uint8_t obj_kind = OBJ_MESH;
     bool is_assembly = false;
     size_t data_index = 0;
     int32_t shader = -1;
@@ -79,12 +85,20 @@ struct Assembly {
     std::vector<Instance> instances;
     std::vector<Transform> xforms;
     std::vector<std::unique_ptr<TriangleMesh>> objects;  // surface objects
+    std::vector<RectangleLight> rect_lights;
+    std::vector<SphereLight> sphere_lights;
     std::vector<std::unique_ptr<Assembly>> assemblies;
     BVH4 object_accel;
+    // Per-instance world bounds, indexed by Instance::id (assembly.rs:354-395)
+    std::vector<size_t> inst_bound_start;
+    std::vector<BBox> inst_bounds;
     const std::vector<BBox>& bounds() const { return object_accel.root_bounds; }
 
     // AssemblyBuilder::add_instance (assembly.rs:223-282)
-    void add_instance(bool is_assembly, size_t data_index, int32_t shader, const Transform* xf, size_t n_xf);
+    void add_instance(bool is_assembly, size_t data_index, int32_t shader, const Transform* xf, size_t n_xf,
+                      uint8_t obj_kind = OBJ_MESH);
+    std::vector<BBox> object_bounds(const Instance& inst) const;
+    float light_energy(const Instance& inst) const;
     // AssemblyBuilder::build (assembly.rs:288-395), object accel only.
     void build();
 };

@@ -197,3 +197,123 @@ def raycast_workload(levels: int = 9, n_rays: int = 1 << 24):
     verts, tris = displaced_sphere(levels)
     lo, hi = verts.min(axis=0), verts.max(axis=0)
     return verts, tris, incoherent_rays(lo, hi, n_rays)
+
+
+# ------------------------------------------------------------- render scenes
+# Colours are (kind, a, b, c): kind 0 = XYZ (x, y, z), 1 = blackbody
+# (temperature K, factor), 2 = normalised temperature (temperature K, factor)
+# (Color, src/color.rs:46-59).  Shaders are (kind, colour, roughness, fresnel)
+# with kind 0 = Lambert, 1 = GGX, 2 = Emit (SimpleSurfaceShader,
+# src/shading/mod.rs:27-40).
+
+COLOR_XYZ, COLOR_BLACKBODY, COLOR_TEMPERATURE = 0, 1, 2
+SHADER_LAMBERT, SHADER_GGX, SHADER_EMIT = 0, 1, 2
+
+
+def rec709_e_to_xyz_matrix() -> np.ndarray:
+    """RGB (Rec.709 primaries, equal-energy white) to CIE XYZ, derived from
+    the primaries' chromaticities: columns are the primaries' XYZ scaled so
+    RGB (1,1,1) maps to the white point's XYZ (1,1,1)."""
+    prim = np.array([[0.64, 0.33], [0.30, 0.60], [0.15, 0.06]])
+    P = np.stack([prim[:, 0] / prim[:, 1], np.ones(3), (1 - prim[:, 0] - prim[:, 1]) / prim[:, 1]])
+    S = np.linalg.solve(P, np.ones(3))
+    return P * S[None, :]
+
+
+def rec709(r, g, b):
+    x, y, z = rec709_e_to_xyz_matrix() @ np.array([r, g, b], np.float64)
+    return (COLOR_XYZ, float(np.float32(x)), float(np.float32(y)), float(np.float32(z)))
+
+
+def translate_xf(t):
+    """World-to-object transform of an object placed at `t` (12 values a..l)."""
+    return np.array([1, 0, 0, -t[0], 0, 1, 0, -t[1], 0, 0, 1, -t[2]], np.float32)
+
+
+
+class RenderScene:
+    """Everything a render needs, in plain arrays: meshes, light objects,
+    instances, shaders, world, camera and settings.  Transforms are the 12
+    values a..l of Transform::new_from_values; instance transforms map world
+    to object space, camera transforms camera to world space."""
+
+    def __init__(self):
+        self.meshes = []          # (verts [S,V,3] or [V,3], tris [T,3])
+        self.rect_lights = []     # (width, height, colour)
+        self.sphere_lights = []   # (radius, colour)
+        self.instances = []       # dict(kind="mesh"|"rect"|"sphere", index, shader, xforms)
+        self.shaders = []         # (kind, colour, roughness, fresnel)
+        self.background = (COLOR_XYZ, 0.0, 0.0, 0.0)
+        self.distant_lights = []  # (radius, direction (3), colour)
+        self.camera = dict(xforms=np.zeros((1, 12), np.float32), fov=1.0, aperture=0.0, focus=1.0)
+        self.settings = dict(width=64, height=64, spp=1, max_bounces=2, seed=0)
+
+    def add_mesh_instance(self, verts, tris, shader, xforms=None):
+        self.meshes.append((np.asarray(verts, np.float32), np.asarray(tris, np.uint32)))
+        self.instances.append(dict(kind="mesh", index=len(self.meshes) - 1, shader=shader, xforms=xforms))
+
+
+def _quad(p0, p1, p2, p3):
+    v = np.array([p0, p1, p2, p3], np.float32)
+    return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def _box(center, size, angle_deg):
+    """An axis-aligned box of `size` rotated about the vertical (z) axis and
+    placed with its base at z = 0: 8 vertices, 12 outward triangles."""
+    sx, sy, sz = size
+    c = np.cos(np.radians(angle_deg))
+    s = np.sin(np.radians(angle_deg))
+    corners = []
+    for dz in (0.0, sz):
+        for dx, dy in ((-0.5, -0.5), (0.5, -0.5), (0.5, 0.5), (-0.5, 0.5)):
+            x, y = dx * sx, dy * sy
+            corners.append((center[0] + c * x - s * y, center[1] + s * x + c * y, dz))
+    v = np.array(corners, np.float32) + np.float32(0.0)
+    f = np.array([[0, 2, 1], [0, 3, 2],            # bottom
+                  [4, 5, 6], [4, 6, 7],            # top
+                  [0, 1, 5], [0, 5, 4], [1, 2, 6], [1, 6, 5],
+                  [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]], np.uint32)
+    return v, f
+
+
+def cornell_box(width=512, height=512, spp=16, max_bounces=4, seed=0) -> RenderScene:
+    """The Cornell-box workload: five axis-aligned walls (two triangles each,
+    Lambertian white/red/green), a tall and a short box (twelve triangles
+    each, rotated 15 and -18 degrees about the vertical), a 0.25 x 0.25
+    rectangle light just under the ceiling, and a pinhole camera.  Spectra
+    (Rec.709 reflectances and light colour), camera transform, field of view
+    and focal distance are those of the reference's example scene
+    (example_scenes/cornell_box.psy:10-34,38); its walls and light are
+    simplified as the workload describes."""
+    sc = RenderScene()
+    white = rec709(0.7295, 0.7355, 0.729)
+    red = rec709(0.611, 0.0555, 0.062)
+    green = rec709(0.117, 0.4125, 0.115)
+    sc.shaders = [(SHADER_LAMBERT, white, 0.0, 0.0), (SHADER_LAMBERT, red, 0.0, 0.0),
+                  (SHADER_LAMBERT, green, 0.0, 0.0)]
+    X0, X1, Y1, Z1 = -5.56, 0.0, 5.592, 5.488
+    walls = [
+        (_quad((X0, 0, 0), (X1, 0, 0), (X1, Y1, 0), (X0, Y1, 0)), 0),          # floor
+        (_quad((X0, 0, Z1), (X0, Y1, Z1), (X1, Y1, Z1), (X1, 0, Z1)), 0),      # ceiling
+        (_quad((X0, Y1, 0), (X1, Y1, 0), (X1, Y1, Z1), (X0, Y1, Z1)), 0),      # back
+        (_quad((X0, 0, 0), (X0, Y1, 0), (X0, Y1, Z1), (X0, 0, Z1)), 1),        # left, red
+        (_quad((X1, 0, 0), (X1, 0, Z1), (X1, Y1, Z1), (X1, Y1, 0)), 2),        # right, green
+    ]
+    for (v, f), shader in walls:
+        sc.add_mesh_instance(v, f, shader)
+    v, f = _box((-3.68, 3.51, 0.0), (1.65, 1.65, 3.30), 15.0)
+    sc.add_mesh_instance(v, f, 0)
+    v, f = _box((-1.86, 1.69, 0.0), (1.65, 1.65, 1.65), -18.0)
+    sc.add_mesh_instance(v, f, 0)
+    sc.rect_lights.append((0.25, 0.25, rec709(84.300003, 53.800003, 18.5)))
+    # The example's light placement (cornell_box.psy:84-86: world to object
+    # is a pure translation), lowered 1 cm under the closed ceiling.
+    light_xf = np.array([1, 0, 0, 2.779475, 0, 1, 0, -2.794788, 0, 0, 1, -5.478], np.float32)
+    sc.instances.append(dict(kind="rect", index=0, shader=None, xforms=light_xf[None]))
+    # Camera (cornell_box.psy:14): camera x -> world x, camera y -> world z,
+    # camera z (view direction) -> world y.
+    sc.camera = dict(xforms=np.array([[1, 0, 0, -2.779998, 0, 0, 1, -8.0, 0, 1, 0, 2.73001]], np.float32),
+                     fov=float(np.float32(39.449188 * np.pi / 180.0)), aperture=0.0, focus=10.62)
+    sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
+    return sc
