@@ -1,0 +1,127 @@
+/*
+ * psy_render.h — C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * Replaces the renderer's per-thread bucket loop above the tracer:
+ *   Renderer::render(..) -> (Image, RenderStats)      (src/renderer.rs:76-192)
+ *   Renderer::render_job / LightPath::{new,next}      (src/renderer.rs:194-692)
+ *   Camera::generate_ray                              (src/camera.rs:74-102)
+ *   Scene::sample_lights / Assembly::sample_lights    (src/scene/mod.rs:28-101,
+ *                                                      src/scene/assembly.rs:46-141)
+ * with the scene's shaders (SimpleSurfaceShader, src/shading/mod.rs:28-61)
+ * and world (World, src/scene/world.rs:4-8) given at renderer creation and
+ * the camera (Camera::new, src/camera.rs:22-72) and render settings
+ * (Renderer, src/renderer.rs:32-38) per render.  The output is the XYZ image
+ * the reference accumulates before writing a file (Image, src/image.rs).
+ *
+ * Light paths follow the reference's state machine exactly (hero
+ * wavelengths, next-event estimation through the light tree, power-heuristic
+ * MIS), with the Faure-permuted Halton sequence in place of the Sobol
+ * sampler for the per-pixel sample dimensions.  Paths are processed as
+ * device-resident wavefronts: camera, closest-hit and shadow traversal,
+ * material-sorted shading, and an in-order film accumulation.
+ */
+#ifndef PSY_RENDER_H
+#define PSY_RENDER_H
+
+#include "psy_rt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psy_renderer psy_renderer;
+
+/* Surface shaders (SimpleSurfaceShader, src/shading/mod.rs:28-40). */
+#define PSY_SHADER_LAMBERT 0u
+#define PSY_SHADER_GGX 1u
+#define PSY_SHADER_EMIT 2u
+typedef struct {
+    uint32_t kind;
+    psy_color color;
+    float roughness;  /* GGX only */
+    float fresnel;    /* GGX only */
+} psy_shader_desc;
+
+/* Distant disk light (DistantDiskLight::new,
+ * src/light/distant_disk_light.rs:24-36): angular radius in radians, the
+ * direction the light travels, colour. */
+typedef struct {
+    float radius;
+    float direction[3];
+    psy_color color;
+} psy_distant_light_desc;
+
+/* Shaders (indexed by psy_instance_desc.shader) and world. */
+typedef struct {
+    uint32_t shader_count;
+    const psy_shader_desc* shaders;
+    psy_color background;
+    uint32_t distant_light_count;
+    const psy_distant_light_desc* distant_lights;
+} psy_world_desc;
+
+/* Camera (Camera::new, src/camera.rs:22-72): camera-to-world transforms
+ * (12 floats each, as in psy_scene_desc), horizontal field of view in
+ * radians, thin-lens aperture radius and focus distance (aperture 0 gives a
+ * pinhole; focus distance 0 disables the lens as in the reference). */
+typedef struct {
+    uint32_t xform_count;
+    const float* xforms;
+    float fov;
+    float aperture_radius;
+    float focus_distance;
+} psy_camera_desc;
+
+/* One render (Renderer, src/renderer.rs:32-38): resolution, samples per
+ * pixel, seed, and the bounce limit of a path (the reference fixes it at 2,
+ * renderer.rs:599). */
+typedef struct {
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;
+    uint32_t max_bounces;
+    uint32_t seed;
+} psy_render_settings;
+
+/* RenderStats (src/renderer.rs:41-49), as measured on the device. */
+typedef struct {
+    uint64_t paths;        /* camera paths (width * height * spp) */
+    uint64_t rays;         /* rays traced: closest-hit and shadow */
+    uint64_t waves;        /* path wavefronts the render was split into */
+    double seconds;        /* wall time of the render call */
+    double trace_ms;       /* traversal kernel time (HIP events) */
+} psy_render_stats;
+
+/* Creates a renderer for `scene` (which must outlive it): uploads the scene
+ * if needed, the shaders, the world and the light tree, and allocates path
+ * state for up to `max_paths` concurrent paths (0 = a default of 4M).
+ * Every mesh instance must name a shader in range. */
+int psy_renderer_create(const psy_scene* scene, const psy_world_desc* world, uint64_t max_paths,
+                        psy_renderer** out);
+int psy_renderer_destroy(psy_renderer* renderer);
+
+/* Renders the image: width*height XYZ triples, row 0 at the top, each pixel
+ * the mean over its samples (accumulated in sample order).  `mem` says
+ * whether `xyz` is a host (PSY_MEM_HOST) or device (PSY_MEM_DEVICE)
+ * pointer.  Synchronous.  `stats` may be NULL. */
+int psy_render(psy_renderer* renderer, const psy_camera_desc* camera, const psy_render_settings* settings,
+               float* xyz, uint32_t mem, psy_render_stats* stats);
+
+/* Launch shape of the renderer's traversal kernel (see psy_trace_tuning;
+ * the renderer always uses the persistent kernel). */
+int psy_renderer_get_tuning(const psy_renderer* renderer, psy_trace_tuning* tuning);
+int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuning);
+
+/* Execution options, none of which changes results: PSY_RENDER_NO_GRAPHS
+ * launches every wave's kernels directly instead of replaying a captured
+ * hipGraph; PSY_RENDER_NO_SORT shades closest hits in queue order instead of
+ * binning them by closure kind first. */
+#define PSY_RENDER_NO_GRAPHS 1u
+#define PSY_RENDER_NO_SORT 2u
+int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSY_RENDER_H */

@@ -67,18 +67,50 @@ typedef struct {
     const uint32_t* indices;
 } psy_mesh_desc;
 
-/* One instance of a mesh (Instance, src/scene/assembly.rs:404-411).
+/* Colours (Color, src/color.rs:46-62): XYZ tristimulus (a, b, c = X, Y, Z),
+ * or a blackbody spectrum of temperature `a` kelvin scaled by `b`, plain
+ * (Blackbody) or normalised to roughly equal energy (Temperature). */
+#define PSY_COLOR_XYZ 0u
+#define PSY_COLOR_BLACKBODY 1u
+#define PSY_COLOR_TEMPERATURE 2u
+typedef struct {
+    uint32_t kind;
+    float a, b, c;
+} psy_color;
+
+/* Light objects of an assembly (RectangleLight::new,
+ * src/light/rectangle_light.rs:31-48; SphereLight::new,
+ * src/light/sphere_light.rs:34-48), one time sample each.  Both emit from
+ * both sides; the rectangle lies in its local xy plane, centred. */
+typedef struct {
+    float width, height;
+    psy_color color;
+} psy_rect_light_desc;
+typedef struct {
+    float radius;
+    psy_color color;
+} psy_sphere_light_desc;
+
+/* Object kinds of an instance (Object, src/scene/assembly.rs:399-402). */
+#define PSY_OBJECT_MESH 0u
+#define PSY_OBJECT_RECT_LIGHT 1u
+#define PSY_OBJECT_SPHERE_LIGHT 2u
+
+/* One instance of an object (Instance, src/scene/assembly.rs:404-411).
+ * `mesh` indexes meshes, rect_lights or sphere_lights according to `kind`.
  * `xform_count` transforms starting at `xform_start` in psy_scene_desc.xforms
- * are its world-to-object time samples (0 = no transform). */
+ * are its world-to-object time samples (0 = no transform; light instances
+ * need at least one). */
 typedef struct {
     uint32_t mesh;
     uint32_t shader;      /* surface shader index, PSY_NONE = unassigned */
     uint32_t xform_start;
     uint32_t xform_count;
+    uint32_t kind;        /* PSY_OBJECT_* */
 } psy_instance_desc;
 
-/* A single-level assembly: meshes and the instances that place them.
- * Transforms are 12 floats each, the arguments a..l of
+/* A single-level assembly: meshes, light objects and the instances that
+ * place them.  Transforms are 12 floats each, the arguments a..l of
  * Transform::new_from_values (sub_crates/math3d/src/transform.rs:29-47). */
 typedef struct {
     uint32_t mesh_count;
@@ -87,6 +119,10 @@ typedef struct {
     const psy_instance_desc* instances;
     uint32_t xform_count;
     const float* xforms;
+    uint32_t rect_light_count;
+    const psy_rect_light_desc* rect_lights;
+    uint32_t sphere_light_count;
+    const psy_sphere_light_desc* sphere_lights;
 } psy_scene_desc;
 
 /* Rays as structure-of-arrays (RayBatch, src/ray.rs:24-44).  `flags` may be
@@ -103,8 +139,10 @@ typedef struct {
  * (winding-swapped) vertex order (src/surface/triangle_mesh.rs:80): b0 to the
  * first vertex, b1 to the file's third, b2 to the file's second.  prim_id is
  * the triangle's index in the mesh description, inst_id the instance's index
- * in psy_scene_desc.instances.  Fields other than `kind` are written for
- * PSY_HIT_SURFACE only. */
+ * in psy_scene_desc.instances.  On a light instance prim_id is the
+ * rectangle's triangle (0: corners 1,2,3; 1: corners 3,4,1,
+ * rectangle_light.rs:288) or 0 for a sphere, whose barycentrics are 0.
+ * Fields other than `kind` are written for PSY_HIT_SURFACE only. */
 typedef struct {
     uint8_t* kind;
     float *t, *b0, *b1, *b2;

@@ -38,12 +38,30 @@ class MeshDesc(C.Structure):
     ]
 
 
+PSY_COLOR_XYZ, PSY_COLOR_BLACKBODY, PSY_COLOR_TEMPERATURE = 0, 1, 2
+PSY_OBJECT_MESH, PSY_OBJECT_RECT_LIGHT, PSY_OBJECT_SPHERE_LIGHT = 0, 1, 2
+PSY_SHADER_LAMBERT, PSY_SHADER_GGX, PSY_SHADER_EMIT = 0, 1, 2
+
+
+class Color(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("a", C.c_float), ("b", C.c_float), ("c", C.c_float)]
+
+
+class RectLightDesc(C.Structure):
+    _fields_ = [("width", C.c_float), ("height", C.c_float), ("color", Color)]
+
+
+class SphereLightDesc(C.Structure):
+    _fields_ = [("radius", C.c_float), ("color", Color)]
+
+
 class InstanceDesc(C.Structure):
     _fields_ = [
         ("mesh", C.c_uint32),
         ("shader", C.c_uint32),
         ("xform_start", C.c_uint32),
         ("xform_count", C.c_uint32),
+        ("kind", C.c_uint32),
     ]
 
 
@@ -55,6 +73,58 @@ class SceneDesc(C.Structure):
         ("instances", C.POINTER(InstanceDesc)),
         ("xform_count", C.c_uint32),
         ("xforms", _fp),
+        ("rect_light_count", C.c_uint32),
+        ("rect_lights", C.POINTER(RectLightDesc)),
+        ("sphere_light_count", C.c_uint32),
+        ("sphere_lights", C.POINTER(SphereLightDesc)),
+    ]
+
+
+class ShaderDesc(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("color", Color), ("roughness", C.c_float), ("fresnel", C.c_float)]
+
+
+class DistantLightDesc(C.Structure):
+    _fields_ = [("radius", C.c_float), ("direction", C.c_float * 3), ("color", Color)]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("shader_count", C.c_uint32),
+        ("shaders", C.POINTER(ShaderDesc)),
+        ("background", Color),
+        ("distant_light_count", C.c_uint32),
+        ("distant_lights", C.POINTER(DistantLightDesc)),
+    ]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [
+        ("xform_count", C.c_uint32),
+        ("xforms", _fp),
+        ("fov", C.c_float),
+        ("aperture_radius", C.c_float),
+        ("focus_distance", C.c_float),
+    ]
+
+
+class RenderSettings(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("max_bounces", C.c_uint32),
+        ("seed", C.c_uint32),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("paths", C.c_uint64),
+        ("rays", C.c_uint64),
+        ("waves", C.c_uint64),
+        ("seconds", C.c_double),
+        ("trace_ms", C.c_double),
     ]
 
 
@@ -140,7 +210,18 @@ _SIGNATURES = {
     "psy_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "psy_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "psy_device_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float)]),
+    # include/psy_render.h
+    "psy_renderer_create": (C.c_int, [C.c_void_p, C.POINTER(WorldDesc), C.c_uint64, C.POINTER(C.c_void_p)]),
+    "psy_renderer_destroy": (C.c_int, [C.c_void_p]),
+    "psy_render": (C.c_int, [C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderSettings), C.c_void_p, C.c_uint32,
+                             C.POINTER(RenderStats)]),
+    "psy_renderer_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
+    "psy_renderer_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
+    "psy_renderer_set_options": (C.c_int, [C.c_void_p, C.c_uint32]),
 }
+
+PSY_RENDER_NO_GRAPHS = 1
+PSY_RENDER_NO_SORT = 2
 
 
 class PsyError(RuntimeError):

@@ -7,13 +7,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/psy_rt.h"
-#include "scene.h"
-#include "trace.h"
+#include "api_internal.h"
 
 using namespace psy;
+using namespace psy::api;
 
-namespace {
+namespace psy {
+namespace api {
 
 thread_local std::string g_error;
 
@@ -22,75 +22,6 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define PSY_HIP(call)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess) return fail(PSY_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-template <typename T>
-int upload(const std::vector<T>& v, T** out) {
-    *out = nullptr;
-    if (v.empty()) return PSY_OK;
-    PSY_HIP(hipMalloc((void**)out, v.size() * sizeof(T)));
-    PSY_HIP(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return PSY_OK;
-}
-
-}  // namespace
-
-struct psy_scene {
-    PackedScene host;
-    GpuNode* nodes = nullptr;
-    GpuBox4* motion = nullptr;
-    uint8_t* tris = nullptr;
-    GpuMesh* meshes = nullptr;
-    GpuInstance* instances = nullptr;
-    GpuXform* xforms = nullptr;
-    GpuXform* inv_static = nullptr;
-    uint64_t device_bytes = 0;
-    bool uploaded = false;
-
-    ~psy_scene() {
-        (void)hipFree(nodes);
-        (void)hipFree(motion);
-        (void)hipFree(tris);
-        (void)hipFree(meshes);
-        (void)hipFree(instances);
-        (void)hipFree(xforms);
-        (void)hipFree(inv_static);
-    }
-};
-
-struct psy_tracer {
-    const psy_scene* scene = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t max_batch = 0;
-    uint64_t rays_traced = 0;
-    bool timed = false;
-    // device staging for host-memory calls: 8 ray floats + flags, hit fields
-    void* staging = nullptr;
-    unsigned long long* counters = nullptr;
-    unsigned int* work = nullptr;   // persistent-kernel ray counter
-    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
-    uint32_t cu_count = 256;
-
-    ~psy_tracer() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        (void)hipFree(staging);
-        (void)hipFree(counters);
-        (void)hipFree(work);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
-
-// Uploads the packed scene on first use, so scenes can be built and
-// inspected on hosts without a GPU.
 int ensure_uploaded(psy_scene* s) {
     if (s->uploaded) return PSY_OK;
     const PackedScene& h = s->host;
@@ -102,6 +33,13 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.instances, &s->instances);
     if (!r) r = upload(h.xforms, &s->xforms);
     if (!r) r = upload(h.inv_static, &s->inv_static);
+    if (!r) r = upload(h.rect_lights, &s->rect_lights);
+    if (!r) r = upload(h.sphere_lights, &s->sphere_lights);
+    if (!r) r = upload(h.inst_pos, &s->inst_pos);
+    if (!r) r = upload(h.tri_rec, &s->tri_rec);
+    if (!r) r = upload(h.light_nodes, &s->light_nodes);
+    if (!r) r = upload(h.light_boxes, &s->light_boxes);
+    if (!r) r = upload(h.light_list, &s->light_list);
     if (r == PSY_OK) s->uploaded = true;
     return r;
 }
@@ -115,9 +53,43 @@ TraceArgs scene_args(const psy_scene* s) {
     a.instances = s->instances;
     a.xforms = s->xforms;
     a.inv_static = s->inv_static;
+    a.rect_lights = s->rect_lights;
+    a.sphere_lights = s->sphere_lights;
     a.top_root = s->host.top_root;
     return a;
 }
+
+int device_cu_count(uint32_t& cus_out) {
+    int dev = 0, cus = 0;
+    PSY_HIP(hipGetDevice(&dev));
+    PSY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus > 0) cus_out = (uint32_t)cus;
+    return PSY_OK;
+}
+
+hipError_t launch_tuned(const psy_scene* scene, const psy_trace_tuning& tuning, uint32_t cu_count, unsigned int* work,
+                        const TraceArgs& a, bool counted, hipStream_t stream) {
+    if (tuning.kernel == PSY_KERNEL_SIMPLE) return launch_trace(a, counted, stream);
+    TraceTuning tt{cu_count * tuning.waves_per_cu, tuning.refill_min, tuning.stack_lds, tuning.simd_waves};
+    return launch_trace_persistent(a, counted, scene->host.has_xf || scene->host.has_lights, scene->host.has_motion,
+                                   work, tt, stream);
+}
+
+int check_tuning(const psy_trace_tuning* t) {
+    if (t->kernel != PSY_KERNEL_PERSISTENT && t->kernel != PSY_KERNEL_SIMPLE)
+        return fail(PSY_E_INVALID, "unknown kernel");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(PSY_E_RANGE, "waves_per_cu must be in [1, 32]");
+    if (t->refill_min < 1 || t->refill_min > 64) return fail(PSY_E_RANGE, "refill_min must be in [1, 64]");
+    if (t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 24)
+        return fail(PSY_E_RANGE, "stack_lds must be 8, 16 or 24");
+    if (t->simd_waves != 4 && t->simd_waves != 8) return fail(PSY_E_RANGE, "simd_waves must be 4 or 8");
+    return PSY_OK;
+}
+
+}  // namespace api
+}  // namespace psy
+
+namespace {
 
 void set_rays(TraceArgs& a, const psy_ray_soa* r) {
     a.ox = r->ox; a.oy = r->oy; a.oz = r->oz;
@@ -145,15 +117,7 @@ int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hi
     a.n = n;
     a.counters = tr->counters;
     PSY_HIP(hipEventRecord(tr->ev0, tr->stream));
-    hipError_t e;
-    if (tr->tuning.kernel == PSY_KERNEL_SIMPLE) {
-        e = launch_trace(a, counted, tr->stream);
-    } else {
-        TraceTuning tt{tr->cu_count * tr->tuning.waves_per_cu, tr->tuning.refill_min, tr->tuning.stack_lds,
-                       tr->tuning.simd_waves};
-        e = launch_trace_persistent(a, counted, tr->scene->host.has_xf, tr->scene->host.has_motion, tr->work, tt,
-                                    tr->stream);
-    }
+    hipError_t e = launch_tuned(tr->scene, tr->tuning, tr->cu_count, tr->work, a, counted, tr->stream);
     if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
     PSY_HIP(hipEventRecord(tr->ev1, tr->stream));
     tr->timed = true;
@@ -237,7 +201,10 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
     const PackedScene& h = s->host;
     s->device_bytes = h.nodes.size() * sizeof(GpuNode) + h.motion.size() * sizeof(GpuBox4) + h.tris.size() +
                       h.meshes.size() * sizeof(GpuMesh) + h.instances.size() * sizeof(GpuInstance) +
-                      (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform);
+                      (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform) +
+                      h.rect_lights.size() * sizeof(GpuRectLight) + h.sphere_lights.size() * sizeof(GpuSphereLight) +
+                      (h.inst_pos.size() + h.tri_rec.size() + h.light_list.size()) * 4 +
+                      h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox);
     *out = s;
     return PSY_OK;
 }
@@ -294,15 +261,14 @@ int psy_tracer_create(const psy_scene* scene, uint64_t max_batch, psy_tracer** o
     if (e == hipSuccess) e = hipEventCreate(&t->ev1);
     if (e == hipSuccess) e = hipMalloc((void**)&t->counters, 8 * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void**)&t->work, 64);
-    if (e == hipSuccess) {
-        int dev = 0, cus = 0;
-        e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess && cus > 0) t->cu_count = (uint32_t)cus;
-    }
     if (e != hipSuccess) {
         delete t;
         return fail(PSY_E_DEVICE, std::string("tracer setup: ") + hipGetErrorString(e));
+    }
+    int rc = device_cu_count(t->cu_count);
+    if (rc) {
+        delete t;
+        return rc;
     }
     *out = t;
     return PSY_OK;
@@ -342,13 +308,8 @@ int psy_tracer_get_tuning(const psy_tracer* tr, psy_trace_tuning* t) {
 
 int psy_tracer_set_tuning(psy_tracer* tr, const psy_trace_tuning* t) {
     if (!tr || !t) return fail(PSY_E_INVALID, "NULL argument");
-    if (t->kernel != PSY_KERNEL_PERSISTENT && t->kernel != PSY_KERNEL_SIMPLE)
-        return fail(PSY_E_INVALID, "unknown kernel");
-    if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(PSY_E_RANGE, "waves_per_cu must be in [1, 32]");
-    if (t->refill_min < 1 || t->refill_min > 64) return fail(PSY_E_RANGE, "refill_min must be in [1, 64]");
-    if (t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 24)
-        return fail(PSY_E_RANGE, "stack_lds must be 8, 16 or 24");
-    if (t->simd_waves != 4 && t->simd_waves != 8) return fail(PSY_E_RANGE, "simd_waves must be 4 or 8");
+    int rc = check_tuning(t);
+    if (rc) return rc;
     tr->tuning = *t;
     return PSY_OK;
 }

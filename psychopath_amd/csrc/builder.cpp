@@ -164,6 +164,13 @@ class Builder {
         return node;
     }
 
+    uint32_t split_sah(uint32_t* keys, uint32_t n) {
+        uint32_t split;
+        uint8_t axis;
+        sah_split(keys, n, split, axis);
+        return split;
+    }
+
    private:
     const BuildInput& in_;
     uint32_t per_leaf_;
@@ -460,6 +467,11 @@ HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads) {
         out.root = c.emit(root.get(), 0);
     }
     return out;
+}
+
+uint32_t sah_split_keys(const BuildInput& in, uint32_t* keys, uint32_t n) {
+    Builder b(in, 1, 1);
+    return b.split_sah(keys, n);
 }
 
 void dump_bvh4(const HostBvh4& bvh, std::vector<uint32_t>& out) {

@@ -41,6 +41,11 @@ struct HostBvh4 {
 // used for independent subtrees (0 = hardware concurrency).
 HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads);
 
+// objects_split::sah_split (src/accel/objects_split.rs:153-238) over the
+// objects `keys[0..n)` of `in` (n >= 2): partitions `keys` in place and
+// returns the split position.  The light tree splits with it too.
+uint32_t sah_split_keys(const BuildInput& in, uint32_t* keys, uint32_t n);
+
 // Canonical pre-order dump (shared with the parity tests):
 //   internal: 0, child_count, traversal_code, bounds_len, bounds_len*24 f32 bits, children...
 //   leaf:     1, obj_start, obj_end

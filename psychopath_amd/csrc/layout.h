@@ -64,7 +64,7 @@ struct GpuMesh {
     uint32_t root;
     uint32_t time_samples;
     uint32_t tri_stride;   // bytes
-    uint32_t pad0;
+    uint32_t rec_map;      // start of this mesh's triangle -> record map
     uint64_t tri_offset;   // bytes into the triangle buffer
     uint64_t pad1;
 };
@@ -80,13 +80,68 @@ struct GpuInstance {
     uint32_t xf_start;
     uint32_t xf_count;
     uint32_t inv_static;
-    uint32_t pad[2];
+    uint32_t kind;     // PSY_OBJECT_*; `mesh` indexes the light arrays for lights
+    uint32_t pad;
 };
 
 // Transform: glam Affine3A columns c0, c1, c2 and translation t, stored as
 // three rows (x: c0.x c1.x c2.x t.x, ...) so each row is one float4.
 struct GpuXform {
     float r[3][4];
+};
+
+// ---- renderer data ---------------------------------------------------------
+
+// Color (src/color.rs:46-62): kind PSY_COLOR_*, then X,Y,Z or
+// (temperature, factor, unused).
+struct GpuColor {
+    uint32_t kind;
+    float a, b, c;
+};
+
+struct GpuRectLight {
+    float w, h;
+    float pad[2];
+    GpuColor color;
+};
+struct GpuSphereLight {
+    float radius;
+    float pad[3];
+    GpuColor color;
+};
+struct GpuDistantLight {
+    float radius;
+    float dir[3];
+    GpuColor color;
+};
+
+// A surface closure as shaded (SimpleSurfaceShader::shade, src/shading/mod.rs:
+// 42-61): kind PSY_SHADER_*.
+struct GpuClosure {
+    uint32_t kind;
+    float roughness, fresnel;
+    uint32_t pad;
+    GpuColor color;
+};
+
+// Light tree node (src/accel/light_tree.rs:17-33) after the collapse to
+// arity 8: children are contiguous nodes [child_first, child_first +
+// child_count); a leaf (child_count 0) names an entry of the light list.
+// Bounds are time samples [bounds_start, bounds_start + bounds_len) of the
+// light-bounds array.
+struct GpuLightNode {
+    uint32_t child_first;
+    uint32_t child_count;
+    uint32_t light;
+    uint32_t bounds_start;
+    uint32_t bounds_len;
+    float energy;
+    uint32_t pad[2];
+};
+
+// A box of the light tree: min xyz, max xyz.
+struct GpuBox {
+    float lo[3], hi[3];
 };
 
 }  // namespace psy

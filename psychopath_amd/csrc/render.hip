@@ -1,0 +1,1150 @@
+// Wavefront path tracer for gfx950 (include/psy_render.h).
+//
+// The reference renders buckets of pixels per CPU thread, advancing every
+// light path of a bucket one event at a time (LightPath::next,
+// src/renderer.rs:449-692) between calls to Tracer::trace.  Here the paths of
+// a whole wave (up to millions) live in HBM as structure-of-arrays and each
+// event is one kernel over the wave:
+//
+//   camera    Camera::generate_ray + LightPath::new for every path slot
+//   trace     closest-hit queue and shadow queue through the persistent
+//             traversal kernel (counts read on the device)
+//   shadow    unoccluded shadow rays add their pending light
+//   sort      closest hits binned by closure kind (miss, light, Lambert,
+//             GGX, emit) so the shading waves run one closure's code
+//   shade     emission / background, next-event estimation through the light
+//             tree, BSDF sampling; appends the next shadow and bounce rays
+//   film      path colour -> XYZ, added per pixel in sample order
+//
+// A path's shadow ray and its next bounce ray are traced in the same step
+// (the reference traces them one after the other); the colour additions keep
+// the reference's order per path, so every path computes what the
+// reference's does.  The steps of one wave are captured once into a hipGraph
+// and replayed per wave; a device-side wave counter selects each replay's
+// plan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/psy_render.h"
+#include "api_internal.h"
+#include "render_device.h"
+
+using namespace psy;
+using namespace psy::api;
+using namespace psy::rd;
+
+namespace {
+
+constexpr int KBLOCK = 256;
+constexpr uint32_t MAX_SEGS = 64;  // requests interleaved in one wave
+constexpr uint32_t NUM_BINS = 5;   // miss, light, Lambert, GGX, emit
+
+// One request's render parameters (device copy).
+struct ReqParams {
+    uint32_t width, height, spp, max_bounces, seed;
+    float cmpx, cmpy, x_extent, y_extent, tfov, aperture, focus;
+    uint32_t cam_xf_count;
+    uint32_t pad;
+    const GpuXform* cam_xf;
+    float* film;
+};
+
+// A run of consecutive path slots of a wave taken by one request: slots
+// [slot0, slot0 + count) carry the request's global paths g0, g0 + 1, ...
+// (path g = pixel * spp + sample).
+struct WaveSeg {
+    uint32_t req;
+    uint32_t slot0;
+    uint32_t count;
+    uint32_t pad;
+    uint64_t g0;
+};
+struct WavePlan {
+    uint32_t nseg;
+    uint32_t total;
+    uint32_t pad[2];
+    WaveSeg seg[MAX_SEGS];
+};
+
+// Scene data the shading kernels read.
+struct ShadeScene {
+    const GpuInstance* instances;
+    const uint32_t* inst_pos;
+    const GpuMesh* meshes;
+    const uint8_t* tris;
+    const uint32_t* tri_rec;
+    const GpuXform* xforms;
+    const GpuXform* inv_static;
+    const GpuRectLight* rects;
+    const GpuSphereLight* spheres;
+    const GpuLightNode* lnodes;
+    const GpuBox* lboxes;
+    const uint32_t* llist;
+    uint32_t has_ltree;
+    uint32_t shader_count;
+    const GpuClosure* shaders;
+    const GpuDistantLight* distant;
+    uint32_t distant_count;
+    uint32_t pad;
+    GpuColor background;
+    const HaltonDim* hdims;
+    const uint16_t* hperms;
+};
+
+// Path state, one slot per path of the wave.
+struct Paths {
+    uint32_t* req;
+    uint32_t* pix;
+    uint32_t* sample;
+    uint32_t* seed;
+    uint32_t* bounces;
+    float* wl;
+    float* time;
+    float* csp;  // closure_sample_pdf
+    float4* color;
+    float4* atten;
+    float4* pending;
+};
+
+// A ray queue: the path slot of each ray and the rays as SoA.
+struct Queue {
+    uint32_t* path;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
+    uint32_t* count;
+};
+struct Hits {
+    uint8_t* kind;
+    float *t, *b0, *b1, *b2;
+    uint32_t *prim, *inst;
+};
+
+__device__ __forceinline__ const WaveSeg& find_seg(const WavePlan& p, uint32_t i) {
+    uint32_t s = 0;
+    while (s + 1 < p.nseg && i >= p.seg[s + 1].slot0) s++;
+    return p.seg[s];
+}
+
+// Wave-aggregated append: one atomic per wave64 for the lanes with `want`.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0xffffffffu;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ void queue_put(const Queue& q, uint32_t at, uint32_t path, V3 o, V3 d, float time,
+                                          float max_t) {
+    q.path[at] = path;
+    q.ox[at] = o.x; q.oy[at] = o.y; q.oz[at] = o.z;
+    q.dx[at] = d.x; q.dy[at] = d.y; q.dz[at] = d.z;
+    q.time[at] = time;
+    q.max_t[at] = max_t;
+}
+
+// ---- kernels -------------------------------------------------------------------
+
+__global__ void wave_begin_kernel(uint32_t* wave_idx, uint32_t value) { *wave_idx = value; }
+__global__ void wave_advance_kernel(uint32_t* wave_idx) { *wave_idx += 1u; }
+
+// Zeroes the closest and shadow output counters and the sort bins.
+__global__ void reset_counts_kernel(uint32_t* a, uint32_t* b, uint32_t* bins) {
+    const int i = threadIdx.x;
+    if (i == 0) *a = 0u;
+    if (i == 1) *b = 0u;
+    if (i < 2 * (int)NUM_BINS) bins[i] = 0u;
+}
+
+__global__ void zero_film_kernel(float* film, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        film[i] = 0.0f;
+}
+
+// Camera::generate_ray and LightPath::new for every slot (renderer.rs:233-
+// 289, camera.rs:74-102).
+__global__ __launch_bounds__(KBLOCK) void camera_kernel(const WavePlan* plans, const uint32_t* wave_idx,
+                                                        const ReqParams* reqs, ShadeScene sc, Paths P, Queue qa,
+                                                        uint32_t* shadow_count) {
+    const WavePlan& plan = plans[*wave_idx];
+    const uint32_t total = plan.total;
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0) {
+        *qa.count = total;
+        *shadow_count = 0u;
+    }
+    for (uint32_t i = i0; i < total; i += gridDim.x * blockDim.x) {
+        const WaveSeg& sg = find_seg(plan, i);
+        const ReqParams rp = reqs[sg.req];
+        const uint64_t g = sg.g0 + (i - sg.slot0);
+        const uint32_t pixel = (uint32_t)(g / rp.spp);
+        const uint32_t si = (uint32_t)(g % rp.spp);
+        const uint32_t x = pixel % rp.width, y = pixel / rp.width;
+
+        const S4 d0 = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, rp.seed);
+        const S4 d1 = sample_4d(sc.hdims, sc.hperms, si, 1, x, y, rp.seed);
+        const float filter_x = probit(d0.v[3], 2.0f / 6.0f) + 0.5f;
+        const float filter_y = probit(d1.v[0], 2.0f / 6.0f) + 0.5f;
+        const float samp_x = (filter_x + (float)x) * rp.cmpx;
+        const float samp_y = (filter_y + (float)y) * rp.cmpy;
+        const float img_x = (samp_x - 0.5f) * rp.x_extent;
+        const float img_y = (0.5f - samp_y) * rp.y_extent;
+        const float time = d0.v[0];
+        const float wl = map_0_1_to_wavelength(golden_ratio_sample(si, hash_u32((x << 16) ^ y, rp.seed)));
+
+        Xf xf;
+        {
+            uint32_t i1;
+            float a2;
+            if (!lerp_slice_pos(rp.cam_xf_count, time, i1, a2)) xf = load_xf(rp.cam_xf + i1);
+            else xf = xf_lerp(load_xf(rp.cam_xf + i1), load_xf(rp.cam_xf + i1 + 1), a2);
+        }
+        float lu, lv;
+        square_to_circle((d0.v[1] * 2.0f) - 1.0f, (d0.v[2] * 2.0f) - 1.0f, lu, lv);
+        const V3 orig = v3(rp.aperture * lu, rp.aperture * lv, 0.0f);
+        const V3 dir = normalize(v3((img_x * rp.tfov) - (orig.x / rp.focus), (img_y * rp.tfov) - (orig.y / rp.focus),
+                                    1.0f));
+
+        P.req[i] = sg.req;
+        P.pix[i] = pixel;
+        P.sample[i] = si;
+        P.seed[i] = rp.seed ^ 0x40d4682bu;
+        P.bounces[i] = 0u;
+        P.wl[i] = wl;
+        P.time[i] = time;
+        P.csp[i] = 1.0f;
+        P.color[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        P.pending[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        queue_put(qa, i, i, xf_point(xf, orig), xf_vector(xf, dir), time, __builtin_huge_valf());
+    }
+}
+
+// Shadow-ray results (renderer.rs:670-690): a miss adds the pending light.
+__global__ __launch_bounds__(KBLOCK) void shadow_kernel(Queue qs, const uint8_t* kind, Paths P) {
+    const uint32_t n = *qs.count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (kind[i] != PSY_HIT_MISS) continue;
+        const uint32_t p = qs.path[i];
+        const S4 c = s4_load(P.color, p) + s4_load(P.pending, p);
+        s4_store(P.color, p, c);
+    }
+}
+
+// Sort key of a closest hit: 0 miss, 1 light surface, 2 + closure kind.
+__device__ __forceinline__ uint32_t hit_key(const ShadeScene& sc, const Hits& h, uint32_t q) {
+    if (h.kind[q] != PSY_HIT_SURFACE) return 0u;
+    const GpuInstance inst = sc.instances[sc.inst_pos[h.inst[q]]];
+    if (inst.kind != PSY_OBJECT_MESH) return 1u;
+    return 2u + sc.shaders[inst.shader].kind;
+}
+
+__global__ __launch_bounds__(KBLOCK) void sort_count_kernel(ShadeScene sc, Hits h, const uint32_t* count,
+                                                            uint32_t* bins) {
+    __shared__ uint32_t local[NUM_BINS];
+    if (threadIdx.x < NUM_BINS) local[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t n = *count;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        atomicAdd(&local[hit_key(sc, h, q)], 1u);
+    __syncthreads();
+    if (threadIdx.x < NUM_BINS && local[threadIdx.x] != 0u) atomicAdd(&bins[threadIdx.x], local[threadIdx.x]);
+}
+
+// bins[0..5) hold the bin sizes, bins[5..10) the fill cursors.
+__global__ __launch_bounds__(KBLOCK) void sort_scatter_kernel(ShadeScene sc, Hits h, const uint32_t* count,
+                                                              uint32_t* bins, uint32_t* order) {
+    __shared__ uint32_t local[NUM_BINS], base[NUM_BINS];
+    const uint32_t n = *count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += stride) {
+        if (threadIdx.x < NUM_BINS) local[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t q = q0 + threadIdx.x;
+        uint32_t key = 0, rank = 0;
+        if (q < n) {
+            key = hit_key(sc, h, q);
+            rank = atomicAdd(&local[key], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < NUM_BINS) {
+            uint32_t start = 0;
+            for (uint32_t b = 0; b < threadIdx.x; b++) start += bins[b];
+            base[threadIdx.x] = local[threadIdx.x] ? start + atomicAdd(&bins[NUM_BINS + threadIdx.x], local[threadIdx.x])
+                                                   : 0u;
+        }
+        __syncthreads();
+        if (q < n) order[base[key] + rank] = q;
+        __syncthreads();
+    }
+}
+
+// Surface intersection data (SurfaceIntersectionData, src/surface/mod.rs:
+// 95-105) rebuilt from a hit record as the reference's mesh and light
+// intersection code computes it (triangle_mesh.rs:174-310,
+// rectangle_light.rs:288-330, sphere_light.rs:283-330).
+struct Isect {
+    V3 pos, nor, nor_g;
+    float pos_err;
+    float sample_pdf;
+    bool light;
+    GpuColor emit;
+    uint32_t shader;
+};
+
+__device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float t, float b0, float b1, float b2,
+                            uint32_t prim, V3 orig, V3 dir, float time) {
+    Isect is;
+    is.light = inst.kind != PSY_OBJECT_MESH;
+    is.shader = inst.shader;
+    is.sample_pdf = 0.0f;
+    if (!is.light) {
+        const GpuMesh mesh = sc.meshes[inst.mesh];
+        const uint32_t rec = sc.tri_rec[mesh.rec_map + prim];
+        const float* f = reinterpret_cast<const float*>(sc.tris + mesh.tri_offset + (uint64_t)rec * mesh.tri_stride);
+        V3 v0, v1, v2;
+        uint32_t i1;
+        float a2;
+        if (!lerp_slice_pos(mesh.time_samples, time, i1, a2)) {
+            const float* s0 = f + i1 * 9;
+            v0 = v3(s0[0], s0[1], s0[2]);
+            v1 = v3(s0[3], s0[4], s0[5]);
+            v2 = v3(s0[6], s0[7], s0[8]);
+        } else {
+            const float* s0 = f + i1 * 9;
+            const float* s1 = s0 + 9;
+            v0 = lerp3(v3(s0[0], s0[1], s0[2]), v3(s1[0], s1[1], s1[2]), a2);
+            v1 = lerp3(v3(s0[3], s0[4], s0[5]), v3(s1[3], s1[4], s1[5]), a2);
+            v2 = lerp3(v3(s0[6], s0[7], s0[8]), v3(s1[6], s1[7], s1[8]), a2);
+        }
+        if (inst.xf_count > 0) {
+            const Xf space = inst.xf_count == 1 ? load_xf(sc.inv_static + inst.inv_static)
+                                                : xf_inverse(instance_xf(sc.xforms, inst, time));
+            v0 = xf_point(space, v0);
+            v1 = xf_point(space, v1);
+            v2 = xf_point(space, v2);
+        }
+        surface_point(v0, v1, v2, b0, b1, b2, is.pos, is.pos_err);
+        is.nor_g = cross(sub(v0, v1), sub(v0, v2));
+        is.nor = is.nor_g;
+        return is;
+    }
+    const Xf xform = instance_xf(sc.xforms, inst, time);
+    if (inst.kind == PSY_OBJECT_RECT_LIGHT) {
+        const GpuRectLight L = sc.rects[inst.mesh];
+        V3 p[4];
+        rect_corners(L, xf_inverse(xform), p);
+        const V3 t0 = prim == 0 ? p[0] : p[2], t1 = prim == 0 ? p[1] : p[3], t2 = prim == 0 ? p[2] : p[0];
+        surface_point(t0, t1, t2, b0, b1, b2, is.pos, is.pos_err);
+        is.nor = cross(sub(t0, t1), sub(t0, t2));
+        is.nor_g = is.nor;
+        is.sample_pdf = rect_sample_pdf(L, xform, orig, dir, is.pos);
+        is.emit = color_scaled(L.color, (float)(1.0 / ((double)L.w * (double)L.h)));
+    } else {
+        const GpuSphereLight L = sc.spheres[inst.mesh];
+        const V3 lo = xf_point(xform, orig);
+        const V3 ld = xf_vector(xform, dir);
+        const Xf inv = xf_inverse(xform);
+        const V3 unit_pos = normalize(add(lo, mul(ld, t)));
+        // `unit_pos * radius * inv_xform` is a Vector times a Transform in the
+        // reference: no translation.
+        is.pos = xf_vector(inv, mul(unit_pos, L.radius));
+        is.pos_err = 0.001f;
+        is.nor = xf_normal(inv, unit_pos);
+        is.nor_g = is.nor;
+        is.sample_pdf = sphere_sample_pdf(L, xform, orig);
+        is.emit = color_scaled(L.color, (float)(1.0 / (((4.0 * PI_D) * (double)L.radius) * (double)L.radius)));
+    }
+    return is;
+}
+
+// LightTree::select (src/accel/light_tree.rs:135-201)
+__device__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, V3 nor_g, const GpuClosure& clo,
+                                  float time, float n, uint32_t& index, float& prob) {
+    if (!sc.has_ltree) return false;
+    uint32_t node = 0;
+    float tot_prob = 1.0f;
+    while (sc.lnodes[node].child_count > 0) {
+        const GpuLightNode nd = sc.lnodes[node];
+        const uint32_t cnt = nd.child_count;
+        float ps[8];
+        float total = 0.0f;
+        for (uint32_t i = 0; i < 8; i++) ps[i] = 0.0f;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const GpuLightNode ch = sc.lnodes[nd.child_first + i];
+            // node_prob: energy / bounding-sphere area * closure estimate
+            Box b;
+            {
+                const GpuBox* bs = sc.lboxes + ch.bounds_start;
+                uint32_t i1;
+                float a2;
+                auto ld = [](const GpuBox& g) { return Box{v3(g.lo[0], g.lo[1], g.lo[2]), v3(g.hi[0], g.hi[1], g.hi[2])}; };
+                if (!lerp_slice_pos(ch.bounds_len, time, i1, a2)) b = ld(bs[i1]);
+                else b = box_lerp(ld(bs[i1]), ld(bs[i1 + 1]), a2);
+            }
+            const V3 d = sub(box_center(b), pos);
+            const float r2 = length2(sub(b.hi, b.lo)) * 0.25f;
+            const float inv_surface_area = 1.0f / r2;
+            const float approx = closure_estimate_sphere_light(clo, inc, d, r2, nor, nor_g);
+            const float p = (ch.energy * inv_surface_area) * approx;
+            ps[i] = p;
+            total += p;
+        }
+        if (total <= 0.0f) {
+            const float p = 1.0f / (float)cnt;
+            for (uint32_t i = 0; i < 8; i++) ps[i] = p;
+        } else {
+            for (uint32_t i = 0; i < 8; i++) ps[i] /= total;
+        }
+        float base = 0.0f;
+        for (uint32_t i = 0; i < 8; i++) {
+            const float p = ps[i];
+            if ((n <= base + p) || (i == cnt - 1)) {
+                tot_prob *= p;
+                node = nd.child_first + i;
+                n = (n - base) / p;
+                break;
+            }
+            base += p;
+        }
+    }
+    index = sc.lnodes[node].light;
+    prob = tot_prob;
+    return true;
+}
+
+// Scene::sample_lights (src/scene/mod.rs:28-101) and Assembly::sample_lights
+// (src/scene/assembly.rs:46-141) for a single-level assembly.
+struct LightPick {
+    bool found;
+    bool distant;
+    S4 color;
+    V3 dir;  // distant
+    V3 point, normal;
+    float point_err;
+    float pdf, selection_pdf;
+};
+
+__device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float v, float wl, float time,
+                                   const Isect& is, V3 incoming, const GpuClosure& clo) {
+    LightPick out;
+    out.found = false;
+    float wl_sum = 0.0f;
+    for (uint32_t i = 0; i < sc.distant_count; i++) wl_sum = wl_sum + ((0.0f + color_energy(sc.distant[i].color)) / 1.0f);
+    const float wl_energy = wl_sum <= 0.0f ? 0.0f : 1.0f;
+    const float ll_energy = (sc.has_ltree ? sc.lnodes[0].energy : 0.0f) <= 0.0f ? 0.0f : 1.0f;
+    const float tot = wl_energy + ll_energy;
+    if (tot <= 0.0f) return out;
+    const float wl_prob = wl_energy / tot;
+    if (n < wl_prob) {
+        const float nn = (n / wl_prob) * wl_sum;  // weighted_choice over approximate energies
+        float x = 0.0f;
+        uint32_t pick = sc.distant_count - 1;
+        float w = 0.0f;
+        for (uint32_t i = 0; i < sc.distant_count; i++) {
+            w = (0.0f + color_energy(sc.distant[i].color)) / 1.0f;
+            x += w;
+            if (x > nn) {
+                pick = i;
+                break;
+            }
+        }
+        w = (0.0f + color_energy(sc.distant[pick].color)) / 1.0f;
+        const float p = w / wl_sum;
+        distant_sample(sc.distant[pick], u, v, wl, out.color, out.dir, out.pdf);
+        out.distant = true;
+        out.selection_pdf = p * wl_prob;
+        out.found = true;
+        return out;
+    }
+    const float nn = (n - wl_prob) / (1.0f - wl_prob);
+    // With an empty transform stack the reference still applies the identity.
+    const Xf ident = xf_identity();
+    uint32_t li;
+    float sel_pdf;
+    if (!light_tree_select(sc, xf_vector(ident, incoming), xf_point(ident, is.pos), xf_normal(ident, is.nor),
+                           xf_normal(ident, is.nor_g), clo, time, nn, li, sel_pdf))
+        return out;
+    const GpuInstance inst = sc.instances[sc.llist[li]];
+    const Xf xform = instance_xf(sc.xforms, inst, time);
+    const LightSample s = inst.kind == PSY_OBJECT_RECT_LIGHT
+                              ? rect_sample_from_point(sc.rects[inst.mesh], xform, is.pos, u, v, wl)
+                              : sphere_sample_from_point(sc.spheres[inst.mesh], xform, is.pos, u, v, wl);
+    out.found = true;
+    out.distant = false;
+    out.color = s.color;
+    out.point = s.point;
+    out.normal = s.normal;
+    out.point_err = s.point_err;
+    out.pdf = s.pdf;
+    out.selection_pdf = sel_pdf * (1.0f - wl_prob);
+    return out;
+}
+
+// LightPath::next for camera and bounce rays (renderer.rs:459-667).
+__global__ __launch_bounds__(KBLOCK) void shade_kernel(ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa,
+                                                       Hits h, const uint32_t* order, Queue qa_out, Queue qs_out) {
+    const uint32_t n = *qa.count;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // Every lane of a wave runs the loop the same number of times so the
+    // wave-wide appends see all lanes.
+    const uint32_t n_round = (n + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        bool want_shadow = false, want_bounce = false;
+        V3 so{}, sd{}, bo{}, bd{};
+        float smax = 1.0f, ptime = 0.0f;
+        uint32_t p = 0;
+        if (i < n) {
+            const uint32_t q = order ? order[i] : i;
+            p = qa.path[q];
+            const ReqParams rp = reqs[P.req[p]];
+            const float wl = P.wl[p];
+            const float time = P.time[p];
+            ptime = time;
+            S4 la = s4_load(P.atten, p);
+            S4 col = s4_load(P.color, p);
+            float csp = P.csp[p];
+            uint32_t seed = P.seed[p];
+            uint32_t bounces = P.bounces[p];
+            const V3 orig = v3(qa.ox[q], qa.oy[q], qa.oz[q]);
+            const V3 dir = v3(qa.dx[q], qa.dy[q], qa.dz[q]);
+            if (h.kind[q] != PSY_HIT_SURFACE) {
+                // Background (renderer.rs:656-666)
+                const S4 bg = color_spectral(sc.background, wl);
+                col = col + ((bg * la) / csp);
+                s4_store(P.color, p, col);
+            } else {
+                const GpuInstance inst = sc.instances[sc.inst_pos[h.inst[q]]];
+                const Isect is = make_isect(sc, inst, h.t[q], h.b0[q], h.b1[q], h.b2[q], h.prim[q], orig, dir, time);
+                GpuClosure clo;
+                if (is.light) {
+                    clo.kind = PSY_SHADER_EMIT;
+                    clo.color = is.emit;
+                    clo.roughness = 0.0f;
+                    clo.fresnel = 0.0f;
+                } else {
+                    clo = sc.shaders[is.shader];
+                }
+                if (clo.kind == PSY_SHADER_EMIT) {
+                    const S4 c = color_spectral(clo.color, wl);
+                    if (bounces == 0u) {
+                        col = col + c;
+                    } else {
+                        const float mis_pdf = power_heuristic(csp, is.sample_pdf);
+                        col = col + ((c * la) / mis_pdf);
+                    }
+                    s4_store(P.color, p, col);
+                } else {
+                    la = la / csp;
+                    const uint32_t x = P.pix[p] % rp.width, y = P.pix[p] / rp.width;
+                    const uint32_t si = P.sample[p];
+                    // Light ray
+                    seed += 1u;
+                    const S4 ls = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, seed);
+                    const LightPick li = sample_lights(sc, ls.v[0], ls.v[1], ls.v[2], wl, time, is, dir, clo);
+                    if (li.found && li.pdf > 0.0f && li.selection_pdf > 0.0f) {
+                        S4 att;
+                        float closure_pdf;
+                        if (li.distant) {
+                            closure_evaluate(clo, dir, li.dir, is.nor, is.nor_g, wl, att, closure_pdf);
+                            so = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), li.dir);
+                            sd = li.dir;
+                            smax = __builtin_huge_valf();
+                        } else {
+                            const V3 ldir = sub(li.point, is.pos);
+                            closure_evaluate(clo, dir, ldir, is.nor, is.nor_g, wl, att, closure_pdf);
+                            const V3 offset_pos = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ldir);
+                            const V3 offset_end = robust_ray_origin(li.point, li.point_err, normalize(li.normal),
+                                                                    neg3(ldir));
+                            so = offset_pos;
+                            sd = sub(offset_end, offset_pos);
+                            smax = 1.0f;
+                        }
+                        if (max_element4(att) > 0.0f) {
+                            const float light_mis_pdf = power_heuristic(li.pdf, closure_pdf);
+                            const S4 pend = ((li.color * att) * la) / (light_mis_pdf * li.selection_pdf);
+                            s4_store(P.pending, p, pend);
+                            want_shadow = true;
+                        }
+                    }
+                    // Bounce ray
+                    if (bounces < rp.max_bounces) {
+                        bounces += 1u;
+                        seed += 1u;
+                        const S4 bs = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, seed);
+                        V3 ndir;
+                        S4 filter;
+                        float pdf;
+                        closure_sample(clo, dir, is.nor, is.nor_g, bs.v[0], bs.v[1], wl, ndir, filter, pdf);
+                        if ((pdf > 0.0f) && (max_element4(filter) > 0.0f)) {
+                            csp = pdf;
+                            bo = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ndir);
+                            bd = ndir;
+                            la = la * filter;
+                            want_bounce = true;
+                        }
+                    }
+                    s4_store(P.atten, p, la);
+                    P.csp[p] = csp;
+                    P.seed[p] = seed;
+                    P.bounces[p] = bounces;
+                }
+            }
+        }
+        const uint32_t at_s = wave_append(qs_out.count, want_shadow);
+        if (want_shadow) queue_put(qs_out, at_s, p, so, sd, ptime, smax);
+        const uint32_t at_b = wave_append(qa_out.count, want_bounce);
+        if (want_bounce) queue_put(qa_out, at_b, p, bo, bd, ptime, __builtin_huge_valf());
+    }
+}
+
+// Film accumulation (renderer.rs:299-310): the first slot of each pixel's
+// run in the wave adds the run's samples in sample order.
+__global__ __launch_bounds__(KBLOCK) void film_kernel(const WavePlan* plans, const uint32_t* wave_idx,
+                                                      const ReqParams* reqs, Paths P) {
+    const WavePlan& plan = plans[*wave_idx];
+    const uint32_t total = plan.total;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const WaveSeg& sg = find_seg(plan, i);
+        const ReqParams rp = reqs[sg.req];
+        const uint64_t g = sg.g0 + (i - sg.slot0);
+        if (i != sg.slot0 && g % rp.spp != 0) continue;
+        const uint64_t pixel = g / rp.spp;
+        const uint32_t end = sg.slot0 + sg.count;
+        float* px = rp.film + 3 * pixel;
+        float fx = px[0], fy = px[1], fz = px[2];
+        const float fspp = (float)rp.spp;
+        for (uint32_t j = i; j < end && (sg.g0 + (j - sg.slot0)) / rp.spp == pixel; j++) {
+            const V3 c = xyz_from_spectral(s4_load(P.color, j), P.wl[j]);
+            fx += c.x / fspp;
+            fy += c.y / fspp;
+            fz += c.z / fspp;
+        }
+        px[0] = fx;
+        px[1] = fy;
+        px[2] = fz;
+    }
+}
+
+// ---- host side -------------------------------------------------------------------
+
+// Faure permutations and the per-dimension tables of the Halton sampler
+// (sub_crates/halton/build.rs:101-291), computed once per renderer.
+void halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perms) {
+    const int NUM = 128;
+    std::vector<uint32_t> primes;
+    for (uint32_t c = 2; (int)primes.size() < NUM; c++) {
+        bool prime = true;
+        for (uint32_t p : primes)
+            if (c % p == 0) {
+                prime = false;
+                break;
+            }
+        if (prime) primes.push_back(c);
+    }
+    // faure[b]: Faure's permutation of the digits of base b.
+    std::vector<std::vector<uint32_t>> faure(primes.back() + 1);
+    faure[1] = {0};
+    faure[2] = {0, 1};
+    for (uint32_t b = 3; b <= primes.back(); b++) {
+        std::vector<uint32_t>& f = faure[b];
+        if (b % 2 == 0) {
+            const std::vector<uint32_t>& h = faure[b / 2];
+            for (uint32_t x : h) f.push_back(2 * x);
+            for (uint32_t x : h) f.push_back(2 * x + 1);
+        } else {
+            const uint32_t c = (b - 1) / 2;
+            for (uint32_t x : faure[b - 1]) f.push_back(x >= c ? x + 1 : x);
+            f.insert(f.begin() + c, c);
+        }
+    }
+    dims.assign(NUM, HaltonDim{2, 2, 0, 0});
+    perms.clear();
+    for (int i = 1; i < NUM; i++) {
+        const uint64_t base = primes[i];
+        uint64_t digits = 1, pow_base = base;
+        while (pow_base * base <= 500) {
+            pow_base *= base;
+            digits++;
+        }
+        uint64_t max_power = pow_base;
+        while (max_power * pow_base < (1ull << 32)) max_power *= pow_base;
+        HaltonDim d{(uint32_t)base, (uint32_t)pow_base, (uint32_t)max_power, (uint32_t)perms.size()};
+        for (uint64_t j = 0; j < pow_base; j++) {
+            // Digits of j, least significant first, permuted and reversed.
+            uint64_t idx = j, r = 0;
+            for (uint64_t k = 0; k < digits; k++) {
+                r = r * base + faure[base][idx % base];
+                idx /= base;
+            }
+            perms.push_back((uint16_t)r);
+        }
+        dims[i] = d;
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct psy_renderer {
+    psy_scene* scene = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t capacity = 0;
+    uint32_t cu_count = 256;
+    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
+    bool use_graphs = true;
+    bool sort_materials = true;
+
+    std::vector<DevBuf*> bufs;
+    ShadeScene sc{};
+    Paths P{};
+    Queue qa[2]{}, qs[2]{};
+    Hits ha{}, hs{};
+    uint32_t* order = nullptr;
+    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins
+    uint32_t* wave_idx = nullptr;
+    unsigned int* work = nullptr;
+    WavePlan* plans = nullptr;   // device plan array
+    uint32_t plan_cap = 0;
+    ReqParams* reqs = nullptr;
+    uint32_t req_cap = 0;
+    GpuXform* cam = nullptr;
+    uint32_t cam_cap = 0;
+
+    // Captured wave graphs, keyed by the number of bounce steps.
+    std::map<uint32_t, hipGraphExec_t> graphs;
+
+    ~psy_renderer() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (DevBuf* b : bufs) delete b;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T** out, uint64_t count) {
+        DevBuf* b = new (std::nothrow) DevBuf();
+        if (!b) return fail(PSY_E_NOMEM, "out of host memory");
+        hipError_t e = hipMalloc(&b->p, std::max<uint64_t>(1, count) * sizeof(T));
+        if (e != hipSuccess) {
+            delete b;
+            return fail(PSY_E_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
+        }
+        bufs.push_back(b);
+        *out = static_cast<T*>(b->p);
+        return PSY_OK;
+    }
+    void release(void* p) {
+        for (size_t i = 0; i < bufs.size(); i++)
+            if (bufs[i]->p == p) {
+                delete bufs[i];
+                bufs.erase(bufs.begin() + i);
+                return;
+            }
+    }
+
+    int alloc_queue(Queue& q) {
+        int r = alloc(&q.path, capacity);
+        float** f[8] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.time, &q.max_t};
+        for (int k = 0; k < 8 && !r; k++) r = alloc(f[k], capacity);
+        return r;
+    }
+};
+
+namespace {
+
+uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + KBLOCK - 1) / KBLOCK, 8192)); }
+
+TraceArgs queue_args(const psy_renderer* r, const Queue& q, const Hits& h, bool occlusion) {
+    TraceArgs a = scene_args(r->scene);
+    a.ox = q.ox; a.oy = q.oy; a.oz = q.oz;
+    a.dx = q.dx; a.dy = q.dy; a.dz = q.dz;
+    a.time = q.time; a.max_t = q.max_t;
+    a.flags = nullptr;
+    a.all_occlusion = occlusion ? 1u : 0u;
+    a.n = r->capacity;
+    a.n_dev = q.count;
+    a.kind = h.kind;
+    a.t = h.t; a.b0 = h.b0; a.b1 = h.b1; a.b2 = h.b2;
+    a.prim_id = h.prim; a.inst_id = h.inst;
+    return a;
+}
+
+// Enqueues the steps of one wave: camera, `steps` closest/shadow rounds, the
+// last shadow round and the film.  The wave's plan is plans[*wave_idx].
+int enqueue_wave(psy_renderer* r, uint32_t steps) {
+    const hipStream_t s = r->stream;
+    const uint32_t grid = grid_for(r->capacity);
+    Queue qa_in = r->qa[0], qa_out = r->qa[1], qs_in = r->qs[0], qs_out = r->qs[1];
+    hipLaunchKernelGGL(camera_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->sc, r->P,
+                       qa_in, qs_in.count);
+    PSY_HIP(hipGetLastError());
+    psy_trace_tuning tune = r->tuning;
+    tune.kernel = PSY_KERNEL_PERSISTENT;
+    for (uint32_t step = 0; step <= steps; step++) {
+        const bool last = step == steps;
+        if (!last) {
+            hipError_t e = launch_tuned(r->scene, tune, r->cu_count, r->work, queue_args(r, qa_in, r->ha, false), false, s);
+            if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+        }
+        hipError_t e = launch_tuned(r->scene, tune, r->cu_count, r->work + 16, queue_args(r, qs_in, r->hs, true), false, s);
+        if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+        hipLaunchKernelGGL(shadow_kernel, dim3(grid), dim3(KBLOCK), 0, s, qs_in, r->hs.kind, r->P);
+        PSY_HIP(hipGetLastError());
+        if (last) break;
+        hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, qa_out.count, qs_out.count, r->counts + 4);
+        PSY_HIP(hipGetLastError());
+        const uint32_t* order = nullptr;
+        if (r->sort_materials) {
+            hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+                               r->counts + 4);
+            PSY_HIP(hipGetLastError());
+            hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+                               r->counts + 4, r->order);
+            PSY_HIP(hipGetLastError());
+            order = r->order;
+        }
+        hipLaunchKernelGGL(shade_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->reqs, r->P, qa_in, r->ha, order,
+                           qa_out, qs_out);
+        PSY_HIP(hipGetLastError());
+        std::swap(qa_in, qa_out);
+        std::swap(qs_in, qs_out);
+    }
+    hipLaunchKernelGGL(film_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->P);
+    PSY_HIP(hipGetLastError());
+    hipLaunchKernelGGL(wave_advance_kernel, dim3(1), dim3(1), 0, s, r->wave_idx);
+    PSY_HIP(hipGetLastError());
+    return PSY_OK;
+}
+
+int run_wave(psy_renderer* r, uint32_t steps) {
+    if (!r->use_graphs) return enqueue_wave(r, steps);
+    auto it = r->graphs.find(steps);
+    if (it == r->graphs.end()) {
+        hipGraph_t g = nullptr;
+        PSY_HIP(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_wave(r, steps);
+        hipError_t e = hipStreamEndCapture(r->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = r->graphs.emplace(steps, ex).first;
+    }
+    PSY_HIP(hipGraphLaunch(it->second, r->stream));
+    return PSY_OK;
+}
+
+int color_check(const psy_color& c) {
+    if (c.kind > PSY_COLOR_TEMPERATURE || !std::isfinite(c.a) || !std::isfinite(c.b) || !std::isfinite(c.c))
+        return fail(PSY_E_INVALID, "bad colour");
+    return PSY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, uint64_t max_paths,
+                        psy_renderer** out) {
+    if (!scene_c || !world || !out) return fail(PSY_E_INVALID, "NULL argument");
+    *out = nullptr;
+    psy_scene* scene = const_cast<psy_scene*>(scene_c);
+    if (max_paths == 0) max_paths = 1ull << 22;
+    if (max_paths > (1ull << 30)) return fail(PSY_E_RANGE, "max_paths must be at most 2^30");
+    if (world->shader_count > 0 && !world->shaders) return fail(PSY_E_INVALID, "shaders is NULL");
+    if (world->distant_light_count > 0 && !world->distant_lights) return fail(PSY_E_INVALID, "distant_lights is NULL");
+    for (const GpuInstance& gi : scene->host.instances)
+        if (gi.kind == PSY_OBJECT_MESH && gi.shader >= world->shader_count)
+            return fail(PSY_E_INVALID, "instance " + std::to_string(gi.id) + " has no valid shader");
+    std::vector<GpuClosure> shaders;
+    for (uint32_t i = 0; i < world->shader_count; i++) {
+        const psy_shader_desc& d = world->shaders[i];
+        if (d.kind > PSY_SHADER_EMIT) return fail(PSY_E_INVALID, "unknown shader kind");
+        int rc = color_check(d.color);
+        if (rc) return rc;
+        shaders.push_back(GpuClosure{d.kind, d.roughness, d.fresnel, 0, GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
+    }
+    std::vector<GpuDistantLight> distant;
+    for (uint32_t i = 0; i < world->distant_light_count; i++) {
+        const psy_distant_light_desc& d = world->distant_lights[i];
+        int rc = color_check(d.color);
+        if (rc) return rc;
+        distant.push_back(GpuDistantLight{d.radius, {d.direction[0], d.direction[1], d.direction[2]},
+                                          GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
+    }
+    int rc = color_check(world->background);
+    if (rc) return rc;
+    if (scene->host.max_stack > (uint32_t)TRACE_STACK_CAPACITY)
+        return fail(PSY_E_RANGE, "scene BVHs are deeper than the traversal stack holds");
+    if (scene->host.has_normals) return fail(PSY_E_INVALID, "the renderer shades with geometric normals: meshes with normals are not supported");
+    rc = ensure_uploaded(scene);
+    if (rc) return rc;
+
+    psy_renderer* r = new (std::nothrow) psy_renderer();
+    if (!r) return fail(PSY_E_NOMEM, "out of host memory");
+    r->scene = scene;
+    r->capacity = max_paths;
+    auto bail = [&](int code) {
+        delete r;
+        return code;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&r->ev1);
+    if (e != hipSuccess) return bail(fail(PSY_E_DEVICE, std::string("renderer setup: ") + hipGetErrorString(e)));
+    rc = device_cu_count(r->cu_count);
+    if (rc) return bail(rc);
+
+    const uint64_t C = max_paths;
+    Paths& P = r->P;
+    rc = r->alloc(&P.req, C);
+    if (!rc) rc = r->alloc(&P.pix, C);
+    if (!rc) rc = r->alloc(&P.sample, C);
+    if (!rc) rc = r->alloc(&P.seed, C);
+    if (!rc) rc = r->alloc(&P.bounces, C);
+    if (!rc) rc = r->alloc(&P.wl, C);
+    if (!rc) rc = r->alloc(&P.time, C);
+    if (!rc) rc = r->alloc(&P.csp, C);
+    if (!rc) rc = r->alloc(&P.color, C);
+    if (!rc) rc = r->alloc(&P.atten, C);
+    if (!rc) rc = r->alloc(&P.pending, C);
+    for (int k = 0; k < 2 && !rc; k++) rc = r->alloc_queue(r->qa[k]);
+    for (int k = 0; k < 2 && !rc; k++) rc = r->alloc_queue(r->qs[k]);
+    if (!rc) rc = r->alloc(&r->ha.kind, C);
+    float** hf[4] = {&r->ha.t, &r->ha.b0, &r->ha.b1, &r->ha.b2};
+    for (int k = 0; k < 4 && !rc; k++) rc = r->alloc(hf[k], C);
+    if (!rc) rc = r->alloc(&r->ha.prim, C);
+    if (!rc) rc = r->alloc(&r->ha.inst, C);
+    if (!rc) rc = r->alloc(&r->hs.kind, C);
+    float** sf[4] = {&r->hs.t, &r->hs.b0, &r->hs.b1, &r->hs.b2};
+    for (int k = 0; k < 4 && !rc; k++) rc = r->alloc(sf[k], C);
+    if (!rc) rc = r->alloc(&r->hs.prim, C);
+    if (!rc) rc = r->alloc(&r->hs.inst, C);
+    if (!rc) rc = r->alloc(&r->order, C);
+    if (!rc) rc = r->alloc(&r->counts, 16);
+    if (!rc) rc = r->alloc(&r->wave_idx, 1);
+    if (!rc) rc = r->alloc(&r->work, 64);
+    if (rc) return bail(rc);
+    r->qa[0].count = r->counts + 0;
+    r->qa[1].count = r->counts + 1;
+    r->qs[0].count = r->counts + 2;
+    r->qs[1].count = r->counts + 3;
+
+    // Scene data for shading.
+    ShadeScene& sc = r->sc;
+    sc.instances = scene->instances;
+    sc.inst_pos = scene->inst_pos;
+    sc.meshes = scene->meshes;
+    sc.tris = scene->tris;
+    sc.tri_rec = scene->tri_rec;
+    sc.xforms = scene->xforms;
+    sc.inv_static = scene->inv_static;
+    sc.rects = scene->rect_lights;
+    sc.spheres = scene->sphere_lights;
+    sc.lnodes = scene->light_nodes;
+    sc.lboxes = scene->light_boxes;
+    sc.llist = scene->light_list;
+    sc.has_ltree = scene->host.light_nodes.empty() ? 0u : 1u;
+    sc.shader_count = (uint32_t)shaders.size();
+    sc.distant_count = (uint32_t)distant.size();
+    sc.background = GpuColor{world->background.kind, world->background.a, world->background.b, world->background.c};
+    GpuClosure* dsh = nullptr;
+    GpuDistantLight* ddl = nullptr;
+    HaltonDim* hd = nullptr;
+    uint16_t* hp = nullptr;
+    std::vector<HaltonDim> hdims;
+    std::vector<uint16_t> hperms;
+    halton_tables(hdims, hperms);
+    rc = r->alloc(&dsh, shaders.size());
+    if (!rc) rc = r->alloc(&ddl, distant.size());
+    if (!rc) rc = r->alloc(&hd, hdims.size());
+    if (!rc) rc = r->alloc(&hp, hperms.size());
+    if (rc) return bail(rc);
+    e = hipSuccess;
+    if (!shaders.empty()) e = hipMemcpy(dsh, shaders.data(), shaders.size() * sizeof(GpuClosure), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !distant.empty())
+        e = hipMemcpy(ddl, distant.data(), distant.size() * sizeof(GpuDistantLight), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(hd, hdims.data(), hdims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(hp, hperms.data(), hperms.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bail(fail(PSY_E_DEVICE, std::string("renderer upload: ") + hipGetErrorString(e)));
+    sc.shaders = dsh;
+    sc.distant = ddl;
+    sc.hdims = hd;
+    sc.hperms = hp;
+    *out = r;
+    return PSY_OK;
+}
+
+int psy_renderer_destroy(psy_renderer* r) {
+    delete r;
+    return PSY_OK;
+}
+
+int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_settings* st, float* xyz, uint32_t mem,
+               psy_render_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!r || !cam || !st || !xyz) return fail(PSY_E_INVALID, "NULL argument");
+    if (mem != PSY_MEM_HOST && mem != PSY_MEM_DEVICE) return fail(PSY_E_INVALID, "mem must be PSY_MEM_HOST or PSY_MEM_DEVICE");
+    if (st->width == 0 || st->height == 0 || st->spp == 0) return fail(PSY_E_INVALID, "empty image or zero spp");
+    if (st->width > 65535 || st->height > 65535) return fail(PSY_E_RANGE, "width and height must be below 65536");
+    const uint64_t pixels = (uint64_t)st->width * st->height;
+    const uint64_t total = pixels * st->spp;
+    if (pixels > 0xffffffffull || total >= (1ull << 48)) return fail(PSY_E_RANGE, "image too large");
+    if (st->max_bounces > 64) return fail(PSY_E_RANGE, "max_bounces must be at most 64");
+    if (cam->xform_count == 0 || !cam->xforms) return fail(PSY_E_INVALID, "camera needs at least one transform");
+    if (cam->xform_count > 64) return fail(PSY_E_RANGE, "at most 64 camera transforms");
+
+    // Camera::new (camera.rs:22-72): tan(fov/2), focus 0 disables the lens.
+    ReqParams rp{};
+    rp.width = st->width;
+    rp.height = st->height;
+    rp.spp = st->spp;
+    rp.max_bounces = st->max_bounces;
+    rp.seed = st->seed;
+    rp.cmpx = 1.0f / (float)st->width;
+    rp.cmpy = 1.0f / (float)st->height;
+    const float min_x = -1.0f, max_x = 1.0f;
+    const float max_y = (float)st->height / (float)st->width, min_y = -max_y;
+    rp.x_extent = max_x - min_x;
+    rp.y_extent = max_y - min_y;
+    const float half = cam->fov / 2.0f;
+    rp.tfov = std::sin(half) / std::cos(half);
+    rp.aperture = cam->aperture_radius;
+    rp.focus = cam->focus_distance;
+    if (rp.focus == 0.0f) {
+        rp.aperture = 0.0f;
+        rp.focus = 1.0f;
+    }
+    rp.cam_xf_count = cam->xform_count;
+
+    int rc = PSY_OK;
+    if (r->cam_cap < cam->xform_count) {
+        if (r->cam) r->release(r->cam);
+        r->cam = nullptr;
+        rc = r->alloc(&r->cam, 64);
+        if (rc) return rc;
+        r->cam_cap = 64;
+    }
+    std::vector<GpuXform> cx(cam->xform_count);
+    for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(cx[i].r, cam->xforms + 12ull * i, 48);
+    rp.cam_xf = r->cam;
+
+    float* film = xyz;
+    float* film_dev = nullptr;
+    if (mem == PSY_MEM_HOST) {
+        rc = r->alloc(&film_dev, pixels * 3);
+        if (rc) return rc;
+        film = film_dev;
+    }
+    rp.film = film;
+    if (!r->reqs) {
+        rc = r->alloc(&r->reqs, 1);
+        if (rc) return rc;
+        r->req_cap = 1;
+    }
+
+    // Wave plans: consecutive path ranges of `capacity` slots.
+    const uint64_t waves = (total + r->capacity - 1) / r->capacity;
+    if (waves > r->plan_cap) {
+        // Captured graphs hold the plan array's address.
+        for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
+        r->graphs.clear();
+        if (r->plans) r->release(r->plans);
+        r->plans = nullptr;
+        rc = r->alloc(&r->plans, waves);
+        if (rc) return rc;
+        r->plan_cap = (uint32_t)waves;
+    }
+    std::vector<WavePlan> plans(waves);
+    for (uint64_t w = 0; w < waves; w++) {
+        std::memset(&plans[w], 0, sizeof(WavePlan));
+        const uint64_t g0 = w * r->capacity;
+        plans[w].nseg = 1;
+        plans[w].total = (uint32_t)std::min<uint64_t>(r->capacity, total - g0);
+        plans[w].seg[0] = WaveSeg{0, 0, plans[w].total, 0, g0};
+    }
+    const hipStream_t s = r->stream;
+    PSY_HIP(hipMemcpyAsync(r->cam, cx.data(), cx.size() * sizeof(GpuXform), hipMemcpyHostToDevice, s));
+    PSY_HIP(hipMemcpyAsync(r->reqs, &rp, sizeof(rp), hipMemcpyHostToDevice, s));
+    PSY_HIP(hipMemcpyAsync(r->plans, plans.data(), waves * sizeof(WavePlan), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(zero_film_kernel, dim3(grid_for(pixels * 3)), dim3(KBLOCK), 0, s, film, pixels * 3);
+    PSY_HIP(hipGetLastError());
+    hipLaunchKernelGGL(wave_begin_kernel, dim3(1), dim3(1), 0, s, r->wave_idx, 0u);
+    PSY_HIP(hipGetLastError());
+    PSY_HIP(hipEventRecord(r->ev0, s));
+    for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
+    PSY_HIP(hipEventRecord(r->ev1, s));
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        if (film_dev) r->release(film_dev);
+        return rc;
+    }
+    if (mem == PSY_MEM_HOST) PSY_HIP(hipMemcpyAsync(xyz, film_dev, pixels * 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    PSY_HIP(hipStreamSynchronize(s));
+    if (film_dev) r->release(film_dev);
+    if (stats) {
+        float ms = 0.0f;
+        PSY_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        stats->paths = total;
+        stats->rays = 0;
+        stats->waves = waves;
+        stats->trace_ms = ms;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return PSY_OK;
+}
+
+int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
+    if (!r) return fail(PSY_E_INVALID, "renderer is NULL");
+    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT)) return fail(PSY_E_INVALID, "unknown option flag");
+    const bool graphs = (flags & PSY_RENDER_NO_GRAPHS) == 0, sort = (flags & PSY_RENDER_NO_SORT) == 0;
+    if (sort != r->sort_materials) {
+        for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
+        r->graphs.clear();
+    }
+    r->use_graphs = graphs;
+    r->sort_materials = sort;
+    return PSY_OK;
+}
+
+int psy_renderer_get_tuning(const psy_renderer* r, psy_trace_tuning* t) {
+    if (!r || !t) return fail(PSY_E_INVALID, "NULL argument");
+    *t = r->tuning;
+    return PSY_OK;
+}
+
+int psy_renderer_set_tuning(psy_renderer* r, const psy_trace_tuning* t) {
+    if (!r || !t) return fail(PSY_E_INVALID, "NULL argument");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    if (t->kernel != PSY_KERNEL_PERSISTENT) return fail(PSY_E_INVALID, "the renderer traces with the persistent kernel");
+    r->tuning = *t;
+    for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
+    r->graphs.clear();
+    return PSY_OK;
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
 namespace psy {
 
@@ -73,6 +74,100 @@ uint32_t append_bvh(PackedScene& s, const HostBvh4& bvh) {
     return bvh.root + node_base;
 }
 
+bool color_ok(const psy_color& c) {
+    return c.kind <= PSY_COLOR_TEMPERATURE && std::isfinite(c.a) && std::isfinite(c.b) && std::isfinite(c.c);
+}
+GpuColor to_gpu(const psy_color& c) { return GpuColor{c.kind, c.a, c.b, c.c}; }
+
+// Color::approximate_energy (src/color.rs:139-152), and the light objects'
+// approximate_energy: the fold over their single colour divided by the
+// colour count (rectangle_light.rs:246-252, sphere_light.rs:200-204).
+float light_energy(const psy_color& c) {
+    float e;
+    if (c.kind == PSY_COLOR_XYZ) e = c.b;
+    else if (c.kind == PSY_COLOR_BLACKBODY) {
+        float t2 = c.a * c.a;
+        e = (t2 * t2) * c.b;
+    } else e = c.b;
+    return (0.0f + e) / 1.0f;
+}
+
+// merge_slices_append with union (src/algorithm.rs:233-262).
+std::vector<Box> merge_union(const std::vector<Box>& a, const std::vector<Box>& b) {
+    std::vector<Box> out;
+    uint32_t na = (uint32_t)a.size(), nb = (uint32_t)b.size();
+    if (na == nb) {
+        for (uint32_t i = 0; i < na; i++) out.push_back(box_union(a[i], b[i]));
+    } else if (na > nb) {
+        float sc = (float)(na - 1);
+        for (uint32_t i = 0; i < na; i++) out.push_back(box_union(a[i], lerp_slice_bx(b.data(), nb, (float)i / sc)));
+    } else {
+        float sc = (float)(nb - 1);
+        for (uint32_t i = 0; i < nb; i++) out.push_back(box_union(lerp_slice_bx(a.data(), na, (float)i / sc), b[i]));
+    }
+    return out;
+}
+
+// Binary light tree (LightTree::from_objects, src/accel/light_tree.rs:
+// 211-290): one light per leaf, SAH splits, bounds merged without the BVH's
+// union rule, energies summed.
+struct LtNode {
+    bool leaf = false;
+    std::vector<Box> bounds;
+    float energy = 0.0f;
+    uint32_t light = 0;
+    std::unique_ptr<LtNode> kid[2];
+};
+
+std::unique_ptr<LtNode> lt_build(const BuildInput& in, const std::vector<float>& energy, uint32_t* keys, uint32_t n,
+                                 uint32_t offset) {
+    auto node = std::make_unique<LtNode>();
+    if (n == 1) {
+        node->leaf = true;
+        node->bounds.assign(in.bounds + in.offsets[keys[0]], in.bounds + in.offsets[keys[0] + 1]);
+        node->energy = energy[keys[0]];
+        node->light = offset;
+        return node;
+    }
+    uint32_t split = sah_split_keys(in, keys, n);
+    node->kid[0] = lt_build(in, energy, keys, split, offset);
+    node->kid[1] = lt_build(in, energy, keys + split, n - split, offset + split);
+    node->bounds = merge_union(node->kid[0]->bounds, node->kid[1]->bounds);
+    node->energy = node->kid[0]->energy + node->kid[1]->energy;
+    return node;
+}
+
+// The collapse to arity 8 (light_tree.rs:292-372): a node's children are its
+// descendants three levels down, or the leaves met before that, left to
+// right.
+void lt_collect(const LtNode* n, int level, std::vector<const LtNode*>& out) {
+    if (level > 0 && !n->leaf) {
+        lt_collect(n->kid[0].get(), level - 1, out);
+        lt_collect(n->kid[1].get(), level - 1, out);
+    } else {
+        out.push_back(n);
+    }
+}
+
+void lt_emit(const LtNode* n, uint32_t slot, PackedScene& s) {
+    GpuLightNode g{};
+    g.bounds_start = (uint32_t)s.light_boxes.size();
+    g.bounds_len = (uint32_t)n->bounds.size();
+    for (const Box& b : n->bounds) s.light_boxes.push_back(GpuBox{{b.lo.x, b.lo.y, b.lo.z}, {b.hi.x, b.hi.y, b.hi.z}});
+    g.energy = n->energy;
+    std::vector<const LtNode*> kids;
+    if (n->leaf) {
+        g.light = n->light;
+    } else {
+        lt_collect(n, 3, kids);
+        g.child_first = (uint32_t)s.light_nodes.size();
+        g.child_count = (uint32_t)kids.size();
+        s.light_nodes.resize(s.light_nodes.size() + kids.size());
+    }
+    s.light_nodes[slot] = g;
+    for (uint32_t i = 0; i < (uint32_t)kids.size(); i++) lt_emit(kids[i], g.child_first + i, s);
+}
+
 }  // namespace
 
 int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
@@ -82,6 +177,25 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     if (d->instance_count > 0 && !d->instances) { err = "instances is NULL"; return PSY_E_INVALID; }
     if (d->xform_count > 0 && !d->xforms) { err = "xforms is NULL"; return PSY_E_INVALID; }
     if (d->instance_count > MAX_LEAF_FIRST) { err = "too many instances"; return PSY_E_RANGE; }
+    if (d->rect_light_count > 0 && !d->rect_lights) { err = "rect_lights is NULL"; return PSY_E_INVALID; }
+    if (d->sphere_light_count > 0 && !d->sphere_lights) { err = "sphere_lights is NULL"; return PSY_E_INVALID; }
+    for (uint32_t i = 0; i < d->rect_light_count; i++) {
+        const psy_rect_light_desc& l = d->rect_lights[i];
+        if (!(l.width > 0.0f) || !(l.height > 0.0f) || !std::isfinite(l.width) || !std::isfinite(l.height) ||
+            !color_ok(l.color)) {
+            err = "rect light " + std::to_string(i) + ": bad size or colour";
+            return PSY_E_INVALID;
+        }
+        s.rect_lights.push_back(GpuRectLight{l.width, l.height, {0.0f, 0.0f}, to_gpu(l.color)});
+    }
+    for (uint32_t i = 0; i < d->sphere_light_count; i++) {
+        const psy_sphere_light_desc& l = d->sphere_lights[i];
+        if (!(l.radius > 0.0f) || !std::isfinite(l.radius) || !color_ok(l.color)) {
+            err = "sphere light " + std::to_string(i) + ": bad radius or colour";
+            return PSY_E_INVALID;
+        }
+        s.sphere_lights.push_back(GpuSphereLight{l.radius, {0.0f, 0.0f, 0.0f}, to_gpu(l.color)});
+    }
 
     s.bvhs.resize(d->mesh_count + 1);
     s.meshes.resize(d->mesh_count);
@@ -99,6 +213,7 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         }
         if (md.tri_count > MAX_LEAF_FIRST) { err = "mesh " + std::to_string(m) + ": too many triangles"; return PSY_E_RANGE; }
         const uint32_t S = md.time_samples, V = md.vert_count, T = md.tri_count;
+        if (md.normals) s.has_normals = true;
         for (uint64_t i = 0; i < 3ull * T; i++)
             if (md.indices[i] >= V) {
                 err = "mesh " + std::to_string(m) + ": vertex index out of range";
@@ -147,6 +262,9 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
                 }
             std::memcpy(rec + 36ull * S, &orig, 4);
         }
+        gm.rec_map = (uint32_t)s.tri_rec.size();
+        s.tri_rec.resize(s.tri_rec.size() + T);
+        for (uint32_t k = 0; k < T; k++) s.tri_rec[gm.rec_map + bvh.order[k]] = k;
         s.meshes[m] = gm;
         s.max_depth = std::max(s.max_depth, bvh.depth);
         s.leaf_count += bvh.leaf_count;
@@ -158,15 +276,36 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     std::vector<uint32_t> ioffs{0};
     for (uint32_t i = 0; i < d->instance_count; i++) {
         const psy_instance_desc& id = d->instances[i];
-        if (id.mesh >= d->mesh_count) { err = "instance " + std::to_string(i) + ": mesh out of range"; return PSY_E_INVALID; }
+        const std::string who = "instance " + std::to_string(i);
+        if (id.kind > PSY_OBJECT_SPHERE_LIGHT) { err = who + ": unknown object kind"; return PSY_E_INVALID; }
+        const uint32_t limit = id.kind == PSY_OBJECT_MESH
+                                   ? d->mesh_count
+                                   : (id.kind == PSY_OBJECT_RECT_LIGHT ? d->rect_light_count : d->sphere_light_count);
+        if (id.mesh >= limit) { err = who + ": object out of range"; return PSY_E_INVALID; }
         if ((uint64_t)id.xform_start + id.xform_count > d->xform_count) {
-            err = "instance " + std::to_string(i) + ": transform range out of bounds";
+            err = who + ": transform range out of bounds";
             return PSY_E_INVALID;
         }
-        if (id.xform_count > MAX_TIME_SAMPLES) { err = "instance " + std::to_string(i) + ": too many transforms"; return PSY_E_RANGE; }
+        if (id.xform_count > MAX_TIME_SAMPLES) { err = who + ": too many transforms"; return PSY_E_RANGE; }
+        if (id.kind != PSY_OBJECT_MESH && id.xform_count == 0) {
+            err = who + ": light instances need a transform";
+            return PSY_E_INVALID;
+        }
         std::vector<Xf> xf;
         for (uint32_t k = 0; k < id.xform_count; k++) xf.push_back(xf_at(d, id.xform_start + k));
-        std::vector<Box> wb = transform_bounds(s.bvhs[id.mesh].root_bounds, xf.data(), id.xform_count);
+        std::vector<Box> ob;
+        if (id.kind == PSY_OBJECT_MESH) {
+            ob = s.bvhs[id.mesh].root_bounds;
+        } else if (id.kind == PSY_OBJECT_RECT_LIGHT) {
+            // RectangleLight bounds (rectangle_light.rs:36-42)
+            const psy_rect_light_desc& l = d->rect_lights[id.mesh];
+            ob.push_back(Box{V3{l.width * -0.5f, l.height * -0.5f, 0.0f}, V3{l.width * 0.5f, l.height * 0.5f, 0.0f}});
+        } else {
+            // SphereLight bounds (sphere_light.rs:35-40)
+            const float r = d->sphere_lights[id.mesh].radius;
+            ob.push_back(Box{V3{-r, -r, -r}, V3{r, r, r}});
+        }
+        std::vector<Box> wb = transform_bounds(ob, xf.data(), id.xform_count);
         ibounds.insert(ibounds.end(), wb.begin(), wb.end());
         ioffs.push_back((uint32_t)ibounds.size());
     }
@@ -177,6 +316,7 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     s.max_depth = std::max(s.max_depth, top.depth);
     s.leaf_count += top.leaf_count;
 
+    s.inst_pos.resize(d->instance_count);
     for (uint32_t k = 0; k < d->instance_count; k++) {
         uint32_t orig = top.order[k];
         const psy_instance_desc& id = d->instances[orig];
@@ -184,6 +324,9 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         gi.mesh = id.mesh;
         gi.id = orig;
         gi.shader = id.shader;
+        gi.kind = id.kind;
+        s.inst_pos[orig] = k;
+        if (id.kind != PSY_OBJECT_MESH) s.has_lights = true;
         gi.xf_start = (uint32_t)s.xforms.size();
         gi.xf_count = id.xform_count;
         gi.inv_static = REF_EMPTY;
@@ -196,6 +339,32 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         s.instances.push_back(gi);
         if (id.xform_count > 0) s.has_xf = true;
     }
+    // Light tree over the light instances in leaf order
+    // (AssemblyBuilder::build, assembly.rs:297-338).
+    std::vector<uint32_t> light_pos;
+    std::vector<Box> lbounds;
+    std::vector<uint32_t> loffs{0};
+    std::vector<float> lenergy;
+    for (uint32_t k = 0; k < (uint32_t)s.instances.size(); k++) {
+        const GpuInstance& gi = s.instances[k];
+        if (gi.kind == PSY_OBJECT_MESH) continue;
+        light_pos.push_back(k);
+        lbounds.insert(lbounds.end(), ibounds.begin() + ioffs[gi.id], ibounds.begin() + ioffs[gi.id + 1]);
+        loffs.push_back((uint32_t)lbounds.size());
+        const psy_color& c = gi.kind == PSY_OBJECT_RECT_LIGHT ? d->rect_lights[gi.mesh].color
+                                                               : d->sphere_lights[gi.mesh].color;
+        lenergy.push_back(light_energy(c));
+    }
+    if (!light_pos.empty()) {
+        std::vector<uint32_t> keys(light_pos.size());
+        for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) keys[i] = i;
+        BuildInput lin{(uint32_t)keys.size(), lbounds.data(), loffs.data()};
+        std::unique_ptr<LtNode> root = lt_build(lin, lenergy, keys.data(), (uint32_t)keys.size(), 0);
+        s.light_nodes.resize(1);
+        lt_emit(root.get(), 0, s);
+        for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) s.light_list.push_back(light_pos[keys[i]]);
+    }
+
     s.has_motion = !s.motion.empty();
     for (const GpuMesh& gm : s.meshes)
         if (gm.time_samples > 1) s.has_motion = true;

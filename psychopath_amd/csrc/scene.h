@@ -24,10 +24,21 @@ struct PackedScene {
     std::vector<GpuXform> inv_static;
     uint32_t top_root = REF_EMPTY;
 
+    // Lights and shading lookups.
+    std::vector<GpuRectLight> rect_lights;
+    std::vector<GpuSphereLight> sphere_lights;
+    std::vector<uint32_t> inst_pos;         // description index -> leaf position
+    std::vector<uint32_t> tri_rec;          // per mesh at GpuMesh::rec_map: triangle -> record
+    std::vector<GpuLightNode> light_nodes;  // light tree, root first; empty without lights
+    std::vector<GpuBox> light_boxes;
+    std::vector<uint32_t> light_list;       // light-tree order -> instance leaf position
+
     // Host builds, kept for parity dumps (index = mesh; the instance BVH last).
     std::vector<HostBvh4> bvhs;
     bool has_xf = false;        // some instance has transforms
     bool has_motion = false;    // time-sampled node bounds or vertices
+    bool has_lights = false;    // some instance places a light
+    bool has_normals = false;   // some mesh has shading normals (not kept)
     uint32_t max_stack = 0;     // deepest pending-node count traversal can need
     uint32_t max_depth = 0;
     uint64_t leaf_count = 0;

@@ -22,11 +22,15 @@ struct TraceArgs {
     const GpuInstance* instances;
     const GpuXform* xforms;
     const GpuXform* inv_static;
+    const GpuRectLight* rect_lights;
+    const GpuSphereLight* sphere_lights;
     uint32_t top_root;
     // rays (device)
     const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
     const uint8_t* flags;
-    uint64_t n;
+    uint32_t all_occlusion;    // every ray is an occlusion ray (flags ignored)
+    uint64_t n;                // ray count, or the capacity when n_dev is set
+    const uint32_t* n_dev;     // device-resident ray count (persistent kernel), or NULL
     // hits (device)
     uint8_t* kind;
     float *t, *b0, *b1, *b2;

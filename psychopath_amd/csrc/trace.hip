@@ -38,13 +38,13 @@ template <bool COUNT>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * BLOCK];
     const uint64_t ray = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (ray >= a.n) return;
+    if (ray >= a.n || (a.n_dev != nullptr && ray >= *a.n_dev)) return;
 
     const V3 o = v3(a.ox[ray], a.oy[ray], a.oz[ray]);
     const V3 d = v3(a.dx[ray], a.dy[ray], a.dz[ray]);
     const float time = a.time[ray];
     float tmax = a.max_t[ray];
-    const bool occl = a.flags != nullptr && (a.flags[ray] & 1u) != 0;
+    const bool occl = a.all_occlusion != 0 || (a.flags != nullptr && (a.flags[ray] & 1u) != 0);
 
     // Top-level local ray: update_local with the identity (tracer.rs:65-70).
     const Xf ident = xf_identity();
@@ -147,6 +147,28 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
 
         const uint32_t first = (cur & ~REF_LEAF) >> 2;
         const uint32_t count = cur & 3u;
+        if (!in_inst && a.instances[first].kind != PSY_OBJECT_MESH) {
+            // Light instance: test the light's surface in place.
+            const GpuInstance inst = a.instances[first];
+            if (COUNT) c_inst++;
+            const Xf x = instance_xf(a.xforms, inst, time);
+            float lt, l0, l1, l2;
+            uint32_t lp;
+            if (light_leaf_test(a.rect_lights, a.sphere_lights, inst, x, o, pre, xf_point(x, o), xf_vector(x, d),
+                                tmax, lt, l0, l1, l2, lp)) {
+                if (occl) {
+                    kind = PSY_HIT_OCCLUDE;
+                    break;
+                }
+                tmax = lt;
+                kind = PSY_HIT_SURFACE;
+                ht = lt; hb0 = l0; hb1 = l1; hb2 = l2;
+                hprim = lp;
+                hinst = inst.id;
+            }
+            cur = REF_EMPTY;
+            continue;
+        }
         if (!in_inst) {
             // Instance leaf (tracer.rs:95-128): move the ray into the
             // instance's space and descend into its mesh.

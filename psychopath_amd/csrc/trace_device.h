@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/psy_rt.h"
 #include "layout.h"
 #include "psymath.h"
 
@@ -111,6 +112,16 @@ __device__ __forceinline__ bool tri_test(V3 o, const TriPre& pre, float tmax, V3
     return true;
 }
 
+// The instance's world-to-object transform at `time` (lerp_slice over its
+// time samples, src/lerp.rs:20-35); xf_count must be at least 1.
+__device__ __forceinline__ Xf load_xf(const GpuXform* x);
+__device__ __forceinline__ Xf instance_xf(const GpuXform* xforms, const GpuInstance& inst, float time) {
+    uint32_t i1;
+    float a2;
+    if (!lerp_slice_pos(inst.xf_count, time, i1, a2)) return load_xf(xforms + inst.xf_start + i1);
+    return xf_lerp(load_xf(xforms + inst.xf_start + i1), load_xf(xforms + inst.xf_start + i1 + 1), a2);
+}
+
 __device__ __forceinline__ Xf load_xf(const GpuXform* x) {
     const float4* p = reinterpret_cast<const float4*>(x);
     float4 a = p[0], b = p[1], c = p[2];
@@ -121,5 +132,58 @@ __device__ __forceinline__ Xf load_xf(const GpuXform* x) {
     return r;
 }
 
+
+// Light surfaces as traversal leaves.  Rectangle (rectangle_light.rs:257-
+// 338): the two world-space triangles (p1,p2,p3), (p3,p4,p1) of the corners
+// mapped through the inverse of the lerped transform, the first hit with
+// t < tmax wins; prim is the triangle.  Sphere (sphere_light.rs:206-282): the
+// quadratic in object space with the object-space origin `lo` and direction
+// `ld`.  Returns true on a hit.
+__device__ __forceinline__ bool light_leaf_test(const GpuRectLight* rects, const GpuSphereLight* spheres,
+                                                const GpuInstance& inst, const Xf& x, V3 o, const TriPre& pre,
+                                                V3 lo, V3 ld, float tmax, float& t_out, float& b0, float& b1,
+                                                float& b2, uint32_t& prim) {
+    if (inst.kind == PSY_OBJECT_RECT_LIGHT) {
+        const GpuRectLight L = rects[inst.mesh];
+        const Xf inv = xf_inverse(x);
+        const float hw = L.w * 0.5f, hh = L.h * 0.5f, nw = L.w * -0.5f, nh = L.h * -0.5f;
+        const V3 p1 = xf_point(inv, v3(hw, hh, 0.0f));
+        const V3 p2 = xf_point(inv, v3(nw, hh, 0.0f));
+        const V3 p3 = xf_point(inv, v3(nw, nh, 0.0f));
+        const V3 p4 = xf_point(inv, v3(hw, nh, 0.0f));
+        float t, c0, c1, c2;
+        if (tri_test(o, pre, tmax, p1, p2, p3, t, c0, c1, c2) && t < tmax) {
+            t_out = t; b0 = c0; b1 = c1; b2 = c2; prim = 0;
+            return true;
+        }
+        if (tri_test(o, pre, tmax, p3, p4, p1, t, c0, c1, c2) && t < tmax) {
+            t_out = t; b0 = c0; b1 = c1; b2 = c2; prim = 1;
+            return true;
+        }
+        return false;
+    }
+    const float r = spheres[inst.mesh].radius;
+    const float qa = dot(ld, ld);
+    const float qb = 2.0f * dot(ld, lo);
+    const float qc = dot(lo, lo) - (r * r);
+    float disc = (qb * qb) - ((4.0f * qa) * qc);
+    if (disc < 0.0f) return false;
+    disc = sqrtf(disc);
+    const float q = qb < 0.0f ? -0.5f * (qb - disc) : -0.5f * (qb + disc);
+    float t0 = q / qa;
+    float t1 = q != 0.0f ? qc / q : tmax;
+    if (t0 > t1) {
+        float tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    if (t0 > tmax || t1 <= 0.0f) return false;
+    float t;
+    if (t0 > 0.0f) t = t0;
+    else if (t1 <= tmax) t = t1;
+    else return false;
+    t_out = t; b0 = 0.0f; b1 = 0.0f; b2 = 0.0f; prim = 0;
+    return true;
+}
 
 }  // namespace psy

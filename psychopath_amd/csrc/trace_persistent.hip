@@ -62,7 +62,7 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
     uint32_t spill[TRACE_STACK_CAPACITY];
     int sp = 0, bot = 0;
 
-    const uint32_t n = (uint32_t)a.n;
+    const uint32_t n = a.n_dev ? *a.n_dev : (uint32_t)a.n;
     const Xf ident = xf_identity();
 
     // Per-lane ray state.
@@ -107,7 +107,7 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                     d = v3(a.dx[ray], a.dy[ray], a.dz[ray]);
                     time = MOTION || XF ? a.time[ray] : 0.0f;
                     tmax = a.max_t[ray];
-                    occl = a.flags != nullptr && (a.flags[ray] & 1u) != 0;
+                    occl = a.all_occlusion != 0 || (a.flags != nullptr && (a.flags[ray] & 1u) != 0);
                     // Top-level local ray: update_local with the identity
                     // (tracer.rs:65-70).
                     lo = xf_point(ident, o);
@@ -178,7 +178,31 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         if (cur != REF_EMPTY) {
             const uint32_t first = (cur & ~REF_LEAF) >> 2;
             const uint32_t count = cur & 3u;
-            if (!in_inst) {
+            if (!in_inst && XF && a.instances[first].kind != PSY_OBJECT_MESH) {
+                // Light instance (tracer.rs:95-128 with a light object; light
+                // instances always carry transforms).
+                const GpuInstance inst = a.instances[first];
+                if (COUNT) c_inst++;
+                const Xf x = instance_xf(a.xforms, inst, time);
+                const V3 llo = xf_point(x, o);
+                const V3 lld = xf_vector(x, d);
+                float lt, l0, l1, l2;
+                uint32_t lp;
+                if (light_leaf_test(a.rect_lights, a.sphere_lights, inst, x, o, pre, llo, lld, tmax, lt, l0, l1, l2,
+                                    lp)) {
+                    if (occl) {
+                        kind = PSY_HIT_OCCLUDE;
+                        finished = true;
+                    } else {
+                        tmax = lt;
+                        kind = PSY_HIT_SURFACE;
+                        ht = lt; hb0 = l0; hb1 = l1; hb2 = l2;
+                        hprim = lp;
+                        hinst = inst.id;
+                    }
+                }
+                cur = REF_EMPTY;
+            } else if (!in_inst) {
                 // Instance leaf (tracer.rs:95-128).
                 const GpuInstance inst = a.instances[first];
                 const GpuMesh mesh = a.meshes[inst.mesh];
@@ -339,12 +363,17 @@ hipError_t launch_feat(const TraceArgs& a, unsigned int* work, const TraceTuning
 
 }  // namespace
 
+// Sets *work = 0 with a kernel, so the launch sequence can be captured in a
+// graph (no memset node).
+__global__ void reset_work_kernel(unsigned int* work) { *work = 0u; }
+
 hipError_t launch_trace_persistent(const TraceArgs& args, bool counted, bool has_xf, bool has_motion,
                                    unsigned int* work, const TraceTuning& tune, hipStream_t stream) {
     if (args.n == 0) return hipSuccess;
     if (args.n >= 0x80000000ull || tune.waves == 0 || tune.refill_min == 0 || tune.refill_min > 64)
         return hipErrorInvalidValue;
-    hipError_t e = hipMemsetAsync(work, 0, sizeof(unsigned int), stream);
+    hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(1), 0, stream, work);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return counted ? launch_feat<true>(args, work, tune, has_xf, has_motion, stream)
                    : launch_feat<false>(args, work, tune, has_xf, has_motion, stream);

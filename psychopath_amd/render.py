@@ -1,0 +1,101 @@
+"""Host-side mirror of the reference's renderer interface over the C ABI
+(include/psy_render.h).
+
+    Renderer       ~ Renderer::render              (src/renderer.rs:32-192)
+    World          ~ World + the scene's shaders   (src/scene/world.rs:4-8,
+                                                    src/shading/mod.rs:28-61)
+    Camera         ~ Camera::new                   (src/camera.rs:22-72)
+
+`build(render_scene)` turns a psychopath_amd.scenes.RenderScene into a
+device scene plus renderer.  Images are XYZ float32 arrays [H, W, 3], row 0
+at the top.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .tracer import Assembly, Mesh, RectLight, SphereLight, color_tuple
+
+
+def _color(c) -> _lib.Color:
+    return _lib.Color(*color_tuple(c))
+
+
+class Renderer:
+    """A device-resident scene with its shaders, world and path state."""
+
+    def __init__(self, scene, shaders, background=(0, 0.0, 0.0, 0.0), distant_lights=(), max_paths: int = 0):
+        self.scene = scene  # keeps the scene alive
+        sh = (_lib.ShaderDesc * max(1, len(shaders)))()
+        for i, (kind, col, rough, fres) in enumerate(shaders):
+            sh[i] = _lib.ShaderDesc(int(kind), _color(col), float(rough), float(fres))
+        dl = (_lib.DistantLightDesc * max(1, len(distant_lights)))()
+        for i, (radius, direction, col) in enumerate(distant_lights):
+            dl[i] = _lib.DistantLightDesc(float(radius), (C.c_float * 3)(*[float(x) for x in direction]), _color(col))
+        world = _lib.WorldDesc(len(shaders), sh, _color(background), len(distant_lights), dl)
+        h = C.c_void_p()
+        check("psy_renderer_create", lib().psy_renderer_create(scene.handle, C.byref(world), int(max_paths),
+                                                                C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib._lib is not None:
+            _lib._lib.psy_renderer_destroy(h)
+            self.handle = None
+
+    def set_options(self, graphs: bool = True, sort: bool = True) -> None:
+        flags = (0 if graphs else _lib.PSY_RENDER_NO_GRAPHS) | (0 if sort else _lib.PSY_RENDER_NO_SORT)
+        check("psy_renderer_set_options", lib().psy_renderer_set_options(self.handle, flags))
+
+    def tuning(self) -> dict:
+        t = _lib.TraceTuning()
+        check("psy_renderer_get_tuning", lib().psy_renderer_get_tuning(self.handle, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        t = _lib.TraceTuning(**{**self.tuning(), **kw})
+        check("psy_renderer_set_tuning", lib().psy_renderer_set_tuning(self.handle, C.byref(t)))
+
+    def render(self, camera: dict, settings: dict, device_out=None) -> tuple[np.ndarray | None, dict]:
+        """Renders one image.  `camera`: xforms [N,12] (camera to world), fov
+        (radians), aperture, focus.  `settings`: width, height, spp,
+        max_bounces, seed.  With `device_out` (a device pointer of
+        width*height*3 floats) the image stays on the device."""
+        xf = np.ascontiguousarray(camera["xforms"], np.float32).reshape(-1, 12)
+        cam = _lib.CameraDesc(xf.shape[0], xf.ctypes.data_as(_lib._fp), float(camera["fov"]),
+                              float(camera.get("aperture", 0.0)), float(camera.get("focus", 1.0)))
+        st = _lib.RenderSettings(int(settings["width"]), int(settings["height"]), int(settings["spp"]),
+                                 int(settings.get("max_bounces", 2)), int(settings.get("seed", 0)))
+        stats = _lib.RenderStats()
+        if device_out is None:
+            img = np.zeros((st.height, st.width, 3), np.float32)
+            ptr, mem = img.ctypes.data, _lib.PSY_MEM_HOST
+        else:
+            img, ptr, mem = None, int(device_out), _lib.PSY_MEM_DEVICE
+        check("psy_render", lib().psy_render(self.handle, C.byref(cam), C.byref(st), C.c_void_p(ptr), mem,
+                                             C.byref(stats)))
+        return img, {f: getattr(stats, f) for f, _ in _lib.RenderStats._fields_}
+
+
+def build(rs, max_paths: int = 0) -> Renderer:
+    """Device scene + renderer for a psychopath_amd.scenes.RenderScene."""
+    asm = Assembly()
+    for v, t in rs.meshes:
+        asm.add_mesh(Mesh(v, t))
+    for (w, h, col) in rs.rect_lights:
+        asm.add_rect_light(RectLight(float(w), float(h), col))
+    for (r, col) in rs.sphere_lights:
+        asm.add_sphere_light(SphereLight(float(r), col))
+    for inst in rs.instances:
+        kind = {"mesh": _lib.PSY_OBJECT_MESH, "rect": _lib.PSY_OBJECT_RECT_LIGHT,
+                "sphere": _lib.PSY_OBJECT_SPHERE_LIGHT}[inst["kind"]]
+        shader = _lib.PSY_NONE if inst.get("shader") is None else int(inst["shader"])
+        asm.add_instance(inst["index"], inst.get("xforms"), shader, kind)
+    scene = asm.build()
+    return Renderer(scene, rs.shaders, rs.background, rs.distant_lights, max_paths)

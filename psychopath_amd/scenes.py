@@ -317,3 +317,94 @@ def cornell_box(width=512, height=512, spp=16, max_bounces=4, seed=0) -> RenderS
                      fov=float(np.float32(39.449188 * np.pi / 180.0)), aperture=0.0, focus=10.62)
     sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
     return sc
+
+
+def normal01(seed: int, n: int) -> np.ndarray:
+    """Seeded standard normal samples (Box-Muller over uniform01)."""
+    u = uniform01(seed, 2 * n).astype(np.float64).reshape(2, n)
+    u1 = np.maximum(u[0], 1e-12)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u[1])
+
+
+def _rotation(axis, angle):
+    """Rodrigues rotation matrix about a unit axis."""
+    x, y, z = axis
+    c, s = np.cos(angle), np.sin(angle)
+    C = 1.0 - c
+    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+
+
+def world_to_object(rot, pos):
+    """12 transform values a..l mapping world to object space for an object
+    placed by p_world = rot @ p_object + pos."""
+    rt = rot.T
+    t = -rt @ pos
+    return np.array([rt[0, 0], rt[0, 1], rt[0, 2], t[0], rt[1, 0], rt[1, 1], rt[1, 2], t[1],
+                     rt[2, 0], rt[2, 1], rt[2, 2], t[2]], np.float32)
+
+
+def look_at(pos, target, up=(0.0, 0.0, 1.0)):
+    """Camera-to-world transform (camera x right, y up, z forward) of a
+    camera at `pos` looking at `target`."""
+    pos = np.asarray(pos, np.float64)
+    fwd = np.asarray(target, np.float64) - pos
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    return np.array([right[0], upv[0], fwd[0], pos[0], right[1], upv[1], fwd[1], pos[1],
+                     right[2], upv[2], fwd[2], pos[2]], np.float32)
+
+
+def motion_instances(grid: int = 16, levels: int = 7, lights: int = 16, width: int = 1920, height: int = 1080,
+                     spp: int = 64, max_bounces: int = 6, seed: int = 0, aperture: float = 0.05) -> RenderScene:
+    """The motion-blur and instancing workload: grid x grid instances of a
+    displaced sphere (4 * 4^levels triangles) with two deformation time
+    samples (vertex offsets N(0, 0.02)), each instance on a seeded jittered
+    grid with three time-sampled transforms (rotation up to 30 degrees,
+    translation up to 0.5), `lights` blackbody sphere lights of radius
+    0.05-0.2 at 2500-9000 K, and a thin-lens camera."""
+    sc = RenderScene()
+    v, f = displaced_sphere(levels, amplitude=0.1, seed=1)
+    off = normal01(seed * 7 + 11, v.size).reshape(v.shape) * 0.02
+    v1 = (v.astype(np.float64) + off).astype(np.float32) + np.float32(0.0)
+    sc.meshes.append((np.stack([v, v1]), f))
+    sc.shaders = [(SHADER_LAMBERT, rec709(0.8, 0.8, 0.8), 0.0, 0.0),
+                  (SHADER_GGX, rec709(1.0, 0.78, 0.34), 0.2, 0.8),
+                  (SHADER_LAMBERT, rec709(0.7, 0.2, 0.15), 0.0, 0.0)]
+    n = grid * grid
+    u = uniform01(seed * 7 + 1, 9 * n).astype(np.float64).reshape(n, 9)
+    spacing = 2.5
+    for i in range(n):
+        gx, gy = i % grid, i // grid
+        base = np.array([(gx - (grid - 1) / 2) * spacing + (u[i, 0] - 0.5) * spacing * 0.4,
+                         (gy - (grid - 1) / 2) * spacing + (u[i, 1] - 0.5) * spacing * 0.4, 0.0])
+        axis = _sphere_point(u[i, 2], u[i, 3])
+        ang = np.radians(30.0) * u[i, 4]
+        move = _sphere_point(u[i, 5], u[i, 6]) * 0.5 * u[i, 7]
+        xfs = [world_to_object(_rotation(axis, ang * k / 2.0), base + move * k / 2.0) for k in range(3)]
+        sc.instances.append(dict(kind="mesh", index=0, shader=i % 3, xforms=np.stack(xfs)))
+    lu = uniform01(seed * 7 + 2, 5 * lights).astype(np.float64).reshape(lights, 5)
+    half = grid * spacing / 2
+    for k in range(lights):
+        radius = 0.05 + 0.15 * lu[k, 0]
+        temp = 2500.0 + 6500.0 * lu[k, 1]
+        sc.sphere_lights.append((radius, (COLOR_TEMPERATURE, temp, 40.0, 0.0)))
+        pos = np.array([(lu[k, 2] - 0.5) * 2 * half, (lu[k, 3] - 0.5) * 2 * half, 2.0 + 2.0 * lu[k, 4]])
+        sc.instances.append(dict(kind="sphere", index=k, shader=None,
+                                 xforms=world_to_object(np.eye(3), pos)[None]))
+    cam_pos = np.array([0.0, -half * 1.6, half * 0.9])
+    focus = float(np.linalg.norm(cam_pos))
+    sc.camera = dict(xforms=look_at(cam_pos, (0.0, 0.0, 0.0))[None], fov=float(np.float32(np.radians(50.0))),
+                     aperture=aperture, focus=focus)
+    sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
+    return sc
+
+
+def _sphere_point(u, v):
+    z = 1.0 - 2.0 * u
+    r = np.sqrt(max(0.0, 1.0 - z * z))
+    phi = 2.0 * np.pi * v
+    return np.array([r * np.cos(phi), r * np.sin(phi), z])

@@ -53,32 +53,67 @@ class Mesh:
             self.normals = n
 
 
+def color_tuple(c) -> tuple:
+    """A colour as (kind, a, b, c): kind 0 = XYZ, 1 = blackbody (temperature,
+    factor), 2 = normalised blackbody (Color, src/color.rs:46-62)."""
+    k, a, b, cc = c
+    return (int(k), float(a), float(b), float(cc))
+
+
+@dataclass
+class RectLight:
+    """RectangleLight (src/light/rectangle_light.rs:24-48)."""
+    width: float
+    height: float
+    color: tuple
+
+
+@dataclass
+class SphereLight:
+    """SphereLight (src/light/sphere_light.rs:27-48)."""
+    radius: float
+    color: tuple
+
+
 @dataclass
 class Instance:
-    """An instance of mesh `mesh` with world-to-object transforms, each the 12
-    values a..l of Transform::new_from_values (math3d transform.rs:29-47)."""
+    """An instance of object `mesh` (a mesh, or a light of kind `kind`) with
+    world-to-object transforms, each the 12 values a..l of
+    Transform::new_from_values (math3d transform.rs:29-47)."""
     mesh: int
     xforms: np.ndarray | None = None
     shader: int = _lib.PSY_NONE
+    kind: int = _lib.PSY_OBJECT_MESH
 
 
 @dataclass
 class Assembly:
-    """Single-level assembly of meshes and instances
+    """Single-level assembly of meshes, light objects and instances
     (AssemblyBuilder::add_object / add_instance, assembly.rs:197-282)."""
     meshes: list = field(default_factory=list)
     instances: list = field(default_factory=list)
+    rect_lights: list = field(default_factory=list)
+    sphere_lights: list = field(default_factory=list)
 
     def add_mesh(self, mesh: Mesh) -> int:
         self.meshes.append(mesh)
         return len(self.meshes) - 1
 
-    def add_instance(self, mesh: int, xforms=None, shader: int = _lib.PSY_NONE) -> int:
+    def add_rect_light(self, light: RectLight) -> int:
+        self.rect_lights.append(light)
+        return len(self.rect_lights) - 1
+
+    def add_sphere_light(self, light: SphereLight) -> int:
+        self.sphere_lights.append(light)
+        return len(self.sphere_lights) - 1
+
+    def add_instance(self, mesh: int, xforms=None, shader: int = _lib.PSY_NONE,
+                     kind: int = _lib.PSY_OBJECT_MESH) -> int:
         if xforms is not None:
             xforms = _f32(xforms).reshape(-1, 12)
             if len(xforms) == 0:
                 xforms = None
-        self.instances.append(Instance(mesh, xforms, shader))
+        self.instances.append(Instance(mesh, xforms, shader, kind))
         return len(self.instances) - 1
 
     def build(self) -> "Scene":
@@ -110,6 +145,7 @@ class Scene:
         for i, inst in enumerate(asm.instances):
             ids[i].mesh = inst.mesh
             ids[i].shader = inst.shader
+            ids[i].kind = inst.kind
             ids[i].xform_start = nxf
             cnt = 0 if inst.xforms is None else len(inst.xforms)
             ids[i].xform_count = cnt
@@ -118,7 +154,15 @@ class Scene:
             nxf += cnt
         xfa = _f32(np.concatenate(xf_list)) if xf_list else np.zeros((1, 12), np.float32)
         self._keep.append(xfa)
-        desc = _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp))
+        rls = (_lib.RectLightDesc * max(1, len(asm.rect_lights)))()
+        for i, l in enumerate(asm.rect_lights):
+            rls[i] = _lib.RectLightDesc(l.width, l.height, _lib.Color(*color_tuple(l.color)))
+        sls = (_lib.SphereLightDesc * max(1, len(asm.sphere_lights)))()
+        for i, l in enumerate(asm.sphere_lights):
+            sls[i] = _lib.SphereLightDesc(l.radius, _lib.Color(*color_tuple(l.color)))
+        self._keep += [rls, sls]
+        desc = _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp),
+                              len(asm.rect_lights), rls, len(asm.sphere_lights), sls)
         h = C.c_void_p()
         check("psy_scene_create", lib().psy_scene_create(C.byref(desc), C.byref(h)))
         self.handle = h

@@ -15,10 +15,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    with open(os.path.join(ROOT, "include", "psy_rt.h")) as f:
-        src = f.read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(psy_[a-z0-9_]+)\s*\(", src)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        with open(os.path.join(ROOT, "include", h)) as f:
+            src = f.read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(psy_[a-z0-9_]+)\s*\(", src))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -75,7 +80,7 @@ def test_rejects_bad_scenes():
     rc, msg = create([mesh_desc(v, t, samples=0)], [ok_inst])
     assert rc == -2 and "time_samples" in msg
     rc, msg = create([mesh_desc(v, t)], [_lib.InstanceDesc(3, _lib.PSY_NONE, 0, 0)])
-    assert rc == -1 and "mesh out of range" in msg
+    assert rc == -1 and "object out of range" in msg
     rc, msg = create([mesh_desc(v, t)], [_lib.InstanceDesc(0, _lib.PSY_NONE, 0, 2)], np.zeros((1, 12), np.float32))
     assert rc == -1 and "transform range" in msg
 
@@ -90,7 +95,9 @@ def test_scene_info_and_dump_on_host():
     info = s.info()
     assert info["tri_count"] == 256
     assert info["node_bytes"] == 128 and info["tri_bytes"] == 48 and info["motion_bounds_bytes"] == 96
-    assert info["device_bytes"] == info["node_count"] * 128 + 256 * 48 + 32 + 32
+    # nodes, triangle records, mesh and instance records, the instance
+    # position map and the triangle -> record map
+    assert info["device_bytes"] == info["node_count"] * 128 + 256 * 48 + 32 + 32 + 4 + 256 * 4
     d, order = s.dump_bvh(0)
     assert sorted(order.tolist()) == list(range(256))
     with pytest.raises(_lib.PsyError):
@@ -104,3 +111,50 @@ def test_null_tracer_calls_fail_cleanly():
     assert L.psy_tracer_sync(None) == -1
     assert L.psy_tracer_rays_traced(None) == 0
     assert L.psy_tracer_create(None, 16, C.byref(C.c_void_p())) == -1
+
+
+def light_scene(rects=(), spheres=(), insts=(), xforms=None):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    t = np.array([[0, 1, 2]], np.uint32)
+    L = _lib.lib()
+    md = mesh_desc(v, t)
+    mds = (_lib.MeshDesc * 1)(md)
+    ids = (_lib.InstanceDesc * max(1, len(insts)))(*insts)
+    rl = (_lib.RectLightDesc * max(1, len(rects)))(*rects)
+    sl = (_lib.SphereLightDesc * max(1, len(spheres)))(*spheres)
+    xf = np.zeros((1, 12), np.float32) if xforms is None else np.ascontiguousarray(xforms, np.float32)
+    desc = _lib.SceneDesc(1, mds, len(insts), ids, 0 if xforms is None else len(xf), xf.ctypes.data_as(_lib._fp),
+                          len(rects), rl, len(spheres), sl)
+    h = C.c_void_p()
+    rc = L.psy_scene_create(C.byref(desc), C.byref(h))
+    if rc == 0:
+        L.psy_scene_destroy(h)
+    return rc, L.psy_last_error().decode()
+
+
+def test_light_objects_validate():
+    white = _lib.Color(_lib.PSY_COLOR_XYZ, 1.0, 1.0, 1.0)
+    ident = np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]], np.float32)
+    rect = _lib.RectLightDesc(1.0, 2.0, white)
+    sph = _lib.SphereLightDesc(0.5, white)
+    mesh_inst = _lib.InstanceDesc(0, 0, 0, 0, _lib.PSY_OBJECT_MESH)
+    rect_inst = _lib.InstanceDesc(0, _lib.PSY_NONE, 0, 1, _lib.PSY_OBJECT_RECT_LIGHT)
+    sph_inst = _lib.InstanceDesc(0, _lib.PSY_NONE, 0, 1, _lib.PSY_OBJECT_SPHERE_LIGHT)
+    assert light_scene([rect], [sph], [mesh_inst, rect_inst, sph_inst], ident)[0] == 0
+    rc, msg = light_scene([rect], [], [_lib.InstanceDesc(0, _lib.PSY_NONE, 0, 0, _lib.PSY_OBJECT_RECT_LIGHT)])
+    assert rc == -1 and "need a transform" in msg
+    rc, msg = light_scene([], [sph], [_lib.InstanceDesc(1, _lib.PSY_NONE, 0, 1, _lib.PSY_OBJECT_SPHERE_LIGHT)], ident)
+    assert rc == -1 and "object out of range" in msg
+    rc, msg = light_scene([_lib.RectLightDesc(0.0, 1.0, white)], [], [rect_inst], ident)
+    assert rc == -1 and "bad size" in msg
+    rc, msg = light_scene([], [_lib.SphereLightDesc(1.0, _lib.Color(7, 1, 1, 1))], [sph_inst], ident)
+    assert rc == -1 and "bad radius or colour" in msg
+    rc, msg = light_scene([rect], [], [_lib.InstanceDesc(0, _lib.PSY_NONE, 0, 1, 9)], ident)
+    assert rc == -1 and "unknown object kind" in msg
+
+
+def test_null_renderer_calls_fail_cleanly():
+    L = _lib.lib()
+    assert L.psy_renderer_create(None, None, 0, C.byref(C.c_void_p())) == -1
+    assert L.psy_render(None, None, None, None, 0, None) == -1
+    assert L.psy_renderer_destroy(None) == 0
