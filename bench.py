@@ -46,6 +46,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run one device copy kernel of this many bytes (HBM counter calibration)")
+    ap.add_argument("--no-render", action="store_true", help="skip the rendering and serving measurements")
+    ap.add_argument("--render-only", action="store_true", help="only the rendering and serving measurements")
+    ap.add_argument("--requests", type=int, default=16, help="requests in the serving run")
+    ap.add_argument("--render-renders", type=int, default=3, help="timed plain Cornell renders")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -57,6 +61,10 @@ def main():
 
     from psychopath_amd import scenes
     from psychopath_amd.tracer import Assembly, DeviceHits, DeviceRays, Mesh, RayBatch, Tracer
+
+    if args.render_only:
+        print(json.dumps(render_bench(args)), flush=True)
+        return
 
     t0 = time.time()
     verts, tris = scenes.displaced_sphere(args.levels)
@@ -215,7 +223,141 @@ def main():
         "kernel_mrays_per_s": round(args.rays / (avg_kernel_ms * 1e-3) / 1e6, 1),
         **extra,
     }
+    del d_rays, d_hits, tracer
+    if not args.no_render:
+        out["render"] = render_bench(args)
     print(json.dumps(out), flush=True)
+
+
+def render_bench(args) -> dict:
+    """Samples/s of the plain renderer on the Cornell-box workload (512x512,
+    16 spp, depth 4) beside the oracle renderer on the host, and the served
+    request mix (Poisson arrivals at 20 req/s over four device-resident
+    scenes) against the same requests rendered one after another."""
+    import threading
+
+    import numpy as np
+
+    from psychopath_amd import render, scenes
+
+    res = {}
+    sc = scenes.cornell_box()
+    r = render.build(sc)
+    r.render(sc.camera, sc.settings)
+    times = []
+    for _ in range(args.render_renders):
+        t = time.perf_counter()
+        img, st = r.render(sc.camera, sc.settings)
+        times.append(time.perf_counter() - t)
+    paths = st["paths"]
+    res["cornell_plain"] = {
+        "workload": "Cornell box 512x512, 16 spp, depth 4, seed 0",
+        "samples_per_s": round(paths / float(np.median(times)), 1),
+        "ms_per_render": round(float(np.median(times)) * 1e3, 2),
+        "device_ms_per_render": round(st["trace_ms"], 2),
+    }
+    log(f"cornell plain: {res['cornell_plain']}")
+    if not args.no_cpu:
+        from oracle import psyref
+        small = scenes.cornell_box(128, 128, 16, 4, 0)
+        ref, info = psyref.render(small, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
+        gimg, _ = r.render(small.camera, small.settings)
+        ok = np.abs(gimg.astype(np.float64) - ref) <= 1e-3 * np.abs(ref.astype(np.float64)) + 1e-9
+        res["cornell_cpu_baseline"] = {
+            "value": round(128 * 128 * 16 / info["wall_seconds"], 1),
+            "unit": "samples/s",
+            "cores": info["threads"],
+            "kind": "port",
+            "sample": "Cornell box at 128x128, 16 spp, depth 4 (1/16 of the workload's paths), psyref renderer "
+                      "(C++ restatement of renderer.rs), buckets of 65536/spp pixels per thread",
+        }
+        res["cornell_parity_128"] = round(float(ok.all(axis=-1).mean()), 5)
+        log(f"cornell cpu: {res['cornell_cpu_baseline']['value']} samples/s; parity {res['cornell_parity_128']}")
+    del r
+
+    # ---- serving ---------------------------------------------------------------
+    t0 = time.time()
+    scene_defs = [
+        ("cornell", scenes.cornell_box()),
+        ("displaced_sphere_1m", scenes.sphere_scene(9)),
+        ("motion_instances", scenes.motion_instances()),
+        ("terrain_5m", scenes.terrain_scene()),
+    ]
+    rens = [render.build(s) for _, s in scene_defs]
+    log(f"serving scenes built in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(20)
+    sizes = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    spps = [16, 64, 256]
+    reqs = []
+    t_arr = 0.0
+    for i in range(args.requests):
+        t_arr += rng.exponential(1.0 / 20.0)
+        k = int(rng.integers(len(scene_defs)))
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        spp = spps[int(rng.integers(len(spps)))]
+        sd = scene_defs[k][1]
+        reqs.append((t_arr, k, sd.camera, dict(width=w, height=h, spp=spp, max_bounces=sd.settings["max_bounces"],
+                                               seed=i)))
+    total_paths = sum(s["width"] * s["height"] * s["spp"] for _, _, _, s in reqs)
+    # warm every renderer's graphs
+    for k, (_, sd) in enumerate(scene_defs):
+        rens[k].render(sd.camera, dict(width=64, height=64, spp=1, max_bounces=sd.settings["max_bounces"], seed=0))
+
+    # plain: one render after another
+    service = []
+    t = time.perf_counter()
+    for _, k, cam, st in reqs:
+        a = time.perf_counter()
+        rens[k].render(cam, st)
+        service.append(time.perf_counter() - a)
+    plain_s = time.perf_counter() - t
+    # the same arrivals served first-come first-served by the plain path
+    done, plain_lat = 0.0, []
+    for (ta, _, _, _), sv in zip(reqs, service):
+        done = max(done, ta) + sv
+        plain_lat.append(done - ta)
+
+    server = render.Server(rens, hbm_budget=200 * (1 << 30))
+    ids = [None] * len(reqs)
+    t_start = time.perf_counter()
+
+    def submitter():
+        for i, (ta, k, cam, st) in enumerate(reqs):
+            delay = t_start + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids[i] = server.submit(k, cam, st)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = []
+    for i in range(len(reqs)):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        _, rs = server.wait(ids[i], keep_image=False)
+        lat.append(rs["latency_ms"] / 1e3)
+    served_s = time.perf_counter() - t_start
+    th.join()
+    sst = server.stats()
+    server.close()
+    res["serving"] = {
+        "workload": f"{len(reqs)} requests, Poisson arrivals at 20 req/s, sizes {{256^2, 512^2, 1024^2, 1920x1080}}, "
+                    "spp {16, 64, 256}, over 4 device-resident scenes (Cornell box, 1M-tri displaced sphere, "
+                    "256 motion-blurred instances of a 65,536-tri mesh with 16 sphere lights, 5.2M-tri terrain)",
+        "paths": int(total_paths),
+        "served_samples_per_s": round(total_paths / served_s, 1),
+        "plain_samples_per_s": round(total_paths / plain_s, 1),
+        "speedup": round(plain_s / served_s, 3),
+        "served_latency_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+        "served_latency_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+        "plain_latency_p50_ms": round(float(np.percentile(plain_lat, 50)) * 1e3, 1),
+        "plain_latency_p99_ms": round(float(np.percentile(plain_lat, 99)) * 1e3, 1),
+        "served_seconds": round(served_s, 3),
+        "plain_seconds": round(plain_s, 3),
+        "waves": sst["waves"],
+    }
+    log(f"serving: {res['serving']}")
+    return res
 
 
 if __name__ == "__main__":

@@ -120,6 +120,52 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
 #define PSY_RENDER_NO_SORT 2u
 int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
+/* ---- request server ---------------------------------------------------
+ * Serves render requests against several renderers (one device-resident
+ * scene each) with continuous batching.  Requests queue on submission and
+ * are admitted in arrival order once the HBM budget has room for their film
+ * and their renderer a free request slot; admitted requests keep path state
+ * and film on the device.  A worker thread fills each renderer's waves with
+ * the next paths of its active requests, so requests that arrive while
+ * others render join the next wave, and the renderers' streams overlap on
+ * the device.  Each image equals what psy_render gives for the request
+ * alone.  Renderers given to a server must not be used otherwise until the
+ * server is destroyed.  There is no counterpart in the reference, which
+ * renders one scene per process (src/main.rs). */
+typedef struct psy_server psy_server;
+
+typedef struct {
+    uint64_t hbm_budget;   /* bytes for scenes, path state and films; 0 = no limit */
+    uint32_t max_active;   /* admitted requests per renderer, 1..64; 0 = 64 */
+} psy_server_config;
+
+typedef struct {
+    uint64_t paths;        /* width * height * spp */
+    uint64_t waves;        /* waves that carried paths of the request */
+    double queued_ms;      /* submission to admission */
+    double latency_ms;     /* submission to image complete on the device */
+} psy_request_stats;
+
+typedef struct {
+    uint64_t waves;        /* waves launched */
+    uint64_t paths;        /* paths rendered */
+    uint64_t queued;       /* requests waiting for admission */
+    uint64_t fixed_bytes;  /* scenes and path state */
+    uint64_t film_bytes;   /* films held (admitted, uncollected or pooled) */
+} psy_server_stats;
+
+int psy_server_create(psy_renderer* const* renderers, uint32_t count, const psy_server_config* config,
+                      psy_server** out);
+/* Waits for every submitted request to finish, then releases the server. */
+int psy_server_destroy(psy_server* server);
+/* Queues a render of `renderer` (an index into the creation list). */
+int psy_server_submit(psy_server* server, uint32_t renderer, const psy_camera_desc* camera,
+                      const psy_render_settings* settings, uint64_t* request_id);
+/* Blocks until the request is complete, copies its XYZ image to host memory
+ * (`xyz` may be NULL to discard it) and releases the request. */
+int psy_server_wait(psy_server* server, uint64_t request_id, float* xyz, psy_request_stats* stats);
+int psy_server_get_stats(psy_server* server, psy_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

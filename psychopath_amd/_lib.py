@@ -224,6 +224,30 @@ PSY_RENDER_NO_GRAPHS = 1
 PSY_RENDER_NO_SORT = 2
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_uint32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("waves", C.c_uint64), ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("waves", C.c_uint64), ("paths", C.c_uint64), ("queued", C.c_uint64), ("fixed_bytes", C.c_uint64),
+                ("film_bytes", C.c_uint64)]
+
+
+_SIGNATURES.update({
+    "psy_server_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(ServerConfig),
+                                    C.POINTER(C.c_void_p)]),
+    "psy_server_destroy": (C.c_int, [C.c_void_p]),
+    "psy_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(CameraDesc), C.POINTER(RenderSettings),
+                                    C.POINTER(C.c_uint64)]),
+    "psy_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(RequestStats)]),
+    "psy_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+})
+
+
 class PsyError(RuntimeError):
     """A psy_rt call returned an error code."""
 

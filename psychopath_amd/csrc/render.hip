@@ -33,98 +33,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/psy_render.h"
-#include "api_internal.h"
-#include "render_device.h"
+#include "render_internal.h"
 
 using namespace psy;
 using namespace psy::api;
 using namespace psy::rd;
+using namespace psy::render;
 
 namespace {
-
-constexpr int KBLOCK = 256;
-constexpr uint32_t MAX_SEGS = 64;  // requests interleaved in one wave
-constexpr uint32_t NUM_BINS = 5;   // miss, light, Lambert, GGX, emit
-
-// One request's render parameters (device copy).
-struct ReqParams {
-    uint32_t width, height, spp, max_bounces, seed;
-    float cmpx, cmpy, x_extent, y_extent, tfov, aperture, focus;
-    uint32_t cam_xf_count;
-    uint32_t pad;
-    const GpuXform* cam_xf;
-    float* film;
-};
-
-// A run of consecutive path slots of a wave taken by one request: slots
-// [slot0, slot0 + count) carry the request's global paths g0, g0 + 1, ...
-// (path g = pixel * spp + sample).
-struct WaveSeg {
-    uint32_t req;
-    uint32_t slot0;
-    uint32_t count;
-    uint32_t pad;
-    uint64_t g0;
-};
-struct WavePlan {
-    uint32_t nseg;
-    uint32_t total;
-    uint32_t pad[2];
-    WaveSeg seg[MAX_SEGS];
-};
-
-// Scene data the shading kernels read.
-struct ShadeScene {
-    const GpuInstance* instances;
-    const uint32_t* inst_pos;
-    const GpuMesh* meshes;
-    const uint8_t* tris;
-    const uint32_t* tri_rec;
-    const GpuXform* xforms;
-    const GpuXform* inv_static;
-    const GpuRectLight* rects;
-    const GpuSphereLight* spheres;
-    const GpuLightNode* lnodes;
-    const GpuBox* lboxes;
-    const uint32_t* llist;
-    uint32_t has_ltree;
-    uint32_t shader_count;
-    const GpuClosure* shaders;
-    const GpuDistantLight* distant;
-    uint32_t distant_count;
-    uint32_t pad;
-    GpuColor background;
-    const HaltonDim* hdims;
-    const uint16_t* hperms;
-};
-
-// Path state, one slot per path of the wave.
-struct Paths {
-    uint32_t* req;
-    uint32_t* pix;
-    uint32_t* sample;
-    uint32_t* seed;
-    uint32_t* bounces;
-    float* wl;
-    float* time;
-    float* csp;  // closure_sample_pdf
-    float4* color;
-    float4* atten;
-    float4* pending;
-};
-
-// A ray queue: the path slot of each ray and the rays as SoA.
-struct Queue {
-    uint32_t* path;
-    float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
-    uint32_t* count;
-};
-struct Hits {
-    uint8_t* kind;
-    float *t, *b0, *b1, *b2;
-    uint32_t *prim, *inst;
-};
 
 __device__ __forceinline__ const WaveSeg& find_seg(const WavePlan& p, uint32_t i) {
     uint32_t s = 0;
@@ -693,84 +609,17 @@ void halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perms) {
     }
 }
 
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { (void)hipFree(p); }
-};
-
 }  // namespace
 
-struct psy_renderer {
-    psy_scene* scene = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint64_t capacity = 0;
-    uint32_t cu_count = 256;
-    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
-    bool use_graphs = true;
-    bool sort_materials = true;
-
-    std::vector<DevBuf*> bufs;
-    ShadeScene sc{};
-    Paths P{};
-    Queue qa[2]{}, qs[2]{};
-    Hits ha{}, hs{};
-    uint32_t* order = nullptr;
-    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins
-    uint32_t* wave_idx = nullptr;
-    unsigned int* work = nullptr;
-    WavePlan* plans = nullptr;   // device plan array
-    uint32_t plan_cap = 0;
-    ReqParams* reqs = nullptr;
-    uint32_t req_cap = 0;
-    GpuXform* cam = nullptr;
-    uint32_t cam_cap = 0;
-
-    // Captured wave graphs, keyed by the number of bounce steps.
-    std::map<uint32_t, hipGraphExec_t> graphs;
-
-    ~psy_renderer() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
-        for (DevBuf* b : bufs) delete b;
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    template <typename T>
-    int alloc(T** out, uint64_t count) {
-        DevBuf* b = new (std::nothrow) DevBuf();
-        if (!b) return fail(PSY_E_NOMEM, "out of host memory");
-        hipError_t e = hipMalloc(&b->p, std::max<uint64_t>(1, count) * sizeof(T));
-        if (e != hipSuccess) {
-            delete b;
-            return fail(PSY_E_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
-        }
-        bufs.push_back(b);
-        *out = static_cast<T*>(b->p);
-        return PSY_OK;
-    }
-    void release(void* p) {
-        for (size_t i = 0; i < bufs.size(); i++)
-            if (bufs[i]->p == p) {
-                delete bufs[i];
-                bufs.erase(bufs.begin() + i);
-                return;
-            }
-    }
-
-    int alloc_queue(Queue& q) {
-        int r = alloc(&q.path, capacity);
-        float** f[8] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.time, &q.max_t};
-        for (int k = 0; k < 8 && !r; k++) r = alloc(f[k], capacity);
-        return r;
-    }
-};
-
-namespace {
+namespace psy {
+namespace render {
 
 uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + KBLOCK - 1) / KBLOCK, 8192)); }
+
+}  // namespace render
+}  // namespace psy
+
+namespace {
 
 TraceArgs queue_args(const psy_renderer* r, const Queue& q, const Hits& h, bool occlusion) {
     TraceArgs a = scene_args(r->scene);
@@ -834,6 +683,63 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
     return PSY_OK;
 }
 
+}  // namespace
+
+namespace psy {
+namespace render {
+
+void drop_graphs(psy_renderer* r) {
+    for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
+    r->graphs.clear();
+}
+
+int launch_set_wave(psy_renderer* r, uint32_t value) {
+    hipLaunchKernelGGL(wave_begin_kernel, dim3(1), dim3(1), 0, r->stream, r->wave_idx, value);
+    PSY_HIP(hipGetLastError());
+    return PSY_OK;
+}
+
+int launch_zero(float* p, uint64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_film_kernel, dim3(grid_for(n)), dim3(KBLOCK), 0, s, p, n);
+    PSY_HIP(hipGetLastError());
+    return PSY_OK;
+}
+
+// Camera::new (camera.rs:22-72) and the image plane of Renderer::render_job
+// (renderer.rs:214-231).
+int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp) {
+    if (!cam || !st) return fail(PSY_E_INVALID, "NULL argument");
+    if (st->width == 0 || st->height == 0 || st->spp == 0) return fail(PSY_E_INVALID, "empty image or zero spp");
+    if (st->width > 65535 || st->height > 65535) return fail(PSY_E_RANGE, "width and height must be below 65536");
+    const uint64_t pixels = (uint64_t)st->width * st->height;
+    if (pixels * st->spp >= (1ull << 48)) return fail(PSY_E_RANGE, "image too large");
+    if (st->max_bounces > 64) return fail(PSY_E_RANGE, "max_bounces must be at most 64");
+    if (cam->xform_count == 0 || !cam->xforms) return fail(PSY_E_INVALID, "camera needs at least one transform");
+    if (cam->xform_count > 64) return fail(PSY_E_RANGE, "at most 64 camera transforms");
+    std::memset(&rp, 0, sizeof(rp));
+    rp.width = st->width;
+    rp.height = st->height;
+    rp.spp = st->spp;
+    rp.max_bounces = st->max_bounces;
+    rp.seed = st->seed;
+    rp.cmpx = 1.0f / (float)st->width;
+    rp.cmpy = 1.0f / (float)st->height;
+    const float min_x = -1.0f, max_x = 1.0f;
+    const float max_y = (float)st->height / (float)st->width, min_y = -max_y;
+    rp.x_extent = max_x - min_x;
+    rp.y_extent = max_y - min_y;
+    const float half = cam->fov / 2.0f;
+    rp.tfov = std::sin(half) / std::cos(half);
+    rp.aperture = cam->aperture_radius;
+    rp.focus = cam->focus_distance;
+    if (rp.focus == 0.0f) {
+        rp.aperture = 0.0f;
+        rp.focus = 1.0f;
+    }
+    rp.cam_xf_count = cam->xform_count;
+    return PSY_OK;
+}
+
 int run_wave(psy_renderer* r, uint32_t steps) {
     if (!r->use_graphs) return enqueue_wave(r, steps);
     auto it = r->graphs.find(steps);
@@ -856,6 +762,11 @@ int run_wave(psy_renderer* r, uint32_t steps) {
     PSY_HIP(hipGraphLaunch(it->second, r->stream));
     return PSY_OK;
 }
+
+}  // namespace render
+}  // namespace psy
+
+namespace {
 
 int color_check(const psy_color& c) {
     if (c.kind > PSY_COLOR_TEMPERATURE || !std::isfinite(c.a) || !std::isfinite(c.b) || !std::isfinite(c.c))
@@ -947,7 +858,13 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!rc) rc = r->alloc(&r->counts, 16);
     if (!rc) rc = r->alloc(&r->wave_idx, 1);
     if (!rc) rc = r->alloc(&r->work, 64);
+    if (!rc) rc = r->alloc(&r->reqs, MAX_SEGS);
+    if (!rc) rc = r->alloc(&r->plans, 2);
+    if (!rc) rc = r->alloc(&r->cam, (uint64_t)MAX_SEGS * 64);
     if (rc) return bail(rc);
+    r->req_cap = MAX_SEGS;
+    r->plan_cap = 2;
+    r->cam_cap = 64;
     r->qa[0].count = r->counts + 0;
     r->qa[1].count = r->counts + 1;
     r->qs[0].count = r->counts + 2;
@@ -1008,46 +925,12 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     auto t0 = std::chrono::steady_clock::now();
     if (!r || !cam || !st || !xyz) return fail(PSY_E_INVALID, "NULL argument");
     if (mem != PSY_MEM_HOST && mem != PSY_MEM_DEVICE) return fail(PSY_E_INVALID, "mem must be PSY_MEM_HOST or PSY_MEM_DEVICE");
-    if (st->width == 0 || st->height == 0 || st->spp == 0) return fail(PSY_E_INVALID, "empty image or zero spp");
-    if (st->width > 65535 || st->height > 65535) return fail(PSY_E_RANGE, "width and height must be below 65536");
+    ReqParams rp;
+    int rc = make_params(cam, st, rp);
+    if (rc) return rc;
     const uint64_t pixels = (uint64_t)st->width * st->height;
     const uint64_t total = pixels * st->spp;
-    if (pixels > 0xffffffffull || total >= (1ull << 48)) return fail(PSY_E_RANGE, "image too large");
-    if (st->max_bounces > 64) return fail(PSY_E_RANGE, "max_bounces must be at most 64");
-    if (cam->xform_count == 0 || !cam->xforms) return fail(PSY_E_INVALID, "camera needs at least one transform");
-    if (cam->xform_count > 64) return fail(PSY_E_RANGE, "at most 64 camera transforms");
 
-    // Camera::new (camera.rs:22-72): tan(fov/2), focus 0 disables the lens.
-    ReqParams rp{};
-    rp.width = st->width;
-    rp.height = st->height;
-    rp.spp = st->spp;
-    rp.max_bounces = st->max_bounces;
-    rp.seed = st->seed;
-    rp.cmpx = 1.0f / (float)st->width;
-    rp.cmpy = 1.0f / (float)st->height;
-    const float min_x = -1.0f, max_x = 1.0f;
-    const float max_y = (float)st->height / (float)st->width, min_y = -max_y;
-    rp.x_extent = max_x - min_x;
-    rp.y_extent = max_y - min_y;
-    const float half = cam->fov / 2.0f;
-    rp.tfov = std::sin(half) / std::cos(half);
-    rp.aperture = cam->aperture_radius;
-    rp.focus = cam->focus_distance;
-    if (rp.focus == 0.0f) {
-        rp.aperture = 0.0f;
-        rp.focus = 1.0f;
-    }
-    rp.cam_xf_count = cam->xform_count;
-
-    int rc = PSY_OK;
-    if (r->cam_cap < cam->xform_count) {
-        if (r->cam) r->release(r->cam);
-        r->cam = nullptr;
-        rc = r->alloc(&r->cam, 64);
-        if (rc) return rc;
-        r->cam_cap = 64;
-    }
     std::vector<GpuXform> cx(cam->xform_count);
     for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(cx[i].r, cam->xforms + 12ull * i, 48);
     rp.cam_xf = r->cam;
@@ -1060,18 +943,12 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
         film = film_dev;
     }
     rp.film = film;
-    if (!r->reqs) {
-        rc = r->alloc(&r->reqs, 1);
-        if (rc) return rc;
-        r->req_cap = 1;
-    }
 
     // Wave plans: consecutive path ranges of `capacity` slots.
     const uint64_t waves = (total + r->capacity - 1) / r->capacity;
     if (waves > r->plan_cap) {
         // Captured graphs hold the plan array's address.
-        for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
-        r->graphs.clear();
+        drop_graphs(r);
         if (r->plans) r->release(r->plans);
         r->plans = nullptr;
         rc = r->alloc(&r->plans, waves);
@@ -1090,10 +967,9 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     PSY_HIP(hipMemcpyAsync(r->cam, cx.data(), cx.size() * sizeof(GpuXform), hipMemcpyHostToDevice, s));
     PSY_HIP(hipMemcpyAsync(r->reqs, &rp, sizeof(rp), hipMemcpyHostToDevice, s));
     PSY_HIP(hipMemcpyAsync(r->plans, plans.data(), waves * sizeof(WavePlan), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(zero_film_kernel, dim3(grid_for(pixels * 3)), dim3(KBLOCK), 0, s, film, pixels * 3);
-    PSY_HIP(hipGetLastError());
-    hipLaunchKernelGGL(wave_begin_kernel, dim3(1), dim3(1), 0, s, r->wave_idx, 0u);
-    PSY_HIP(hipGetLastError());
+    rc = launch_zero(film, pixels * 3, s);
+    if (!rc) rc = launch_set_wave(r, 0u);
+    if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
     for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
     PSY_HIP(hipEventRecord(r->ev1, s));

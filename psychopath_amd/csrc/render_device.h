@@ -2,13 +2,16 @@
 // lights, surface closures and the light tree.  Every function keeps the
 // reference's operation order and precision (f32, with f64 where the
 // reference computes in f64), so a path computes the same values as the
-// reference's CPU path; transcendental functions are the device library's.
+// reference's CPU path.  Transcendental functions are the device library's,
+// except asinf/acosf, whose last-bit differences the light tree's estimates
+// amplify: those follow the host library's rounding (libm_f32.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "../../include/psy_render.h"
 #include "layout.h"
+#include "libm_f32.h"
 #include "psymath.h"
 #include "trace_device.h"
 
@@ -831,10 +834,10 @@ RD float sphere_lambert(float nlcos, float rcos) {
     float ysin = rcos / nlsin;
     float ycos2 = 1.0f - (ysin * ysin);
     float ycos = sqrtf(ycos2);
-    float g = ((((-2.0f * nlsin) * rcos) * ycos) + H_PI_F - asinf(ysin)) + (ysin * ycos);
-    float h = nlcos * ((ycos * sqrtf(rsin2 - ycos2)) + (rsin2 * asinf(ycos / rsin)));
-    float nl = acosf(nlcos);
-    float r = acosf(rcos);
+    float g = ((((-2.0f * nlsin) * rcos) * ycos) + H_PI_F - psy_asinf(ysin)) + (ysin * ycos);
+    float h = nlcos * ((ycos * sqrtf(rsin2 - ycos2)) + (rsin2 * psy_asinf(ycos / rsin)));
+    float nl = psy_acosf(nlcos);
+    float r = psy_acosf(rcos);
     if (nl < (H_PI_F - r)) return nlcos * rsin2;
     if (nl < H_PI_F) return ((nlcos * rsin2) + g) - h;
     if (nl < (H_PI_F + r)) return (g + h) * INV_PI_F;
@@ -861,7 +864,7 @@ RD float closure_estimate_sphere_light(const GpuClosure& c, V3 inc, V3 to_light_
         V3 nn = dot(nor, inc) < 0.0f ? normalize(nor) : neg3(normalize(nor));
         V3 aa = neg3(normalize(inc));
         V3 bb = normalize(to_light_center);
-        float theta = acosf(cos_theta_max);
+        float theta = psy_acosf(cos_theta_max);
         V3 hh = normalize(add(aa, bb));
         float nh = clampf(dot(nn, hh), -1.0f, 1.0f);
         float fac = ggx_d(nh, fminn(1.0f, sqrtf(c.roughness) + ((2.0f * theta) / PI_F)));

@@ -1,0 +1,216 @@
+// Internal state of the wavefront renderer, shared by the single-render
+// path (render.hip) and the request server (serve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/psy_render.h"
+#include "api_internal.h"
+#include "render_device.h"
+
+namespace psy {
+namespace render {
+
+using rd::HaltonDim;
+
+constexpr int KBLOCK = 256;
+constexpr uint32_t MAX_SEGS = 64;  // requests interleaved in one wave
+constexpr uint32_t NUM_BINS = 5;   // miss, light, Lambert, GGX, emit
+
+// One request's render parameters (device copy).
+struct ReqParams {
+    uint32_t width, height, spp, max_bounces, seed;
+    float cmpx, cmpy, x_extent, y_extent, tfov, aperture, focus;
+    uint32_t cam_xf_count;
+    uint32_t pad;
+    const GpuXform* cam_xf;
+    float* film;
+};
+
+// A run of consecutive path slots of a wave taken by one request: slots
+// [slot0, slot0 + count) carry the request's global paths g0, g0 + 1, ...
+// (path g = pixel * spp + sample).
+struct WaveSeg {
+    uint32_t req;
+    uint32_t slot0;
+    uint32_t count;
+    uint32_t pad;
+    uint64_t g0;
+};
+struct WavePlan {
+    uint32_t nseg;
+    uint32_t total;
+    uint32_t pad[2];
+    WaveSeg seg[MAX_SEGS];
+};
+
+// Scene data the shading kernels read.
+struct ShadeScene {
+    const GpuInstance* instances;
+    const uint32_t* inst_pos;
+    const GpuMesh* meshes;
+    const uint8_t* tris;
+    const uint32_t* tri_rec;
+    const GpuXform* xforms;
+    const GpuXform* inv_static;
+    const GpuRectLight* rects;
+    const GpuSphereLight* spheres;
+    const GpuLightNode* lnodes;
+    const GpuBox* lboxes;
+    const uint32_t* llist;
+    uint32_t has_ltree;
+    uint32_t shader_count;
+    const GpuClosure* shaders;
+    const GpuDistantLight* distant;
+    uint32_t distant_count;
+    uint32_t pad;
+    GpuColor background;
+    const HaltonDim* hdims;
+    const uint16_t* hperms;
+};
+
+// Path state, one slot per path of the wave.
+struct Paths {
+    uint32_t* req;
+    uint32_t* pix;
+    uint32_t* sample;
+    uint32_t* seed;
+    uint32_t* bounces;
+    float* wl;
+    float* time;
+    float* csp;  // closure_sample_pdf
+    float4* color;
+    float4* atten;
+    float4* pending;
+};
+
+// A ray queue: the path slot of each ray and the rays as SoA.
+struct Queue {
+    uint32_t* path;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
+    uint32_t* count;
+};
+struct Hits {
+    uint8_t* kind;
+    float *t, *b0, *b1, *b2;
+    uint32_t *prim, *inst;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    uint64_t bytes = 0;
+    ~DevBuf() { (void)hipFree(p); }
+};
+
+}  // namespace render
+}  // namespace psy
+
+struct psy_renderer {
+    using ShadeScene = psy::render::ShadeScene;
+    using Paths = psy::render::Paths;
+    using Queue = psy::render::Queue;
+    using Hits = psy::render::Hits;
+    using DevBuf = psy::render::DevBuf;
+    using WavePlan = psy::render::WavePlan;
+    using ReqParams = psy::render::ReqParams;
+
+    psy_scene* scene = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t capacity = 0;
+    uint32_t cu_count = 256;
+    psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
+    bool use_graphs = true;
+    bool sort_materials = true;
+
+    std::vector<DevBuf*> bufs;
+    ShadeScene sc{};
+    Paths P{};
+    Queue qa[2]{}, qs[2]{};
+    Hits ha{}, hs{};
+    uint32_t* order = nullptr;
+    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins
+    uint32_t* wave_idx = nullptr;
+    unsigned int* work = nullptr;
+    WavePlan* plans = nullptr;   // device plan array
+    uint32_t plan_cap = 0;
+    ReqParams* reqs = nullptr;
+    uint32_t req_cap = 0;
+    psy::GpuXform* cam = nullptr;
+    uint32_t cam_cap = 0;
+
+    // Captured wave graphs, keyed by the number of bounce steps.
+    std::map<uint32_t, hipGraphExec_t> graphs;
+
+    ~psy_renderer() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (DevBuf* b : bufs) delete b;
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint64_t device_bytes() const {
+        uint64_t n = 0;
+        for (const DevBuf* b : bufs) n += b->bytes;
+        return n;
+    }
+
+    template <typename T>
+    int alloc(T** out, uint64_t count) {
+        DevBuf* b = new (std::nothrow) DevBuf();
+        if (!b) return psy::api::fail(PSY_E_NOMEM, "out of host memory");
+        b->bytes = std::max<uint64_t>(1, count) * sizeof(T);
+        hipError_t e = hipMalloc(&b->p, b->bytes);
+        if (e != hipSuccess) {
+            delete b;
+            return psy::api::fail(PSY_E_NOMEM, std::string("device allocation: ") + hipGetErrorString(e));
+        }
+        bufs.push_back(b);
+        *out = static_cast<T*>(b->p);
+        return PSY_OK;
+    }
+    void release(void* p) {
+        for (size_t i = 0; i < bufs.size(); i++)
+            if (bufs[i]->p == p) {
+                delete bufs[i];
+                bufs.erase(bufs.begin() + i);
+                return;
+            }
+    }
+
+    int alloc_queue(Queue& q) {
+        int r = alloc(&q.path, capacity);
+        float** f[8] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.time, &q.max_t};
+        for (int k = 0; k < 8 && !r; k++) r = alloc(f[k], capacity);
+        return r;
+    }
+};
+
+
+namespace psy {
+namespace render {
+
+// Grid of the queue kernels for `n` items (grid-stride loops).
+uint32_t grid_for(uint64_t n);
+// Enqueues one wave (camera, bounce steps, film) on the renderer's stream,
+// replaying or capturing its hipGraph; the wave's plan is
+// plans[*wave_idx], and *wave_idx is incremented at the end.
+int run_wave(psy_renderer* r, uint32_t steps);
+// Kernels the server launches around waves.
+int launch_set_wave(psy_renderer* r, uint32_t value);
+int launch_zero(float* p, uint64_t n, hipStream_t s);
+// Fills a request's parameters from camera and settings (validated).
+int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp);
+// Drops captured graphs (after a buffer they name was replaced).
+void drop_graphs(psy_renderer* r);
+
+}  // namespace render
+}  // namespace psy

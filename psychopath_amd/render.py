@@ -62,16 +62,25 @@ class Renderer:
         t = _lib.TraceTuning(**{**self.tuning(), **kw})
         check("psy_renderer_set_tuning", lib().psy_renderer_set_tuning(self.handle, C.byref(t)))
 
+    @staticmethod
+    def _camera(camera: dict):
+        xf = np.ascontiguousarray(camera["xforms"], np.float32).reshape(-1, 12)
+        cam = _lib.CameraDesc(xf.shape[0], xf.ctypes.data_as(_lib._fp), float(camera["fov"]),
+                              float(camera.get("aperture", 0.0)), float(camera.get("focus", 1.0)))
+        return cam, xf
+
+    @staticmethod
+    def _settings(settings: dict):
+        return _lib.RenderSettings(int(settings["width"]), int(settings["height"]), int(settings["spp"]),
+                                   int(settings.get("max_bounces", 2)), int(settings.get("seed", 0)))
+
     def render(self, camera: dict, settings: dict, device_out=None) -> tuple[np.ndarray | None, dict]:
         """Renders one image.  `camera`: xforms [N,12] (camera to world), fov
         (radians), aperture, focus.  `settings`: width, height, spp,
         max_bounces, seed.  With `device_out` (a device pointer of
         width*height*3 floats) the image stays on the device."""
-        xf = np.ascontiguousarray(camera["xforms"], np.float32).reshape(-1, 12)
-        cam = _lib.CameraDesc(xf.shape[0], xf.ctypes.data_as(_lib._fp), float(camera["fov"]),
-                              float(camera.get("aperture", 0.0)), float(camera.get("focus", 1.0)))
-        st = _lib.RenderSettings(int(settings["width"]), int(settings["height"]), int(settings["spp"]),
-                                 int(settings.get("max_bounces", 2)), int(settings.get("seed", 0)))
+        cam, _xf = self._camera(camera)
+        st = self._settings(settings)
         stats = _lib.RenderStats()
         if device_out is None:
             img = np.zeros((st.height, st.width, 3), np.float32)
@@ -99,3 +108,50 @@ def build(rs, max_paths: int = 0) -> Renderer:
         asm.add_instance(inst["index"], inst.get("xforms"), shader, kind)
     scene = asm.build()
     return Renderer(scene, rs.shaders, rs.background, rs.distant_lights, max_paths)
+
+
+class Server:
+    """Render request server over several renderers (psy_server_*):
+    continuous batching of requests into shared waves, HBM-budget
+    admission, one stream per scene."""
+
+    def __init__(self, renderers, hbm_budget: int = 0, max_active: int = 0):
+        self.renderers = list(renderers)  # keeps them alive
+        arr = (C.c_void_p * len(self.renderers))(*[r.handle.value for r in self.renderers])
+        cfg = _lib.ServerConfig(int(hbm_budget), int(max_active))
+        h = C.c_void_p()
+        check("psy_server_create", lib().psy_server_create(arr, len(self.renderers), C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._settings = {}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().psy_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        if _lib._lib is not None:
+            self.close()
+
+    def submit(self, renderer: int, camera: dict, settings: dict) -> int:
+        cam, _xf = Renderer._camera(camera)
+        st = Renderer._settings(settings)
+        rid = C.c_uint64()
+        check("psy_server_submit", lib().psy_server_submit(self.handle, int(renderer), C.byref(cam), C.byref(st),
+                                                            C.byref(rid)))
+        self._settings[rid.value] = (st.height, st.width)
+        return rid.value
+
+    def wait(self, request_id: int, keep_image: bool = True) -> tuple[np.ndarray | None, dict]:
+        h, w = self._settings.pop(request_id)
+        img = np.zeros((h, w, 3), np.float32) if keep_image else None
+        rs = _lib.RequestStats()
+        check("psy_server_wait", lib().psy_server_wait(self.handle, int(request_id),
+                                                        None if img is None else img.ctypes.data_as(C.c_void_p),
+                                                        C.byref(rs)))
+        return img, {f: getattr(rs, f) for f, _ in _lib.RequestStats._fields_}
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        check("psy_server_get_stats", lib().psy_server_get_stats(self.handle, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _lib.ServerStats._fields_}

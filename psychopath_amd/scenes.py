@@ -408,3 +408,94 @@ def _sphere_point(u, v):
     r = np.sqrt(max(0.0, 1.0 - z * z))
     phi = 2.0 * np.pi * v
     return np.array([r * np.cos(phi), r * np.sin(phi), z])
+
+
+def value_noise2(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
+    """Seeded 2D value noise in [-1, 1]: hashed lattice values blended with
+    a smoothstep."""
+    x0 = np.floor(x)
+    y0 = np.floor(y)
+    fx = x - x0
+    fy = y - y0
+    ix = x0.astype(np.int64)
+    iy = y0.astype(np.int64)
+    ox, oy = ix.min(), iy.min()
+    w, h = ix.max() - ox + 2, iy.max() - oy + 2
+    lattice = uniform01(seed, int(w * h)).astype(np.float64).reshape(h, w) * 2.0 - 1.0
+    gx, gy = ix - ox, iy - oy
+    sx = fx * fx * (3.0 - 2.0 * fx)
+    sy = fy * fy * (3.0 - 2.0 * fy)
+    a = lattice[gy, gx]
+    b = lattice[gy, gx + 1]
+    c = lattice[gy + 1, gx]
+    d = lattice[gy + 1, gx + 1]
+    return (a * (1 - sx) + b * sx) * (1 - sy) + (c * (1 - sx) + d * sx) * sy
+
+
+def heightfield(nx: int = 2048, ny: int = 1280, seed: int = 3, spacing: float = 0.01,
+                amplitude: float = 1.5) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded terrain: (nx+1) x (ny+1) vertices on a grid of `spacing`,
+    heights from four octaves of Perlin noise, two triangles per quad
+    (2 * nx * ny triangles)."""
+    xs = np.arange(nx + 1, dtype=np.float64) * spacing
+    ys = np.arange(ny + 1, dtype=np.float64) * spacing
+    gx, gy = np.meshgrid(xs, ys)
+    h = np.zeros(gx.shape)
+    freq, amp = 0.35, 1.0
+    for octave in range(4):
+        h += amp * value_noise2(gx * freq, gy * freq, seed * 16 + octave)
+        freq *= 2.0
+        amp *= 0.5
+    v = np.stack([gx.ravel(), gy.ravel(), h.ravel() * amplitude], axis=1).astype(np.float32) + np.float32(0.0)
+    i = np.arange(nx, dtype=np.uint32)
+    j = np.arange(ny, dtype=np.uint32)
+    ii, jj = np.meshgrid(i, j)
+    a = (jj * (nx + 1) + ii).ravel()
+    b = a + 1
+    c = a + (nx + 1)
+    d = c + 1
+    f = np.empty((2 * a.size, 3), np.uint32)
+    f[0::2] = np.stack([a, b, d], axis=1)
+    f[1::2] = np.stack([a, d, c], axis=1)
+    return v, f
+
+
+def terrain_scene(nx: int = 2048, ny: int = 1280, width: int = 1024, height: int = 1024, spp: int = 16,
+                  max_bounces: int = 4, seed: int = 0) -> RenderScene:
+    """The serving workload's terrain: a 2 * nx * ny triangle heightfield,
+    Lambertian, lit by a distant disk light (the sun) under a sky-coloured
+    background, seen from above at a grazing angle."""
+    sc = RenderScene()
+    v, f = heightfield(nx, ny)
+    sc.meshes.append((v, f))
+    sc.shaders = [(SHADER_LAMBERT, rec709(0.35, 0.3, 0.22), 0.0, 0.0)]
+    sc.instances.append(dict(kind="mesh", index=0, shader=0, xforms=None))
+    sc.background = rec709(0.25, 0.35, 0.6)
+    sc.distant_lights.append((0.02, (-0.4, 0.3, -0.6), (COLOR_TEMPERATURE, 5800.0, 100.0, 0.0)))
+    lo, hi = v.min(axis=0).astype(np.float64), v.max(axis=0).astype(np.float64)
+    center = (lo + hi) / 2
+    ext = hi - lo
+    cam = center + np.array([0.0, -0.75 * ext[1], 0.35 * ext[0]])
+    sc.camera = dict(xforms=look_at(cam, center)[None], fov=float(np.float32(np.radians(55.0))), aperture=0.0,
+                     focus=1.0)
+    sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
+    return sc
+
+
+def sphere_scene(levels: int = 9, width: int = 512, height: int = 512, spp: int = 16, max_bounces: int = 4,
+                 seed: int = 0) -> RenderScene:
+    """The ray-cast mesh (1M-triangle displaced sphere) as a render scene:
+    Lambertian, under a 2 x 2 rectangle light, dim background."""
+    sc = RenderScene()
+    v, f = displaced_sphere(levels)
+    sc.meshes.append((v, f))
+    sc.shaders = [(SHADER_LAMBERT, rec709(0.75, 0.72, 0.7), 0.0, 0.0)]
+    sc.instances.append(dict(kind="mesh", index=0, shader=0, xforms=None))
+    sc.background = rec709(0.03, 0.03, 0.04)
+    sc.rect_lights.append((2.0, 2.0, rec709(40.0, 38.0, 33.0)))
+    sc.instances.append(dict(kind="rect", index=0, shader=None,
+                             xforms=np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, -2.5]], np.float32)))
+    sc.camera = dict(xforms=look_at((0.0, -3.2, 1.2), (0.0, 0.0, 0.0))[None], fov=float(np.float32(np.radians(45.0))),
+                     aperture=0.0, focus=1.0)
+    sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
+    return sc

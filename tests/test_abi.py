@@ -158,3 +158,11 @@ def test_null_renderer_calls_fail_cleanly():
     assert L.psy_renderer_create(None, None, 0, C.byref(C.c_void_p())) == -1
     assert L.psy_render(None, None, None, None, 0, None) == -1
     assert L.psy_renderer_destroy(None) == 0
+
+
+def test_null_server_calls_fail_cleanly():
+    L = _lib.lib()
+    assert L.psy_server_create(None, 0, None, C.byref(C.c_void_p())) == -1
+    assert L.psy_server_submit(None, 0, None, None, None) == -1
+    assert L.psy_server_wait(None, 1, None, None) == -1
+    assert L.psy_server_destroy(None) == 0

@@ -116,3 +116,54 @@ def test_full_resolution_cornell_properties():
     r.set_options(graphs=False)
     img3, _ = r.render(sc.camera, sc.settings)
     assert np.array_equal(img, img3)
+
+
+def test_served_requests_match_renders_alone():
+    """Requests of different sizes, spp, bounce limits and scenes interleaved
+    in shared waves give the images their plain renders give."""
+    cornell = scenes.cornell_box(64, 64, 4, 4, 0)
+    motion = scenes.motion_instances(grid=3, levels=3, lights=3, width=48, height=32, spp=4, max_bounces=3)
+    ra = render.build(cornell, max_paths=5000)
+    rb = render.build(motion, max_paths=5000)
+    reqs = [
+        (0, cornell.camera, dict(width=64, height=64, spp=4, max_bounces=4, seed=0)),
+        (1, motion.camera, dict(width=48, height=32, spp=4, max_bounces=3, seed=0)),
+        (0, cornell.camera, dict(width=40, height=30, spp=9, max_bounces=2, seed=3)),
+        (0, cornell.camera, dict(width=17, height=13, spp=1, max_bounces=5, seed=1)),
+        (1, motion.camera, dict(width=33, height=21, spp=7, max_bounces=6, seed=2)),
+    ]
+    alone = [(ra if k == 0 else rb).render(cam, st)[0] for k, cam, st in reqs]
+    server = render.Server([ra, rb])
+    ids = [server.submit(k, cam, st) for k, cam, st in reqs]
+    waves = []
+    for rid, ref in zip(ids, alone):
+        img, stats = server.wait(rid)
+        waves.append(stats["waves"])
+        assert np.array_equal(img, ref)
+        assert stats["latency_ms"] >= stats["queued_ms"] >= 0
+    st = server.stats()
+    server.close()
+    # the small path pools force the larger requests over several waves
+    assert max(waves) > 1 and st["paths"] == sum(s["width"] * s["height"] * s["spp"] for _, _, s in reqs)
+
+
+def test_server_budget_admission():
+    """A budget that fits one film at a time still completes every request,
+    in order, with the same images; a request that can never fit fails."""
+    cornell = scenes.cornell_box(32, 32, 2, 2, 0)
+    r = render.build(cornell, max_paths=4096)
+    settings = [dict(width=32, height=32, spp=2, max_bounces=2, seed=s) for s in range(4)]
+    refs = [r.render(cornell.camera, st)[0] for st in settings]
+    probe = render.Server([r])
+    fixed = probe.stats()["fixed_bytes"]
+    probe.close()
+    film = 32 * 32 * 3 * 4
+    server = render.Server([r], hbm_budget=fixed + film + film // 2)
+    ids = [server.submit(0, cornell.camera, st) for st in settings]
+    for rid, ref in zip(ids, refs):
+        img, _ = server.wait(rid)
+        assert np.array_equal(img, ref)
+    from psychopath_amd._lib import PsyError
+    with pytest.raises(PsyError):
+        server.submit(0, cornell.camera, dict(width=256, height=256, spp=1, max_bounces=1, seed=0))
+    server.close()
