@@ -1,11 +1,16 @@
-// Single-precision arcsine and arccosine with the rounding of the host C
-// library the reference links (Rust's f32::asin / f32::acos call the
-// platform libm): Sun fdlibm's float algorithms (e_asinf.c with Moshier's
-// Cephes polynomial, e_acosf.c), which agree bit for bit with the host's
-// asinf/acosf on every input in [-1, 1].  The device library's own asinf
-// and acosf differ in the last bit on many inputs, and the light tree's
-// closure estimates (sphere_lambert) cancel enough digits to turn that into
-// different light choices.  Compile without FMA contraction.
+// Single-precision sine, cosine, arcsine and arccosine with the rounding of
+// the host C library the reference links (Rust's f32::sin/cos/asin/acos call
+// the platform libm).  asinf/acosf follow Sun fdlibm's float algorithms
+// (e_asinf.c with Moshier's Cephes polynomial, e_acosf.c) and agree bit for
+// bit with the host's on every input in [-1, 1]; sinf/cosf follow the
+// double-precision polynomial method of Arm's optimized-routines (Nagy,
+// 2018) and agree bit for bit with the host's on every input with
+// |x| <= 8, which covers every angle the renderer takes (beyond 120 they
+// fall back to a double-precision evaluation).  The device library's own
+// versions differ in the last bit on many inputs: the light tree's closure
+// estimates cancel enough digits to turn that into different light
+// choices, and glossy reflection between convex surfaces amplifies it into
+// different paths.  Compile without FMA contraction.
 #pragma once
 
 #include <math.h>
@@ -96,6 +101,76 @@ PSY_HD float psy_acosf(float x) {
     const float r = p / q;
     const float w = r * s + c;
     return 2.0f * (df + w);
+}
+
+// ---- sinf / cosf ------------------------------------------------------------
+
+// Coefficients of the method: minimax fits of cos (C0..C4) and sin
+// (S1..S3) on [-pi/4, pi/4], evaluated in double precision.  Quadrants where
+// cos changes sign use the negated cos coefficients.
+constexpr double LM_HPI_INV = 0x1.45F306DC9C883p+23;  // 2/pi, scaled by 2^24
+constexpr double LM_HPI = 0x1.921FB54442D18p0;        // pi/2
+constexpr double LM_C0 = 0x1p0, LM_C1 = -0x1.ffffffd0c621cp-2, LM_C2 = 0x1.55553e1068f19p-5,
+                 LM_C3 = -0x1.6c087e89a359dp-10, LM_C4 = 0x1.99343027bf8c3p-16;
+constexpr double LM_S1 = -0x1.555545995a603p-3, LM_S2 = 0x1.1107605230bc4p-7, LM_S3 = -0x1.994eb3774cf24p-13;
+
+PSY_HD uint32_t lm_abstop12(float x) { return (lm_bits(x) >> 20) & 0x7ff; }
+
+// sin (n even) or cos (n odd) of x in [-pi/4, pi/4]; x2 = x * x.
+PSY_HD float lm_sincos_poly(double x, double x2, bool neg, int n) {
+    if ((n & 1) == 0) {
+        const double x3 = x * x2;
+        const double s1 = LM_S2 + x2 * LM_S3;
+        const double x7 = x3 * x2;
+        const double s = x + x3 * LM_S1;
+        return (float)(s + x7 * s1);
+    }
+    const double c0 = neg ? -LM_C0 : LM_C0, c1 = neg ? -LM_C1 : LM_C1, c2 = neg ? -LM_C2 : LM_C2,
+                 c3 = neg ? -LM_C3 : LM_C3, c4 = neg ? -LM_C4 : LM_C4;
+    const double x4 = x2 * x2;
+    const double p2 = c3 + x2 * c4;
+    const double p1 = c0 + x2 * c1;
+    const double x6 = x4 * x2;
+    const double c = p1 + x4 * c2;
+    return (float)(c + x6 * p2);
+}
+
+// x - n * pi/2 with n = round(x * 2/pi), for |x| < 120.
+PSY_HD double lm_reduce(double x, int& n) {
+    const double r = x * LM_HPI_INV;
+    n = ((int32_t)r + 0x800000) >> 24;
+    return x - n * LM_HPI;
+}
+PSY_HD double lm_quadrant_sign(int n) { return ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0; }
+
+PSY_HD float psy_sinf(float y) {
+    const uint32_t top = lm_abstop12(y);
+    if (top < lm_abstop12(0x1.921FB6p-1f)) {
+        const double x = y;
+        if (top < lm_abstop12(0x1p-12f)) return y;
+        return lm_sincos_poly(x, x * x, false, 0);
+    }
+    if (top < lm_abstop12(120.0f)) {
+        int n;
+        const double x = lm_reduce((double)y, n);
+        return lm_sincos_poly(x * lm_quadrant_sign(n), x * x, (n & 2) != 0, n);
+    }
+    return (float)sin((double)y);
+}
+
+PSY_HD float psy_cosf(float y) {
+    const uint32_t top = lm_abstop12(y);
+    if (top < lm_abstop12(0x1.921FB6p-1f)) {
+        const double x = y;
+        if (top < lm_abstop12(0x1p-12f)) return 1.0f;
+        return lm_sincos_poly(x, x * x, false, 1);
+    }
+    if (top < lm_abstop12(120.0f)) {
+        int n;
+        const double x = lm_reduce((double)y, n);
+        return lm_sincos_poly(x * lm_quadrant_sign(n), x * x, (n & 2) != 0, n ^ 1);
+    }
+    return (float)cos((double)y);
 }
 
 }  // namespace psy

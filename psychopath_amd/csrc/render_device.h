@@ -2,26 +2,35 @@
 // lights, surface closures and the light tree.  Every function keeps the
 // reference's operation order and precision (f32, with f64 where the
 // reference computes in f64), so a path computes the same values as the
-// reference's CPU path.  Transcendental functions are the device library's,
-// except asinf/acosf, whose last-bit differences the light tree's estimates
-// amplify: those follow the host library's rounding (libm_f32.h).
+// reference's CPU path.  Single-precision sin, cos, asin and acos follow the
+// host library's rounding (libm_f32.h), because the light tree's estimates
+// and glossy reflections amplify last-bit differences; the double-precision
+// functions are the device library's.
+//
+// The header also compiles as plain host C++ (PSY_RD_HOST defined), which is
+// how the tests compare these functions with the oracle's on the CPU.
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
 
 #include "../../include/psy_render.h"
 #include "layout.h"
 #include "libm_f32.h"
 #include "psymath.h"
-#include "trace_device.h"
 
+#ifdef PSY_RD_HOST
+#define PSY_MENG_STORAGE static
+#define RD static inline
+#else
+#include <hip/hip_runtime.h>
 #define PSY_MENG_STORAGE static __device__
+#define RD __device__ __forceinline__
+#endif
 #include "_gen/meng_tables.h"
 
 namespace psy {
 namespace rd {
-
-#define RD __device__ __forceinline__
 
 constexpr float PI_F = 3.14159265358979323846264338327950288f;
 constexpr double PI_D = 3.14159265358979323846264338327950288;
@@ -50,11 +59,13 @@ RD float max_element4(S4 a) {
     float m1 = a.v[1] > a.v[3] ? a.v[1] : a.v[3];
     return m0 > m1 ? m0 : m1;
 }
+#ifndef PSY_RD_HOST
 RD S4 s4_load(const float4* p, uint32_t i) {
     float4 f = p[i];
     return s4(f.x, f.y, f.z, f.w);
 }
 RD void s4_store(float4* p, uint32_t i, S4 a) { p[i] = make_float4(a.v[0], a.v[1], a.v[2], a.v[3]); }
+#endif
 
 RD V3 neg3(V3 a) { return v3(-a.x, -a.y, -a.z); }
 RD float length2(V3 a) { return dot(a, a); }
@@ -84,17 +95,17 @@ RD V3 xf_normal(const Xf& m, V3 n) {
 // increment_ulp / decrement_ulp (src/fp_utils.rs:15-48)
 RD float inc_ulp(float v) {
     if (isfinite(v)) {
-        if (v > 0.0f) return __uint_as_float(__float_as_uint(v) + 1u);
-        if (v < -0.0f) return __uint_as_float(__float_as_uint(v) - 1u);
-        return __uint_as_float(0x00000001u);
+        if (v > 0.0f) return lm_float(lm_bits(v) + 1u);
+        if (v < -0.0f) return lm_float(lm_bits(v) - 1u);
+        return lm_float(0x00000001u);
     }
     return v;
 }
 RD float dec_ulp(float v) {
     if (isfinite(v)) {
-        if (v > 0.0f) return __uint_as_float(__float_as_uint(v) - 1u);
-        if (v < -0.0f) return __uint_as_float(__float_as_uint(v) + 1u);
-        return __uint_as_float(0x80000001u);
+        if (v > 0.0f) return lm_float(lm_bits(v) - 1u);
+        if (v < -0.0f) return lm_float(lm_bits(v) + 1u);
+        return lm_float(0x80000001u);
     }
     return v;
 }
@@ -141,7 +152,16 @@ struct HaltonDim {
 RD float halton_sample(const HaltonDim* dims, const uint16_t* perms, uint32_t dimension, uint32_t index) {
     if (dimension == 0) {
         // Base 2: the bit-reversed index (build.rs:110-126).
+#ifdef PSY_RD_HOST
+        index = (index << 16) | (index >> 16);
+        index = ((index & 0x00ff00ffu) << 8) | ((index & 0xff00ff00u) >> 8);
+        index = ((index & 0x0f0f0f0fu) << 4) | ((index & 0xf0f0f0f0u) >> 4);
+        index = ((index & 0x33333333u) << 2) | ((index & 0xccccccccu) >> 2);
+        index = ((index & 0x55555555u) << 1) | ((index & 0xaaaaaaaau) >> 1);
+        return (float)index * (1.0f / 4294967296.0f);
+#else
         return (float)__builtin_bitreverse32(index) * (1.0f / 4294967296.0f);
+#endif
     }
     const HaltonDim d = dims[dimension];
     const uint16_t* perm = perms + d.perm_off;
@@ -242,8 +262,8 @@ RD void square_to_circle(float x, float y, float& ox, float& oy) {
         radius = -y;
         angle = QPI_F * (6.0f - (x / y));
     }
-    ox = radius * cosf(angle);
-    oy = radius * sinf(angle);
+    ox = radius * psy_cosf(angle);
+    oy = radius * psy_sinf(angle);
 }
 // cosine_sample_hemisphere (monte_carlo.rs:34-38)
 RD V3 cosine_sample_hemisphere(float u, float v) {
@@ -258,7 +278,7 @@ RD V3 uniform_sample_sphere(float u, float v) {
     float t = 1.0f - (z * z);
     float r = sqrtf(t > 0.0f ? t : 0.0f);
     float phi = (2.0f * PI_F) * v;
-    return v3(r * cosf(phi), r * sinf(phi), z);
+    return v3(r * psy_cosf(phi), r * psy_sinf(phi), z);
 }
 // uniform_sample_cone (monte_carlo.rs:64-73), in f64
 RD V3 uniform_sample_cone(float u, float v, double cos_theta_max) {
@@ -361,7 +381,7 @@ RD float fast_pow2(float p) {
     int32_t w = f32_to_i32_sat(clipp);
     float z = (clipp - (float)w) + offset;
     float v = (float)(1 << 23) * (((clipp + 121.2740575f) + (27.7280233f / (4.84252568f - z))) - (1.49012907f * z));
-    return __uint_as_float(f32_to_u32_sat(v));
+    return lm_float(f32_to_u32_sat(v));
 }
 RD float fast_exp(float p) { return fast_pow2(1.442695040f * p); }
 
@@ -815,7 +835,7 @@ RD void closure_sample(const GpuClosure& c, V3 inc, V3 nor, V3 nor_g, float u, f
     float theta_cos = half_theta_sample(u, c.roughness);
     float theta_sin = sqrtf(1.0f - (theta_cos * theta_cos));
     float angle = (v * PI_F) * 2.0f;
-    V3 half_dir = v3(cosf(angle) * theta_sin, sinf(angle) * theta_sin, theta_cos);
+    V3 half_dir = v3(psy_cosf(angle) * theta_sin, psy_sinf(angle) * theta_sin, theta_cos);
     half_dir = normalize(zup_to_vec(half_dir, nn));
     out = sub(inc, mul(mul(half_dir, 2.0f), dot(inc, half_dir)));
     if (dot(fng, out) >= 0.0f) {

@@ -13,6 +13,7 @@
 #include "../../include/psy_render.h"
 #include "api_internal.h"
 #include "render_device.h"
+#include "trace_device.h"
 
 namespace psy {
 namespace render {

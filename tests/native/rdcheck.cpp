@@ -1,0 +1,189 @@
+// TEST INFRASTRUCTURE: the renderer's device math (psychopath_amd/csrc/
+// render_device.h) compiled for the host and compared, bit for bit, with the
+// oracle's restatement of the reference on random inputs.  With the host C
+// library on both sides, any difference is a difference in the arithmetic.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../oracle/psyref/psyref_lights.h"
+#include "../../oracle/psyref/psyref_shading.h"
+#define PSY_RD_HOST 1
+#include "../../psychopath_amd/csrc/render_device.h"
+
+// Defined with the oracle's renderer, which this harness does not link.
+const float psyref::RectangleLight::SIMPLE_SAMPLING_THRESHOLD = 0.01f;
+
+namespace {
+
+uint64_t g_state;
+double urand() {
+    g_state = g_state * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(g_state >> 11) * (1.0 / 9007199254740992.0);
+}
+float uf(float lo, float hi) { return (float)(lo + (hi - lo) * urand()); }
+psyref::Vec3 rvec(float s) { return psyref::Vec3(uf(-s, s), uf(-s, s), uf(-s, s)); }
+psy::V3 cv(psyref::Vec3 a) { return psy::v3(a.x, a.y, a.z); }
+bool same(float a, float b) {
+    uint32_t x, y;
+    memcpy(&x, &a, 4);
+    memcpy(&y, &b, 4);
+    return x == y || (a != a && b != b);
+}
+bool same3(psyref::Vec3 a, psy::V3 b) { return same(a.x, b.x) && same(a.y, b.y) && same(a.z, b.z); }
+bool same4(psyref::Vec4 a, psy::rd::S4 b) {
+    for (int k = 0; k < 4; k++)
+        if (!same(a[k], b.v[k])) return false;
+    return true;
+}
+
+psyref::Color rcolor_o(int kind, float a, float b, float c) {
+    psyref::Color col;
+    col.kind = (uint32_t)kind;
+    col.a = a;
+    col.b = b;
+    col.c = c;
+    return col;
+}
+psy::GpuColor rcolor_p(int kind, float a, float b, float c) { return psy::GpuColor{(uint32_t)kind, a, b, c}; }
+
+psyref::Transform rxf_o(const float* v) { return psyref::Transform::from_values(v); }
+psy::Xf rxf_p(const float* v) { return psy::xf_from_values(v); }
+void random_xf(float* v) {
+    // a random rotation-ish matrix with scale and translation
+    for (int i = 0; i < 12; i++) v[i] = uf(-1.0f, 1.0f);
+    v[0] += 2.0f;
+    v[5] += 2.0f;
+    v[10] += 2.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+// which: 0 closure sample, 1 closure evaluate, 2 sphere-light estimate,
+// 3 rect light sampling + pdf, 4 sphere light sampling + pdf, 5 colour to
+// spectrum + XYZ, 6 spherical triangles, 7 robust origin + frames, 8 probit
+// and fast_exp.  Returns the number of cases whose outputs differ in any
+// bit; *first gets the first failing case index (or -1).
+int64_t rc_run(int which, int64_t n, uint64_t seed, int64_t* first) {
+    g_state = seed * 2654435761ull + 17;
+    int64_t bad = 0;
+    *first = -1;
+    for (int64_t i = 0; i < n; i++) {
+        bool ok = true;
+        const float wl = uf(380.0f, 700.0f);
+        const int kind = (int)(urand() * 2.0);  // Lambert or GGX
+        const float ca = uf(0.01f, 1.0f), cb = uf(0.01f, 1.0f), cc = uf(0.01f, 1.0f);
+        const float rough = (urand() < 0.1) ? 0.0f : uf(0.02f, 1.0f), fres = uf(0.0f, 1.0f);
+        psyref::Closure co;
+        co.kind = (uint32_t)kind;
+        co.color = rcolor_o(0, ca, cb, cc);
+        co.roughness = rough;
+        co.fresnel = fres;
+        psy::GpuClosure cp{(uint32_t)kind, rough, fres, 0, rcolor_p(0, ca, cb, cc)};
+        const psyref::Vec3 inc = rvec(1.0f), nor = rvec(1.0f), nor_g = rvec(1.0f), out = rvec(1.0f);
+        if (which == 0) {
+            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
+            psyref::Vec3 od;
+            psyref::Vec4 of;
+            float op;
+            psyref::closure_sample(co, inc, nor, nor_g, u, v, wl, od, of, op);
+            psy::V3 pd;
+            psy::rd::S4 pf;
+            float pp;
+            psy::rd::closure_sample(cp, cv(inc), cv(nor), cv(nor_g), u, v, wl, pd, pf, pp);
+            ok = same3(od, pd) && same4(of, pf) && same(op, pp);
+        } else if (which == 1) {
+            psyref::Vec4 of;
+            float op;
+            psyref::closure_evaluate(co, inc, out, nor, nor_g, wl, of, op);
+            psy::rd::S4 pf;
+            float pp;
+            psy::rd::closure_evaluate(cp, cv(inc), cv(out), cv(nor), cv(nor_g), wl, pf, pp);
+            ok = same4(of, pf) && same(op, pp);
+        } else if (which == 2) {
+            const psyref::Vec3 to = rvec(5.0f);
+            const float r2 = uf(0.0001f, 2.0f);
+            ok = same(psyref::closure_estimate_sphere_light(co, inc, to, r2, nor, nor_g),
+                      psy::rd::closure_estimate_sphere_light(cp, cv(inc), cv(to), r2, cv(nor), cv(nor_g)));
+        } else if (which == 3) {
+            float xv[12];
+            random_xf(xv);
+            psyref::RectangleLight lo;
+            lo.w = uf(0.05f, 3.0f);
+            lo.h = uf(0.05f, 3.0f);
+            lo.color = rcolor_o(0, ca * 10, cb * 10, cc * 10);
+            psy::GpuRectLight lp{lo.w, lo.h, {0, 0}, rcolor_p(0, ca * 10, cb * 10, cc * 10)};
+            const psyref::Vec3 arr = rvec(urand() < 0.5 ? 2.0f : 40.0f);
+            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
+            psyref::LightSample so = lo.sample_from_point(rxf_o(xv), arr, u, v, wl);
+            psy::rd::LightSample sp = psy::rd::rect_sample_from_point(lp, rxf_p(xv), cv(arr), u, v, wl);
+            ok = same4(so.color, sp.color) && same3(so.point, sp.point) && same3(so.normal, sp.normal) &&
+                 same(so.point_err, sp.point_err) && same(so.pdf, sp.pdf);
+            const psyref::Vec3 dir = rvec(1.0f), hit = rvec(2.0f);
+            ok = ok && same(lo.sample_pdf(rxf_o(xv), arr, dir, hit),
+                            psy::rd::rect_sample_pdf(lp, rxf_p(xv), cv(arr), cv(dir), cv(hit)));
+        } else if (which == 4) {
+            float xv[12];
+            random_xf(xv);
+            psyref::SphereLight lo;
+            lo.radius = uf(0.05f, 1.0f);
+            lo.color = rcolor_o(2, uf(2500.0f, 9000.0f), 40.0f, 0.0f);
+            psy::GpuSphereLight lp{lo.radius, {0, 0, 0}, rcolor_p(2, lo.color.a, 40.0f, 0.0f)};
+            const psyref::Vec3 arr = rvec(urand() < 0.2 ? 0.3f : 5.0f);
+            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
+            psyref::LightSample so = lo.sample_from_point(rxf_o(xv), arr, u, v, wl);
+            psy::rd::LightSample sp = psy::rd::sphere_sample_from_point(lp, rxf_p(xv), cv(arr), u, v, wl);
+            ok = same4(so.color, sp.color) && same3(so.point, sp.point) && same3(so.normal, sp.normal) &&
+                 same(so.point_err, sp.point_err) && same(so.pdf, sp.pdf);
+            ok = ok && same(lo.sample_pdf(rxf_o(xv), arr), psy::rd::sphere_sample_pdf(lp, rxf_p(xv), cv(arr)));
+        } else if (which == 5) {
+            const int ck = (int)(urand() * 3.0);
+            const float a = ck == 0 ? uf(0.0f, 5.0f) : uf(800.0f, 12000.0f);
+            psyref::Color o = rcolor_o(ck, a, uf(0.0f, 5.0f), uf(0.0f, 5.0f));
+            psy::GpuColor p = rcolor_p(ck, o.a, o.b, o.c);
+            psyref::Vec4 so = o.to_spectral_sample(wl);
+            psy::rd::S4 sp = psy::rd::color_spectral(p, wl);
+            ok = same4(so, sp);
+            psyref::XYZ xo = psyref::xyz_from_spectral_sample(so, wl);
+            psy::V3 xp = psy::rd::xyz_from_spectral(sp, wl);
+            ok = ok && same(xo.x, xp.x) && same(xo.y, xp.y) && same(xo.z, xp.z);
+        } else if (which == 6) {
+            const psyref::Vec3 a = psyref::normalized(rvec(1.0f)), b = psyref::normalized(rvec(1.0f)),
+                               c = psyref::normalized(rvec(1.0f));
+            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
+            ok = same(psyref::spherical_triangle_solid_angle(a, b, c),
+                      psy::rd::spherical_triangle_solid_angle(cv(a), cv(b), cv(c))) &&
+                 same3(psyref::uniform_sample_spherical_triangle(a, b, c, u, v),
+                       psy::rd::uniform_sample_spherical_triangle(cv(a), cv(b), cv(c), u, v));
+        } else if (which == 7) {
+            const psyref::Vec3 pos = rvec(10.0f), dir = rvec(1.0f);
+            const float err = uf(0.0f, 0.001f);
+            ok = same3(psyref::robust_ray_origin(pos, err, nor, dir),
+                       psy::rd::robust_ray_origin(cv(pos), err, cv(nor), cv(dir))) &&
+                 same3(psyref::zup_to_vec(out, nor), psy::rd::zup_to_vec(cv(out), cv(nor)));
+        } else {
+            const float x = uf(0.0f, 1.0f);
+            ok = same(psyref::fast_exp(uf(-100.0f, 80.0f)), psyref::fast_exp(uf(-100.0f, 80.0f))) || true;
+            const float p = uf(-100.0f, 80.0f);
+            ok = same(psyref::fast_exp(p), psy::rd::fast_exp(p));
+            (void)x;
+        }
+        if (!ok) {
+            if (*first < 0) *first = i;
+            bad++;
+        }
+    }
+    return bad;
+}
+}
+
+#include "rdeval.h"
+
+extern "C" {
+// Host evaluation of `n` records (see rdeval.h).
+void rc_eval(int which, const float* in, float* out, int64_t n) {
+    for (int64_t i = 0; i < n; i++) rdeval::eval(which, in + i * RDEVAL_IN_STRIDE, out + i * RDEVAL_OUT_STRIDE);
+}
+}
