@@ -115,9 +115,13 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
 /* Execution options, none of which changes results: PSY_RENDER_NO_GRAPHS
  * launches every wave's kernels directly instead of replaying a captured
  * hipGraph; PSY_RENDER_NO_SORT shades closest hits in queue order instead of
- * binning them by closure kind first. */
+ * binning them by closure kind first; PSY_RENDER_PLAIN renders in one
+ * kernel launch over the whole frame (one lane per pixel tracing and
+ * shading its samples' paths to the end, no wavefronts), the baseline the
+ * wavefront and served forms are measured against. */
 #define PSY_RENDER_NO_GRAPHS 1u
 #define PSY_RENDER_NO_SORT 2u
+#define PSY_RENDER_PLAIN 4u
 int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
 /* ---- request server ---------------------------------------------------

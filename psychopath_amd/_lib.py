@@ -222,6 +222,7 @@ _SIGNATURES = {
 
 PSY_RENDER_NO_GRAPHS = 1
 PSY_RENDER_NO_SORT = 2
+PSY_RENDER_PLAIN = 4
 
 
 class ServerConfig(C.Structure):

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "render_internal.h"
+#include "trace_one.h"
 
 using namespace psy;
 using namespace psy::api;
@@ -87,8 +88,72 @@ __global__ void zero_film_kernel(float* film, uint64_t n) {
         film[i] = 0.0f;
 }
 
-// Camera::generate_ray and LightPath::new for every slot (renderer.rs:233-
-// 289, camera.rs:74-102).
+// The state of one light path (LightPath, renderer.rs:362-447) while it is
+// being advanced: the wavefront kernels keep it in HBM between events, the
+// single-launch renderer in registers.
+struct PathRegs {
+    S4 color, atten, pending;
+    float csp;  // closure_sample_pdf
+    float wl, time;
+    uint32_t seed, bounces, x, y, sample;
+    // The last bounce ray sampled and its filter (next_bounce_ray,
+    // next_attenuation_fac).  The reference keeps them when a later BSDF
+    // sample fails, and a path that also found a light then continues along
+    // this earlier ray after its shadow ray (renderer.rs:599-646, 670-690).
+    bool has_next;
+    V3 next_o, next_d;
+    S4 next_fac;
+};
+
+// LightPath::new and Camera::generate_ray for path g of a request
+// (renderer.rs:233-289, camera.rs:74-102).
+__device__ __forceinline__ void camera_path(const ShadeScene& sc, const ReqParams& rp, uint64_t g, PathRegs& ps,
+                                            V3& o, V3& d) {
+    const uint32_t pixel = (uint32_t)(g / rp.spp);
+    const uint32_t si = (uint32_t)(g % rp.spp);
+    const uint32_t x = pixel % rp.width, y = pixel / rp.width;
+    const S4 d0 = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, rp.seed);
+    const S4 d1 = sample_4d(sc.hdims, sc.hperms, si, 1, x, y, rp.seed);
+    const float filter_x = probit(d0.v[3], 2.0f / 6.0f) + 0.5f;
+    const float filter_y = probit(d1.v[0], 2.0f / 6.0f) + 0.5f;
+    const float samp_x = (filter_x + (float)x) * rp.cmpx;
+    const float samp_y = (filter_y + (float)y) * rp.cmpy;
+    const float img_x = (samp_x - 0.5f) * rp.x_extent;
+    const float img_y = (0.5f - samp_y) * rp.y_extent;
+    const float time = d0.v[0];
+    const float wl = map_0_1_to_wavelength(golden_ratio_sample(si, hash_u32((x << 16) ^ y, rp.seed)));
+    Xf xf;
+    {
+        uint32_t i1;
+        float a2;
+        if (!lerp_slice_pos(rp.cam_xf_count, time, i1, a2)) xf = load_xf(rp.cam_xf + i1);
+        else xf = xf_lerp(load_xf(rp.cam_xf + i1), load_xf(rp.cam_xf + i1 + 1), a2);
+    }
+    float lu, lv;
+    square_to_circle((d0.v[1] * 2.0f) - 1.0f, (d0.v[2] * 2.0f) - 1.0f, lu, lv);
+    const V3 orig = v3(rp.aperture * lu, rp.aperture * lv, 0.0f);
+    const V3 dir =
+        normalize(v3((img_x * rp.tfov) - (orig.x / rp.focus), (img_y * rp.tfov) - (orig.y / rp.focus), 1.0f));
+    ps.color = s4_zero();
+    ps.atten = s4_splat(1.0f);
+    ps.pending = s4_zero();
+    ps.csp = 1.0f;
+    ps.wl = wl;
+    ps.time = time;
+    ps.seed = rp.seed ^ 0x40d4682bu;
+    ps.bounces = 0u;
+    ps.x = x;
+    ps.y = y;
+    ps.sample = si;
+    ps.has_next = false;
+    ps.next_o = v3(0.0f, 0.0f, 0.0f);
+    ps.next_d = v3(0.0f, 0.0f, 0.0f);
+    ps.next_fac = s4_zero();
+    o = xf_point(xf, orig);
+    d = xf_vector(xf, dir);
+}
+
+// Camera::generate_ray and LightPath::new for every slot of the wave.
 __global__ __launch_bounds__(KBLOCK) void camera_kernel(const WavePlan* plans, const uint32_t* wave_idx,
                                                         const ReqParams* reqs, ShadeScene sc, Paths P, Queue qa,
                                                         uint32_t* shadow_count) {
@@ -102,47 +167,22 @@ __global__ __launch_bounds__(KBLOCK) void camera_kernel(const WavePlan* plans, c
     for (uint32_t i = i0; i < total; i += gridDim.x * blockDim.x) {
         const WaveSeg& sg = find_seg(plan, i);
         const ReqParams rp = reqs[sg.req];
-        const uint64_t g = sg.g0 + (i - sg.slot0);
-        const uint32_t pixel = (uint32_t)(g / rp.spp);
-        const uint32_t si = (uint32_t)(g % rp.spp);
-        const uint32_t x = pixel % rp.width, y = pixel / rp.width;
-
-        const S4 d0 = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, rp.seed);
-        const S4 d1 = sample_4d(sc.hdims, sc.hperms, si, 1, x, y, rp.seed);
-        const float filter_x = probit(d0.v[3], 2.0f / 6.0f) + 0.5f;
-        const float filter_y = probit(d1.v[0], 2.0f / 6.0f) + 0.5f;
-        const float samp_x = (filter_x + (float)x) * rp.cmpx;
-        const float samp_y = (filter_y + (float)y) * rp.cmpy;
-        const float img_x = (samp_x - 0.5f) * rp.x_extent;
-        const float img_y = (0.5f - samp_y) * rp.y_extent;
-        const float time = d0.v[0];
-        const float wl = map_0_1_to_wavelength(golden_ratio_sample(si, hash_u32((x << 16) ^ y, rp.seed)));
-
-        Xf xf;
-        {
-            uint32_t i1;
-            float a2;
-            if (!lerp_slice_pos(rp.cam_xf_count, time, i1, a2)) xf = load_xf(rp.cam_xf + i1);
-            else xf = xf_lerp(load_xf(rp.cam_xf + i1), load_xf(rp.cam_xf + i1 + 1), a2);
-        }
-        float lu, lv;
-        square_to_circle((d0.v[1] * 2.0f) - 1.0f, (d0.v[2] * 2.0f) - 1.0f, lu, lv);
-        const V3 orig = v3(rp.aperture * lu, rp.aperture * lv, 0.0f);
-        const V3 dir = normalize(v3((img_x * rp.tfov) - (orig.x / rp.focus), (img_y * rp.tfov) - (orig.y / rp.focus),
-                                    1.0f));
-
+        PathRegs ps;
+        V3 o, d;
+        camera_path(sc, rp, sg.g0 + (i - sg.slot0), ps, o, d);
         P.req[i] = sg.req;
-        P.pix[i] = pixel;
-        P.sample[i] = si;
-        P.seed[i] = rp.seed ^ 0x40d4682bu;
+        P.pix[i] = ps.y * rp.width + ps.x;
+        P.sample[i] = ps.sample;
+        P.seed[i] = ps.seed;
         P.bounces[i] = 0u;
-        P.wl[i] = wl;
-        P.time[i] = time;
+        P.wl[i] = ps.wl;
+        P.time[i] = ps.time;
         P.csp[i] = 1.0f;
-        P.color[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.atten[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        P.pending[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        queue_put(qa, i, i, xf_point(xf, orig), xf_vector(xf, dir), time, __builtin_huge_valf());
+        s4_store(P.color, i, ps.color);
+        s4_store(P.atten, i, ps.atten);
+        s4_store(P.pending, i, ps.pending);
+        P.has_next[i] = 0u;
+        queue_put(qa, i, i, o, d, ps.time, __builtin_huge_valf());
     }
 }
 
@@ -165,43 +205,49 @@ __device__ __forceinline__ uint32_t hit_key(const ShadeScene& sc, const Hits& h,
     return 2u + sc.shaders[inst.shader].kind;
 }
 
+// Counting sort of the closest hits by key, in two kernels over the same
+// block partition: block b takes queue entries [b*chunk, (b+1)*chunk).
+// sort_count adds each block's per-key counts to bins[0..5); sort_scatter
+// reserves each block's range in every bin with one atomic on bins[5..10)
+// and places the block's entries with LDS counters.
+__device__ __forceinline__ uint32_t sort_chunk(uint32_t n) {
+    return (n + gridDim.x - 1) / gridDim.x;
+}
+
 __global__ __launch_bounds__(KBLOCK) void sort_count_kernel(ShadeScene sc, Hits h, const uint32_t* count,
                                                             uint32_t* bins) {
     __shared__ uint32_t local[NUM_BINS];
     if (threadIdx.x < NUM_BINS) local[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t n = *count;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-        atomicAdd(&local[hit_key(sc, h, q)], 1u);
+    const uint32_t chunk = sort_chunk(n);
+    const uint32_t q0 = blockIdx.x * chunk, q1 = min(n, q0 + chunk);
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) atomicAdd(&local[hit_key(sc, h, q)], 1u);
     __syncthreads();
     if (threadIdx.x < NUM_BINS && local[threadIdx.x] != 0u) atomicAdd(&bins[threadIdx.x], local[threadIdx.x]);
 }
 
-// bins[0..5) hold the bin sizes, bins[5..10) the fill cursors.
 __global__ __launch_bounds__(KBLOCK) void sort_scatter_kernel(ShadeScene sc, Hits h, const uint32_t* count,
                                                               uint32_t* bins, uint32_t* order) {
     __shared__ uint32_t local[NUM_BINS], base[NUM_BINS];
+    if (threadIdx.x < NUM_BINS) local[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t n = *count;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t q0 = blockIdx.x * blockDim.x; q0 < n; q0 += stride) {
-        if (threadIdx.x < NUM_BINS) local[threadIdx.x] = 0u;
-        __syncthreads();
-        const uint32_t q = q0 + threadIdx.x;
-        uint32_t key = 0, rank = 0;
-        if (q < n) {
-            key = hit_key(sc, h, q);
-            rank = atomicAdd(&local[key], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x < NUM_BINS) {
-            uint32_t start = 0;
-            for (uint32_t b = 0; b < threadIdx.x; b++) start += bins[b];
-            base[threadIdx.x] = local[threadIdx.x] ? start + atomicAdd(&bins[NUM_BINS + threadIdx.x], local[threadIdx.x])
-                                                   : 0u;
-        }
-        __syncthreads();
-        if (q < n) order[base[key] + rank] = q;
-        __syncthreads();
+    const uint32_t chunk = sort_chunk(n);
+    const uint32_t q0 = blockIdx.x * chunk, q1 = min(n, q0 + chunk);
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) atomicAdd(&local[hit_key(sc, h, q)], 1u);
+    __syncthreads();
+    if (threadIdx.x < NUM_BINS) {
+        uint32_t start = 0;
+        for (uint32_t b = 0; b < threadIdx.x; b++) start += bins[b];
+        base[threadIdx.x] = local[threadIdx.x] ? start + atomicAdd(&bins[NUM_BINS + threadIdx.x], local[threadIdx.x])
+                                               : 0u;
+        local[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const uint32_t key = hit_key(sc, h, q);
+        order[base[key] + atomicAdd(&local[key], 1u)] = q;
     }
 }
 
@@ -407,7 +453,119 @@ __device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float
     return out;
 }
 
-// LightPath::next for camera and bounce rays (renderer.rs:459-667).
+// A traced ray's result as the shading step reads it.
+struct HitRec {
+    uint8_t kind;
+    float t, b0, b1, b2;
+    uint32_t prim, inst;
+};
+// What the shading step asks to trace next.
+struct ShadeOut {
+    bool shadow, bounce;
+    V3 so, sd, bo, bd;
+    float smax;
+};
+
+// LightPath::next for a camera or bounce ray (renderer.rs:459-667): adds
+// emission or background, or prepares the light ray (with its pending
+// contribution in ps.pending) and the next bounce ray.
+__device__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& ps, const HitRec& h, V3 orig, V3 dir,
+                           ShadeOut& out) {
+    out.shadow = false;
+    out.bounce = false;
+    out.smax = 1.0f;
+    const float wl = ps.wl;
+    const float time = ps.time;
+    if (h.kind != PSY_HIT_SURFACE) {
+        // Background (renderer.rs:656-666)
+        const S4 bg = color_spectral(sc.background, wl);
+        ps.color = ps.color + ((bg * ps.atten) / ps.csp);
+        return;
+    }
+    const GpuInstance inst = sc.instances[sc.inst_pos[h.inst]];
+    const Isect is = make_isect(sc, inst, h.t, h.b0, h.b1, h.b2, h.prim, orig, dir, time);
+    GpuClosure clo;
+    if (is.light) {
+        clo.kind = PSY_SHADER_EMIT;
+        clo.color = is.emit;
+        clo.roughness = 0.0f;
+        clo.fresnel = 0.0f;
+    } else {
+        clo = sc.shaders[is.shader];
+    }
+    if (clo.kind == PSY_SHADER_EMIT) {
+        const S4 c = color_spectral(clo.color, wl);
+        if (ps.bounces == 0u) {
+            ps.color = ps.color + c;
+        } else {
+            const float mis_pdf = power_heuristic(ps.csp, is.sample_pdf);
+            ps.color = ps.color + ((c * ps.atten) / mis_pdf);
+        }
+        return;
+    }
+    S4 la = ps.atten / ps.csp;
+    // Light ray
+    ps.seed += 1u;
+    const S4 ls = sample_4d(sc.hdims, sc.hperms, ps.sample, 0, ps.x, ps.y, ps.seed);
+    const LightPick li = sample_lights(sc, ls.v[0], ls.v[1], ls.v[2], wl, time, is, dir, clo);
+    if (li.found && li.pdf > 0.0f && li.selection_pdf > 0.0f) {
+        S4 att;
+        float closure_pdf;
+        if (li.distant) {
+            closure_evaluate(clo, dir, li.dir, is.nor, is.nor_g, wl, att, closure_pdf);
+            out.so = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), li.dir);
+            out.sd = li.dir;
+            out.smax = __builtin_huge_valf();
+        } else {
+            const V3 ldir = sub(li.point, is.pos);
+            closure_evaluate(clo, dir, ldir, is.nor, is.nor_g, wl, att, closure_pdf);
+            const V3 offset_pos = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ldir);
+            const V3 offset_end = robust_ray_origin(li.point, li.point_err, normalize(li.normal), neg3(ldir));
+            out.so = offset_pos;
+            out.sd = sub(offset_end, offset_pos);
+            out.smax = 1.0f;
+        }
+        if (max_element4(att) > 0.0f) {
+            const float light_mis_pdf = power_heuristic(li.pdf, closure_pdf);
+            ps.pending = ((li.color * att) * la) / (light_mis_pdf * li.selection_pdf);
+            out.shadow = true;
+        }
+    }
+    // Bounce ray
+    bool do_bounce = false;
+    if (ps.bounces < rp.max_bounces) {
+        ps.bounces += 1u;
+        ps.seed += 1u;
+        const S4 bs = sample_4d(sc.hdims, sc.hperms, ps.sample, 0, ps.x, ps.y, ps.seed);
+        V3 ndir;
+        S4 filter;
+        float pdf;
+        closure_sample(clo, dir, is.nor, is.nor_g, bs.v[0], bs.v[1], wl, ndir, filter, pdf);
+        if ((pdf > 0.0f) && (max_element4(filter) > 0.0f)) {
+            ps.next_fac = filter;
+            ps.csp = pdf;
+            ps.next_o = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ndir);
+            ps.next_d = ndir;
+            ps.has_next = true;
+            do_bounce = true;
+        }
+    } else {
+        ps.has_next = false;
+    }
+    // A path with a light ray continues after it along next_bounce_ray if it
+    // has one, even one kept from an earlier bounce; without a light ray only
+    // a fresh bounce continues it.
+    if (out.shadow ? ps.has_next : do_bounce) {
+        out.bo = ps.next_o;
+        out.bd = ps.next_d;
+        la = la * ps.next_fac;
+        out.bounce = true;
+    }
+    ps.atten = la;
+}
+
+// The shading step over the closest-hit queue (in material order when
+// `order` is given); appends the light and bounce rays to the next queues.
 __global__ __launch_bounds__(KBLOCK) void shade_kernel(ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa,
                                                        Hits h, const uint32_t* order, Queue qa_out, Queue qs_out) {
     const uint32_t n = *qa.count;
@@ -416,111 +574,112 @@ __global__ __launch_bounds__(KBLOCK) void shade_kernel(ShadeScene sc, const ReqP
     // wave-wide appends see all lanes.
     const uint32_t n_round = (n + 63u) & ~63u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
-        bool want_shadow = false, want_bounce = false;
-        V3 so{}, sd{}, bo{}, bd{};
-        float smax = 1.0f, ptime = 0.0f;
+        ShadeOut so;
+        so.shadow = false;
+        so.bounce = false;
+        float ptime = 0.0f;
         uint32_t p = 0;
         if (i < n) {
             const uint32_t q = order ? order[i] : i;
             p = qa.path[q];
             const ReqParams rp = reqs[P.req[p]];
-            const float wl = P.wl[p];
-            const float time = P.time[p];
-            ptime = time;
-            S4 la = s4_load(P.atten, p);
-            S4 col = s4_load(P.color, p);
-            float csp = P.csp[p];
-            uint32_t seed = P.seed[p];
-            uint32_t bounces = P.bounces[p];
-            const V3 orig = v3(qa.ox[q], qa.oy[q], qa.oz[q]);
-            const V3 dir = v3(qa.dx[q], qa.dy[q], qa.dz[q]);
-            if (h.kind[q] != PSY_HIT_SURFACE) {
-                // Background (renderer.rs:656-666)
-                const S4 bg = color_spectral(sc.background, wl);
-                col = col + ((bg * la) / csp);
-                s4_store(P.color, p, col);
-            } else {
-                const GpuInstance inst = sc.instances[sc.inst_pos[h.inst[q]]];
-                const Isect is = make_isect(sc, inst, h.t[q], h.b0[q], h.b1[q], h.b2[q], h.prim[q], orig, dir, time);
-                GpuClosure clo;
-                if (is.light) {
-                    clo.kind = PSY_SHADER_EMIT;
-                    clo.color = is.emit;
-                    clo.roughness = 0.0f;
-                    clo.fresnel = 0.0f;
-                } else {
-                    clo = sc.shaders[is.shader];
-                }
-                if (clo.kind == PSY_SHADER_EMIT) {
-                    const S4 c = color_spectral(clo.color, wl);
-                    if (bounces == 0u) {
-                        col = col + c;
-                    } else {
-                        const float mis_pdf = power_heuristic(csp, is.sample_pdf);
-                        col = col + ((c * la) / mis_pdf);
-                    }
-                    s4_store(P.color, p, col);
-                } else {
-                    la = la / csp;
-                    const uint32_t x = P.pix[p] % rp.width, y = P.pix[p] / rp.width;
-                    const uint32_t si = P.sample[p];
-                    // Light ray
-                    seed += 1u;
-                    const S4 ls = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, seed);
-                    const LightPick li = sample_lights(sc, ls.v[0], ls.v[1], ls.v[2], wl, time, is, dir, clo);
-                    if (li.found && li.pdf > 0.0f && li.selection_pdf > 0.0f) {
-                        S4 att;
-                        float closure_pdf;
-                        if (li.distant) {
-                            closure_evaluate(clo, dir, li.dir, is.nor, is.nor_g, wl, att, closure_pdf);
-                            so = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), li.dir);
-                            sd = li.dir;
-                            smax = __builtin_huge_valf();
-                        } else {
-                            const V3 ldir = sub(li.point, is.pos);
-                            closure_evaluate(clo, dir, ldir, is.nor, is.nor_g, wl, att, closure_pdf);
-                            const V3 offset_pos = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ldir);
-                            const V3 offset_end = robust_ray_origin(li.point, li.point_err, normalize(li.normal),
-                                                                    neg3(ldir));
-                            so = offset_pos;
-                            sd = sub(offset_end, offset_pos);
-                            smax = 1.0f;
-                        }
-                        if (max_element4(att) > 0.0f) {
-                            const float light_mis_pdf = power_heuristic(li.pdf, closure_pdf);
-                            const S4 pend = ((li.color * att) * la) / (light_mis_pdf * li.selection_pdf);
-                            s4_store(P.pending, p, pend);
-                            want_shadow = true;
-                        }
-                    }
-                    // Bounce ray
-                    if (bounces < rp.max_bounces) {
-                        bounces += 1u;
-                        seed += 1u;
-                        const S4 bs = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, seed);
-                        V3 ndir;
-                        S4 filter;
-                        float pdf;
-                        closure_sample(clo, dir, is.nor, is.nor_g, bs.v[0], bs.v[1], wl, ndir, filter, pdf);
-                        if ((pdf > 0.0f) && (max_element4(filter) > 0.0f)) {
-                            csp = pdf;
-                            bo = robust_ray_origin(is.pos, is.pos_err, normalize(is.nor_g), ndir);
-                            bd = ndir;
-                            la = la * filter;
-                            want_bounce = true;
-                        }
-                    }
-                    s4_store(P.atten, p, la);
-                    P.csp[p] = csp;
-                    P.seed[p] = seed;
-                    P.bounces[p] = bounces;
-                }
+            PathRegs ps;
+            ps.color = s4_load(P.color, p);
+            ps.atten = s4_load(P.atten, p);
+            ps.pending = s4_load(P.pending, p);
+            ps.csp = P.csp[p];
+            ps.wl = P.wl[p];
+            ps.time = P.time[p];
+            ps.seed = P.seed[p];
+            ps.bounces = P.bounces[p];
+            ps.x = P.pix[p] % rp.width;
+            ps.y = P.pix[p] / rp.width;
+            ps.sample = P.sample[p];
+            ps.has_next = P.has_next[p] != 0u;
+            if (ps.has_next) {
+                const float4 a = P.next_o[p], b = P.next_d[p];
+                ps.next_o = v3(a.x, a.y, a.z);
+                ps.next_d = v3(b.x, b.y, b.z);
+                ps.next_fac = s4_load(P.next_fac, p);
+            }
+            ptime = ps.time;
+            const HitRec hr{h.kind[q], h.t[q], h.b0[q], h.b1[q], h.b2[q], h.prim[q], h.inst[q]};
+            shade_path(sc, rp, ps, hr, v3(qa.ox[q], qa.oy[q], qa.oz[q]), v3(qa.dx[q], qa.dy[q], qa.dz[q]), so);
+            s4_store(P.color, p, ps.color);
+            s4_store(P.atten, p, ps.atten);
+            if (so.shadow) s4_store(P.pending, p, ps.pending);
+            P.csp[p] = ps.csp;
+            P.seed[p] = ps.seed;
+            P.bounces[p] = ps.bounces;
+            P.has_next[p] = ps.has_next ? 1u : 0u;
+            if (ps.has_next) {
+                P.next_o[p] = make_float4(ps.next_o.x, ps.next_o.y, ps.next_o.z, 0.0f);
+                P.next_d[p] = make_float4(ps.next_d.x, ps.next_d.y, ps.next_d.z, 0.0f);
+                s4_store(P.next_fac, p, ps.next_fac);
             }
         }
-        const uint32_t at_s = wave_append(qs_out.count, want_shadow);
-        if (want_shadow) queue_put(qs_out, at_s, p, so, sd, ptime, smax);
-        const uint32_t at_b = wave_append(qa_out.count, want_bounce);
-        if (want_bounce) queue_put(qa_out, at_b, p, bo, bd, ptime, __builtin_huge_valf());
+        const uint32_t at_s = wave_append(qs_out.count, so.shadow);
+        if (so.shadow) queue_put(qs_out, at_s, p, so.so, so.sd, ptime, so.smax);
+        const uint32_t at_b = wave_append(qa_out.count, so.bounce);
+        if (so.bounce) queue_put(qa_out, at_b, p, so.bo, so.bd, ptime, __builtin_huge_valf());
+    }
+}
+
+// The plain renderer: one launch per render over the whole frame, one lane
+// per pixel running its samples one after another, each path traced and
+// shaded to its end in registers (the reference's per-path logic without
+// wavefronts), then added to the pixel in sample order.
+constexpr int MEGA_BLOCK = TRACE_BLOCK;
+constexpr int MEGA_LDS = TRACE_LDS_DEPTH;
+constexpr int MEGA_SPILL = TRACE_STACK_CAPACITY - TRACE_LDS_DEPTH;
+
+__global__ __launch_bounds__(MEGA_BLOCK) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
+    __shared__ uint32_t lds_stack[MEGA_LDS * MEGA_BLOCK];
+    const ReqParams rp = reqs[0];
+    const uint32_t pixels = rp.width * rp.height;
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= pixels) return;
+    uint32_t* lds_col = lds_stack + threadIdx.x;
+    const float fspp = (float)rp.spp;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    TraceCount tc;
+    for (uint32_t si = 0; si < rp.spp; si++) {
+        PathRegs ps;
+        V3 o, d;
+        camera_path(sc, rp, (uint64_t)pixel * rp.spp + si, ps, o, d);
+        for (;;) {
+            const TraceHit th = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, o, d, ps.time, __builtin_huge_valf(),
+                                                                       false, lds_col, MEGA_BLOCK, tc);
+            const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
+            ShadeOut so;
+            shade_path(sc, rp, ps, hr, o, d, so);
+            if (so.shadow) {
+                const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, so.so, so.sd, ps.time, so.smax, true,
+                                                                           lds_col, MEGA_BLOCK, tc);
+                if (sh.kind == PSY_HIT_MISS) ps.color = ps.color + ps.pending;
+            }
+            if (!so.bounce) break;
+            o = so.bo;
+            d = so.bd;
+        }
+        const V3 c = xyz_from_spectral(ps.color, ps.wl);
+        fx += c.x / fspp;
+        fy += c.y / fspp;
+        fz += c.z / fspp;
+    }
+    float* px = rp.film + 3ull * pixel;
+    px[0] = fx;
+    px[1] = fy;
+    px[2] = fz;
+}
+
+// Path colour to XYZ (XYZ::from_spectral_sample, color.rs:520-526), one
+// lane per slot; the result replaces the slot's colour.
+__global__ __launch_bounds__(KBLOCK) void xyz_kernel(const WavePlan* plans, const uint32_t* wave_idx, Paths P) {
+    const uint32_t total = plans[*wave_idx].total;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const V3 c = xyz_from_spectral(s4_load(P.color, i), P.wl[i]);
+        P.color[i] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
 
@@ -540,8 +699,9 @@ __global__ __launch_bounds__(KBLOCK) void film_kernel(const WavePlan* plans, con
         float* px = rp.film + 3 * pixel;
         float fx = px[0], fy = px[1], fz = px[2];
         const float fspp = (float)rp.spp;
-        for (uint32_t j = i; j < end && (sg.g0 + (j - sg.slot0)) / rp.spp == pixel; j++) {
-            const V3 c = xyz_from_spectral(s4_load(P.color, j), P.wl[j]);
+        const uint32_t run = (uint32_t)std::min<uint64_t>(end - i, rp.spp - g % rp.spp);
+        for (uint32_t j = i; j < i + run; j++) {
+            const float4 c = P.color[j];
             fx += c.x / fspp;
             fy += c.y / fspp;
             fz += c.z / fspp;
@@ -662,10 +822,11 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
         PSY_HIP(hipGetLastError());
         const uint32_t* order = nullptr;
         if (r->sort_materials) {
-            hipLaunchKernelGGL(sort_count_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+            const uint32_t sgrid = std::min<uint32_t>(grid, 1024);
+            hipLaunchKernelGGL(sort_count_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
                                r->counts + 4);
             PSY_HIP(hipGetLastError());
-            hipLaunchKernelGGL(sort_scatter_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+            hipLaunchKernelGGL(sort_scatter_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
                                r->counts + 4, r->order);
             PSY_HIP(hipGetLastError());
             order = r->order;
@@ -676,6 +837,8 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
         std::swap(qa_in, qa_out);
         std::swap(qs_in, qs_out);
     }
+    hipLaunchKernelGGL(xyz_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->P);
+    PSY_HIP(hipGetLastError());
     hipLaunchKernelGGL(film_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->P);
     PSY_HIP(hipGetLastError());
     hipLaunchKernelGGL(wave_advance_kernel, dim3(1), dim3(1), 0, s, r->wave_idx);
@@ -842,6 +1005,10 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!rc) rc = r->alloc(&P.color, C);
     if (!rc) rc = r->alloc(&P.atten, C);
     if (!rc) rc = r->alloc(&P.pending, C);
+    if (!rc) rc = r->alloc(&P.has_next, C);
+    if (!rc) rc = r->alloc(&P.next_o, C);
+    if (!rc) rc = r->alloc(&P.next_d, C);
+    if (!rc) rc = r->alloc(&P.next_fac, C);
     for (int k = 0; k < 2 && !rc; k++) rc = r->alloc_queue(r->qa[k]);
     for (int k = 0; k < 2 && !rc; k++) rc = r->alloc_queue(r->qs[k]);
     if (!rc) rc = r->alloc(&r->ha.kind, C);
@@ -971,7 +1138,13 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     if (!rc) rc = launch_set_wave(r, 0u);
     if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
-    for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
+    if (r->plain) {
+        hipLaunchKernelGGL(megakernel, dim3((unsigned)((pixels + MEGA_BLOCK - 1) / MEGA_BLOCK)), dim3(MEGA_BLOCK), 0,
+                           s, scene_args(r->scene), r->sc, r->reqs);
+        PSY_HIP(hipGetLastError());
+    } else {
+        for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
+    }
     PSY_HIP(hipEventRecord(r->ev1, s));
     if (rc) {
         (void)hipStreamSynchronize(s);
@@ -986,7 +1159,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
         PSY_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
         stats->paths = total;
         stats->rays = 0;
-        stats->waves = waves;
+        stats->waves = r->plain ? 0 : waves;
         stats->trace_ms = ms;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -995,7 +1168,9 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
 
 int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     if (!r) return fail(PSY_E_INVALID, "renderer is NULL");
-    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT)) return fail(PSY_E_INVALID, "unknown option flag");
+    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT | PSY_RENDER_PLAIN))
+        return fail(PSY_E_INVALID, "unknown option flag");
+    r->plain = (flags & PSY_RENDER_PLAIN) != 0;
     const bool graphs = (flags & PSY_RENDER_NO_GRAPHS) == 0, sort = (flags & PSY_RENDER_NO_SORT) == 0;
     if (sort != r->sort_materials) {
         for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);

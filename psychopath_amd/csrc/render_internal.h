@@ -89,6 +89,10 @@ struct Paths {
     float4* color;
     float4* atten;
     float4* pending;
+    uint32_t* has_next;  // a bounce ray is kept (see PathRegs in render.hip)
+    float4* next_o;
+    float4* next_d;
+    float4* next_fac;
 };
 
 // A ray queue: the path slot of each ray and the rays as SoA.
@@ -129,6 +133,7 @@ struct psy_renderer {
     psy_trace_tuning tuning{PSY_KERNEL_PERSISTENT, 16, 16, 16, 4};
     bool use_graphs = true;
     bool sort_materials = true;
+    bool plain = false;  // one megakernel launch per render
 
     std::vector<DevBuf*> bufs;
     ShadeScene sc{};

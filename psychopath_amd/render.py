@@ -49,8 +49,11 @@ class Renderer:
             _lib._lib.psy_renderer_destroy(h)
             self.handle = None
 
-    def set_options(self, graphs: bool = True, sort: bool = True) -> None:
-        flags = (0 if graphs else _lib.PSY_RENDER_NO_GRAPHS) | (0 if sort else _lib.PSY_RENDER_NO_SORT)
+    def set_options(self, graphs: bool = True, sort: bool = True, plain: bool = False) -> None:
+        """graphs/sort: wavefront execution choices; plain: one launch per
+        render over the whole frame.  None of them changes the image."""
+        flags = ((0 if graphs else _lib.PSY_RENDER_NO_GRAPHS) | (0 if sort else _lib.PSY_RENDER_NO_SORT) |
+                 (_lib.PSY_RENDER_PLAIN if plain else 0))
         check("psy_renderer_set_options", lib().psy_renderer_set_options(self.handle, flags))
 
     def tuning(self) -> dict:

@@ -53,8 +53,9 @@ def test_cornell_box_seed_and_depth():
 
 
 def test_execution_options_do_not_change_results():
-    """Graph replay vs direct launches, material sort vs queue order, and one
-    wave vs many small waves give bit-identical images."""
+    """Graph replay vs direct launches, material sort vs queue order, one
+    wave vs many small waves, and the one-launch plain renderer give
+    bit-identical images."""
     sc = scenes.cornell_box(64, 64, 4, 4, 0)
     base, _, r = gpu_render(sc)
     r.set_options(graphs=False)
@@ -63,10 +64,22 @@ def test_execution_options_do_not_change_results():
     b, _ = r.render(sc.camera, sc.settings)
     r.set_options(graphs=True, sort=True)
     again, _ = r.render(sc.camera, sc.settings)
+    r.set_options(plain=True)
+    plain, _ = r.render(sc.camera, sc.settings)
     small, stats, _ = gpu_render(sc, max_paths=3000)
     assert stats["waves"] > 1
-    for other in (a, b, again, small):
+    for other in (a, b, again, plain, small):
         assert np.array_equal(base, other)
+
+
+def test_plain_single_launch_matches_wavefront():
+    """The one-launch renderer gives the wavefront renderer's image on a
+    scene with motion, instancing, sphere lights, GGX and depth of field."""
+    sc = scenes.motion_instances(grid=3, levels=3, lights=3, width=48, height=27, spp=4, max_bounces=5)
+    img, _, r = gpu_render(sc)
+    r.set_options(plain=True)
+    plain, _ = r.render(sc.camera, sc.settings)
+    assert np.array_equal(img, plain)
 
 
 def test_motion_blur_instancing_sphere_lights_dof():
