@@ -250,13 +250,27 @@ def render_bench(args) -> dict:
         img, st = r.render(sc.camera, sc.settings)
         times.append(time.perf_counter() - t)
     paths = st["paths"]
-    res["cornell_plain"] = {
+    res["cornell_wavefront"] = {
         "workload": "Cornell box 512x512, 16 spp, depth 4, seed 0",
         "samples_per_s": round(paths / float(np.median(times)), 1),
         "ms_per_render": round(float(np.median(times)) * 1e3, 2),
         "device_ms_per_render": round(st["trace_ms"], 2),
     }
-    log(f"cornell plain: {res['cornell_plain']}")
+    r.set_options(plain=True)
+    pimg, _ = r.render(sc.camera, sc.settings)
+    ptimes = []
+    for _ in range(args.render_renders):
+        t = time.perf_counter()
+        r.render(sc.camera, sc.settings)
+        ptimes.append(time.perf_counter() - t)
+    r.set_options(plain=False)
+    res["cornell_plain"] = {
+        "workload": "Cornell box 512x512, 16 spp, depth 4, seed 0, one launch per render",
+        "samples_per_s": round(paths / float(np.median(ptimes)), 1),
+        "ms_per_render": round(float(np.median(ptimes)) * 1e3, 2),
+        "same_image_as_wavefront": bool(np.array_equal(pimg, img)),
+    }
+    log(f"cornell wavefront: {res['cornell_wavefront']}; plain: {res['cornell_plain']}")
     if not args.no_cpu:
         from oracle import psyref
         small = scenes.cornell_box(128, 128, 16, 4, 0)
@@ -303,19 +317,33 @@ def render_bench(args) -> dict:
     for k, (_, sd) in enumerate(scene_defs):
         rens[k].render(sd.camera, dict(width=64, height=64, spp=1, max_bounces=sd.settings["max_bounces"], seed=0))
 
-    # plain: one render after another
-    service = []
-    t = time.perf_counter()
-    for _, k, cam, st in reqs:
-        a = time.perf_counter()
-        rens[k].render(cam, st)
-        service.append(time.perf_counter() - a)
-    plain_s = time.perf_counter() - t
-    # the same arrivals served first-come first-served by the plain path
-    done, plain_lat = 0.0, []
-    for (ta, _, _, _), sv in zip(reqs, service):
-        done = max(done, ta) + sv
-        plain_lat.append(done - ta)
+    def sequential(plain):
+        """One render after another; returns (seconds, service times)."""
+        for rr in rens:
+            rr.set_options(plain=plain)
+        service = []
+        t = time.perf_counter()
+        for _, k, cam, st in reqs:
+            a = time.perf_counter()
+            rens[k].render(cam, st)
+            service.append(time.perf_counter() - a)
+        total = time.perf_counter() - t
+        for rr in rens:
+            rr.set_options(plain=False)
+        return total, service
+
+    def fcfs_latency(service):
+        """The same arrivals served first-come first-served, one at a time."""
+        done, lat = 0.0, []
+        for (ta, _, _, _), sv in zip(reqs, service):
+            done = max(done, ta) + sv
+            lat.append(done - ta)
+        return lat
+
+    plain_s, plain_service = sequential(True)
+    wave_s, wave_service = sequential(False)
+    plain_lat = fcfs_latency(plain_service)
+    wave_lat = fcfs_latency(wave_service)
 
     server = render.Server(rens, hbm_budget=200 * (1 << 30))
     ids = [None] * len(reqs)
@@ -347,13 +375,17 @@ def render_bench(args) -> dict:
         "paths": int(total_paths),
         "served_samples_per_s": round(total_paths / served_s, 1),
         "plain_samples_per_s": round(total_paths / plain_s, 1),
-        "speedup": round(plain_s / served_s, 3),
+        "speedup_vs_plain": round(plain_s / served_s, 3),
+        "wavefront_sequential_samples_per_s": round(total_paths / wave_s, 1),
         "served_latency_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
         "served_latency_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
         "plain_latency_p50_ms": round(float(np.percentile(plain_lat, 50)) * 1e3, 1),
         "plain_latency_p99_ms": round(float(np.percentile(plain_lat, 99)) * 1e3, 1),
+        "wavefront_latency_p50_ms": round(float(np.percentile(wave_lat, 50)) * 1e3, 1),
+        "wavefront_latency_p99_ms": round(float(np.percentile(wave_lat, 99)) * 1e3, 1),
         "served_seconds": round(served_s, 3),
         "plain_seconds": round(plain_s, 3),
+        "plain_form": "one megakernel launch per render over the whole frame, requests one after another",
         "waves": sst["waves"],
     }
     log(f"serving: {res['serving']}")

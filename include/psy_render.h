@@ -124,6 +124,15 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
 #define PSY_RENDER_PLAIN 4u
 int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
+/* Autotuner: renders `trial` (a reduced version of the intended renders)
+ * with every candidate launch shape of the traversal kernel (waves per CU,
+ * refill threshold, LDS stack depth) with and without material sorting,
+ * keeps the fastest on the renderer and returns it (either output may be
+ * NULL; `best_flags` is 0 or PSY_RENDER_NO_SORT).  Images do not depend on
+ * the choice. */
+int psy_renderer_autotune(psy_renderer* renderer, const psy_camera_desc* camera, const psy_render_settings* trial,
+                          psy_trace_tuning* best_tuning, uint32_t* best_flags);
+
 /* ---- request server ---------------------------------------------------
  * Serves render requests against several renderers (one device-resident
  * scene each) with continuous batching.  Requests queue on submission and

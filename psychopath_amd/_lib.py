@@ -218,6 +218,8 @@ _SIGNATURES = {
     "psy_renderer_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
     "psy_renderer_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
     "psy_renderer_set_options": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "psy_renderer_autotune": (C.c_int, [C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderSettings),
+                                        C.POINTER(TraceTuning), C.POINTER(C.c_uint32)]),
 }
 
 PSY_RENDER_NO_GRAPHS = 1

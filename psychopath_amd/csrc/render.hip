@@ -1181,6 +1181,59 @@ int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     return PSY_OK;
 }
 
+int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy_render_settings* trial,
+                          psy_trace_tuning* best_tuning, uint32_t* best_flags) {
+    if (!r || !cam || !trial) return fail(PSY_E_INVALID, "NULL argument");
+    ReqParams rp;
+    int rc = make_params(cam, trial, rp);
+    if (rc) return rc;
+    const uint64_t pixels = (uint64_t)trial->width * trial->height;
+    float* film = nullptr;
+    rc = r->alloc(&film, pixels * 3);
+    if (rc) return rc;
+    const psy_trace_tuning saved_t = r->tuning;
+    const bool saved_sort = r->sort_materials, saved_plain = r->plain;
+    r->plain = false;
+    psy_trace_tuning best_t = saved_t;
+    bool best_sort = saved_sort;
+    float best_ms = 1e30f;
+    for (uint32_t sort = 0; sort < 2 && !rc; sort++) {
+        for (uint32_t w : {8u, 16u, 32u}) {
+            for (uint32_t refill : {8u, 16u, 32u}) {
+                for (uint32_t lds : {16u, 24u}) {
+                    const psy_trace_tuning t{PSY_KERNEL_PERSISTENT, w, refill, lds, 4};
+                    r->tuning = t;
+                    r->sort_materials = sort != 0;
+                    drop_graphs(r);
+                    float ms_min = 1e30f;
+                    for (int rep = 0; rep < 3 && !rc; rep++) {
+                        psy_render_stats st;
+                        rc = psy_render(r, cam, trial, film, PSY_MEM_DEVICE, &st);
+                        if (rep > 0) ms_min = std::min(ms_min, (float)st.trace_ms);
+                    }
+                    if (rc) break;
+                    if (ms_min < best_ms) {
+                        best_ms = ms_min;
+                        best_t = t;
+                        best_sort = sort != 0;
+                    }
+                }
+                if (rc) break;
+            }
+            if (rc) break;
+        }
+    }
+    r->release(film);
+    r->tuning = rc ? saved_t : best_t;
+    r->sort_materials = rc ? saved_sort : best_sort;
+    r->plain = saved_plain;
+    drop_graphs(r);
+    if (rc) return rc;
+    if (best_tuning) *best_tuning = best_t;
+    if (best_flags) *best_flags = best_sort ? 0u : PSY_RENDER_NO_SORT;
+    return PSY_OK;
+}
+
 int psy_renderer_get_tuning(const psy_renderer* r, psy_trace_tuning* t) {
     if (!r || !t) return fail(PSY_E_INVALID, "NULL argument");
     *t = r->tuning;

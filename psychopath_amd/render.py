@@ -56,6 +56,19 @@ class Renderer:
                  (_lib.PSY_RENDER_PLAIN if plain else 0))
         check("psy_renderer_set_options", lib().psy_renderer_set_options(self.handle, flags))
 
+    def autotune(self, camera: dict, trial: dict) -> dict:
+        """Picks the traversal launch shape and material sorting by timing
+        `trial` renders; returns the choice."""
+        cam, _xf = self._camera(camera)
+        st = self._settings(trial)
+        t = _lib.TraceTuning()
+        flags = C.c_uint32()
+        check("psy_renderer_autotune", lib().psy_renderer_autotune(self.handle, C.byref(cam), C.byref(st),
+                                                                    C.byref(t), C.byref(flags)))
+        out = {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
+        out["sort"] = not (flags.value & _lib.PSY_RENDER_NO_SORT)
+        return out
+
     def tuning(self) -> dict:
         t = _lib.TraceTuning()
         check("psy_renderer_get_tuning", lib().psy_renderer_get_tuning(self.handle, C.byref(t)))

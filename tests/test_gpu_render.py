@@ -180,3 +180,13 @@ def test_server_budget_admission():
     with pytest.raises(PsyError):
         server.submit(0, cornell.camera, dict(width=256, height=256, spp=1, max_bounces=1, seed=0))
     server.close()
+
+
+def test_autotune_keeps_images():
+    sc = scenes.cornell_box(48, 48, 4, 3, 0)
+    r = render.build(sc)
+    before, _ = r.render(sc.camera, sc.settings)
+    choice = r.autotune(sc.camera, dict(width=32, height=32, spp=2, max_bounces=3, seed=0))
+    assert choice["waves_per_cu"] in (8, 16, 32) and choice["stack_lds"] in (16, 24)
+    after, _ = r.render(sc.camera, sc.settings)
+    assert np.array_equal(before, after)

@@ -262,8 +262,8 @@ def test_small_light_is_sampled_by_nee(light_radiance):
     """A light of tiny solid angle is found by light sampling alone: the
     plane's radiance is albedo * F * L_e / 2.  The half is the reference's
     own: rectangle-light samples carry colour/area * 0.5
-    (rectangle_light.rs:186,232) while a ray that hits the light sees
-    colour/area (rectangle_light.rs:314-316)."""
+    (rectangle_light.rs:191,235) while a ray that hits the light sees
+    colour/area (rectangle_light.rs:317-318)."""
     img = _plane_under_light(0.05, 1)
     ratio = img.mean(0) / light_radiance
     exp = 0.5 * _view_factor_centre(0.05) * 0.5
