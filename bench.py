@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--render-only", action="store_true", help="only the rendering and serving measurements")
     ap.add_argument("--requests", type=int, default=16, help="requests in the serving run")
     ap.add_argument("--render-renders", type=int, default=3, help="timed plain Cornell renders")
+    ap.add_argument("--no-large", action="store_true", help="skip the large instanced scene")
+    ap.add_argument("--large-spp", type=int, default=128, help="spp of the large instanced scene at 3840x2160")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -389,7 +391,73 @@ def render_bench(args) -> dict:
         "waves": sst["waves"],
     }
     log(f"serving: {res['serving']}")
+    del rens, server
+    if not args.no_large:
+        res["large_scene"] = large_bench(args)
     return res
+
+
+def large_bench(args) -> dict:
+    """The batched large-scene workload (4096 instances of eight meshes of
+    10K-1M triangles, four material families, 1024 sphere lights and a sun,
+    3840x2160, depth 8): the wavefront renderer with its autotuned launch
+    shape against the one-launch plain renderer, one render each, and the
+    oracle renderer on the host over a small sample of the same frame."""
+    import numpy as np
+
+    from psychopath_amd import render, scenes
+
+    t0 = time.time()
+    sc = scenes.large_scene(spp=args.large_spp)
+    r = render.build(sc)
+    log(f"large scene built in {time.time() - t0:.1f}s")
+    trial = dict(width=480, height=270, spp=4, max_bounces=8, seed=0)
+    t1 = time.time()
+    choice = r.autotune(sc.camera, trial)
+    log(f"large scene autotune {time.time() - t1:.1f}s -> {choice}")
+    t = time.perf_counter()
+    img, st = r.render(sc.camera, sc.settings)
+    wave_s = time.perf_counter() - t
+    log(f"large scene wavefront: {wave_s:.2f}s, {st}")
+    r.set_options(plain=True)
+    t = time.perf_counter()
+    pimg, _ = r.render(sc.camera, sc.settings)
+    plain_s = time.perf_counter() - t
+    log(f"large scene plain: {plain_s:.2f}s")
+    paths = st["paths"]
+    out = {
+        "workload": f"large instanced scene 3840x2160, {args.large_spp} spp, depth 8: 4096 instances of 8 meshes "
+                    "(2,099,040 unique, 1.07G instanced triangles), Lambertian/GGX metal/dielectric coat/emissive, "
+                    "1024 sphere lights + sun",
+        "paths": int(paths),
+        "wavefront_samples_per_s": round(paths / wave_s, 1),
+        "plain_samples_per_s": round(paths / plain_s, 1),
+        "speedup_vs_plain": round(plain_s / wave_s, 3),
+        "wavefront_seconds": round(wave_s, 3),
+        "plain_seconds": round(plain_s, 3),
+        "autotuned": choice,
+        "same_image_as_plain": bool(np.array_equal(img, pimg)),
+        "finite": bool(np.isfinite(img).all()),
+    }
+    del img, pimg
+    if not args.no_cpu:
+        from oracle import psyref
+        small = scenes.large_scene(width=384, height=216, spp=32)
+        ref, info = psyref.render(small, threads=args.cpu_threads or min(16, os.cpu_count() or 1))
+        r.set_options(plain=False)
+        gimg, _ = r.render(small.camera, small.settings)
+        ok = np.abs(gimg.astype(np.float64) - ref) <= 1e-3 * np.abs(ref.astype(np.float64)) + 1e-9
+        out["cpu_baseline"] = {
+            "value": round(384 * 216 * 32 / info["wall_seconds"], 1),
+            "unit": "samples/s",
+            "cores": info["threads"],
+            "kind": "port",
+            "sample": "the same scene at 384x216, 32 spp, depth 8 (1/400 of the workload's paths), psyref "
+                      "renderer (C++ restatement of renderer.rs)",
+        }
+        out["parity_384x216"] = round(float(ok.all(axis=-1).mean()), 5)
+        log(f"large scene cpu: {out['cpu_baseline']['value']} samples/s; parity {out['parity_384x216']}")
+    return out
 
 
 if __name__ == "__main__":

@@ -118,18 +118,26 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
  * binning them by closure kind first; PSY_RENDER_PLAIN renders in one
  * kernel launch over the whole frame (one lane per pixel tracing and
  * shading its samples' paths to the end, no wavefronts), the baseline the
- * wavefront and served forms are measured against. */
+ * wavefront and served forms are measured against.
+ * PSY_RENDER_SHADE_WAVES(n), n = 1..4, selects the build of the shading
+ * kernels compiled for at least n waves per SIMD (a register budget of
+ * 512/n per lane; tighter budgets spill to scratch); 0 in those bits keeps
+ * the current choice. */
 #define PSY_RENDER_NO_GRAPHS 1u
 #define PSY_RENDER_NO_SORT 2u
 #define PSY_RENDER_PLAIN 4u
+#define PSY_RENDER_SHADE_WAVES_SHIFT 8u
+#define PSY_RENDER_SHADE_WAVES_MASK (7u << PSY_RENDER_SHADE_WAVES_SHIFT)
+#define PSY_RENDER_SHADE_WAVES(n) ((uint32_t)(n) << PSY_RENDER_SHADE_WAVES_SHIFT)
 int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
 /* Autotuner: renders `trial` (a reduced version of the intended renders)
  * with every candidate launch shape of the traversal kernel (waves per CU,
- * refill threshold, LDS stack depth) with and without material sorting,
- * keeps the fastest on the renderer and returns it (either output may be
- * NULL; `best_flags` is 0 or PSY_RENDER_NO_SORT).  Images do not depend on
- * the choice. */
+ * refill threshold, LDS stack depth), then every shading register budget,
+ * then with and without material sorting, keeps the fastest on the renderer
+ * and returns it (either output may be NULL; `best_flags` holds
+ * PSY_RENDER_NO_SORT if sorting lost and PSY_RENDER_SHADE_WAVES(n) of the
+ * chosen budget).  Images do not depend on the choice. */
 int psy_renderer_autotune(psy_renderer* renderer, const psy_camera_desc* camera, const psy_render_settings* trial,
                           psy_trace_tuning* best_tuning, uint32_t* best_flags);
 

@@ -1,0 +1,784 @@
+"""Oracle restatement of the reference's .psy scene reader (TEST
+INFRASTRUCTURE ONLY: imported by tests/ as the checker of the product's
+C++ reader, never by the product).
+
+Follows:
+    DataTree::from_str, parse_node, parse_leaf_content, next_token,
+    skip_ws_and_comments                  src/parse/data_tree.rs:22-45, 292-563
+    ws_f32 / ws_u32 / ws_usize            src/parse/basics.rs:16-26
+    parse_scene, parse_output_info, parse_render_settings, parse_camera,
+    parse_world, parse_matrix, parse_color src/parse/psy.rs:99-615
+    parse_assembly                         src/parse/psy_assembly.rs:18-188
+    parse_mesh_surface                     src/parse/psy_mesh_surface.rs:25-127
+    parse_surface_shader                   src/parse/psy_surface_shader.rs:21-139
+    parse_distant_disk_light, parse_sphere_light, parse_rectangle_light
+                                           src/parse/psy_light.rs:16-184
+    AssemblyBuilder name rules             src/scene/assembly.rs:185-284
+    Camera::new defaults                   src/camera.rs:22-72
+
+Floats: nom 5.1.2's `float` hands the text to lexical-core 0.7.6 (neither is
+vendored); the grammar restated here is sign, digits with an optional
+fraction (or a fraction alone), an optional exponent with digits, or
+nan / inf / infinity, parsed with correct rounding to f32.  Parity of the
+corner cases of that dependency (its exact special-value spellings) is
+unpinned; the reference's own basics.rs tests pin the rest.
+
+Not restated: nested assemblies (parse_assembly's "Assembly" children) raise
+here as they do in the product reader, which flattens no assembly tree.
+
+The result is a plain dict: settings, camera, world, shaders, meshes,
+lights and instances in the layout of psychopath_amd.scenes.RenderScene
+(transforms are the 12 values a..l of Transform::new_from_values).
+"""
+
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+
+WS = "\n\r\t "
+RESERVED = "{}[]$#\\"
+
+
+class DataTreeError(Exception):
+    """data_tree.rs ParseError: kind and byte offset."""
+
+    def __init__(self, kind: str, offset: int):
+        super().__init__(f"{kind} at byte {offset}")
+        self.kind = kind
+        self.offset = offset
+
+
+class PsyParseError(Exception):
+    """psy.rs PsyParseError: kind, byte offset and message."""
+
+    def __init__(self, kind: str, offset: int, message: str):
+        super().__init__(f"{kind} at byte {offset}: {message}")
+        self.kind = kind
+        self.offset = offset
+        self.message = message
+
+
+def _is_ident_char(c: str) -> bool:
+    return c not in WS and c not in RESERVED
+
+
+# ---------------------------------------------------------------- data tree
+# Offsets are in bytes of the UTF-8 text, as the reference's are; the text is
+# handled as a bytes object so that slicing matches.
+
+def _skip_ws_and_comments(b: bytes, pos: int) -> int:
+    while True:
+        start = pos
+        while pos < len(b) and chr(b[pos]) in WS:
+            pos += 1
+        if pos < len(b) and b[pos] == ord("#"):
+            while pos < len(b) and b[pos] not in (10, 13):
+                pos += 1
+        if pos == start:
+            return pos
+
+
+def next_token(b: bytes, pos: int):
+    """(kind, text, new_pos); kind in OpenInner CloseInner OpenLeaf CloseLeaf
+    TypeName Ident End Unknown (data_tree.rs:411-484)."""
+    p = _skip_ws_and_comments(b, pos)
+    if p >= len(b):
+        return "End", None, p
+    c = chr(b[p])
+    if c == "{":
+        return "OpenInner", None, p + 1
+    if c == "}":
+        return "CloseInner", None, p + 1
+    if c == "[":
+        return "OpenLeaf", None, p + 1
+    if c == "]":
+        return "CloseLeaf", None, p + 1
+    if c == "$":
+        q = p + 1
+        escaped = False
+        while q < len(b):
+            ch = chr(b[q])
+            if escaped:
+                escaped = False
+            elif ch == "\\":
+                escaped = True
+            elif not _is_ident_char(ch):
+                break
+            q += 1
+        return "Ident", b[p:q].decode("utf-8"), q
+    if _is_ident_char(c):
+        q = p
+        while q < len(b) and _is_ident_char(chr(b[q])):
+            q += 1
+        return "TypeName", b[p:q].decode("utf-8"), q
+    return "Unknown", None, p
+
+
+def _parse_leaf_content(b: bytes, pos: int):
+    q = pos
+    escaped = False
+    while q < len(b):
+        ch = chr(b[q])
+        if escaped:
+            escaped = False
+        elif ch == "\\":
+            escaped = True
+        elif ch == "]":
+            break
+        q += 1
+    return b[pos:q].decode("utf-8"), q
+
+
+def _parse_node(b: bytes, pos: int):
+    tok, text, p1 = next_token(b, pos)
+    if tok != "TypeName":
+        return None
+    type_name = text
+    tok2, text2, p2 = next_token(b, p1)
+    if tok2 == "Ident":
+        tok3, _, p3 = next_token(b, p2)
+        if tok3 != "OpenInner":
+            raise DataTreeError("MissingOpenInternal", p2)
+        children, rem = _parse_children(b, p3)
+        tok4, _, p4 = next_token(b, rem)
+        if tok4 != "CloseInner":
+            raise DataTreeError("MissingCloseInternal", rem)
+        return dict(internal=True, type=type_name, ident=text2, children=children, offset=p1), p4
+    if tok2 == "OpenInner":
+        children, rem = _parse_children(b, p2)
+        tok3, _, p3 = next_token(b, rem)
+        if tok3 != "CloseInner":
+            raise DataTreeError("MissingCloseInternal", rem)
+        return dict(internal=True, type=type_name, ident=None, children=children, offset=p1), p3
+    if tok2 == "OpenLeaf":
+        contents, p3 = _parse_leaf_content(b, p2)
+        tok4, _, p4 = next_token(b, p3)
+        if tok4 != "CloseLeaf":
+            raise DataTreeError("MissingCloseLeaf", p3)
+        return dict(internal=False, type=type_name, contents=contents, offset=p1), p4
+    raise DataTreeError("MissingOpener", p1)
+
+
+def _parse_children(b: bytes, pos: int):
+    children = []
+    while True:
+        r = _parse_node(b, pos)
+        if r is None:
+            return children, pos
+        node, pos = r
+        children.append(node)
+
+
+def parse_data_tree(text) -> dict:
+    """DataTree::from_str (data_tree.rs:22-45)."""
+    b = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+    items, pos = _parse_children(b, 0)
+    pos = _skip_ws_and_comments(b, pos)
+    if pos != len(b):
+        raise DataTreeError("Other", 0)
+    return dict(internal=True, type="ROOT", ident=None, children=items, offset=0)
+
+
+def dump_tree(node: dict, depth: int = 0) -> str:
+    """Canonical text of a tree: one line per node, `I type ident offset` or
+    `L type offset contents` (contents with backslash and newline escaped)."""
+    pad = "  " * depth
+    if node["internal"]:
+        out = [f"{pad}I {node['type']} {node['ident'] if node['ident'] is not None else '-'} {node['offset']}\n"]
+        for c in node["children"]:
+            out.append(dump_tree(c, depth + 1))
+        return "".join(out)
+    cont = node["contents"].replace("\\", "\\\\").replace("\n", "\\n").replace("\r", "\\r")
+    return f"{pad}L {node['type']} {node['offset']} {cont}\n"
+
+
+# ------------------------------------------------------------- leaf numbers
+
+_MULTISPACE = " \t\r\n"
+
+
+def _skip_ms(s: str, i: int) -> int:
+    while i < len(s) and s[i] in _MULTISPACE:
+        i += 1
+    return i
+
+
+def _f32(x: float) -> float:
+    return struct.unpack("<f", struct.pack("<f", x))[0] if math.isfinite(x) else x
+
+
+def _round_f32(text: str) -> float:
+    """Correctly rounded decimal -> f32 (round half to even on the exact
+    value, overflow to infinity)."""
+    from fractions import Fraction
+    fr = Fraction(text)
+    if fr == 0:
+        return -0.0 if text.lstrip().startswith("-") else 0.0
+    neg = fr < 0
+    fr = abs(fr)
+    # f32: 24-bit significand, exponents -149 (subnormal) .. 127
+    e = fr.numerator.bit_length() - fr.denominator.bit_length()
+    if Fraction(2) ** e > fr:
+        e -= 1
+    e = max(e, -126)
+    scale = Fraction(2) ** (e - 23)
+    q = fr / scale
+    n = q.numerator // q.denominator
+    rem = q - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    val = n * scale
+    if val >= Fraction(2) ** 128:
+        v = math.inf
+    else:
+        v = float(val)
+    return -v if neg else v
+
+
+def parse_float_prefix(s: str, i: int):
+    """The float grammar at s[i:]: (value, end) or None."""
+    j = i
+    if j < len(s) and s[j] in "+-":
+        j += 1
+    low = s[j:j + 8].lower()
+    for word in ("infinity", "inf", "nan"):
+        if low.startswith(word):
+            end = j + len(word)
+            neg = s[i] == "-"
+            v = math.nan if word == "nan" else math.inf
+            return (-v if neg else v), end
+    k = j
+    while k < len(s) and s[k].isdigit() and s[k].isascii():
+        k += 1
+    nd = k - j
+    frac = 0
+    if k < len(s) and s[k] == ".":
+        m = k + 1
+        while m < len(s) and s[m].isdigit() and s[m].isascii():
+            m += 1
+        frac = m - k - 1
+        if nd == 0 and frac == 0:
+            return None
+        k = m
+    elif nd == 0:
+        return None
+    if k < len(s) and s[k] in "eE":
+        m = k + 1
+        if m < len(s) and s[m] in "+-":
+            m += 1
+        e0 = m
+        while m < len(s) and s[m].isdigit() and s[m].isascii():
+            m += 1
+        if m == e0:
+            return None
+        k = m
+    return _round_f32(s[i:k]), k
+
+
+def ws_f32(s: str, i: int = 0):
+    """basics.rs:16-18: (value, rest index) or None."""
+    j = _skip_ms(s, i)
+    r = parse_float_prefix(s, j)
+    if r is None:
+        return None
+    v, k = r
+    return v, _skip_ms(s, k)
+
+
+def ws_uint(s: str, i: int = 0, limit: int = (1 << 32) - 1):
+    """basics.rs:20-26 (u32, or usize with limit 2^64-1)."""
+    j = _skip_ms(s, i)
+    k = j
+    while k < len(s) and s[k].isdigit() and s[k].isascii():
+        k += 1
+    if k == j:
+        return None
+    v = int(s[j:k])
+    if v > limit:
+        return None
+    return v, _skip_ms(s, k)
+
+
+def _all_f32(s: str, n: int):
+    """all_consuming(tuple of n ws_f32)."""
+    vals, i = [], 0
+    for _ in range(n):
+        r = ws_f32(s, i)
+        if r is None:
+            return None
+        v, i = r
+        vals.append(v)
+    return vals if i == len(s) else None
+
+
+def _tuple_f32(s: str, n: int):
+    """tuple of n ws_f32, not necessarily consuming all: (vals, rest)."""
+    vals, i = [], 0
+    for _ in range(n):
+        r = ws_f32(s, i)
+        if r is None:
+            return None
+        v, i = r
+        vals.append(v)
+    return vals, i
+
+
+# -------------------------------------------------------------- scene parse
+
+def _children(node, type_name):
+    return [c for c in node.get("children", []) if c["type"] == type_name]
+
+
+def _leaves(node, type_name):
+    return [c for c in node.get("children", []) if c["type"] == type_name and not c["internal"]]
+
+
+def parse_color(contents: str):
+    """psy.rs:577-615 -> (kind, a, b, c) with rec709 converted to XYZ
+    (color.rs rec709_e_to_xyz, restated in oracle psyref)."""
+    items = [t.strip() for t in contents.split(",")]
+    if len(items) != 2:
+        raise PsyParseError("UnknownError", 0, "bad colour")
+    if items[0] == "rec709":
+        r = _tuple_f32(items[1], 3)
+        if r is None:
+            raise PsyParseError("UnknownError", 0, "bad colour")
+        return ("rec709",) + tuple(r[0])
+    if items[0] in ("blackbody", "color_temperature"):
+        r = _tuple_f32(items[1], 2)
+        if r is None:
+            raise PsyParseError("UnknownError", 0, "bad colour")
+        return (items[0],) + tuple(r[0])
+    raise PsyParseError("UnknownError", 0, "bad colour")
+
+
+def parse_matrix(contents: str):
+    """psy.rs:556-572: 16 column-major values -> a..l."""
+    v = _all_f32(contents, 16)
+    if v is None:
+        raise PsyParseError("UnknownError", 0, "bad matrix")
+    return [v[0], v[4], v[8], v[12], v[1], v[5], v[9], v[13], v[2], v[6], v[10], v[14]]
+
+
+def _transform_error(offset):
+    return PsyParseError("IncorrectLeafData", offset,
+                         "Transform should be sixteen integers specified in the form "
+                         "'[# # # # # # # # # # # # # # # #]'.")
+
+
+def parse_scene(tree: dict) -> dict:
+    """psy.rs:99-200 for the Scene node `tree`."""
+    for sec, msg in (("Output", "Scene should have precisely one Output section."),
+                     ("RenderSettings", "Scene should have precisely one RenderSettings section."),
+                     ("Camera", "Scene should have precisely one Camera section."),
+                     ("World", "Scene should have precisely one World section."),
+                     ("Assembly", "Scene should have precisely one Root Assembly section.")):
+        n = len(_children(tree, sec))
+        if n != 1:
+            raise PsyParseError("WrongNodeCount", tree["offset"], msg)
+    out = _parse_output(_children(tree, "Output")[0])
+    res, spp, seed = _parse_render_settings(_children(tree, "RenderSettings")[0])
+    camera = _parse_camera(_children(tree, "Camera")[0])
+    world = _parse_world(_children(tree, "World")[0])
+    asm = _parse_assembly(_children(tree, "Assembly")[0])
+    return dict(name=tree.get("ident"), output=out, width=res[0], height=res[1], spp=spp, seed=seed,
+                camera=camera, world=world, assembly=asm)
+
+
+def _parse_output(node):
+    if not node["internal"]:
+        raise PsyParseError("ExpectedInternalNode", node["offset"], "Output section should be an internal node")
+    path = None
+    for c in node["children"]:
+        if not c["internal"] and c["type"] == "Path":
+            tc = c["contents"].strip()
+            if len(tc) < 2:
+                raise PsyParseError("IncorrectLeafData", c["offset"], "File path format is incorrect.")
+            if tc[0] != '"' or not tc.endswith('"'):
+                raise PsyParseError("IncorrectLeafData", c["offset"], "File paths must be surrounded by quotes.")
+            path = tc[1:-1]
+    if path is None:
+        raise PsyParseError("MissingNode", node["offset"], "Output section must contain a Path.")
+    return path
+
+
+def _parse_render_settings(node):
+    if not node["internal"]:
+        raise PsyParseError("ExpectedInternalNode", node["offset"], "RenderSettings section should be internal")
+    res = spp = None
+    seed = 0
+    for c in node["children"]:
+        if c["internal"]:
+            continue
+        if c["type"] == "Resolution":
+            a = ws_uint(c["contents"])
+            b = ws_uint(c["contents"], a[1]) if a else None
+            if not a or not b or b[1] != len(c["contents"]):
+                raise PsyParseError("IncorrectLeafData", c["offset"], "Resolution")
+            res = (a[0], b[0])
+        elif c["type"] == "SamplesPerPixel":
+            a = ws_uint(c["contents"])
+            if not a or a[1] != len(c["contents"]):
+                raise PsyParseError("IncorrectLeafData", c["offset"], "SamplesPerPixel")
+            spp = a[0]
+        elif c["type"] == "Seed":
+            a = ws_uint(c["contents"])
+            if not a or a[1] != len(c["contents"]):
+                raise PsyParseError("IncorrectLeafData", c["offset"], "Seed")
+            seed = a[0]
+    if res is None or spp is None:
+        raise PsyParseError("MissingNode", node["offset"],
+                            "RenderSettings must have both Resolution and SamplesPerPixel specified.")
+    return res, spp, seed
+
+
+def _parse_camera(node):
+    if not node["internal"]:
+        raise PsyParseError("ExpectedInternalNode", node["offset"], "Camera section should be internal")
+    mats, fovs, fds, ars = [], [], [], []
+    for c in node["children"]:
+        if c["internal"]:
+            continue
+        t = c["type"]
+        if t in ("Fov", "FocalDistance", "ApertureRadius"):
+            v = _all_f32(c["contents"], 1)
+            if v is None:
+                raise PsyParseError("IncorrectLeafData", c["offset"], t)
+            if t == "Fov":
+                fovs.append(_f32(v[0] * _f32(math.pi / 180.0)))
+            elif t == "FocalDistance":
+                fds.append(v[0])
+            else:
+                ars.append(v[0])
+        elif t == "Transform":
+            try:
+                mats.append(parse_matrix(c["contents"]))
+            except PsyParseError:
+                raise _transform_error(c["offset"])
+    if not mats or not fovs:
+        raise PsyParseError("MissingNode", node["offset"], "Camera has no transform(s) or fov(s)")
+    # Camera::new (camera.rs:31-58)
+    if not ars or not fds:
+        ars, fds = [0.0], [1.0]
+    if any(d == 0.0 for d in fds):
+        ars, fds = [0.0], [1.0]
+    return dict(xforms=mats, fovs=fovs, apertures=ars, focus=fds)
+
+
+def _parse_world(node):
+    if not node["internal"]:
+        raise PsyParseError("ExpectedInternalNode", node["offset"], "World section should be internal")
+    bgs = _children(node, "BackgroundShader")
+    if len(bgs) != 1:
+        raise PsyParseError("WrongNodeCount", node["offset"],
+                            "World should have precisely one BackgroundShader section.")
+    bg = bgs[0]
+    types = _children(bg, "Type")
+    if len(types) != 1:
+        raise PsyParseError("WrongNodeCount", bg["offset"], "BackgroundShader should have precisely one Type.")
+    if types[0]["internal"]:
+        raise PsyParseError("ExpectedLeafNode", bg["offset"], "BackgroundShader's Type should be a leaf node.")
+    if types[0]["contents"].strip() != "Color":
+        raise PsyParseError("UnknownVariant", bg["offset"], "The specified BackgroundShader Type isn't recognized.")
+    cols = _children(bg, "Color")
+    if not cols or cols[0]["internal"]:
+        raise PsyParseError("MissingNode", bg["offset"], "BackgroundShader's Type is Color, but no Color is given.")
+    try:
+        background = parse_color(cols[0]["contents"])
+    except PsyParseError:
+        raise PsyParseError("IncorrectLeafData", cols[0]["offset"], "Color")
+    lights = []
+    for c in node["children"]:
+        if c["internal"] and c["type"] == "DistantDiskLight":
+            lights.append(_parse_light(c, "distant"))
+    return dict(background=background, distant_lights=lights)
+
+
+def _parse_light(node, kind):
+    if not node["internal"]:
+        raise PsyParseError("UnknownError", node["offset"], "light")
+    out = dict(radius=[], direction=[], color=[], dims=[])
+    for c in node["children"]:
+        if c["internal"]:
+            continue
+        t = c["type"]
+        if t == "Radius" and kind in ("distant", "sphere"):
+            v = _all_f32(c["contents"], 1)
+            if v is None:
+                raise PsyParseError("UnknownError", c["offset"], "Radius")
+            out["radius"].append(v[0])
+        elif t == "Direction" and kind == "distant":
+            v = _all_f32(c["contents"], 3)
+            if v is None:
+                raise PsyParseError("UnknownError", c["offset"], "Direction")
+            out["direction"].append(v)
+        elif t == "Dimensions" and kind == "rect":
+            v = _all_f32(c["contents"], 2)
+            if v is None:
+                raise PsyParseError("UnknownError", c["offset"], "Dimensions")
+            out["dims"].append(v)
+        elif t == "Color":
+            try:
+                out["color"].append(parse_color(c["contents"]))
+            except PsyParseError:
+                raise PsyParseError("UnknownError", c["offset"], "Color")
+    return out
+
+
+def _parse_shader(node):
+    types = _leaves(node, "Type")
+    if not types:
+        raise PsyParseError("MissingNode", node["offset"], "Expected a Type field in SurfaceShader.")
+    t = types[0]["contents"].strip()
+
+    def color():
+        cs = _leaves(node, "Color")
+        if not cs:
+            raise PsyParseError("MissingNode", node["offset"], f"Expected a Color field in {t} SurfaceShader.")
+        try:
+            return parse_color(cs[0]["contents"])
+        except PsyParseError:
+            raise PsyParseError("UnknownError", cs[0]["offset"], "Color")
+
+    def scalar(name):
+        cs = _leaves(node, name)
+        if not cs:
+            raise PsyParseError("MissingNode", node["offset"], f"Expected a {name} field in GTR SurfaceShader.")
+        v = _all_f32(cs[0]["contents"], 1)
+        if v is None:
+            raise PsyParseError("UnknownError", cs[0]["offset"], name)
+        return v[0]
+
+    if t == "Lambert":
+        return ("Lambert", color(), 0.0, 0.0)
+    if t == "GGX":
+        c = color()
+        return ("GGX", c, scalar("Roughness"), scalar("Fresnel"))
+    if t == "Emit":
+        return ("Emit", color(), 0.0, 0.0)
+    raise PsyParseError("UnknownVariant", node["offset"], "unknown SurfaceShader Type")
+
+
+def _parse_mesh(node):
+    verts = []
+    for c in _leaves(node, "Vertices"):
+        s, i, tv = c["contents"], 0, []
+        while True:
+            r = _tuple_f32(s[i:], 3)
+            if r is None:
+                break
+            tv.append(r[0])
+            i += r[1]
+        verts.append(tv)
+    if not verts:
+        raise PsyParseError("MissingNode", node["offset"], "MeshSurface has no Vertices")
+    n = len(verts[0])
+    if any(len(v) != n for v in verts):
+        raise PsyParseError("IncorrectLeafData", node["offset"], "time samples differ in vertex count")
+    normals = []
+    for c in _leaves(node, "Normals"):
+        s, i, tn = c["contents"], 0, []
+        while True:
+            r = _tuple_f32(s[i:], 3)
+            if r is None:
+                break
+            tn.append(normalized(r[0]))
+            i += r[1]
+        normals.append(tn)
+    if normals and (len(normals) != len(verts) or any(len(x) != n for x in normals)):
+        raise PsyParseError("IncorrectLeafData", node["offset"], "normals do not match the vertices")
+
+    def uints(name):
+        cs = _leaves(node, name)
+        out = []
+        if cs:
+            s, i = cs[0]["contents"], 0
+            while True:
+                r = ws_uint(s, i, (1 << 64) - 1)
+                if r is None:
+                    break
+                out.append(r[0])
+                i = r[1]
+        return out
+
+    counts, idx = uints("FaceVertCounts"), uints("FaceVertIndices")
+    tris, ii = [], 0
+    for fvc in counts:
+        if fvc < 3:
+            raise PsyParseError("IncorrectLeafData", node["offset"], "polygon with fewer than three vertices")
+        if ii + fvc > len(idx):
+            raise PsyParseError("IncorrectLeafData", node["offset"], "FaceVertIndices too short")
+        for vi in range(fvc - 2):
+            tris.append((idx[ii], idx[ii + vi + 1], idx[ii + vi + 2]))
+        ii += fvc
+    if any(v >= n for t in tris for v in t):
+        raise PsyParseError("IncorrectLeafData", node["offset"], "vertex index out of range")
+    return dict(verts=verts, normals=normals or None, tris=tris)
+
+
+def _parse_assembly(node):
+    """psy_assembly.rs:18-188 for a single-level assembly."""
+    if not node["internal"]:
+        raise PsyParseError("UnknownError", node["offset"], "Assembly")
+    shaders, shader_map = [], {}
+    objects, object_map = [], {}
+    instances = []
+    for c in node["children"]:
+        t = c["type"]
+        if t == "Assembly":
+            raise PsyParseError("UnknownVariant", c["offset"], "nested assemblies are not supported")
+        if t == "Instance":
+            if not c["internal"]:
+                raise PsyParseError("UnknownError", c["offset"], "Instance")
+            data = _leaves(c, "Data")
+            if len(data) != 1:
+                raise PsyParseError("UnknownError", c["offset"], "Instance")
+            name = data[0]["contents"]
+            binds = _leaves(c, "SurfaceShaderBind")
+            shader_name = binds[0]["contents"] if binds else None
+            xforms = []
+            for x in _leaves(c, "Transform"):
+                try:
+                    xforms.append(parse_matrix(x["contents"]))
+                except PsyParseError:
+                    raise PsyParseError("UnknownError", 0, "bad matrix")
+            if name not in object_map:
+                raise PsyParseError("InstancedMissingData", data[0]["offset"],
+                                    "Attempted to add instance for data with a name that doesn't exist.")
+            if shader_name is not None and shader_name not in shader_map:
+                raise PsyParseError("UnknownVariant", c["offset"], f"Unknown surface shader '{shader_name}'.")
+            instances.append(dict(object=object_map[name], shader=None if shader_name is None else
+                                  shader_map[shader_name], xforms=xforms))
+        elif t == "SurfaceShader":
+            if not c["internal"] or c["ident"] is None:
+                raise PsyParseError("UnknownError", c["offset"], "SurfaceShader")
+            if c["ident"] in shader_map:
+                raise PsyParseError("UnknownError", c["offset"], "duplicate shader name")
+            shader_map[c["ident"]] = len(shaders)
+            shaders.append(_parse_shader(c))
+        elif t in ("MeshSurface", "SphereLight", "RectangleLight"):
+            if not c["internal"] or c["ident"] is None:
+                raise PsyParseError("UnknownError", c["offset"], t)
+            if c["ident"] in object_map:
+                raise PsyParseError("UnknownError", c["offset"], "duplicate object name")
+            if t == "MeshSurface":
+                obj = ("mesh", _parse_mesh(c))
+            else:
+                obj = ("sphere" if t == "SphereLight" else "rect",
+                       _parse_light(c, "sphere" if t == "SphereLight" else "rect"))
+            object_map[c["ident"]] = len(objects)
+            objects.append(obj)
+    return dict(shaders=shaders, objects=objects, instances=instances)
+
+
+def parse_file(text) -> dict:
+    """DataTree::from_str, then parse_scene on each Scene child of the root
+    (main.rs:231-240 renders every Scene in a file; this returns the
+    first)."""
+    tree = parse_data_tree(text)
+    scenes = [c for c in tree["children"] if c["type"] == "Scene"]
+    if not scenes:
+        raise PsyParseError("MissingNode", 0, "no Scene in file")
+    return parse_scene(scenes[0])
+
+
+def _rgb_to_xyz_matrix(r, g, b, w, y=1.0):
+    """The colour crate's build script (sub_crates/color/build.rs:178-225,
+    RGBtoXYZ of the ACES CTL): f64 matrix from chromaticities."""
+    x = w[0] * y / w[1]
+    z = (1.0 - w[0] - w[1]) * y / w[1]
+    d = r[0] * (b[1] - g[1]) + b[0] * (g[1] - r[1]) + g[0] * (r[1] - b[1])
+    sr = (x * (b[1] - g[1]) - g[0] * (y * (b[1] - 1.0) + b[1] * (x + z)) + b[0] * (y * (g[1] - 1.0) + g[1] * (x + z))) / d
+    sg = (x * (r[1] - b[1]) + r[0] * (y * (b[1] - 1.0) + b[1] * (x + z)) - b[0] * (y * (r[1] - 1.0) + r[1] * (x + z))) / d
+    sb = (x * (g[1] - r[1]) - r[0] * (y * (g[1] - 1.0) + g[1] * (x + z)) + g[0] * (y * (r[1] - 1.0) + r[1] * (x + z))) / d
+    return [[sr * r[0], sg * g[0], sb * b[0]],
+            [sr * r[1], sg * g[1], sb * b[1]],
+            [sr * (1.0 - r[0] - r[1]), sg * (1.0 - g[0] - g[1]), sb * (1.0 - b[0] - b[1])]]
+
+
+def rec709_e_matrix():
+    """rec709_e_to_xyz's constants: the build script prints the f64 matrix
+    (Rec.709 primaries, equal-energy white) with 10 decimals
+    (build.rs:132-153) and rustc reads them as f32 literals."""
+    m = _rgb_to_xyz_matrix((0.640, 0.330), (0.300, 0.600), (0.150, 0.060), (1.0 / 3.0, 1.0 / 3.0))
+    return np.array([[_round_f32("%.10f" % v) for v in row] for row in m], np.float32)
+
+
+def rec709_e_to_xyz(rgb):
+    """(r*m00 + g*m01) + b*m02 per row, in f32 without contraction."""
+    m = rec709_e_matrix()
+    r, g, b = (np.float32(v) for v in rgb)
+    return tuple(float((r * m[i, 0] + g * m[i, 1]) + b * m[i, 2]) for i in range(3))
+
+
+def color_desc(c):
+    """A parsed colour as psy_color (kind, a, b, c): rec709 -> XYZ,
+    blackbody -> 1, color_temperature -> 2 (color.rs:46-62)."""
+    if c[0] == "rec709":
+        return (0,) + rec709_e_to_xyz(c[1:])
+    return (1 if c[0] == "blackbody" else 2, c[1], c[2], 0.0)
+
+
+def normalized(v):
+    """Normal::normalized (sub_crates/math3d/src/normal.rs:37-41) as the
+    oracle restates glam's normalize: v * (1 / sqrt((x*x + y*y) + z*z))."""
+    x, y, z = (np.float32(t) for t in v)
+    inv = np.float32(1.0) / np.sqrt((x * x + y * y) + z * z)
+    return [float(x * inv), float(y * inv), float(z * inv)]
+
+
+def to_scene(parsed: dict):
+    """The parse as the attributes the oracle renderer reads (meshes,
+    lights, instances, shaders, world, camera, settings with the reference
+    renderer's fixed bounce limit of 2, renderer.rs:599)."""
+    from types import SimpleNamespace
+    asm = parsed["assembly"]
+    kinds = {"Lambert": 0, "GGX": 1, "Emit": 2}
+    sc = SimpleNamespace(meshes=[], rect_lights=[], sphere_lights=[], instances=[])
+    where = []
+    for kind, o in asm["objects"]:
+        if kind == "mesh":
+            where.append(("mesh", len(sc.meshes)))
+            sc.meshes.append((np.asarray(o["verts"], np.float32), np.asarray(o["tris"], np.uint32).reshape(-1, 3)))
+        elif kind == "rect":
+            where.append(("rect", len(sc.rect_lights)))
+            sc.rect_lights.append((o["dims"][0][0], o["dims"][0][1], color_desc(o["color"][0])))
+        else:
+            where.append(("sphere", len(sc.sphere_lights)))
+            sc.sphere_lights.append((o["radius"][0], color_desc(o["color"][0])))
+    for i in asm["instances"]:
+        kind, idx = where[i["object"]]
+        xf = i["xforms"]
+        if not xf and kind != "mesh":
+            xf = [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]]
+        sc.instances.append(dict(kind=kind, index=idx, shader=i["shader"],
+                                 xforms=np.asarray(xf, np.float32).reshape(-1, 12) if xf else None))
+    sc.shaders = [(kinds[k], color_desc(c), r, f) for k, c, r, f in asm["shaders"]]
+    w = parsed["world"]
+    sc.background = color_desc(w["background"])
+    sc.distant_lights = [(d["radius"][0], tuple(d["direction"][0]), color_desc(d["color"][0]))
+                         for d in w["distant_lights"]]
+    cam = parsed["camera"]
+    sc.camera = dict(xforms=np.asarray(cam["xforms"], np.float32), fov=cam["fovs"][0], aperture=cam["apertures"][0],
+                     focus=cam["focus"][0])
+    sc.settings = dict(width=parsed["width"], height=parsed["height"], spp=parsed["spp"], max_bounces=2,
+                       seed=parsed["seed"])
+    return sc
+
+
+def as_arrays(parsed: dict) -> dict:
+    """Numpy views of the parsed scene for comparisons."""
+    asm = parsed["assembly"]
+    out = dict(settings=(parsed["width"], parsed["height"], parsed["spp"], parsed["seed"]),
+               camera=parsed["camera"], world=parsed["world"], shaders=asm["shaders"], objects=[],
+               instances=asm["instances"])
+    for kind, o in asm["objects"]:
+        if kind == "mesh":
+            out["objects"].append((kind, np.asarray(o["verts"], np.float32), None if o["normals"] is None else
+                                   np.asarray(o["normals"], np.float32), np.asarray(o["tris"], np.uint64)))
+        else:
+            out["objects"].append((kind, o))
+    return out

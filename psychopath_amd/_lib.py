@@ -225,6 +225,7 @@ _SIGNATURES = {
 PSY_RENDER_NO_GRAPHS = 1
 PSY_RENDER_NO_SORT = 2
 PSY_RENDER_PLAIN = 4
+PSY_RENDER_SHADE_WAVES_SHIFT = 8
 
 
 class ServerConfig(C.Structure):
@@ -248,6 +249,29 @@ _SIGNATURES.update({
                                     C.POINTER(C.c_uint64)]),
     "psy_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(RequestStats)]),
     "psy_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+})
+
+
+class FileScene(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("output_path", C.c_char_p),
+        ("scene", SceneDesc),
+        ("world", WorldDesc),
+        ("camera", CameraDesc),
+        ("settings", RenderSettings),
+    ]
+
+
+PSY_E_PARSE = -6
+
+_SIGNATURES.update({
+    # include/psy_file.h
+    "psy_file_parse": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "psy_file_destroy": (C.c_int, [C.c_void_p]),
+    "psy_file_scene_count": (C.c_uint32, [C.c_void_p]),
+    "psy_file_get_scene": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(FileScene)]),
+    "psy_file_dump_tree": (C.c_int64, [C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64]),
 })
 
 

@@ -566,8 +566,12 @@ __device__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& 
 
 // The shading step over the closest-hit queue (in material order when
 // `order` is given); appends the light and bounce rays to the next queues.
-__global__ __launch_bounds__(KBLOCK) void shade_kernel(ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa,
-                                                       Hits h, const uint32_t* order, Queue qa_out, Queue qs_out) {
+// MINW: minimum waves per SIMD the build is compiled for (its register
+// budget); every build computes the same thing.
+template <int MINW>
+__global__ __launch_bounds__(KBLOCK, MINW) void shade_kernel(ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa,
+                                                             Hits h, const uint32_t* order, Queue qa_out,
+                                                             Queue qs_out) {
     const uint32_t n = *qa.count;
     const uint32_t stride = gridDim.x * blockDim.x;
     // Every lane of a wave runs the loop the same number of times so the
@@ -633,7 +637,8 @@ constexpr int MEGA_BLOCK = TRACE_BLOCK;
 constexpr int MEGA_LDS = TRACE_LDS_DEPTH;
 constexpr int MEGA_SPILL = TRACE_STACK_CAPACITY - TRACE_LDS_DEPTH;
 
-__global__ __launch_bounds__(MEGA_BLOCK) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
+template <int MINW>
+__global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
     __shared__ uint32_t lds_stack[MEGA_LDS * MEGA_BLOCK];
     const ReqParams rp = reqs[0];
     const uint32_t pixels = rp.width * rp.height;
@@ -831,8 +836,12 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
             PSY_HIP(hipGetLastError());
             order = r->order;
         }
-        hipLaunchKernelGGL(shade_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->reqs, r->P, qa_in, r->ha, order,
-                           qa_out, qs_out);
+        auto shade = r->shade_waves >= 4 ? shade_kernel<4>
+                     : r->shade_waves == 3 ? shade_kernel<3>
+                     : r->shade_waves == 2 ? shade_kernel<2>
+                                           : shade_kernel<1>;
+        hipLaunchKernelGGL(shade, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->reqs, r->P, qa_in, r->ha, order, qa_out,
+                           qs_out);
         PSY_HIP(hipGetLastError());
         std::swap(qa_in, qa_out);
         std::swap(qs_in, qs_out);
@@ -1139,8 +1148,12 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
     if (r->plain) {
-        hipLaunchKernelGGL(megakernel, dim3((unsigned)((pixels + MEGA_BLOCK - 1) / MEGA_BLOCK)), dim3(MEGA_BLOCK), 0,
-                           s, scene_args(r->scene), r->sc, r->reqs);
+        auto mega = r->shade_waves >= 4 ? megakernel<4>
+                    : r->shade_waves == 3 ? megakernel<3>
+                    : r->shade_waves == 2 ? megakernel<2>
+                                          : megakernel<1>;
+        hipLaunchKernelGGL(mega, dim3((unsigned)((pixels + MEGA_BLOCK - 1) / MEGA_BLOCK)), dim3(MEGA_BLOCK), 0, s,
+                           scene_args(r->scene), r->sc, r->reqs);
         PSY_HIP(hipGetLastError());
     } else {
         for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
@@ -1168,10 +1181,16 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
 
 int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     if (!r) return fail(PSY_E_INVALID, "renderer is NULL");
-    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT | PSY_RENDER_PLAIN))
+    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT | PSY_RENDER_PLAIN | PSY_RENDER_SHADE_WAVES_MASK))
         return fail(PSY_E_INVALID, "unknown option flag");
+    const uint32_t sw = (flags & PSY_RENDER_SHADE_WAVES_MASK) >> PSY_RENDER_SHADE_WAVES_SHIFT;
+    if (sw > 4) return fail(PSY_E_INVALID, "shading waves per SIMD must be 1 to 4");
     r->plain = (flags & PSY_RENDER_PLAIN) != 0;
     const bool graphs = (flags & PSY_RENDER_NO_GRAPHS) == 0, sort = (flags & PSY_RENDER_NO_SORT) == 0;
+    if (sw != 0 && sw != r->shade_waves) {
+        drop_graphs(r);
+        r->shade_waves = sw;
+    }
     if (sort != r->sort_materials) {
         for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
         r->graphs.clear();
@@ -1193,44 +1212,72 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
     if (rc) return rc;
     const psy_trace_tuning saved_t = r->tuning;
     const bool saved_sort = r->sort_materials, saved_plain = r->plain;
+    const uint32_t saved_sw = r->shade_waves;
     r->plain = false;
     psy_trace_tuning best_t = saved_t;
     bool best_sort = saved_sort;
+    uint32_t best_sw = saved_sw;
     float best_ms = 1e30f;
-    for (uint32_t sort = 0; sort < 2 && !rc; sort++) {
-        for (uint32_t w : {8u, 16u, 32u}) {
-            for (uint32_t refill : {8u, 16u, 32u}) {
-                for (uint32_t lds : {16u, 24u}) {
-                    const psy_trace_tuning t{PSY_KERNEL_PERSISTENT, w, refill, lds, 4};
-                    r->tuning = t;
-                    r->sort_materials = sort != 0;
-                    drop_graphs(r);
-                    float ms_min = 1e30f;
-                    for (int rep = 0; rep < 3 && !rc; rep++) {
-                        psy_render_stats st;
-                        rc = psy_render(r, cam, trial, film, PSY_MEM_DEVICE, &st);
-                        if (rep > 0) ms_min = std::min(ms_min, (float)st.trace_ms);
-                    }
-                    if (rc) break;
-                    if (ms_min < best_ms) {
-                        best_ms = ms_min;
-                        best_t = t;
-                        best_sort = sort != 0;
-                    }
-                }
-                if (rc) break;
-            }
-            if (rc) break;
+    // Best of two timed renders after a warm-up (which also captures the
+    // wave's graph) of the current choice.
+    auto time_trial = [&](float& out_ms) {
+        drop_graphs(r);
+        out_ms = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            psy_render_stats st;
+            const int e = psy_render(r, cam, trial, film, PSY_MEM_DEVICE, &st);
+            if (e) return e;
+            if (rep > 0) out_ms = std::min(out_ms, (float)st.trace_ms);
         }
+        return PSY_OK;
+    };
+    auto consider = [&](float ms) {
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_t = r->tuning;
+            best_sort = r->sort_materials;
+            best_sw = r->shade_waves;
+        }
+    };
+    // Stage 1: the traversal launch shape, sorting on, current shading build.
+    r->sort_materials = true;
+    for (uint32_t w : {8u, 16u, 32u}) {
+        for (uint32_t refill : {8u, 16u, 32u}) {
+            for (uint32_t lds : {16u, 24u}) {
+                if (rc) break;
+                r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, 4};
+                float ms;
+                rc = time_trial(ms);
+                if (!rc) consider(ms);
+            }
+        }
+    }
+    // Stage 2: the shading kernels' register budget.
+    for (uint32_t sw = 1; sw <= 4 && !rc; sw++) {
+        r->tuning = best_t;
+        r->shade_waves = sw;
+        float ms;
+        rc = time_trial(ms);
+        if (!rc) consider(ms);
+    }
+    // Stage 3: material sorting off.
+    if (!rc) {
+        r->tuning = best_t;
+        r->shade_waves = best_sw;
+        r->sort_materials = false;
+        float ms;
+        rc = time_trial(ms);
+        if (!rc) consider(ms);
     }
     r->release(film);
     r->tuning = rc ? saved_t : best_t;
     r->sort_materials = rc ? saved_sort : best_sort;
+    r->shade_waves = rc ? saved_sw : best_sw;
     r->plain = saved_plain;
     drop_graphs(r);
     if (rc) return rc;
     if (best_tuning) *best_tuning = best_t;
-    if (best_flags) *best_flags = best_sort ? 0u : PSY_RENDER_NO_SORT;
+    if (best_flags) *best_flags = (best_sort ? 0u : PSY_RENDER_NO_SORT) | PSY_RENDER_SHADE_WAVES(best_sw);
     return PSY_OK;
 }
 

@@ -134,6 +134,7 @@ struct psy_renderer {
     bool use_graphs = true;
     bool sort_materials = true;
     bool plain = false;  // one megakernel launch per render
+    uint32_t shade_waves = 2;  // minimum waves per SIMD of the shading kernels' build (1-4)
 
     std::vector<DevBuf*> bufs;
     ShadeScene sc{};

@@ -49,11 +49,13 @@ class Renderer:
             _lib._lib.psy_renderer_destroy(h)
             self.handle = None
 
-    def set_options(self, graphs: bool = True, sort: bool = True, plain: bool = False) -> None:
+    def set_options(self, graphs: bool = True, sort: bool = True, plain: bool = False, shade_waves: int = 0) -> None:
         """graphs/sort: wavefront execution choices; plain: one launch per
-        render over the whole frame.  None of them changes the image."""
+        render over the whole frame; shade_waves (1-4, 0 keeps the current):
+        the shading kernels' build by minimum waves per SIMD.  None of them
+        changes the image."""
         flags = ((0 if graphs else _lib.PSY_RENDER_NO_GRAPHS) | (0 if sort else _lib.PSY_RENDER_NO_SORT) |
-                 (_lib.PSY_RENDER_PLAIN if plain else 0))
+                 (_lib.PSY_RENDER_PLAIN if plain else 0) | (int(shade_waves) << _lib.PSY_RENDER_SHADE_WAVES_SHIFT))
         check("psy_renderer_set_options", lib().psy_renderer_set_options(self.handle, flags))
 
     def autotune(self, camera: dict, trial: dict) -> dict:
@@ -67,6 +69,7 @@ class Renderer:
                                                                     C.byref(t), C.byref(flags)))
         out = {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
         out["sort"] = not (flags.value & _lib.PSY_RENDER_NO_SORT)
+        out["shade_waves"] = (flags.value >> _lib.PSY_RENDER_SHADE_WAVES_SHIFT) & 7
         return out
 
     def tuning(self) -> dict:
@@ -111,8 +114,8 @@ class Renderer:
 def build(rs, max_paths: int = 0) -> Renderer:
     """Device scene + renderer for a psychopath_amd.scenes.RenderScene."""
     asm = Assembly()
-    for v, t in rs.meshes:
-        asm.add_mesh(Mesh(v, t))
+    for m in rs.meshes:
+        asm.add_mesh(Mesh(m[0], m[1], m[2] if len(m) > 2 else None))
     for (w, h, col) in rs.rect_lights:
         asm.add_rect_light(RectLight(float(w), float(h), col))
     for (r, col) in rs.sphere_lights:
@@ -171,3 +174,65 @@ class Server:
         st = _lib.ServerStats()
         check("psy_server_get_stats", lib().psy_server_get_stats(self.handle, C.byref(st)))
         return {f: getattr(st, f) for f, _ in _lib.ServerStats._fields_}
+
+
+def main(argv=None) -> int:
+    """Renders one of the built-in workload scenes and prints per-render
+    timing, the way the reference's command line renders a scene file
+    (src/main.rs: spp and output options); the image is written as a
+    float32 XYZ .npy array."""
+    import argparse
+    import time
+
+    from . import scenes
+
+    makers = {
+        "cornell": scenes.cornell_box,
+        "sphere": scenes.sphere_scene,
+        "motion": scenes.motion_instances,
+        "terrain": scenes.terrain_scene,
+        "large": scenes.large_scene,
+    }
+    ap = argparse.ArgumentParser(prog="python -m psychopath_amd.render")
+    ap.add_argument("scene", choices=sorted(makers))
+    ap.add_argument("--width", type=int)
+    ap.add_argument("--height", type=int)
+    ap.add_argument("-s", "--spp", type=int)
+    ap.add_argument("--max-bounces", type=int)
+    ap.add_argument("--seed", type=int)
+    ap.add_argument("--renders", type=int, default=1, help="renders after one untimed warm-up render")
+    ap.add_argument("--plain", action="store_true", help="one launch per render over the whole frame")
+    ap.add_argument("--autotune", action="store_true", help="pick the launch shape on a 1/16-size trial render")
+    ap.add_argument("--shade-waves", type=int, default=0, help="shading kernels' build by waves per SIMD (1-4)")
+    ap.add_argument("-o", "--output", help="write the XYZ image of the last render (.npy)")
+    a = ap.parse_args(argv)
+    t0 = time.time()
+    sc = makers[a.scene]()
+    st = dict(sc.settings)
+    for key, val in (("width", a.width), ("height", a.height), ("spp", a.spp), ("max_bounces", a.max_bounces),
+                     ("seed", a.seed)):
+        if val is not None:
+            st[key] = val
+    r = build(sc)
+    print(f"{a.scene}: scene ready in {time.time() - t0:.1f}s", flush=True)
+    if a.autotune:
+        trial = dict(st, width=max(1, st["width"] // 4), height=max(1, st["height"] // 4),
+                     spp=max(1, st["spp"] // 16))
+        print(f"autotune: {r.autotune(sc.camera, trial)}", flush=True)
+    if a.plain or a.shade_waves:
+        r.set_options(plain=a.plain, shade_waves=a.shade_waves)
+    img, stats = r.render(sc.camera, st)
+    for _ in range(a.renders):
+        t = time.perf_counter()
+        img, stats = r.render(sc.camera, st)
+        s = time.perf_counter() - t
+        print(f"{a.scene} {st['width']}x{st['height']} {st['spp']} spp depth {st['max_bounces']}"
+              f"{' plain' if a.plain else ''}: {s * 1e3:.2f} ms, {stats['paths'] / s / 1e6:.1f} M samples/s, "
+              f"{stats['waves']} waves", flush=True)
+    if a.output:
+        np.save(a.output, img)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

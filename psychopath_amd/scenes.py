@@ -499,3 +499,107 @@ def sphere_scene(levels: int = 9, width: int = 512, height: int = 512, spp: int 
                      aperture=0.0, focus=1.0)
     sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
     return sc
+
+
+# Mesh library of the large instanced scene: (kind, size, seed).  Sphere
+# sizes are 4 * 4^levels triangles, heightfield patches 2 * nx * ny.
+LARGE_MESHES = (
+    ("patch", (72, 70), 31),     # 10,080
+    ("sphere", 6, 32),           # 16,384
+    ("patch", (160, 128), 33),   # 40,960
+    ("sphere", 7, 34),           # 65,536
+    ("patch", (256, 256), 35),   # 131,072
+    ("sphere", 8, 36),           # 262,144
+    ("patch", (512, 512), 37),   # 524,288
+    ("sphere", 9, 38),           # 1,048,576
+)
+
+
+def _rock_patch(nx: int, ny: int, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """A heightfield of 2 * nx * ny triangles centred on the origin, about
+    1.6 units across and 0.5 high."""
+    v, f = heightfield(nx, ny, seed=seed, spacing=1.6 / max(nx, ny), amplitude=0.35)
+    v = v.astype(np.float64)
+    v[:, :2] -= (v[:, :2].max(axis=0) + v[:, :2].min(axis=0)) / 2
+    v[:, 2] -= v[:, 2].min()
+    return v.astype(np.float32) + np.float32(0.0), f
+
+
+def large_scene(grid: int = 64, lights: int = 1024, width: int = 3840, height: int = 2160, spp: int = 128,
+                max_bounces: int = 8, seed: int = 0, mesh_levels_cap: int = 9) -> RenderScene:
+    """The batched large-scene workload: grid x grid instances (4096 by
+    default) on a seeded jittered grid, each drawing one of eight seeded
+    meshes of 10K-1M triangles (four displaced spheres, four heightfield
+    patches; 2,099,040 unique triangles) with a random orientation and
+    scale, and one of four material families: Lambertian, GGX metal of
+    roughness 0.05-0.8, a smooth dielectric coat (GGX, normal-incidence
+    reflectance 0.04 = IOR 1.5, the reference having no transmissive
+    closure) and emissive.  Lit by `lights` blackbody sphere lights and one
+    distant disk light (the sun).  `mesh_levels_cap` lowers the sphere
+    meshes for small test renders."""
+    sc = RenderScene()
+    for kind, size, mseed in LARGE_MESHES:
+        if kind == "sphere":
+            v, f = displaced_sphere(min(size, mesh_levels_cap), amplitude=0.25, seed=mseed, frequency=2.0)
+            v = v * np.float32(0.7)
+        else:
+            nx, ny = size
+            if mesh_levels_cap < 9:
+                nx, ny = max(8, nx >> (9 - mesh_levels_cap)), max(8, ny >> (9 - mesh_levels_cap))
+            v, f = _rock_patch(nx, ny, mseed)
+        sc.meshes.append((v, f))
+    # Shaders: 0-5 Lambertian, 6-13 GGX metal, 14 dielectric coat, 15-16 emissive.
+    lam = [(0.8, 0.8, 0.8), (0.7, 0.25, 0.2), (0.25, 0.6, 0.3), (0.2, 0.3, 0.7), (0.75, 0.65, 0.3),
+           (0.45, 0.4, 0.35)]
+    sc.shaders = [(SHADER_LAMBERT, rec709(*c), 0.0, 0.0) for c in lam]
+    metals = [(1.0, 0.78, 0.34), (0.95, 0.64, 0.54), (0.91, 0.92, 0.92), (0.56, 0.57, 0.58)]
+    for k in range(8):
+        rough = 0.05 + 0.75 * k / 7.0
+        sc.shaders.append((SHADER_GGX, rec709(*metals[k % 4]), float(np.float32(rough)), 1.0))
+    sc.shaders.append((SHADER_GGX, rec709(0.9, 0.9, 0.9), 0.05, 0.04))
+    sc.shaders.append((SHADER_EMIT, (COLOR_TEMPERATURE, 3200.0, 4.0, 0.0), 0.0, 0.0))
+    sc.shaders.append((SHADER_EMIT, (COLOR_TEMPERATURE, 7500.0, 4.0, 0.0), 0.0, 0.0))
+    n = grid * grid
+    u = uniform01(seed * 11 + 41, 10 * n).astype(np.float64).reshape(n, 10)
+    spacing = 2.0
+    half = grid * spacing / 2
+    for i in range(n):
+        gx, gy = i % grid, i // grid
+        base = np.array([(gx - (grid - 1) / 2) * spacing + (u[i, 0] - 0.5) * spacing * 0.5,
+                         (gy - (grid - 1) / 2) * spacing + (u[i, 1] - 0.5) * spacing * 0.5, 0.0])
+        mesh = min(7, int(u[i, 2] * 8))
+        rot = _rotation(_sphere_point(u[i, 3], u[i, 4]), 2.0 * np.pi * u[i, 5])
+        scale = 0.6 + 0.5 * u[i, 6]
+        m = u[i, 7]
+        if m < 0.4:
+            shader = min(5, int(u[i, 8] * 6))
+        elif m < 0.7:
+            shader = 6 + min(7, int(u[i, 8] * 8))
+        elif m < 0.92:
+            shader = 14
+        else:
+            shader = 15 + int(u[i, 8] < 0.5)
+        if LARGE_MESHES[mesh][0] == "patch":
+            rot = _rotation((0.0, 0.0, 1.0), 2.0 * np.pi * u[i, 5])
+        else:
+            base[2] = 0.7 * scale
+        inv = rot.T / scale
+        t = -inv @ base
+        xf = np.concatenate([inv, t[:, None]], axis=1).astype(np.float32).ravel()
+        sc.instances.append(dict(kind="mesh", index=mesh, shader=shader, xforms=xf[None]))
+    lu = uniform01(seed * 11 + 42, 5 * lights).astype(np.float64).reshape(lights, 5)
+    for k in range(lights):
+        radius = 0.05 + 0.15 * lu[k, 0]
+        temp = 2500.0 + 6500.0 * lu[k, 1]
+        sc.sphere_lights.append((radius, (COLOR_TEMPERATURE, temp, 30.0, 0.0)))
+        pos = np.array([(lu[k, 2] - 0.5) * 2 * half, (lu[k, 3] - 0.5) * 2 * half, 1.8 + 2.2 * lu[k, 4]])
+        sc.instances.append(dict(kind="sphere", index=k, shader=None,
+                                 xforms=world_to_object(np.eye(3), pos)[None]))
+    sc.background = rec709(0.02, 0.025, 0.04)
+    sc.distant_lights.append((0.02, (-0.3, 0.45, -0.4), (COLOR_TEMPERATURE, 5800.0, 2.0, 0.0)))
+    cam_pos = np.array([-half * 0.55, -half * 0.75, half * 0.32])
+    target = np.array([half * 0.05, half * 0.05, 0.0])
+    sc.camera = dict(xforms=look_at(cam_pos, target)[None], fov=float(np.float32(np.radians(50.0))),
+                     aperture=0.0, focus=1.0)
+    sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
+    return sc

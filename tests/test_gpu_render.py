@@ -66,9 +66,18 @@ def test_execution_options_do_not_change_results():
     again, _ = r.render(sc.camera, sc.settings)
     r.set_options(plain=True)
     plain, _ = r.render(sc.camera, sc.settings)
+    budgets = []
+    for w in (1, 2, 3, 4):
+        r.set_options(shade_waves=w)
+        budgets.append(r.render(sc.camera, sc.settings)[0])
+        r.set_options(plain=True, shade_waves=w)
+        budgets.append(r.render(sc.camera, sc.settings)[0])
+    from psychopath_amd._lib import PsyError
+    with pytest.raises(PsyError):
+        r.set_options(shade_waves=5)
     small, stats, _ = gpu_render(sc, max_paths=3000)
     assert stats["waves"] > 1
-    for other in (a, b, again, plain, small):
+    for other in (a, b, again, plain, small, *budgets):
         assert np.array_equal(base, other)
 
 
@@ -188,5 +197,44 @@ def test_autotune_keeps_images():
     before, _ = r.render(sc.camera, sc.settings)
     choice = r.autotune(sc.camera, dict(width=32, height=32, spp=2, max_bounces=3, seed=0))
     assert choice["waves_per_cu"] in (8, 16, 32) and choice["stack_lds"] in (16, 24)
+    assert choice["shade_waves"] in (1, 2, 3, 4)
     after, _ = r.render(sc.camera, sc.settings)
     assert np.array_equal(before, after)
+
+
+def test_large_instanced_scene_reduced():
+    """The large-scene workload's structure (eight meshes, Lambertian, GGX
+    metal, dielectric coat and emissive instances, sphere lights and a sun)
+    at reduced size: 256 instances of meshes capped at 65,536 triangles, 64
+    lights, depth 8."""
+    sc = scenes.large_scene(grid=16, lights=64, width=96, height=54, spp=8, mesh_levels_cap=7)
+    check_parity(sc, "large scene reduced 96x54x8")
+
+
+def test_large_instanced_scene_full_geometry():
+    """The large-scene workload's full geometry and lighting (4096 instances
+    over 2,099,040 unique triangles, 1024 sphere lights and a sun, depth 8)
+    at a small image; the plain one-launch renderer gives the same image."""
+    sc = scenes.large_scene(width=64, height=36, spp=2)
+    img, _ = check_parity(sc, "large scene full 64x36x2")
+    r = render.build(sc)
+    r.set_options(plain=True)
+    plain, _ = r.render(sc.camera, sc.settings)
+    assert np.array_equal(img, plain)
+
+
+def test_scene_file_renders_like_the_oracle():
+    """A Cornell box written as a .psy file (parser layout), read by the
+    library's reader and rendered on the GPU, against the oracle's reader
+    and renderer on the same text; the reader's scene gives the same image
+    as the scene it was written from when the colours read back exactly."""
+    from oracle import psyfile as opf
+    from psychopath_amd import psyfile
+    text = psyfile.write(scenes.cornell_box(64, 48, 8, 2, 3), "cornell")
+    sc = psyfile.read(text)[0]
+    assert sc.settings == dict(width=64, height=48, spp=8, max_bounces=2, seed=3)
+    ref, _ = psyref.render(opf.to_scene(opf.parse_file(text)))
+    img, stats, _ = gpu_render(sc)
+    frac = matching_fraction(img, ref)
+    print(f"cornell .psy 64x48x8: {frac * 100:.3f}% pixels within {REL}; {stats}", flush=True)
+    assert frac >= FRACTION
