@@ -201,8 +201,8 @@ def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
     """Renders a psychopath_amd.scenes.RenderScene with the oracle.
     Returns the XYZ image [H, W, 3] and timing figures."""
     o = OracleScene()
-    for v, t in scene.meshes:
-        o.add_mesh(v, t)
+    for m in scene.meshes:
+        o.add_mesh(m[0], m[1], m[2] if len(m) > 2 else None)
     L = lib()
     for (w, h, col) in scene.rect_lights:
         L.psyref_scene_add_rect_light(o.h, w, h, C.byref(_color(col)))

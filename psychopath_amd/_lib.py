@@ -272,7 +272,13 @@ _SIGNATURES.update({
     "psy_file_scene_count": (C.c_uint32, [C.c_void_p]),
     "psy_file_get_scene": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(FileScene)]),
     "psy_file_dump_tree": (C.c_int64, [C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64]),
+    # include/psy_image.h
+    "psy_image_encode": (C.c_int64, [_fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "psy_image_bucket_base64": (C.c_int64, [_fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_char_p, C.c_uint64]),
 })
+
+PSY_IMAGE_PPM_ASCII, PSY_IMAGE_PPM, PSY_IMAGE_PNG, PSY_IMAGE_EXR = 0, 1, 2, 3
 
 
 class PsyError(RuntimeError):

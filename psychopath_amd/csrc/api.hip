@@ -40,6 +40,8 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.light_nodes, &s->light_nodes);
     if (!r) r = upload(h.light_boxes, &s->light_boxes);
     if (!r) r = upload(h.light_list, &s->light_list);
+    if (!r) r = upload(h.normals, &s->normals);
+    if (!r) r = upload(h.nidx, &s->nidx);
     if (r == PSY_OK) s->uploaded = true;
     return r;
 }
@@ -204,7 +206,8 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
                       (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform) +
                       h.rect_lights.size() * sizeof(GpuRectLight) + h.sphere_lights.size() * sizeof(GpuSphereLight) +
                       (h.inst_pos.size() + h.tri_rec.size() + h.light_list.size()) * 4 +
-                      h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox);
+                      h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox) +
+                      (h.normals.size() + h.nidx.size()) * 4;
     *out = s;
     return PSY_OK;
 }

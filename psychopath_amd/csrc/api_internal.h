@@ -52,12 +52,15 @@ struct psy_scene {
     psy::GpuLightNode* light_nodes = nullptr;
     psy::GpuBox* light_boxes = nullptr;
     uint32_t* light_list = nullptr;
+    float* normals = nullptr;
+    uint32_t* nidx = nullptr;
     uint64_t device_bytes = 0;
     bool uploaded = false;
 
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,
-                        sphere_lights, inst_pos, tri_rec, light_nodes, light_boxes, light_list};
+                        sphere_lights, inst_pos, tri_rec, light_nodes, light_boxes, light_list,
+                        normals,   nidx};
         for (void* b : bufs) (void)hipFree(b);
     }
 };

@@ -66,7 +66,12 @@ struct GpuMesh {
     uint32_t tri_stride;   // bytes
     uint32_t rec_map;      // start of this mesh's triangle -> record map
     uint64_t tri_offset;   // bytes into the triangle buffer
-    uint64_t pad1;
+    // Shading normals (triangle_mesh.rs:29-116, 272-296), PSY_NONE if none:
+    // normal_base indexes the xyz normal array, vertex-major with the time
+    // samples of a vertex together; nidx_base the per-triangle vertex
+    // triples (stored order v0, v2, v1) indexed by original triangle.
+    uint32_t normal_base;
+    uint32_t nidx_base;
 };
 
 // Per instance (src/scene/assembly.rs:404-411).  `xf_start/xf_count` index

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/psy_file.h"
+#include "color_host.h"
 
 namespace psy {
 namespace api {
@@ -316,30 +317,6 @@ bool all_f32(const std::string& s, int n, float* out) {
 
 // ---------------------------------------------------------------- scene
 
-// rec709_e_to_xyz (sub_crates/color/build.rs:132-153): the build script
-// prints the f64 matrix of the Rec.709 primaries with equal-energy white to
-// ten decimals and the compiler reads them as f32; rows are evaluated as
-// (r*m0 + g*m1) + b*m2 in f32.
-struct Rec709E {
-    float m[9];
-    Rec709E() {
-        const double rx = 0.640, ry = 0.330, gx = 0.300, gy = 0.600, bx = 0.150, by = 0.060;
-        const double wx = 1.0 / 3.0, wy = 1.0 / 3.0, Y = 1.0;
-        const double X = wx * Y / wy, Z = (1.0 - wx - wy) * Y / wy;
-        const double d = rx * (by - gy) + bx * (gy - ry) + gx * (ry - by);
-        const double sr = (X * (by - gy) - gx * (Y * (by - 1.0) + by * (X + Z)) + bx * (Y * (gy - 1.0) + gy * (X + Z))) / d;
-        const double sg = (X * (ry - by) + rx * (Y * (by - 1.0) + by * (X + Z)) - bx * (Y * (ry - 1.0) + ry * (X + Z))) / d;
-        const double sb = (X * (gy - ry) - rx * (Y * (gy - 1.0) + gy * (X + Z)) + gx * (Y * (ry - 1.0) + ry * (X + Z))) / d;
-        const double md[9] = {sr * rx, sg * gx, sb * bx, sr * ry, sg * gy, sb * by,
-                              sr * (1.0 - rx - ry), sg * (1.0 - gx - gy), sb * (1.0 - bx - by)};
-        for (int i = 0; i < 9; i++) {
-            char buf[64];
-            std::snprintf(buf, sizeof buf, "%.10f", md[i]);
-            m[i] = std::strtof(buf, nullptr);
-        }
-    }
-};
-
 psy_color parse_color(const std::string& contents, uint64_t offset, const char* kind) {
     // psy.rs:577-615: "<space>, <values>" split on commas, both trimmed.
     std::vector<std::string> items;
@@ -368,11 +345,13 @@ psy_color parse_color(const std::string& contents, uint64_t offset, const char* 
         float rgb[3];
         for (float& v : rgb)
             if (!c.f32(v)) raise(kind, offset, msg);
-        static const Rec709E M;
+        // rec709_e_to_xyz (sub_crates/color/build.rs:132-153)
+        float xyz[3];
+        psy::color::apply(psy::color::matrices().rec709_e_to_xyz, rgb[0], rgb[1], rgb[2], xyz);
         out.kind = PSY_COLOR_XYZ;
-        out.a = (rgb[0] * M.m[0] + rgb[1] * M.m[1]) + rgb[2] * M.m[2];
-        out.b = (rgb[0] * M.m[3] + rgb[1] * M.m[4]) + rgb[2] * M.m[5];
-        out.c = (rgb[0] * M.m[6] + rgb[1] * M.m[7]) + rgb[2] * M.m[8];
+        out.a = xyz[0];
+        out.b = xyz[1];
+        out.c = xyz[2];
         return out;
     }
     if (items[0] == "blackbody" || items[0] == "color_temperature") {

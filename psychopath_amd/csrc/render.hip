@@ -187,8 +187,10 @@ __global__ __launch_bounds__(KBLOCK) void camera_kernel(const WavePlan* plans, c
 }
 
 // Shadow-ray results (renderer.rs:670-690): a miss adds the pending light.
-__global__ __launch_bounds__(KBLOCK) void shadow_kernel(Queue qs, const uint8_t* kind, Paths P) {
+__global__ __launch_bounds__(KBLOCK) void shadow_kernel(Queue qs, const uint8_t* kind, Paths P,
+                                                        unsigned long long* rays) {
     const uint32_t n = *qs.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(rays, (unsigned long long)n);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (kind[i] != PSY_HIT_MISS) continue;
         const uint32_t p = qs.path[i];
@@ -289,9 +291,12 @@ __device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float
             v1 = lerp3(v3(s0[3], s0[4], s0[5]), v3(s1[3], s1[4], s1[5]), a2);
             v2 = lerp3(v3(s0[6], s0[7], s0[8]), v3(s1[6], s1[7], s1[8]), a2);
         }
+        // mat_space: the instance transform's inverse, or the identity
+        // (triangle_mesh.rs:134-138, 183-189).
+        Xf space = xf_identity();
         if (inst.xf_count > 0) {
-            const Xf space = inst.xf_count == 1 ? load_xf(sc.inv_static + inst.inv_static)
-                                                : xf_inverse(instance_xf(sc.xforms, inst, time));
+            space = inst.xf_count == 1 ? load_xf(sc.inv_static + inst.inv_static)
+                                       : xf_inverse(instance_xf(sc.xforms, inst, time));
             v0 = xf_point(space, v0);
             v1 = xf_point(space, v1);
             v2 = xf_point(space, v2);
@@ -299,6 +304,26 @@ __device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float
         surface_point(v0, v1, v2, b0, b1, b2, is.pos, is.pos_err);
         is.nor_g = cross(sub(v0, v1), sub(v0, v2));
         is.nor = is.nor_g;
+        if (mesh.normal_base != PSY_NONE) {
+            // Interpolated shading normal, flipped to the geometric side
+            // (triangle_mesh.rs:272-296).
+            const uint32_t* ix = sc.nidx + 3ull * ((uint64_t)mesh.nidx_base + prim);
+            const uint32_t S = mesh.time_samples;
+            auto vnormal = [&](uint32_t vi) {
+                const float* n = sc.normals + 3ull * ((uint64_t)mesh.normal_base + (uint64_t)vi * S);
+                V3 r;
+                if (!lerp_slice_pos(S, time, i1, a2)) {
+                    r = v3(n[3 * i1], n[3 * i1 + 1], n[3 * i1 + 2]);
+                } else {
+                    r = lerp3(v3(n[3 * i1], n[3 * i1 + 1], n[3 * i1 + 2]),
+                              v3(n[3 * i1 + 3], n[3 * i1 + 4], n[3 * i1 + 5]), a2);
+                }
+                return normalize(r);
+            };
+            const V3 n0 = vnormal(ix[0]), n1 = vnormal(ix[1]), n2 = vnormal(ix[2]);
+            const V3 s_nor = xf_normal(space, add(add(mul(n0, b0), mul(n1, b1)), mul(n2, b2)));
+            is.nor = dot(s_nor, is.nor_g) >= 0.0f ? s_nor : neg3(s_nor);
+        }
         return is;
     }
     const Xf xform = instance_xf(sc.xforms, inst, time);
@@ -573,6 +598,7 @@ __global__ __launch_bounds__(KBLOCK, MINW) void shade_kernel(ShadeScene sc, cons
                                                              Hits h, const uint32_t* order, Queue qa_out,
                                                              Queue qs_out) {
     const uint32_t n = *qa.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(sc.rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
     // Every lane of a wave runs the loop the same number of times so the
     // wave-wide appends see all lanes.
@@ -643,12 +669,12 @@ __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, Sha
     const ReqParams rp = reqs[0];
     const uint32_t pixels = rp.width * rp.height;
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pixel >= pixels) return;
     uint32_t* lds_col = lds_stack + threadIdx.x;
     const float fspp = (float)rp.spp;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     TraceCount tc;
-    for (uint32_t si = 0; si < rp.spp; si++) {
+    uint32_t traced = 0;
+    for (uint32_t si = 0; pixel < pixels && si < rp.spp; si++) {
         PathRegs ps;
         V3 o, d;
         camera_path(sc, rp, (uint64_t)pixel * rp.spp + si, ps, o, d);
@@ -658,6 +684,7 @@ __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, Sha
             const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
             ShadeOut so;
             shade_path(sc, rp, ps, hr, o, d, so);
+            traced += so.shadow ? 2u : 1u;
             if (so.shadow) {
                 const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, so.so, so.sd, ps.time, so.smax, true,
                                                                            lds_col, MEGA_BLOCK, tc);
@@ -672,6 +699,10 @@ __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, Sha
         fy += c.y / fspp;
         fz += c.z / fspp;
     }
+    // One atomic per wave for the ray count.
+    for (int off = 32; off > 0; off >>= 1) traced += __shfl_xor(traced, off);
+    if ((threadIdx.x & 63u) == 0u && traced) atomicAdd(sc.rays, (unsigned long long)traced);
+    if (pixel >= pixels) return;
     float* px = rp.film + 3ull * pixel;
     px[0] = fx;
     px[1] = fy;
@@ -820,7 +851,7 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
         }
         hipError_t e = launch_tuned(r->scene, tune, r->cu_count, r->work + 16, queue_args(r, qs_in, r->hs, true), false, s);
         if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
-        hipLaunchKernelGGL(shadow_kernel, dim3(grid), dim3(KBLOCK), 0, s, qs_in, r->hs.kind, r->P);
+        hipLaunchKernelGGL(shadow_kernel, dim3(grid), dim3(KBLOCK), 0, s, qs_in, r->hs.kind, r->P, r->sc.rays);
         PSY_HIP(hipGetLastError());
         if (last) break;
         hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, qa_out.count, qs_out.count, r->counts + 4);
@@ -982,7 +1013,6 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (rc) return rc;
     if (scene->host.max_stack > (uint32_t)TRACE_STACK_CAPACITY)
         return fail(PSY_E_RANGE, "scene BVHs are deeper than the traversal stack holds");
-    if (scene->host.has_normals) return fail(PSY_E_INVALID, "the renderer shades with geometric normals: meshes with normals are not supported");
     rc = ensure_uploaded(scene);
     if (rc) return rc;
 
@@ -1053,6 +1083,9 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.meshes = scene->meshes;
     sc.tris = scene->tris;
     sc.tri_rec = scene->tri_rec;
+    sc.normals = scene->normals;
+    sc.nidx = scene->nidx;
+    sc.rays = reinterpret_cast<unsigned long long*>(r->counts + 14);
     sc.xforms = scene->xforms;
     sc.inv_static = scene->inv_static;
     sc.rects = scene->rect_lights;
@@ -1144,6 +1177,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     PSY_HIP(hipMemcpyAsync(r->reqs, &rp, sizeof(rp), hipMemcpyHostToDevice, s));
     PSY_HIP(hipMemcpyAsync(r->plans, plans.data(), waves * sizeof(WavePlan), hipMemcpyHostToDevice, s));
     rc = launch_zero(film, pixels * 3, s);
+    if (!rc) rc = launch_zero(reinterpret_cast<float*>(r->counts + 14), 2, s);  // ray counter
     if (!rc) rc = launch_set_wave(r, 0u);
     if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
@@ -1170,8 +1204,10 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     if (stats) {
         float ms = 0.0f;
         PSY_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        unsigned long long rays = 0;
+        PSY_HIP(hipMemcpy(&rays, r->counts + 14, sizeof(rays), hipMemcpyDeviceToHost));
         stats->paths = total;
-        stats->rays = 0;
+        stats->rays = rays;
         stats->waves = r->plain ? 0 : waves;
         stats->trace_ms = ms;
         stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

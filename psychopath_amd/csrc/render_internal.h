@@ -58,6 +58,8 @@ struct ShadeScene {
     const GpuMesh* meshes;
     const uint8_t* tris;
     const uint32_t* tri_rec;
+    const float* normals;
+    const uint32_t* nidx;
     const GpuXform* xforms;
     const GpuXform* inv_static;
     const GpuRectLight* rects;
@@ -74,6 +76,7 @@ struct ShadeScene {
     GpuColor background;
     const HaltonDim* hdims;
     const uint16_t* hperms;
+    unsigned long long* rays;  // rays traced (RenderStats::ray_count), summed by the kernels
 };
 
 // Path state, one slot per path of the wave.
@@ -142,7 +145,7 @@ struct psy_renderer {
     Queue qa[2]{}, qs[2]{};
     Hits ha{}, hs{};
     uint32_t* order = nullptr;
-    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins
+    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins, [14..16) rays traced (u64)
     uint32_t* wave_idx = nullptr;
     unsigned int* work = nullptr;
     WavePlan* plans = nullptr;   // device plan array

@@ -265,6 +265,28 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         gm.rec_map = (uint32_t)s.tri_rec.size();
         s.tri_rec.resize(s.tri_rec.size() + T);
         for (uint32_t k = 0; k < T; k++) s.tri_rec[gm.rec_map + bvh.order[k]] = k;
+        gm.normal_base = PSY_NONE;
+        gm.nidx_base = PSY_NONE;
+        if (md.normals) {
+            // Vertex-major normals and stored-order triples (triangle_mesh.rs:
+            // 56-83); index limits keep both bases within 32 bits.
+            if ((s.normals.size() / 3 + (uint64_t)V * S) >= PSY_NONE || (s.nidx.size() / 3 + T) >= PSY_NONE) {
+                err = "mesh " + std::to_string(m) + ": too many vertices with normals";
+                return PSY_E_RANGE;
+            }
+            gm.normal_base = (uint32_t)(s.normals.size() / 3);
+            gm.nidx_base = (uint32_t)(s.nidx.size() / 3);
+            for (uint32_t vi = 0; vi < V; vi++)
+                for (uint32_t ti = 0; ti < S; ti++) {
+                    const float* p = md.normals + 3ull * ((uint64_t)ti * V + vi);
+                    s.normals.insert(s.normals.end(), p, p + 3);
+                }
+            for (uint32_t i = 0; i < T; i++) {
+                s.nidx.push_back(md.indices[3ull * i]);
+                s.nidx.push_back(md.indices[3ull * i + 2]);
+                s.nidx.push_back(md.indices[3ull * i + 1]);
+            }
+        }
         s.meshes[m] = gm;
         s.max_depth = std::max(s.max_depth, bvh.depth);
         s.leaf_count += bvh.leaf_count;

@@ -22,6 +22,8 @@ struct PackedScene {
     std::vector<GpuInstance> instances;  // in instance-BVH leaf order
     std::vector<GpuXform> xforms;
     std::vector<GpuXform> inv_static;
+    std::vector<float> normals;        // xyz per (vertex, time sample), see GpuMesh
+    std::vector<uint32_t> nidx;        // stored-order vertex triples per triangle
     uint32_t top_root = REF_EMPTY;
 
     // Lights and shading lookups.
@@ -38,7 +40,7 @@ struct PackedScene {
     bool has_xf = false;        // some instance has transforms
     bool has_motion = false;    // time-sampled node bounds or vertices
     bool has_lights = false;    // some instance places a light
-    bool has_normals = false;   // some mesh has shading normals (not kept)
+    bool has_normals = false;   // some mesh has shading normals
     uint32_t max_stack = 0;     // deepest pending-node count traversal can need
     uint32_t max_depth = 0;
     uint64_t leaf_count = 0;

@@ -204,7 +204,10 @@ def main(argv=None) -> int:
     ap.add_argument("--plain", action="store_true", help="one launch per render over the whole frame")
     ap.add_argument("--autotune", action="store_true", help="pick the launch shape on a 1/16-size trial render")
     ap.add_argument("--shade-waves", type=int, default=0, help="shading kernels' build by waves per SIMD (1-4)")
-    ap.add_argument("-o", "--output", help="write the XYZ image of the last render (.npy)")
+    ap.add_argument("-o", "--output", help="write the last render: .png, .ppm or .exr as the reference writes "
+                                           "them, or .npy for the raw XYZ film")
+    ap.add_argument("--blender", action="store_true",
+                    help="print the last render as the reference's serialized bucket output")
     a = ap.parse_args(argv)
     t0 = time.time()
     sc = makers[a.scene]()
@@ -230,7 +233,18 @@ def main(argv=None) -> int:
               f"{' plain' if a.plain else ''}: {s * 1e3:.2f} ms, {stats['paths'] / s / 1e6:.1f} M samples/s, "
               f"{stats['waves']} waves", flush=True)
     if a.output:
-        np.save(a.output, img)
+        if a.output.lower().endswith(".npy"):
+            np.save(a.output, img)
+        else:
+            from . import film
+            film.write(a.output, img)
+    if a.blender:
+        import sys
+
+        from . import film
+        for block in film.blender_buckets(img):
+            sys.stdout.write(block)
+        sys.stdout.flush()
     return 0
 
 

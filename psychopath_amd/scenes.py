@@ -359,7 +359,8 @@ def look_at(pos, target, up=(0.0, 0.0, 1.0)):
 
 
 def motion_instances(grid: int = 16, levels: int = 7, lights: int = 16, width: int = 1920, height: int = 1080,
-                     spp: int = 64, max_bounces: int = 6, seed: int = 0, aperture: float = 0.05) -> RenderScene:
+                     spp: int = 64, max_bounces: int = 6, seed: int = 0, aperture: float = 0.05,
+                     smooth: bool = False) -> RenderScene:
     """The motion-blur and instancing workload: grid x grid instances of a
     displaced sphere (4 * 4^levels triangles) with two deformation time
     samples (vertex offsets N(0, 0.02)), each instance on a seeded jittered
@@ -370,7 +371,8 @@ def motion_instances(grid: int = 16, levels: int = 7, lights: int = 16, width: i
     v, f = displaced_sphere(levels, amplitude=0.1, seed=1)
     off = normal01(seed * 7 + 11, v.size).reshape(v.shape) * 0.02
     v1 = (v.astype(np.float64) + off).astype(np.float32) + np.float32(0.0)
-    sc.meshes.append((np.stack([v, v1]), f))
+    vs = np.stack([v, v1])
+    sc.meshes.append((vs, f, vertex_normals(vs, f)) if smooth else (vs, f))
     sc.shaders = [(SHADER_LAMBERT, rec709(0.8, 0.8, 0.8), 0.0, 0.0),
                   (SHADER_GGX, rec709(1.0, 0.78, 0.34), 0.2, 0.8),
                   (SHADER_LAMBERT, rec709(0.7, 0.2, 0.15), 0.0, 0.0)]
@@ -401,6 +403,25 @@ def motion_instances(grid: int = 16, levels: int = 7, lights: int = 16, width: i
                      aperture=aperture, focus=focus)
     sc.settings = dict(width=width, height=height, spp=spp, max_bounces=max_bounces, seed=seed)
     return sc
+
+
+def vertex_normals(verts: np.ndarray, tris: np.ndarray) -> np.ndarray:
+    """Smooth-shading normals: area-weighted face normals summed per vertex
+    and normalised (float64, returned as float32), for each time sample of
+    `verts` ([V,3] or [S,V,3])."""
+    v = np.asarray(verts, np.float64)
+    single = v.ndim == 2
+    if single:
+        v = v[None]
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    out = np.zeros_like(v)
+    for s in range(v.shape[0]):
+        fn = np.cross(v[s, t[:, 1]] - v[s, t[:, 0]], v[s, t[:, 2]] - v[s, t[:, 0]])
+        for k in range(3):
+            np.add.at(out[s], t[:, k], fn)
+    out /= np.maximum(np.linalg.norm(out, axis=-1, keepdims=True), 1e-30)
+    out = out.astype(np.float32) + np.float32(0.0)
+    return out[0] if single else out
 
 
 def _sphere_point(u, v):
@@ -483,12 +504,13 @@ def terrain_scene(nx: int = 2048, ny: int = 1280, width: int = 1024, height: int
 
 
 def sphere_scene(levels: int = 9, width: int = 512, height: int = 512, spp: int = 16, max_bounces: int = 4,
-                 seed: int = 0) -> RenderScene:
+                 seed: int = 0, smooth: bool = False) -> RenderScene:
     """The ray-cast mesh (1M-triangle displaced sphere) as a render scene:
-    Lambertian, under a 2 x 2 rectangle light, dim background."""
+    Lambertian, under a 2 x 2 rectangle light, dim background; `smooth`
+    adds vertex normals for interpolated shading."""
     sc = RenderScene()
     v, f = displaced_sphere(levels)
-    sc.meshes.append((v, f))
+    sc.meshes.append((v, f, vertex_normals(v, f)) if smooth else (v, f))
     sc.shaders = [(SHADER_LAMBERT, rec709(0.75, 0.72, 0.7), 0.0, 0.0)]
     sc.instances.append(dict(kind="mesh", index=0, shader=0, xforms=None))
     sc.background = rec709(0.03, 0.03, 0.04)

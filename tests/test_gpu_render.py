@@ -34,8 +34,10 @@ def gpu_render(sc, **kw):
 
 
 def check_parity(sc, label):
-    ref, _ = psyref.render(sc)
+    ref, info = psyref.render(sc)
     img, stats, _ = gpu_render(sc)
+    # Rays traced (RenderStats::ray_count): camera, bounce and light rays.
+    assert stats["rays"] == info["rays"], (stats["rays"], info["rays"])
     frac = matching_fraction(img, ref)
     print(f"{label}: {frac * 100:.3f}% pixels within {REL} relative; mean xyz gpu {img.mean((0, 1))} "
           f"oracle {ref.mean((0, 1))}; {stats}", flush=True)
@@ -87,8 +89,12 @@ def test_plain_single_launch_matches_wavefront():
     sc = scenes.motion_instances(grid=3, levels=3, lights=3, width=48, height=27, spp=4, max_bounces=5)
     img, _, r = gpu_render(sc)
     r.set_options(plain=True)
-    plain, _ = r.render(sc.camera, sc.settings)
+    plain, pstats = r.render(sc.camera, sc.settings)
     assert np.array_equal(img, plain)
+    _, wstats = r.render(sc.camera, sc.settings)
+    r.set_options(plain=False)
+    _, wstats = r.render(sc.camera, sc.settings)
+    assert pstats["rays"] == wstats["rays"] > sc.settings["width"] * sc.settings["height"] * sc.settings["spp"]
 
 
 def test_motion_blur_instancing_sphere_lights_dof():
@@ -238,3 +244,13 @@ def test_scene_file_renders_like_the_oracle():
     frac = matching_fraction(img, ref)
     print(f"cornell .psy 64x48x8: {frac * 100:.3f}% pixels within {REL}; {stats}", flush=True)
     assert frac >= FRACTION
+
+
+def test_smooth_shading_normals():
+    """Interpolated vertex normals (triangle_mesh.rs:272-296): a static
+    smooth sphere and motion-blurred instances whose normals have two time
+    samples."""
+    check_parity(scenes.sphere_scene(levels=4, width=48, height=48, spp=8, max_bounces=3, smooth=True),
+                 "smooth sphere 48x48x8")
+    check_parity(scenes.motion_instances(grid=3, levels=3, lights=3, width=48, height=27, spp=4, max_bounces=4,
+                                         smooth=True), "smooth motion instances 48x27x4")
