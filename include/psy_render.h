@@ -120,9 +120,10 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
  * shading its samples' paths to the end, no wavefronts), the baseline the
  * wavefront and served forms are measured against.
  * PSY_RENDER_SHADE_WAVES(n), n = 1..4, selects the build of the shading
- * kernels compiled for at least n waves per SIMD (a register budget of
- * 512/n per lane; tighter budgets spill to scratch); 0 in those bits keeps
- * the current choice. */
+ * kernel compiled for at least n waves per SIMD (a register budget of 512/n
+ * per lane; tighter budgets spill to scratch): of the one-launch kernel
+ * when given with PSY_RENDER_PLAIN, of the wavefront's shading kernel
+ * otherwise; 0 in those bits keeps the current choice (2 for both). */
 #define PSY_RENDER_NO_GRAPHS 1u
 #define PSY_RENDER_NO_SORT 2u
 #define PSY_RENDER_PLAIN 4u
@@ -133,7 +134,8 @@ int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
 /* Autotuner: renders `trial` (a reduced version of the intended renders)
  * with every candidate launch shape of the traversal kernel (waves per CU,
- * refill threshold, LDS stack depth), then every shading register budget,
+ * refill threshold, LDS stack depth, register budget), then every shading
+ * register budget,
  * then with and without material sorting, keeps the fastest on the renderer
  * and returns it (either output may be NULL; `best_flags` holds
  * PSY_RENDER_NO_SORT if sorting lost and PSY_RENDER_SHADE_WAVES(n) of the

@@ -1182,9 +1182,9 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
     if (r->plain) {
-        auto mega = r->shade_waves >= 4 ? megakernel<4>
-                    : r->shade_waves == 3 ? megakernel<3>
-                    : r->shade_waves == 2 ? megakernel<2>
+        auto mega = r->plain_waves >= 4 ? megakernel<4>
+                    : r->plain_waves == 3 ? megakernel<3>
+                    : r->plain_waves == 2 ? megakernel<2>
                                           : megakernel<1>;
         hipLaunchKernelGGL(mega, dim3((unsigned)((pixels + MEGA_BLOCK - 1) / MEGA_BLOCK)), dim3(MEGA_BLOCK), 0, s,
                            scene_args(r->scene), r->sc, r->reqs);
@@ -1223,7 +1223,11 @@ int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     if (sw > 4) return fail(PSY_E_INVALID, "shading waves per SIMD must be 1 to 4");
     r->plain = (flags & PSY_RENDER_PLAIN) != 0;
     const bool graphs = (flags & PSY_RENDER_NO_GRAPHS) == 0, sort = (flags & PSY_RENDER_NO_SORT) == 0;
-    if (sw != 0 && sw != r->shade_waves) {
+    // The register budget applies to the form being selected: the
+    // megakernel with PSY_RENDER_PLAIN, shade_kernel otherwise.
+    if (sw != 0 && (flags & PSY_RENDER_PLAIN)) {
+        r->plain_waves = sw;
+    } else if (sw != 0 && sw != r->shade_waves) {
         drop_graphs(r);
         r->shade_waves = sw;
     }
@@ -1277,14 +1281,16 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
     };
     // Stage 1: the traversal launch shape, sorting on, current shading build.
     r->sort_materials = true;
-    for (uint32_t w : {8u, 16u, 32u}) {
-        for (uint32_t refill : {8u, 16u, 32u}) {
-            for (uint32_t lds : {16u, 24u}) {
-                if (rc) break;
-                r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, 4};
-                float ms;
-                rc = time_trial(ms);
-                if (!rc) consider(ms);
+    for (uint32_t sw : {4u, 8u}) {
+        for (uint32_t w : {8u, 16u, 32u}) {
+            for (uint32_t refill : {8u, 16u, 32u}) {
+                for (uint32_t lds : {16u, 24u}) {
+                    if (rc) break;
+                    r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw};
+                    float ms;
+                    rc = time_trial(ms);
+                    if (!rc) consider(ms);
+                }
             }
         }
     }

@@ -137,7 +137,8 @@ struct psy_renderer {
     bool use_graphs = true;
     bool sort_materials = true;
     bool plain = false;  // one megakernel launch per render
-    uint32_t shade_waves = 2;  // minimum waves per SIMD of the shading kernels' build (1-4)
+    uint32_t shade_waves = 2;  // minimum waves per SIMD of shade_kernel's build (1-4)
+    uint32_t plain_waves = 2;  // the same for the one-launch megakernel
 
     std::vector<DevBuf*> bufs;
     ShadeScene sc{};
