@@ -281,7 +281,7 @@ int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, cons
         rsc.shaders.push_back(c);
     }
     for (const Instance& inst : sc->root.instances)
-        if (inst.obj_kind == OBJ_MESH && (inst.shader < 0 || (uint32_t)inst.shader >= n_shaders)) return -2;
+        if (inst.obj_kind == OBJ_MESH && inst.shader >= 0 && (uint32_t)inst.shader >= n_shaders) return -2;
     rsc.world.background = to_color(*background);
     for (uint32_t i = 0; i < n_distant; i++) {
         DistantDiskLight l;

@@ -1,6 +1,9 @@
 // psyref — TEST INFRASTRUCTURE ONLY (parity oracle / CPU baseline).
 #include "psyref_render.h"
 
+#include <cstdio>
+#include <cstdlib>
+
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -320,11 +323,50 @@ struct LightPath {
     void next_lds_samp(float out[4]) { get_sample_4d(sample_number, dim_offset++, px, py, sampling_seed, out); }
 };
 
+// The colour the reference gives mesh surfaces without a shader:
+// Emit(rec709_to_xyz((1, 0, 1))) (tracer.rs:161-164).  rec709_to_xyz's
+// constants come from the colour crate's build script (Rec.709 primaries,
+// D65 white, RGBtoXYZ of the ACES CTL, printed to ten decimals and read as
+// f32 literals; sub_crates/color/build.rs:14-24, 78-99, 178-225).
+static Color unassigned_color() {
+    const double rx = 0.640, ry = 0.330, gx = 0.300, gy = 0.600, bx = 0.150, by = 0.060;
+    const double wx = 0.3127, wy = 0.3290, Y = 1.0;
+    const double X = wx * Y / wy, Z = (1.0 - wx - wy) * Y / wy;
+    const double d = rx * (by - gy) + bx * (gy - ry) + gx * (ry - by);
+    const double sr = (X * (by - gy) - gx * (Y * (by - 1.0) + by * (X + Z)) + bx * (Y * (gy - 1.0) + gy * (X + Z))) / d;
+    const double sg = (X * (ry - by) + rx * (Y * (by - 1.0) + by * (X + Z)) - bx * (Y * (ry - 1.0) + ry * (X + Z))) / d;
+    const double sb = (X * (gy - ry) - rx * (Y * (gy - 1.0) + gy * (X + Z)) + gx * (Y * (ry - 1.0) + ry * (X + Z))) / d;
+    const double m[3][3] = {{sr * rx, sg * gx, sb * bx},
+                            {sr * ry, sg * gy, sb * by},
+                            {sr * (1.0 - rx - ry), sg * (1.0 - gx - gy), sb * (1.0 - bx - by)}};
+    float f[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            char buf[64];
+            snprintf(buf, sizeof buf, "%.10f", m[i][j]);
+            f[i][j] = strtof(buf, nullptr);
+        }
+    const float r = 1.0f, g = 0.0f, b = 1.0f;
+    Color c;
+    c.kind = COLOR_XYZ;
+    c.a = (r * f[0][0] + g * f[0][1]) + b * f[0][2];
+    c.b = (r * f[1][0] + g * f[1][1]) + b * f[1][2];
+    c.c = (r * f[2][0] + g * f[2][1]) + b * f[2][2];
+    return c;
+}
+
 Closure closure_of(const RenderScene& sc, const Isect& is) {
     if (is.light) {
         Closure c;
         c.kind = CLOSURE_EMIT;
         c.color = is.emit;
+        return c;
+    }
+    if (is.shader < 0) {
+        static const Color magenta = unassigned_color();
+        Closure c;
+        c.kind = CLOSURE_EMIT;
+        c.color = magenta;
         return c;
     }
     return sc.shaders[(size_t)is.shader];

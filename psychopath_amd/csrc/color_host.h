@@ -13,17 +13,21 @@ namespace psy {
 namespace color {
 
 struct Matrices {
-    float rec709_e_to_xyz[9];
+    float rec709_e_to_xyz[9];  // equal-energy white
     float xyz_to_rec709_e[9];
+    float rec709_to_xyz[9];    // D65 white
 
     Matrices() {
         double m[3][3];
-        rgb_to_xyz(m);
+        rgb_to_xyz(m, 1.0 / 3.0, 1.0 / 3.0);
         double inv[3][3];
         gj_inverse(m, inv);
+        double d65[3][3];
+        rgb_to_xyz(d65, 0.3127, 0.3290);
         for (int i = 0; i < 9; i++) {
             rec709_e_to_xyz[i] = literal(m[i / 3][i % 3]);
             xyz_to_rec709_e[i] = literal(inv[i / 3][i % 3]);
+            rec709_to_xyz[i] = literal(d65[i / 3][i % 3]);
         }
     }
 
@@ -33,10 +37,10 @@ private:
         std::snprintf(buf, sizeof buf, "%.10f", v);
         return std::strtof(buf, nullptr);
     }
-    // Rec.709 primaries with the equal-energy white point (x = y = 1/3).
-    static void rgb_to_xyz(double mat[3][3]) {
+    // Rec.709 primaries with white point (wx, wy).
+    static void rgb_to_xyz(double mat[3][3], double wx, double wy) {
         const double rx = 0.640, ry = 0.330, gx = 0.300, gy = 0.600, bx = 0.150, by = 0.060;
-        const double wx = 1.0 / 3.0, wy = 1.0 / 3.0, Y = 1.0;
+        const double Y = 1.0;
         const double X = wx * Y / wy, Z = (1.0 - wx - wy) * Y / wy;
         const double d = rx * (by - gy) + bx * (gy - ry) + gx * (ry - by);
         const double sr = (X * (by - gy) - gx * (Y * (by - 1.0) + by * (X + Z)) + bx * (Y * (gy - 1.0) + gy * (X + Z))) / d;

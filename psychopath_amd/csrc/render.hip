@@ -33,6 +33,7 @@
 #include <string>
 #include <vector>
 
+#include "color_host.h"
 #include "render_internal.h"
 #include "trace_one.h"
 
@@ -204,7 +205,7 @@ __device__ __forceinline__ uint32_t hit_key(const ShadeScene& sc, const Hits& h,
     if (h.kind[q] != PSY_HIT_SURFACE) return 0u;
     const GpuInstance inst = sc.instances[sc.inst_pos[h.inst[q]]];
     if (inst.kind != PSY_OBJECT_MESH) return 1u;
-    return 2u + sc.shaders[inst.shader].kind;
+    return 2u + (inst.shader == PSY_NONE ? sc.unassigned.kind : sc.shaders[inst.shader].kind);
 }
 
 // Counting sort of the closest hits by key, in two kernels over the same
@@ -516,7 +517,7 @@ __device__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& 
         clo.roughness = 0.0f;
         clo.fresnel = 0.0f;
     } else {
-        clo = sc.shaders[is.shader];
+        clo = is.shader == PSY_NONE ? sc.unassigned : sc.shaders[is.shader];
     }
     if (clo.kind == PSY_SHADER_EMIT) {
         const S4 c = color_spectral(clo.color, wl);
@@ -991,7 +992,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (world->shader_count > 0 && !world->shaders) return fail(PSY_E_INVALID, "shaders is NULL");
     if (world->distant_light_count > 0 && !world->distant_lights) return fail(PSY_E_INVALID, "distant_lights is NULL");
     for (const GpuInstance& gi : scene->host.instances)
-        if (gi.kind == PSY_OBJECT_MESH && gi.shader >= world->shader_count)
+        if (gi.kind == PSY_OBJECT_MESH && gi.shader != PSY_NONE && gi.shader >= world->shader_count)
             return fail(PSY_E_INVALID, "instance " + std::to_string(gi.id) + " has no valid shader");
     std::vector<GpuClosure> shaders;
     for (uint32_t i = 0; i < world->shader_count; i++) {
@@ -1095,6 +1096,13 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.llist = scene->light_list;
     sc.has_ltree = scene->host.light_nodes.empty() ? 0u : 1u;
     sc.shader_count = (uint32_t)shaders.size();
+    {
+        // Emit(rec709_to_xyz((1, 0, 1))), the reference's colour for mesh
+        // surfaces without a shader (tracer.rs:161-164).
+        float m[3];
+        psy::color::apply(psy::color::matrices().rec709_to_xyz, 1.0f, 0.0f, 1.0f, m);
+        sc.unassigned = GpuClosure{PSY_SHADER_EMIT, 0.0f, 0.0f, 0, GpuColor{PSY_COLOR_XYZ, m[0], m[1], m[2]}};
+    }
     sc.distant_count = (uint32_t)distant.size();
     sc.background = GpuColor{world->background.kind, world->background.a, world->background.b, world->background.c};
     GpuClosure* dsh = nullptr;

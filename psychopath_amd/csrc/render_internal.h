@@ -70,6 +70,7 @@ struct ShadeScene {
     uint32_t has_ltree;
     uint32_t shader_count;
     const GpuClosure* shaders;
+    GpuClosure unassigned;  // mesh surfaces without a shader (tracer.rs:161-164)
     const GpuDistantLight* distant;
     uint32_t distant_count;
     uint32_t pad;

@@ -175,3 +175,12 @@ def test_encode_errors():
         film.encode(np.zeros((2, 2, 3), np.float32), 9)
     with pytest.raises(ValueError):
         film.write("/tmp/x.jpg", np.zeros((2, 2, 3), np.float32))
+
+
+def test_rgb_to_xyz_derivation_matches_the_srgb_matrix():
+    """The build script's RGBtoXYZ with Rec.709 primaries and D65 white is
+    the textbook sRGB matrix (IEC 61966-2-1), pinning the derivation the
+    unassigned-shader colour and the equal-energy matrices share."""
+    m = np.array(opf._rgb_to_xyz_matrix((0.640, 0.330), (0.300, 0.600), (0.150, 0.060), (0.3127, 0.3290)))
+    srgb = np.array([[0.4124, 0.3576, 0.1805], [0.2126, 0.7152, 0.0722], [0.0193, 0.1192, 0.9505]])
+    assert np.allclose(m, srgb, atol=1.5e-4)

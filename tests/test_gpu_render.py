@@ -254,3 +254,15 @@ def test_smooth_shading_normals():
                  "smooth sphere 48x48x8")
     check_parity(scenes.motion_instances(grid=3, levels=3, lights=3, width=48, height=27, spp=4, max_bounces=4,
                                          smooth=True), "smooth motion instances 48x27x4")
+
+
+def test_mesh_without_shader_emits_magenta():
+    """A mesh instance without a shader renders as the reference's magenta
+    emitter (tracer.rs:161-164)."""
+    sc = scenes.cornell_box(48, 48, 4, 3, 2)
+    sc.instances[5]["shader"] = None   # the tall box
+    img, _ = check_parity(sc, "cornell, tall box without a shader")
+    shaded, _, _ = gpu_render(scenes.cornell_box(48, 48, 4, 3, 2))
+    # the emitting box adds light, more in X and Z (magenta) than in Y
+    gain = img.mean((0, 1)) - shaded.mean((0, 1))
+    assert gain[0] > gain[1] > 0 and gain[2] > gain[1]
