@@ -16,6 +16,7 @@
 #include <atomic>
 #include <cstring>
 #include <future>
+#include <cmath>
 #include <memory>
 #include <thread>
 
@@ -122,11 +123,26 @@ class Builder {
             o.mid = lerp_slice_box(s, n, 0.5f);
             o.mid_center = box_center(o.mid);
             o.centroid = mul(add(o.mid.lo, o.mid.hi), 0.5f);
+            const float v[12] = {o.mid.lo.x, o.mid.lo.y, o.mid.lo.z, o.mid.hi.x, o.mid.hi.y, o.mid.hi.z,
+                                 o.mid_center.x, o.mid_center.y, o.mid_center.z, o.centroid.x, o.centroid.y,
+                                 o.centroid.z};
+            for (float f : v)
+                if (f != f || (f == 0.0f && std::signbit(f))) order_free_ = false;
         }
     }
 
-    std::unique_ptr<BaseNode> build(uint32_t* keys, uint32_t n, uint32_t offset, uint32_t depth, unsigned spawn_budget,
-                                    uint32_t& max_depth) {
+    // Builds over keys[0..n), keeping each object's derived data in a
+    // parallel array permuted with the keys, so the split passes stream
+    // through memory instead of gathering.
+    std::unique_ptr<BaseNode> build_all(uint32_t* keys, uint32_t n, unsigned spawn_budget, uint32_t& max_depth) {
+        std::vector<ObjInfo> recs(n);
+        for (uint32_t i = 0; i < n; i++) recs[i] = info_[keys[i]];
+        std::vector<ObjInfo>().swap(info_);
+        return build(keys, recs.data(), n, 0, 0, spawn_budget, max_depth);
+    }
+
+    std::unique_ptr<BaseNode> build(uint32_t* keys, ObjInfo* recs, uint32_t n, uint32_t offset, uint32_t depth,
+                                    unsigned spawn_budget, uint32_t& max_depth) {
         auto node = std::make_unique<BaseNode>();
         if (n <= per_leaf_) {
             leaf_bounds(keys, n, node->bounds);
@@ -139,22 +155,22 @@ class Builder {
         uint32_t split;
         uint8_t axis;
         if ((uint32_t)(63 - __builtin_clzll((unsigned long long)n)) < (MAX_BASE_DEPTH - depth))
-            sah_split(keys, n, split, axis);
+            sah_split(keys, recs, n, split, axis);
         else
-            median_split(keys, n, split, axis);
+            median_split(keys, recs, n, split, axis);
         node->axis = axis;
 
         uint32_t d0 = 0, d1 = 0;
         if (spawn_budget > 1 && n > 20000) {
             unsigned half = spawn_budget / 2;
             auto fut = std::async(std::launch::async, [&] {
-                return build(keys + split, n - split, offset + split, depth + 1, spawn_budget - half, d1);
+                return build(keys + split, recs + split, n - split, offset + split, depth + 1, spawn_budget - half, d1);
             });
-            node->kid[0] = build(keys, split, offset, depth + 1, half, d0);
+            node->kid[0] = build(keys, recs, split, offset, depth + 1, half, d0);
             node->kid[1] = fut.get();
         } else {
-            node->kid[0] = build(keys, split, offset, depth + 1, 1, d0);
-            node->kid[1] = build(keys + split, n - split, offset + split, depth + 1, 1, d1);
+            node->kid[0] = build(keys, recs, split, offset, depth + 1, 1, d0);
+            node->kid[1] = build(keys + split, recs + split, n - split, offset + split, depth + 1, 1, d1);
         }
         max_depth = std::max(max_depth, std::max(d0, d1));
 
@@ -165,9 +181,11 @@ class Builder {
     }
 
     uint32_t split_sah(uint32_t* keys, uint32_t n) {
+        std::vector<ObjInfo> recs(n);
+        for (uint32_t i = 0; i < n; i++) recs[i] = info_[keys[i]];
         uint32_t split;
         uint8_t axis;
-        sah_split(keys, n, split, axis);
+        sah_split(keys, recs.data(), n, split, axis);
         return split;
     }
 
@@ -176,6 +194,10 @@ class Builder {
     uint32_t per_leaf_;
     unsigned threads_;
     std::vector<ObjInfo> info_;
+    // Without NaN or negative zero in the per-object boxes and centroids,
+    // min/max unions give the same box in any order, so the SAH candidates
+    // can be accumulated per bin and combined afterwards.
+    bool order_free_ = true;
 
     // acc_bounds (bvh_base.rs:64-88) followed by the union rule.
     void leaf_bounds(const uint32_t* keys, uint32_t n, std::vector<Box>& out) {
@@ -211,30 +233,60 @@ class Builder {
         }
     }
 
-    // Hoare-style partition of algorithm.rs:44-81.
+    // Hoare-style partition of algorithm.rs:44-81 over the keys and their
+    // records together; `pred` looks at a record.
     template <typename Pred>
-    static uint32_t hoare_partition(uint32_t* s, uint32_t n, Pred&& pred) {
+    static uint32_t hoare_partition(uint32_t* s, ObjInfo* r, uint32_t n, Pred&& pred) {
         uint32_t a = 0, b = n;
         for (;;) {
             while (true) {
                 if (a == b) return a;
-                if (!pred(s[a])) break;
+                if (!pred(r[a])) break;
                 a++;
             }
             while (true) {
                 b--;
                 if (a == b) return a;
-                if (pred(s[b])) break;
+                if (pred(r[b])) break;
             }
             std::swap(s[a], s[b]);
+            std::swap(r[a], r[b]);
             a++;
         }
     }
 
+    // Per-bin boxes and counts of one axis: an object goes to the first bin
+    // k whose divider it does not exceed (c <= divs[k]), or to the last.
+    struct Bins {
+        Box box[3][SAH_BINS];
+        uint32_t count[3][SAH_BINS];
+        Bins() {
+            for (int d = 0; d < 3; d++)
+                for (uint32_t k = 0; k < SAH_BINS; k++) {
+                    box[d][k] = box_empty();
+                    count[d][k] = 0;
+                }
+        }
+    };
+    static void bin_range(const ObjInfo* recs, uint32_t lo, uint32_t hi, const float (&divs)[3][SAH_BINS - 1],
+                          Bins& b) {
+        for (uint32_t i = lo; i < hi; i++) {
+            const ObjInfo& o = recs[i];
+            for (int d = 0; d < 3; d++) {
+                const float c = comp(o.centroid, d);
+                uint32_t k = SAH_BINS - 1;
+                for (int j = (int)SAH_BINS - 2; j >= 0; j--)
+                    if (c <= divs[d][j]) k = (uint32_t)j;
+                b.box[d][k] = box_union(b.box[d][k], o.mid);
+                b.count[d][k]++;
+            }
+        }
+    }
+
     // sah_split (objects_split.rs:153-238)
-    void sah_split(uint32_t* keys, uint32_t n, uint32_t& split, uint8_t& axis) {
+    void sah_split(uint32_t* keys, ObjInfo* recs, uint32_t n, uint32_t& split, uint8_t& axis) {
         Box cb = box_empty();
-        for (uint32_t i = 0; i < n; i++) cb = box_union_pt(cb, info_[keys[i]].mid_center);
+        for (uint32_t i = 0; i < n; i++) cb = box_union_pt(cb, recs[i].mid_center);
 
         float divs[3][SAH_BINS - 1];
         for (int d = 0; d < 3; d++) {
@@ -246,17 +298,60 @@ class Builder {
         uint32_t nl[3][SAH_BINS - 1] = {}, nr[3][SAH_BINS - 1] = {};
         for (int d = 0; d < 3; d++)
             for (uint32_t k = 0; k < SAH_BINS - 1; k++) left[d][k] = right[d][k] = box_empty();
-        for (uint32_t i = 0; i < n; i++) {
-            const ObjInfo& o = info_[keys[i]];
+        if (order_free_) {
+            // Bin once per object, split large nodes over threads, then
+            // combine the bins into the candidates' left and right sides.
+            Bins bins;
+            const unsigned parts = n >= (1u << 18) ? std::max(1u, std::min(threads_, 16u)) : 1u;
+            if (parts > 1) {
+                std::vector<Bins> part(parts);
+                std::vector<std::thread> pool;
+                for (unsigned t = 0; t < parts; t++)
+                    pool.emplace_back([&, t] {
+                        bin_range(recs, (uint32_t)((uint64_t)n * t / parts), (uint32_t)((uint64_t)n * (t + 1) / parts),
+                                  divs, part[t]);
+                    });
+                for (auto& th : pool) th.join();
+                for (unsigned t = 0; t < parts; t++)
+                    for (int d = 0; d < 3; d++)
+                        for (uint32_t k = 0; k < SAH_BINS; k++) {
+                            bins.box[d][k] = box_union(bins.box[d][k], part[t].box[d][k]);
+                            bins.count[d][k] += part[t].count[d][k];
+                        }
+            } else {
+                bin_range(recs, 0, n, divs, bins);
+            }
             for (int d = 0; d < 3; d++) {
-                float c = comp(o.centroid, d);
+                Box acc = box_empty();
+                uint32_t cnt = 0;
                 for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
-                    if (c <= divs[d][k]) {
-                        left[d][k] = box_union(left[d][k], o.mid);
-                        nl[d][k]++;
-                    } else {
-                        right[d][k] = box_union(right[d][k], o.mid);
-                        nr[d][k]++;
+                    acc = box_union(acc, bins.box[d][k]);
+                    cnt += bins.count[d][k];
+                    left[d][k] = acc;
+                    nl[d][k] = cnt;
+                }
+                acc = box_empty();
+                cnt = 0;
+                for (int k = (int)SAH_BINS - 2; k >= 0; k--) {
+                    acc = box_union(acc, bins.box[d][k + 1]);
+                    cnt += bins.count[d][k + 1];
+                    right[d][k] = acc;
+                    nr[d][k] = cnt;
+                }
+            }
+        } else {
+            for (uint32_t i = 0; i < n; i++) {
+                const ObjInfo& o = recs[i];
+                for (int d = 0; d < 3; d++) {
+                    float c = comp(o.centroid, d);
+                    for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
+                        if (c <= divs[d][k]) {
+                            left[d][k] = box_union(left[d][k], o.mid);
+                            nl[d][k]++;
+                        } else {
+                            right[d][k] = box_union(right[d][k], o.mid);
+                            nr[d][k]++;
+                        }
                     }
                 }
             }
@@ -277,7 +372,7 @@ class Builder {
                 }
             }
 
-        uint32_t s = hoare_partition(keys, n, [&](uint32_t k) { return comp(info_[k].centroid, best_d) < best_div; });
+        uint32_t s = hoare_partition(keys, recs, n, [&](const ObjInfo& o) { return comp(o.centroid, best_d) < best_div; });
         if (s < 1) s = 1;
         else if (s >= n) s = n - 1;
         split = s;
@@ -296,9 +391,9 @@ class Builder {
 
     // median_split (objects_split.rs:291-341) with quick_select
     // (algorithm.rs:203-231).
-    void median_split(uint32_t* keys, uint32_t n, uint32_t& split, uint8_t& axis_out) {
+    void median_split(uint32_t* keys, ObjInfo* recs, uint32_t n, uint32_t& split, uint8_t& axis_out) {
         Box b = box_empty();
-        for (uint32_t i = 0; i < n; i++) b = box_union(b, info_[keys[i]].mid);
+        for (uint32_t i = 0; i < n; i++) b = box_union(b, recs[i].mid);
         int axis = 0;
         float largest = -__builtin_huge_valf();
         for (int d = 0; d < 3; d++) {
@@ -308,16 +403,18 @@ class Builder {
         uint32_t place = n / 2;
         if (place == 0) place = 1;
 
-        auto key_of = [&](uint32_t k) { return comp(info_[k].centroid, axis); };
+        auto key_of = [&](const ObjInfo& o) { return comp(o.centroid, axis); };
         uint32_t left = 0, right = n;
         uint64_t seed = place;
         for (;;) {
             uint32_t i = left + (uint32_t)(hash64(right, seed) % (uint64_t)(right - left));
             std::swap(keys[i], keys[right - 1]);
-            float pivot = key_of(keys[right - 1]);
-            uint32_t ii = left + hoare_partition(keys + left, right - 1 - left,
-                                                 [&](uint32_t k) { return key_of(k) < pivot; });
+            std::swap(recs[i], recs[right - 1]);
+            float pivot = key_of(recs[right - 1]);
+            uint32_t ii = left + hoare_partition(keys + left, recs + left, right - 1 - left,
+                                                 [&](const ObjInfo& o) { return key_of(o) < pivot; });
             std::swap(keys[ii], keys[right - 1]);
+            std::swap(recs[ii], recs[right - 1]);
             if (ii == place) break;
             if (ii > place) right = ii; else left = ii + 1;
             seed += 1;
@@ -455,7 +552,7 @@ HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads) {
 
     Builder b(in, per_leaf, threads);
     uint32_t base_depth = 0;
-    std::unique_ptr<BaseNode> root = b.build(out.order.data(), in.count, 0, 0, threads, base_depth);
+    std::unique_ptr<BaseNode> root = b.build_all(out.order.data(), in.count, threads, base_depth);
     out.depth = base_depth / 2 + 1;
     out.root_bounds = root->bounds;
 

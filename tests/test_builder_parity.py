@@ -111,3 +111,30 @@ def test_single_instance_leaf_root():
     assert_same(s, o, 1)
     d, _ = s.dump_bvh(0)
     assert d.tolist() == [1, 1, 1, 1, 0, 1]  # has_root, depth, node_count, leaf(0..1)
+
+
+def test_signed_zeros_take_the_sequential_accumulation():
+    """Coordinates of -0.0 and +0.0 make min/max unions depend on order, so
+    the builder accumulates SAH candidates object by object as the
+    reference does; the trees still match."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    v = rng.uniform(-1, 1, size=(3 * n, 3)).astype(np.float32)
+    v[rng.random(v.shape) < 0.3] = np.float32(0.0)
+    neg = rng.random(v.shape) < 0.5
+    v[(v == 0) & neg] = np.float32(-0.0)
+    assert np.signbit(v[v == 0]).any() and (~np.signbit(v[v == 0])).any()
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
+
+
+def test_large_soup_bins_over_threads():
+    """Nodes of 2^18 objects or more are binned by several threads."""
+    rng = np.random.default_rng(8)
+    n = 300_000
+    c = rng.uniform(-50, 50, size=(n, 1, 3))
+    v = (c + rng.uniform(-0.5, 0.5, size=(n, 3, 3))).astype(np.float32).reshape(-1, 3) + np.float32(0.0)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    s, o = both([(v[None], f)], [(0, None)])
+    assert_same(s, o, 1)
