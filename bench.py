@@ -411,7 +411,7 @@ def large_bench(args) -> dict:
     sc = scenes.large_scene(spp=args.large_spp)
     r = render.build(sc)
     log(f"large scene built in {time.time() - t0:.1f}s")
-    trial = dict(width=480, height=270, spp=4, max_bounces=8, seed=0)
+    trial = dict(width=960, height=540, spp=4, max_bounces=8, seed=0)
     t1 = time.time()
     choice = r.autotune(sc.camera, trial)
     log(f"large scene autotune {time.time() - t1:.1f}s -> {choice}")

@@ -187,6 +187,16 @@ int psy_server_submit(psy_server* server, uint32_t renderer, const psy_camera_de
 /* Blocks until the request is complete, copies its XYZ image to host memory
  * (`xyz` may be NULL to discard it) and releases the request. */
 int psy_server_wait(psy_server* server, uint64_t request_id, float* xyz, psy_request_stats* stats);
+/* Incremental delivery: the number of leading pixels of request `id`
+ * (row-major from the top row) that are final, i.e. whose samples have all
+ * been rendered, and a copy of final pixels [pixel_begin, pixel_end) as XYZ
+ * floats into host memory.  Valid until psy_server_wait collects the
+ * request; reading pixels that are not final fails with PSY_E_RANGE.  The
+ * finished rows are what the reference streams as buckets to Blender
+ * (src/renderer.rs:312-346). */
+int psy_server_progress(psy_server* server, uint64_t id, uint64_t* pixels_done);
+int psy_server_read(psy_server* server, uint64_t id, uint64_t pixel_begin, uint64_t pixel_end, float* xyz);
+
 int psy_server_get_stats(psy_server* server, psy_server_stats* stats);
 
 #ifdef __cplusplus

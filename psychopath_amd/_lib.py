@@ -249,6 +249,8 @@ _SIGNATURES.update({
                                     C.POINTER(C.c_uint64)]),
     "psy_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(RequestStats)]),
     "psy_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "psy_server_progress": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "psy_server_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
 })
 
 

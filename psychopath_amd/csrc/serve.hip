@@ -44,6 +44,7 @@ struct Request {
     ReqParams rp{};
     std::vector<GpuXform> cam;
     uint64_t total = 0, next = 0;
+    uint64_t retired = 0;  // paths of this request in waves that have finished
     uint64_t film_bytes = 0;
     float* film = nullptr;
     uint32_t slot = 0xffffffffu;
@@ -56,6 +57,7 @@ struct Request {
 struct InFlight {
     uint32_t buf;                    // plan buffer / event index
     std::vector<Request*> finishing; // requests whose last paths are in this wave
+    std::vector<std::pair<Request*, uint64_t>> reached;  // each request's dispatched paths after this wave
 };
 
 struct RenState {
@@ -166,6 +168,7 @@ bool retire(psy_server* sv, RenState& rs) {
     while (!rs.inflight.empty()) {
         InFlight& f = rs.inflight.front();
         if (hipEventQuery(rs.ev[f.buf]) != hipSuccess) break;
+        for (auto& qr : f.reached) qr.first->retired = qr.second;
         for (Request* q : f.finishing) {
             q->done = true;
             q->t_done = Clock::now();
@@ -185,7 +188,7 @@ int launch(psy_server* sv, RenState& rs, bool& launched) {
     const uint32_t buf = (uint32_t)(rs.seq % 2);
     WavePlan& plan = rs.host_plans[buf];
     std::memset(&plan, 0, sizeof(plan));
-    InFlight f{buf, {}};
+    InFlight f{buf, {}, {}};
     uint32_t filled = 0, steps = 1;
     for (Request* q : rs.active) {
         if (filled == r->capacity || plan.nseg == MAX_SEGS) break;
@@ -194,6 +197,7 @@ int launch(psy_server* sv, RenState& rs, bool& launched) {
         plan.seg[plan.nseg++] = WaveSeg{q->slot, filled, (uint32_t)m, 0, q->next};
         q->next += m;
         q->waves++;
+        f.reached.emplace_back(q, q->next);
         filled += (uint32_t)m;
         steps = std::max(steps, q->rp.max_bounces + 1);
         if (q->next == q->total) f.finishing.push_back(q);
@@ -409,6 +413,41 @@ int psy_server_wait(psy_server* sv, uint64_t id, float* xyz, psy_request_stats* 
     delete q;
     if (rc) return fail(rc, "request failed: " + sv->worker_msg);
     if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("image copy: ") + hipGetErrorString(e));
+    return PSY_OK;
+}
+
+int psy_server_progress(psy_server* sv, uint64_t id, uint64_t* pixels_done) {
+    if (!sv || !pixels_done) return fail(PSY_E_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(id);
+    if (it == sv->reqs.end()) return fail(PSY_E_INVALID, "unknown request id");
+    const Request* q = it->second;
+    if (q->error) return fail(q->error, "request failed: " + sv->worker_msg);
+    // Waves finish in dispatch order and a pixel's samples are consecutive
+    // paths, so every pixel whose last sample has retired is final.
+    *pixels_done = q->done ? q->total / q->rp.spp : q->retired / q->rp.spp;
+    return PSY_OK;
+}
+
+int psy_server_read(psy_server* sv, uint64_t id, uint64_t pixel_begin, uint64_t pixel_end, float* xyz) {
+    if (!sv || (!xyz && pixel_end > pixel_begin)) return fail(PSY_E_INVALID, "NULL argument");
+    const float* film;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        auto it = sv->reqs.find(id);
+        if (it == sv->reqs.end()) return fail(PSY_E_INVALID, "unknown request id");
+        const Request* q = it->second;
+        if (q->error) return fail(q->error, "request failed: " + sv->worker_msg);
+        const uint64_t done = q->done ? q->total / q->rp.spp : q->retired / q->rp.spp;
+        if (pixel_begin > pixel_end || pixel_end > done) return fail(PSY_E_RANGE, "pixels not finished yet");
+        film = q->film;
+    }
+    // The film stays allocated until psy_server_wait collects the request;
+    // the finished pixels are no longer written, so the copy can run beside
+    // the waves still rendering the rest.
+    if (pixel_end > pixel_begin)
+        PSY_HIP(hipMemcpy(xyz, film + 3 * pixel_begin, (pixel_end - pixel_begin) * 3 * sizeof(float),
+                          hipMemcpyDeviceToHost));
     return PSY_OK;
 }
 

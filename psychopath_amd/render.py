@@ -170,6 +170,36 @@ class Server:
                                                         C.byref(rs)))
         return img, {f: getattr(rs, f) for f, _ in _lib.RequestStats._fields_}
 
+    def progress(self, request_id: int) -> int:
+        """Leading pixels of the request (row-major from the top) that are
+        final."""
+        n = C.c_uint64()
+        check("psy_server_progress", lib().psy_server_progress(self.handle, int(request_id), C.byref(n)))
+        return n.value
+
+    def read(self, request_id: int, begin: int, end: int) -> np.ndarray:
+        """Final pixels [begin, end) as XYZ [end - begin, 3]."""
+        out = np.zeros((max(0, end - begin), 3), np.float32)
+        check("psy_server_read", lib().psy_server_read(self.handle, int(request_id), int(begin), int(end),
+                                                        out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def stream(self, request_id: int, poll_s: float = 0.002):
+        """Yields (row_begin, rows) as whole rows of the request become
+        final, then collects it; rows are XYZ [n, width, 3].  The rows are
+        what the reference streams to Blender bucket by bucket."""
+        import time
+        h, w = self._settings[request_id]
+        sent = 0
+        while sent < h:
+            done_rows = min(h, self.progress(request_id) // w)
+            if done_rows > sent:
+                yield sent, self.read(request_id, sent * w, done_rows * w).reshape(done_rows - sent, w, 3)
+                sent = done_rows
+            else:
+                time.sleep(poll_s)
+        self.wait(request_id, keep_image=False)
+
     def stats(self) -> dict:
         st = _lib.ServerStats()
         check("psy_server_get_stats", lib().psy_server_get_stats(self.handle, C.byref(st)))

@@ -266,3 +266,29 @@ def test_mesh_without_shader_emits_magenta():
     # the emitting box adds light, more in X and Z (magenta) than in Y
     gain = img.mean((0, 1)) - shaded.mean((0, 1))
     assert gain[0] > gain[1] > 0 and gain[2] > gain[1]
+
+
+def test_served_rows_stream_as_they_finish():
+    """A request rendered over many waves delivers its finished rows while
+    later rows still render; the rows put together equal the image the
+    request gives when collected whole, and equal a plain render."""
+    sc = scenes.cornell_box(64, 48, 8, 3, 5)
+    r = render.build(sc, max_paths=2048)
+    alone, _ = r.render(sc.camera, sc.settings)
+    server = render.Server([r])
+    rid = server.submit(0, sc.camera, sc.settings)
+    parts, seen = [], []
+    for row, rows in server.stream(rid):
+        seen.append(row)
+        parts.append(rows)
+    server.close()
+    img = np.concatenate(parts, axis=0)
+    assert img.shape == alone.shape and np.array_equal(img, alone)
+    assert len(parts) > 1 and seen == sorted(seen)
+    from psychopath_amd._lib import PsyError
+    server = render.Server([r])
+    rid = server.submit(0, sc.camera, sc.settings)
+    with pytest.raises(PsyError):
+        server.read(rid, 0, 64 * 48 + 1)
+    server.wait(rid)
+    server.close()
