@@ -292,3 +292,29 @@ def test_served_rows_stream_as_they_finish():
         server.read(rid, 0, 64 * 48 + 1)
     server.wait(rid)
     server.close()
+
+
+def test_command_line_renders_a_scene_file(tmp_path, capsys):
+    """`python -m psychopath_amd -i scene.psy` writes the scene's Output
+    path as Image::write_png does; --serialized_output streams the same
+    pixels in the bucket protocol; --crop keeps the rectangle."""
+    from psychopath_amd import __main__ as cli
+    from psychopath_amd import film, psyfile
+    out = tmp_path / "cornell.png"
+    text = psyfile.write(scenes.cornell_box(40, 30, 2, 2, 0), "cornell", str(out))
+    src = tmp_path / "cornell.psy"
+    src.write_text(text)
+    assert cli.main(["-i", str(src), "-s", "4"]) == 0
+    sc = psyfile.read(text)[0]
+    sc.settings["spp"] = 4
+    img, _, _ = gpu_render(sc)
+    assert out.read_bytes() == film.encode(img, film.FORMATS[".png"])
+    capsys.readouterr()
+    assert cli.main(["-i", str(src), "-s", "4", "--serialized_output"]) == 0
+    lines = capsys.readouterr().out.splitlines()
+    assert lines[0] == "DIV" and lines[-2:] == ["BUCKET_END", "DIV"]
+    assert lines[3] == film.bucket_base64(img, 0, 0, 32, 30)
+    assert cli.main(["-i", str(src), "-s", "4", "--crop", "5", "6", "20", "10"]) == 0
+    cropped = np.zeros_like(img)
+    cropped[6:11, 5:21] = img[6:11, 5:21]
+    assert out.read_bytes() == film.encode(cropped, film.FORMATS[".png"])
