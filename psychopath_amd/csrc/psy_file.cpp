@@ -80,8 +80,12 @@ public:
     }
 
 private:
+    // Nesting deeper than this is refused instead of recursing further (the
+    // reference recurses without a limit and overflows its stack).
+    static constexpr int MAX_NESTING = 2048;
     const char* s_;
     uint64_t n_;
+    int depth_ = 0;
 
     // Whitespace and comments (data_tree.rs:536-553).
     uint64_t skip(uint64_t p) const {
@@ -159,7 +163,9 @@ private:
                 if (token(p2, b, e) != Tok::OpenInner) raise("MissingOpenInternal", p2, "expected '{'");
                 p2 = e;
             }
+            if (++depth_ > MAX_NESTING) raise("NestingTooDeep", p1, "nodes nested more than 2048 deep");
             const uint64_t rem = read_children(p2, node.children);
+            --depth_;
             if (token(rem, b, e) != Tok::CloseInner) raise("MissingCloseInternal", rem, "expected '}'");
             after = e;
             return true;

@@ -386,3 +386,11 @@ def test_unsupported_time_samples_are_rejected():
         with pytest.raises(psyfile.PsyFileError) as ce:
             psyfile.read(text)
         assert ce.value.kind == "UnknownVariant" and "not supported" in str(ce.value)
+
+
+def test_deep_nesting_is_refused_not_crashed():
+    text = "A { " * 5000 + "} " * 5000
+    with pytest.raises(psyfile.PsyFileError) as e:
+        psyfile.tree(text)
+    assert e.value.kind == "NestingTooDeep"
+    assert psyfile.tree("A { " * 100 + "} " * 100).count("\n") == 101
