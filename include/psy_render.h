@@ -74,13 +74,18 @@ typedef struct {
 
 /* One render (Renderer, src/renderer.rs:32-38): resolution, samples per
  * pixel, seed, and the bounce limit of a path (the reference fixes it at 2,
- * renderer.rs:599). */
+ * renderer.rs:599).  A crop region (crop_width > 0) renders only the pixels
+ * [crop_x, crop_x + crop_width) x [crop_y, crop_y + crop_height) of the
+ * width x height image, which keeps its size with the rest black, as the
+ * reference's --crop does (renderer.rs:97-108); pixels inside are the same
+ * as in the full render. */
 typedef struct {
     uint32_t width;
     uint32_t height;
     uint32_t spp;
     uint32_t max_bounces;
     uint32_t seed;
+    uint32_t crop_x, crop_y, crop_width, crop_height;
 } psy_render_settings;
 
 /* RenderStats (src/renderer.rs:41-49), as measured on the device. */
@@ -188,7 +193,8 @@ int psy_server_submit(psy_server* server, uint32_t renderer, const psy_camera_de
  * (`xyz` may be NULL to discard it) and releases the request. */
 int psy_server_wait(psy_server* server, uint64_t request_id, float* xyz, psy_request_stats* stats);
 /* Incremental delivery: the number of leading pixels of request `id`
- * (row-major from the top row) that are final, i.e. whose samples have all
+ * (row-major from the top row of its crop region, or of the image) that are
+ * final, i.e. whose samples have all
  * been rendered, and a copy of final pixels [pixel_begin, pixel_end) as XYZ
  * floats into host memory.  Valid until psy_server_wait collects the
  * request; reading pixels that are not final fails with PSY_E_RANGE.  The

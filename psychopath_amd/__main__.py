@@ -5,9 +5,8 @@ path: `python -m psychopath_amd -i scene.psy [-s N] [--crop X1 Y1 X2 Y2]
 Every Scene of the file is built, rendered on the GPU and written to its
 Output path (.png or .exr, as Image::write_png / write_exr).  `-b/--spb` and
 `-t/--threads` are accepted for compatibility: the GPU renders in waves of
-paths instead of CPU buckets and threads.  `--crop` keeps the pixels of the
-inclusive rectangle and leaves the rest black, as the reference's cropped
-render does (the GPU renders the whole frame; cropped pixels are the same).
+paths instead of CPU buckets and threads.  `--crop` renders only the
+inclusive rectangle and leaves the rest black, as the reference does.
 With `--serialized_output` the image goes to standard output in the
 reference's bucket protocol (DIV / percentage / rectangle / base64 RGBA /
 BUCKET_END / DIV, renderer.rs:328-336) instead of to disk.
@@ -18,8 +17,6 @@ from __future__ import annotations
 import argparse
 import sys
 import time
-
-import numpy as np
 
 
 def _read_stdin() -> str:
@@ -79,17 +76,17 @@ def main(argv=None) -> int:
             sc.settings["spp"] = a.spp
         r = render.build(sc)
         say(f"\tBuilt scene in {time.perf_counter() - t:.3f}s")
-        say("Rendering scene on the GPU...")
-        t = time.perf_counter()
-        img, st = r.render(sc.camera, sc.settings)
-        rtime = time.perf_counter() - t
+        settings = dict(sc.settings)
         if a.crop:
-            h, w, _ = img.shape
+            # Inclusive corners clamped to the image (renderer.rs:97-108).
+            w, h = settings["width"], settings["height"]
             x1, y1, x2, y2 = (min(a.crop[0], w - 1), min(a.crop[1], h - 1), min(a.crop[2], w - 1),
                               min(a.crop[3], h - 1))
-            keep = np.zeros_like(img)
-            keep[y1:y2 + 1, x1:x2 + 1] = img[y1:y2 + 1, x1:x2 + 1]
-            img = keep
+            settings["crop"] = (x1, y1, x2 - x1 + 1, y2 - y1 + 1)
+        say("Rendering scene on the GPU...")
+        t = time.perf_counter()
+        img, st = r.render(sc.camera, settings)
+        rtime = time.perf_counter() - t
         say(f"\tRendered scene in {rtime:.3f}s")
         say(f"\t\tTrace:                  {st['trace_ms'] / 1e3:.3f}s (device time of the whole render)")
         say(f"\t\t\tRays traced:          {st['rays']}")

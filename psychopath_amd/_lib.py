@@ -115,6 +115,10 @@ class RenderSettings(C.Structure):
         ("spp", C.c_uint32),
         ("max_bounces", C.c_uint32),
         ("seed", C.c_uint32),
+        ("crop_x", C.c_uint32),
+        ("crop_y", C.c_uint32),
+        ("crop_width", C.c_uint32),
+        ("crop_height", C.c_uint32),
     ]
 
 

@@ -112,7 +112,7 @@ __device__ __forceinline__ void camera_path(const ShadeScene& sc, const ReqParam
                                             V3& o, V3& d) {
     const uint32_t pixel = (uint32_t)(g / rp.spp);
     const uint32_t si = (uint32_t)(g % rp.spp);
-    const uint32_t x = pixel % rp.width, y = pixel / rp.width;
+    const uint32_t x = rp.rx + pixel % rp.rw, y = rp.ry + pixel / rp.rw;
     const S4 d0 = sample_4d(sc.hdims, sc.hperms, si, 0, x, y, rp.seed);
     const S4 d1 = sample_4d(sc.hdims, sc.hperms, si, 1, x, y, rp.seed);
     const float filter_x = probit(d0.v[3], 2.0f / 6.0f) + 0.5f;
@@ -668,7 +668,7 @@ template <int MINW>
 __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
     __shared__ uint32_t lds_stack[MEGA_LDS * MEGA_BLOCK];
     const ReqParams rp = reqs[0];
-    const uint32_t pixels = rp.width * rp.height;
+    const uint32_t pixels = rp.rw * rp.rh;
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t* lds_col = lds_stack + threadIdx.x;
     const float fspp = (float)rp.spp;
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, Sha
     for (int off = 32; off > 0; off >>= 1) traced += __shfl_xor(traced, off);
     if ((threadIdx.x & 63u) == 0u && traced) atomicAdd(sc.rays, (unsigned long long)traced);
     if (pixel >= pixels) return;
-    float* px = rp.film + 3ull * pixel;
+    float* px = rp.film + 3 * region_pixel(rp, pixel);
     px[0] = fx;
     px[1] = fy;
     px[2] = fz;
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(KBLOCK) void film_kernel(const WavePlan* plans, con
         const ReqParams rp = reqs[sg.req];
         const uint64_t g = sg.g0 + (i - sg.slot0);
         if (i != sg.slot0 && g % rp.spp != 0) continue;
-        const uint64_t pixel = g / rp.spp;
+        const uint64_t pixel = region_pixel(rp, g / rp.spp);
         const uint32_t end = sg.slot0 + sg.count;
         float* px = rp.film + 3 * pixel;
         float fx = px[0], fy = px[1], fz = px[2];
@@ -941,6 +941,19 @@ int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqPa
         rp.focus = 1.0f;
     }
     rp.cam_xf_count = cam->xform_count;
+    rp.rx = 0;
+    rp.ry = 0;
+    rp.rw = st->width;
+    rp.rh = st->height;
+    if (st->crop_width != 0 || st->crop_height != 0) {
+        if (st->crop_width == 0 || st->crop_height == 0 || (uint64_t)st->crop_x + st->crop_width > st->width ||
+            (uint64_t)st->crop_y + st->crop_height > st->height)
+            return fail(PSY_E_RANGE, "crop region outside the image");
+        rp.rx = st->crop_x;
+        rp.ry = st->crop_y;
+        rp.rw = st->crop_width;
+        rp.rh = st->crop_height;
+    }
     return PSY_OK;
 }
 
@@ -1146,7 +1159,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     int rc = make_params(cam, st, rp);
     if (rc) return rc;
     const uint64_t pixels = (uint64_t)st->width * st->height;
-    const uint64_t total = pixels * st->spp;
+    const uint64_t total = (uint64_t)rp.rw * rp.rh * st->spp;
 
     std::vector<GpuXform> cx(cam->xform_count);
     for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(cx[i].r, cam->xforms + 12ull * i, 48);
@@ -1194,7 +1207,8 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
                     : r->plain_waves == 3 ? megakernel<3>
                     : r->plain_waves == 2 ? megakernel<2>
                                           : megakernel<1>;
-        hipLaunchKernelGGL(mega, dim3((unsigned)((pixels + MEGA_BLOCK - 1) / MEGA_BLOCK)), dim3(MEGA_BLOCK), 0, s,
+        hipLaunchKernelGGL(mega, dim3((unsigned)(((uint64_t)rp.rw * rp.rh + MEGA_BLOCK - 1) / MEGA_BLOCK)),
+                           dim3(MEGA_BLOCK), 0, s,
                            scene_args(r->scene), r->sc, r->reqs);
         PSY_HIP(hipGetLastError());
     } else {

@@ -29,10 +29,18 @@ struct ReqParams {
     uint32_t width, height, spp, max_bounces, seed;
     float cmpx, cmpy, x_extent, y_extent, tfov, aperture, focus;
     uint32_t cam_xf_count;
+    // Rendered region (the whole image unless cropped): path g belongs to
+    // region pixel g / spp, row-major in the region.
+    uint32_t rx, ry, rw, rh;
     uint32_t pad;
     const GpuXform* cam_xf;
     float* film;
 };
+
+// Image pixel index of region pixel `p`.
+__host__ __device__ inline uint64_t region_pixel(const ReqParams& rp, uint64_t p) {
+    return (uint64_t)(rp.ry + (uint32_t)(p / rp.rw)) * rp.width + rp.rx + (uint32_t)(p % rp.rw);
+}
 
 // A run of consecutive path slots of a wave taken by one request: slots
 // [slot0, slot0 + count) carry the request's global paths g0, g0 + 1, ...

@@ -357,7 +357,7 @@ int psy_server_submit(psy_server* sv, uint32_t renderer, const psy_camera_desc* 
     q->ren = renderer;
     q->cam.resize(cam->xform_count);
     for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(q->cam[i].r, cam->xforms + 12ull * i, 48);
-    q->total = (uint64_t)st->width * st->height * st->spp;
+    q->total = (uint64_t)q->rp.rw * q->rp.rh * st->spp;
     q->film_bytes = (uint64_t)st->width * st->height * 3 * sizeof(float);
     q->t_submit = Clock::now();
     {
@@ -432,6 +432,7 @@ int psy_server_progress(psy_server* sv, uint64_t id, uint64_t* pixels_done) {
 int psy_server_read(psy_server* sv, uint64_t id, uint64_t pixel_begin, uint64_t pixel_end, float* xyz) {
     if (!sv || (!xyz && pixel_end > pixel_begin)) return fail(PSY_E_INVALID, "NULL argument");
     const float* film;
+    ReqParams rp;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         auto it = sv->reqs.find(id);
@@ -441,13 +442,19 @@ int psy_server_read(psy_server* sv, uint64_t id, uint64_t pixel_begin, uint64_t 
         const uint64_t done = q->done ? q->total / q->rp.spp : q->retired / q->rp.spp;
         if (pixel_begin > pixel_end || pixel_end > done) return fail(PSY_E_RANGE, "pixels not finished yet");
         film = q->film;
+        rp = q->rp;
     }
     // The film stays allocated until psy_server_wait collects the request;
     // the finished pixels are no longer written, so the copy can run beside
-    // the waves still rendering the rest.
-    if (pixel_end > pixel_begin)
-        PSY_HIP(hipMemcpy(xyz, film + 3 * pixel_begin, (pixel_end - pixel_begin) * 3 * sizeof(float),
+    // the waves still rendering the rest.  Region pixels are contiguous in
+    // the film within a row of the region.
+    uint64_t p = pixel_begin;
+    while (p < pixel_end) {
+        const uint64_t row_end = std::min<uint64_t>(pixel_end, (p / rp.rw + 1) * rp.rw);
+        PSY_HIP(hipMemcpy(xyz + 3 * (p - pixel_begin), film + 3 * region_pixel(rp, p), (row_end - p) * 3 * sizeof(float),
                           hipMemcpyDeviceToHost));
+        p = row_end;
+    }
     return PSY_OK;
 }
 

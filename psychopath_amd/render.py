@@ -90,8 +90,11 @@ class Renderer:
 
     @staticmethod
     def _settings(settings: dict):
+        """`crop` (optional): (x, y, width, height) of the region to render."""
+        cx, cy, cw, ch = settings.get("crop") or (0, 0, 0, 0)
         return _lib.RenderSettings(int(settings["width"]), int(settings["height"]), int(settings["spp"]),
-                                   int(settings.get("max_bounces", 2)), int(settings.get("seed", 0)))
+                                   int(settings.get("max_bounces", 2)), int(settings.get("seed", 0)), int(cx), int(cy),
+                                   int(cw), int(ch))
 
     def render(self, camera: dict, settings: dict, device_out=None) -> tuple[np.ndarray | None, dict]:
         """Renders one image.  `camera`: xforms [N,12] (camera to world), fov
@@ -158,11 +161,12 @@ class Server:
         rid = C.c_uint64()
         check("psy_server_submit", lib().psy_server_submit(self.handle, int(renderer), C.byref(cam), C.byref(st),
                                                             C.byref(rid)))
-        self._settings[rid.value] = (st.height, st.width)
+        rw, rh = (st.crop_width, st.crop_height) if st.crop_width else (st.width, st.height)
+        self._settings[rid.value] = (st.height, st.width, rh, rw)
         return rid.value
 
     def wait(self, request_id: int, keep_image: bool = True) -> tuple[np.ndarray | None, dict]:
-        h, w = self._settings.pop(request_id)
+        h, w, _, _ = self._settings.pop(request_id)
         img = np.zeros((h, w, 3), np.float32) if keep_image else None
         rs = _lib.RequestStats()
         check("psy_server_wait", lib().psy_server_wait(self.handle, int(request_id),
@@ -185,11 +189,12 @@ class Server:
         return out
 
     def stream(self, request_id: int, poll_s: float = 0.002):
-        """Yields (row_begin, rows) as whole rows of the request become
-        final, then collects it; rows are XYZ [n, width, 3].  The rows are
-        what the reference streams to Blender bucket by bucket."""
+        """Yields (row_begin, rows) as whole rows of the request (of its crop
+        region, if cropped) become final, then collects it; rows are XYZ
+        [n, width, 3].  The rows are what the reference streams to Blender
+        bucket by bucket."""
         import time
-        h, w = self._settings[request_id]
+        _, _, h, w = self._settings[request_id]
         sent = 0
         while sent < h:
             done_rows = min(h, self.progress(request_id) // w)

@@ -318,3 +318,29 @@ def test_command_line_renders_a_scene_file(tmp_path, capsys):
     cropped = np.zeros_like(img)
     cropped[6:11, 5:21] = img[6:11, 5:21]
     assert out.read_bytes() == film.encode(cropped, film.FORMATS[".png"])
+
+
+def test_crop_region_renders_the_same_pixels():
+    """A crop region renders only its pixels, leaving the rest black, with
+    the values of the full render (both forms, and served with rows
+    streamed from the region)."""
+    sc = scenes.cornell_box(48, 40, 4, 3, 1)
+    full, stats, r = gpu_render(sc)
+    crop = dict(sc.settings, crop=(7, 5, 20, 13))
+    want = np.zeros_like(full)
+    want[5:18, 7:27] = full[5:18, 7:27]
+    img, cstats = r.render(sc.camera, crop)
+    assert np.array_equal(img, want) and cstats["paths"] == 20 * 13 * 4
+    r.set_options(plain=True)
+    img, _ = r.render(sc.camera, crop)
+    assert np.array_equal(img, want)
+    r.set_options(plain=False)
+    small = render.build(sc, max_paths=512)
+    server = render.Server([small])
+    rid = server.submit(0, sc.camera, crop)
+    rows = np.concatenate([part for _, part in server.stream(rid)], axis=0)
+    server.close()
+    assert np.array_equal(rows, full[5:18, 7:27])
+    from psychopath_amd._lib import PsyError
+    with pytest.raises(PsyError):
+        r.render(sc.camera, dict(sc.settings, crop=(40, 0, 10, 5)))
