@@ -23,7 +23,12 @@
  * lists) come back as PSY_E_PARSE with psy_last_error() giving
  * "<kind> at byte <offset> (line <n>): <message>".
  *
- * Not supported (PSY_E_PARSE with a message saying so): nested assemblies,
+ * Nested assemblies are flattened into the one-level scene: every object
+ * instance inside them gets the time-sampled transforms the reference's
+ * transform stack would compose for it (transform_stack.rs:33-60), and their
+ * shaders and meshes are appended after the root's, depth first in
+ * definition order.  Not supported (PSY_E_PARSE with a message saying so):
+ * lights inside nested assemblies (the flattened scene has one light tree),
  * and more than one time sample of camera field of view, aperture or focus
  * distance or of a light's size, direction or colour.  Files carry no
  * bounce limit: settings.max_bounces is the reference renderer's fixed 2

@@ -23,8 +23,10 @@ nan / inf / infinity, parsed with correct rounding to f32.  Parity of the
 corner cases of that dependency (its exact special-value spellings) is
 unpinned; the reference's own basics.rs tests pin the rest.
 
-Not restated: nested assemblies (parse_assembly's "Assembly" children) raise
-here as they do in the product reader, which flattens no assembly tree.
+Nested assemblies are parsed as the reference parses them (each with its
+own shaders and objects); to_scene() keeps them nested for the oracle
+renderer, whose tracer walks them with a transform stack like tracer.rs.
+Lights inside sub-assemblies are refused there, as in the product reader.
 
 The result is a plain dict: settings, camera, world, shaders, meshes,
 lights and instances in the layout of psychopath_amd.scenes.RenderScene
@@ -620,7 +622,8 @@ def _parse_mesh(node):
 
 
 def _parse_assembly(node):
-    """psy_assembly.rs:18-188 for a single-level assembly."""
+    """psy_assembly.rs:18-188; sub-assemblies recurse and share the object
+    namespace (AssemblyBuilder::name_exists, scene/assembly.rs:284-286)."""
     if not node["internal"]:
         raise PsyParseError("UnknownError", node["offset"], "Assembly")
     shaders, shader_map = [], {}
@@ -629,7 +632,13 @@ def _parse_assembly(node):
     for c in node["children"]:
         t = c["type"]
         if t == "Assembly":
-            raise PsyParseError("UnknownVariant", c["offset"], "nested assemblies are not supported")
+            if not c["internal"] or c["ident"] is None:
+                raise PsyParseError("UnknownError", c["offset"], "Assembly")
+            if c["ident"] in object_map:
+                raise PsyParseError("UnknownError", c["offset"], "duplicate assembly name")
+            object_map[c["ident"]] = len(objects)
+            objects.append(("assembly", _parse_assembly(c)))
+            continue
         if t == "Instance":
             if not c["internal"]:
                 raise PsyParseError("UnknownError", c["offset"], "Instance")
@@ -732,31 +741,51 @@ def normalized(v):
 
 def to_scene(parsed: dict):
     """The parse as the attributes the oracle renderer reads (meshes,
-    lights, instances, shaders, world, camera, settings with the reference
-    renderer's fixed bounce limit of 2, renderer.rs:599)."""
+    lights, instances, nested `sub_assemblies`, shaders, world, camera,
+    settings with the reference renderer's fixed bounce limit of 2,
+    renderer.rs:599).  Shader indices are global: the root assembly's
+    shaders first, then each sub-assembly's, depth first in definition
+    order."""
     from types import SimpleNamespace
-    asm = parsed["assembly"]
     kinds = {"Lambert": 0, "GGX": 1, "Emit": 2}
-    sc = SimpleNamespace(meshes=[], rect_lights=[], sphere_lights=[], instances=[])
-    where = []
-    for kind, o in asm["objects"]:
-        if kind == "mesh":
-            where.append(("mesh", len(sc.meshes)))
-            sc.meshes.append((np.asarray(o["verts"], np.float32), np.asarray(o["tris"], np.uint32).reshape(-1, 3)))
-        elif kind == "rect":
-            where.append(("rect", len(sc.rect_lights)))
-            sc.rect_lights.append((o["dims"][0][0], o["dims"][0][1], color_desc(o["color"][0])))
-        else:
-            where.append(("sphere", len(sc.sphere_lights)))
-            sc.sphere_lights.append((o["radius"][0], color_desc(o["color"][0])))
-    for i in asm["instances"]:
-        kind, idx = where[i["object"]]
-        xf = i["xforms"]
-        if not xf and kind != "mesh":
-            xf = [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]]
-        sc.instances.append(dict(kind=kind, index=idx, shader=i["shader"],
-                                 xforms=np.asarray(xf, np.float32).reshape(-1, 12) if xf else None))
-    sc.shaders = [(kinds[k], color_desc(c), r, f) for k, c, r, f in asm["shaders"]]
+    shaders = []
+
+    def build(asm, root):
+        base = len(shaders)
+        shaders.extend((kinds[k], color_desc(c), r, f) for k, c, r, f in asm["shaders"])
+        ns = SimpleNamespace(meshes=[], rect_lights=[], sphere_lights=[], instances=[], sub_assemblies=[])
+        where = []
+        for kind, o in asm["objects"]:
+            if kind == "mesh":
+                where.append(("mesh", len(ns.meshes)))
+                m = (np.asarray(o["verts"], np.float32), np.asarray(o["tris"], np.uint32).reshape(-1, 3))
+                if o["normals"] is not None:
+                    m = m + (np.asarray(o["normals"], np.float32),)
+                ns.meshes.append(m)
+            elif kind == "assembly":
+                where.append(("assembly", len(ns.sub_assemblies)))
+                ns.sub_assemblies.append(build(o, False))
+            else:
+                if not root:
+                    raise PsyParseError("UnknownVariant", 0, "lights inside nested assemblies are not supported")
+                if kind == "rect":
+                    where.append(("rect", len(ns.rect_lights)))
+                    ns.rect_lights.append((o["dims"][0][0], o["dims"][0][1], color_desc(o["color"][0])))
+                else:
+                    where.append(("sphere", len(ns.sphere_lights)))
+                    ns.sphere_lights.append((o["radius"][0], color_desc(o["color"][0])))
+        for i in asm["instances"]:
+            kind, idx = where[i["object"]]
+            xf = i["xforms"]
+            if not xf and kind in ("rect", "sphere"):
+                xf = [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]]
+            shader = None if (i["shader"] is None or kind != "mesh") else base + i["shader"]
+            ns.instances.append(dict(kind=kind, index=idx, shader=shader,
+                                     xforms=np.asarray(xf, np.float32).reshape(-1, 12) if xf else None))
+        return ns
+
+    sc = build(parsed["assembly"], True)
+    sc.shaders = shaders
     w = parsed["world"]
     sc.background = color_desc(w["background"])
     sc.distant_lights = [(d["radius"][0], tuple(d["direction"][0]), color_desc(d["color"][0]))
@@ -767,18 +796,3 @@ def to_scene(parsed: dict):
     sc.settings = dict(width=parsed["width"], height=parsed["height"], spp=parsed["spp"], max_bounces=2,
                        seed=parsed["seed"])
     return sc
-
-
-def as_arrays(parsed: dict) -> dict:
-    """Numpy views of the parsed scene for comparisons."""
-    asm = parsed["assembly"]
-    out = dict(settings=(parsed["width"], parsed["height"], parsed["spp"], parsed["seed"]),
-               camera=parsed["camera"], world=parsed["world"], shaders=asm["shaders"], objects=[],
-               instances=asm["instances"])
-    for kind, o in asm["objects"]:
-        if kind == "mesh":
-            out["objects"].append((kind, np.asarray(o["verts"], np.float32), None if o["normals"] is None else
-                                   np.asarray(o["normals"], np.float32), np.asarray(o["tris"], np.uint64)))
-        else:
-            out["objects"].append((kind, o))
-    return out

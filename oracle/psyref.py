@@ -48,6 +48,8 @@ _SIGS = {
     "psyref_scene_add_mesh": (C.c_int, [_vp, _fp, C.c_uint32, C.c_uint32, _fp, _u32p, C.c_uint32]),
     "psyref_scene_add_instance": (C.c_int, [_vp, C.c_uint32, C.c_int32, _fp, C.c_uint32]),
     "psyref_scene_build": (C.c_int, [_vp]),
+    "psyref_scene_add_assembly": (C.c_int, [_vp, _vp]),
+    "psyref_scene_add_assembly_instance": (C.c_int, [_vp, C.c_uint32, _fp, C.c_uint32]),
     "psyref_scene_dump_bvh": (C.c_uint64, [_vp, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_uint64]),
     "psyref_trace": (C.c_int, [_vp] + [_fp] * 8 + [_u8p, C.c_uint64, C.c_uint32, C.c_uint32, _u8p] + [_fp] * 4
                      + [_u32p, _u32p, _fp, _fp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
@@ -113,6 +115,23 @@ class OracleScene:
     def build(self) -> "OracleScene":
         lib().psyref_scene_build(self.h)
         return self
+
+    def add_assembly(self, child: "OracleScene") -> int:
+        """Moves the built `child` in as a sub-assembly; returns its index."""
+        r = lib().psyref_scene_add_assembly(self.h, child.h)
+        if r < 0:
+            raise ValueError("sub-assembly must be built first")
+        child.h = None
+        return r
+
+    def add_assembly_instance(self, assembly: int, xforms=None) -> int:
+        x = None if xforms is None else f32(xforms).reshape(-1, 12)
+        r = lib().psyref_scene_add_assembly_instance(self.h, assembly, None if x is None else fptr(x),
+                                                     0 if x is None else x.shape[0])
+        if r < 0:
+            raise ValueError("unknown sub-assembly")
+        self.instance_count += 1
+        return r
 
     def dump_bvh(self, mesh: int):
         n = lib().psyref_scene_dump_bvh(self.h, mesh, None, 0, None, 0)
@@ -197,28 +216,40 @@ def _color(c):
     return _Color(int(c[0]), float(c[1]), float(c[2]), float(c[3]))
 
 
-def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
-    """Renders a psychopath_amd.scenes.RenderScene with the oracle.
-    Returns the XYZ image [H, W, 3] and timing figures."""
+def _assembly(asm) -> OracleScene:
+    """An oracle assembly from `asm` (a RenderScene, or one of its optional
+    `sub_assemblies`: objects with meshes, rect_lights, sphere_lights,
+    instances and sub_assemblies; instances of kind "assembly" name a
+    sub-assembly by index), built."""
     o = OracleScene()
-    for m in scene.meshes:
+    for m in asm.meshes:
         o.add_mesh(m[0], m[1], m[2] if len(m) > 2 else None)
     L = lib()
-    for (w, h, col) in scene.rect_lights:
+    for (w, h, col) in getattr(asm, "rect_lights", []):
         L.psyref_scene_add_rect_light(o.h, w, h, C.byref(_color(col)))
-    for (r, col) in scene.sphere_lights:
+    for (r, col) in getattr(asm, "sphere_lights", []):
         L.psyref_scene_add_sphere_light(o.h, r, C.byref(_color(col)))
-    for inst in scene.instances:
+    subs = [o.add_assembly(_assembly(sub)) for sub in getattr(asm, "sub_assemblies", [])]
+    for inst in asm.instances:
         xf = inst.get("xforms")
         if inst["kind"] == "mesh":
             o.add_instance(inst["index"], xf, -1 if inst["shader"] is None else inst["shader"])
+        elif inst["kind"] == "assembly":
+            o.add_assembly_instance(subs[inst["index"]], xf)
         else:
             x = f32(xf).reshape(-1, 12)
             kind = 1 if inst["kind"] == "rect" else 2
             if L.psyref_scene_add_light_instance(o.h, kind, inst["index"], fptr(x), x.shape[0]) < 0:
                 raise ValueError("light instances need transforms")
             o.instance_count += 1
-    o.build()
+    return o.build()
+
+
+def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
+    """Renders a psychopath_amd.scenes.RenderScene with the oracle.
+    Returns the XYZ image [H, W, 3] and timing figures."""
+    o = _assembly(scene)
+    L = lib()
     sh = (_Shader * max(1, len(scene.shaders)))()
     for i, (k, col, rough, fres) in enumerate(scene.shaders):
         sh[i] = _Shader(k, _color(col), rough, fres)

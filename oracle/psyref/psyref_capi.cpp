@@ -129,6 +129,29 @@ int psyref_scene_add_instance(void* s, uint32_t mesh, int32_t shader, const floa
     return (int)sc->root.instances.size() - 1;
 }
 
+// Nested assemblies (parse/psy_assembly.rs:26-37, AssemblyBuilder::
+// add_assembly, scene/assembly.rs:204-217): moves the built root assembly of
+// `child` into `parent` as a sub-assembly (and frees `child`); returns its
+// index for psyref_scene_add_assembly_instance.
+int psyref_scene_add_assembly(void* parent, void* child) {
+    OracleScene* p = static_cast<OracleScene*>(parent);
+    OracleScene* c = static_cast<OracleScene*>(child);
+    if (!c->built) return -1;
+    auto a = std::make_unique<Assembly>(std::move(c->root));
+    p->root.assemblies.push_back(std::move(a));
+    delete c;
+    return (int)p->root.assemblies.size() - 1;
+}
+
+int psyref_scene_add_assembly_instance(void* s, uint32_t assembly, const float* xforms, uint32_t n_xf) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    if (assembly >= sc->root.assemblies.size()) return -1;
+    std::vector<Transform> xf;
+    for (uint32_t i = 0; i < n_xf; i++) xf.push_back(xf_from(xforms + 12 * i));
+    sc->root.add_instance(true, assembly, -1, xf.data(), n_xf);
+    return (int)sc->root.instances.size() - 1;
+}
+
 int psyref_scene_build(void* s) {
     OracleScene* sc = static_cast<OracleScene*>(s);
     sc->root.build();

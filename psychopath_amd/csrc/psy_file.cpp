@@ -23,6 +23,7 @@
 
 #include "../../include/psy_file.h"
 #include "color_host.h"
+#include "psymath.h"
 
 namespace psy {
 namespace api {
@@ -682,79 +683,184 @@ MeshData parse_mesh_surface(const Node& n) {
     return m;
 }
 
-// parse_assembly (psy_assembly.rs:18-188) for the root assembly, with the
-// AssemblyBuilder's naming rules (scene/assembly.rs:185-284).
-void parse_assembly(const Node& n, psy_file_scene_data& sd) {
+// One assembly as the reference's AssemblyBuilder holds it (scene/
+// assembly.rs:150-286): its own shaders and objects, sub-assemblies sharing
+// the object namespace, and instances of objects or sub-assemblies.
+constexpr uint32_t KIND_ASSEMBLY = 3;
+struct AsmDef {
+    std::vector<psy_shader_desc> shaders;
+    std::vector<MeshData> meshes;
+    std::vector<psy_rect_light_desc> rects;
+    std::vector<psy_sphere_light_desc> spheres;
+    std::vector<std::unique_ptr<AsmDef>> subs;
+    struct Inst {
+        uint32_t kind;    // PSY_OBJECT_* or KIND_ASSEMBLY
+        uint32_t index;   // into meshes / rects / spheres / subs
+        uint32_t shader;  // into shaders, PSY_NONE if unbound
+        std::vector<psy::Xf> xf;
+    };
+    std::vector<Inst> instances;
+    uint32_t shader_base = 0, mesh_base = 0;  // positions in the flattened scene
+};
+
+// parse_assembly (psy_assembly.rs:18-188).  Lights are read in the root
+// assembly only: the flattened scene samples lights from one light tree,
+// where the reference keeps one per assembly.
+std::unique_ptr<AsmDef> parse_assembly(const Node& n, bool root) {
     if (!n.internal) raise("UnknownError", n.offset, "Assembly should be an internal node");
+    auto def = std::make_unique<AsmDef>();
     std::unordered_map<std::string, uint32_t> shader_map;
     std::unordered_map<std::string, ObjectRef> object_map;
     for (const Node& c : n.children) {
         if (c.type == "Assembly") {
-            raise("UnknownVariant", c.offset, "nested assemblies are not supported");
+            if (!c.internal || !c.has_ident) raise("UnknownError", c.offset, "Assembly should be an internal node with a name");
+            if (object_map.count(c.ident))
+                raise("UnknownError", c.offset, "Attempted to add assembly to another assembly with a name that already exists.");
+            object_map[c.ident] = ObjectRef{KIND_ASSEMBLY, (uint32_t)def->subs.size()};
+            def->subs.push_back(parse_assembly(c, false));
         } else if (c.type == "Instance") {
             if (!c.internal) raise("UnknownError", c.offset, "Instance should be an internal node");
             const auto data = children_of(c, "Data", 2);
             if (data.size() != 1) raise("UnknownError", c.offset, "Instance should have precisely one Data leaf");
             const std::string& name = data[0]->contents;
             const auto binds = children_of(c, "SurfaceShaderBind", 2);
-            std::vector<float> xf;
+            AsmDef::Inst inst{};
             for (const Node* t : children_of(c, "Transform", 2)) {
                 float m[12];
                 parse_matrix(t->contents, 0, "UnknownError", m);
-                xf.insert(xf.end(), m, m + 12);
+                inst.xf.push_back(psy::xf_from_values(m));
             }
             auto it = object_map.find(name);
             if (it == object_map.end())
                 raise("InstancedMissingData", data[0]->offset,
                       "Attempted to add instance for data with a name that doesn't exist. Data name: '" + name + "'");
-            psy_instance_desc inst{};
             inst.kind = it->second.kind;
-            inst.mesh = it->second.index;
+            inst.index = it->second.index;
             inst.shader = PSY_NONE;
             if (!binds.empty()) {
-                auto s = shader_map.find(binds[0]->contents);
-                if (s == shader_map.end())
+                auto sh = shader_map.find(binds[0]->contents);
+                if (sh == shader_map.end())
                     raise("UnknownVariant", c.offset, "Unknown surface shader '" + binds[0]->contents + "'.");
-                inst.shader = s->second;
+                // An assembly instance's binding is checked but, as in
+                // tracer.rs:114-128, not used.
+                if (inst.kind == PSY_OBJECT_MESH) inst.shader = sh->second;
             }
-            if (xf.empty() && inst.kind != PSY_OBJECT_MESH) {
-                // Light instances are placed by a transform on the device;
-                // an identity one moves nothing.
-                const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
-                xf.assign(id, id + 12);
-            }
-            inst.xform_start = (uint32_t)(sd.xforms.size() / 12);
-            inst.xform_count = (uint32_t)(xf.size() / 12);
-            sd.xforms.insert(sd.xforms.end(), xf.begin(), xf.end());
-            sd.instances.push_back(inst);
+            def->instances.push_back(std::move(inst));
         } else if (c.type == "SurfaceShader") {
             if (!c.internal || !c.has_ident) raise("UnknownError", c.offset, "SurfaceShader should be an internal node with a name");
             if (shader_map.count(c.ident))
                 raise("UnknownError", c.offset, "Attempted to add surface shader to assembly with a name that already exists.");
-            shader_map[c.ident] = (uint32_t)sd.shaders.size();
-            sd.shaders.push_back(parse_surface_shader(c));
+            shader_map[c.ident] = (uint32_t)def->shaders.size();
+            def->shaders.push_back(parse_surface_shader(c));
         } else if (c.type == "MeshSurface" || c.type == "SphereLight" || c.type == "RectangleLight") {
             if (!c.internal || !c.has_ident) raise("UnknownError", c.offset, c.type + " should be an internal node with a name");
             if (object_map.count(c.ident))
                 raise("UnknownError", c.offset, "Attempted to add object to assembly with a name that already exists.");
+            if (c.type != "MeshSurface" && !root)
+                raise("UnknownVariant", c.offset, "lights inside nested assemblies are not supported");
             ObjectRef ref{};
             if (c.type == "MeshSurface") {
-                ref = ObjectRef{PSY_OBJECT_MESH, (uint32_t)sd.mesh_data.size()};
-                sd.mesh_data.push_back(parse_mesh_surface(c));
+                ref = ObjectRef{PSY_OBJECT_MESH, (uint32_t)def->meshes.size()};
+                def->meshes.push_back(parse_mesh_surface(c));
             } else if (c.type == "SphereLight") {
                 const LightFields f = parse_light(c, "SphereLight");
-                ref = ObjectRef{PSY_OBJECT_SPHERE_LIGHT, (uint32_t)sd.spheres.size()};
-                sd.spheres.push_back(psy_sphere_light_desc{single(f.radius, c, "radius"), single(f.color, c, "colour")});
+                ref = ObjectRef{PSY_OBJECT_SPHERE_LIGHT, (uint32_t)def->spheres.size()};
+                def->spheres.push_back(psy_sphere_light_desc{single(f.radius, c, "radius"), single(f.color, c, "colour")});
             } else {
                 const LightFields f = parse_light(c, "RectangleLight");
                 const auto& d = single(f.dims, c, "dimensions");
-                ref = ObjectRef{PSY_OBJECT_RECT_LIGHT, (uint32_t)sd.rects.size()};
-                sd.rects.push_back(psy_rect_light_desc{d[0], d[1], single(f.color, c, "colour")});
+                ref = ObjectRef{PSY_OBJECT_RECT_LIGHT, (uint32_t)def->rects.size()};
+                def->rects.push_back(psy_rect_light_desc{d[0], d[1], single(f.color, c, "colour")});
             }
             object_map[c.ident] = ref;
         }
         // Other node types are ignored, as in the reference.
     }
+    return def;
+}
+
+// Shaders and meshes of every assembly go to the flattened scene depth
+// first in definition order (the root's first).
+void place_definitions(AsmDef& d, psy_file_scene_data& sd) {
+    d.shader_base = (uint32_t)sd.shaders.size();
+    sd.shaders.insert(sd.shaders.end(), d.shaders.begin(), d.shaders.end());
+    d.mesh_base = (uint32_t)sd.mesh_data.size();
+    for (MeshData& m : d.meshes) sd.mesh_data.push_back(std::move(m));
+    for (auto& sub : d.subs) place_definitions(*sub, sd);
+}
+
+// math3d `a * b` (transform.rs:89-96): Self(b.0 * a.0), glam's Affine3A
+// product, which applies a, then b.
+psy::Xf xf_then(const psy::Xf& a, const psy::Xf& b) {
+    psy::Xf r;
+    for (int j = 0; j < 4; j++) {
+        const psy::V3 c = psy::xf_mat_mul(b, psy::V3{a.m[0][j], a.m[1][j], a.m[2][j]});
+        r.m[0][j] = c.x;
+        r.m[1][j] = c.y;
+        r.m[2][j] = c.z;
+    }
+    r.m[0][3] += b.m[0][3];
+    r.m[1][3] += b.m[1][3];
+    r.m[2][3] += b.m[2][3];
+    return r;
+}
+
+psy::Xf xf_lerp_slice(const std::vector<psy::Xf>& s, float alpha) {
+    uint32_t i1;
+    float a2;
+    if (!psy::lerp_slice_pos((uint32_t)s.size(), alpha, i1, a2)) return s[i1];
+    return psy::xf_lerp(s[i1], s[i1 + 1], a2);
+}
+
+// TransformStack::push (transform_stack.rs:33-60): the parent's and the
+// instance's time samples combined by merge_slices_to (algorithm.rs:
+// 266-305), the shorter list resampled with lerp_slice.
+std::vector<psy::Xf> push_xforms(const std::vector<psy::Xf>& top, const std::vector<psy::Xf>& own) {
+    if (own.empty()) return top;
+    if (top.empty()) return own;
+    std::vector<psy::Xf> out;
+    if (top.size() == own.size()) {
+        for (size_t i = 0; i < own.size(); i++) out.push_back(xf_then(top[i], own[i]));
+    } else if (top.size() > own.size()) {
+        const float s = (float)(top.size() - 1);
+        for (size_t i = 0; i < top.size(); i++) out.push_back(xf_then(top[i], xf_lerp_slice(own, (float)i / s)));
+    } else {
+        const float s = (float)(own.size() - 1);
+        for (size_t i = 0; i < own.size(); i++) out.push_back(xf_then(xf_lerp_slice(top, (float)i / s), own[i]));
+    }
+    return out;
+}
+
+// The instances of an assembly under the transform stack `top`, sub-
+// assemblies expanded in place (tracer.rs:87-150 walks them the same way).
+void flatten(const AsmDef& d, const std::vector<psy::Xf>& top, psy_file_scene_data& sd) {
+    for (const AsmDef::Inst& inst : d.instances) {
+        const std::vector<psy::Xf> xf = push_xforms(top, inst.xf);
+        if (inst.kind == KIND_ASSEMBLY) {
+            flatten(*d.subs[inst.index], xf, sd);
+            continue;
+        }
+        psy_instance_desc out{};
+        out.kind = inst.kind;
+        out.mesh = inst.kind == PSY_OBJECT_MESH ? d.mesh_base + inst.index : inst.index;
+        out.shader = inst.shader == PSY_NONE ? PSY_NONE : d.shader_base + inst.shader;
+        std::vector<psy::Xf> placed = xf;
+        if (placed.empty() && inst.kind != PSY_OBJECT_MESH) placed.push_back(psy::xf_identity());
+        out.xform_start = (uint32_t)(sd.xforms.size() / 12);
+        out.xform_count = (uint32_t)placed.size();
+        for (const psy::Xf& x : placed)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 4; c++) sd.xforms.push_back(x.m[r][c]);
+        sd.instances.push_back(out);
+    }
+}
+
+void read_assembly(const Node& n, psy_file_scene_data& sd) {
+    std::unique_ptr<AsmDef> root = parse_assembly(n, true);
+    sd.rects = root->rects;
+    sd.spheres = root->spheres;
+    place_definitions(*root, sd);
+    flatten(*root, {}, sd);
 }
 
 // parse_scene (psy.rs:99-200).
@@ -778,7 +884,7 @@ std::unique_ptr<psy_file_scene_data> parse_scene(const Node& n) {
     parse_render_settings(*children_of(n, "RenderSettings", 0)[0], *sd);
     parse_camera(*children_of(n, "Camera", 0)[0], *sd);
     parse_world(*children_of(n, "World", 0)[0], *sd);
-    parse_assembly(*children_of(n, "Assembly", 0)[0], *sd);
+    read_assembly(*children_of(n, "Assembly", 0)[0], *sd);
     if (n.has_ident) sd->name = n.ident.substr(1);
     // Descriptors over the owned arrays.
     for (const MeshData& m : sd->mesh_data)

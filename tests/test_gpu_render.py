@@ -344,3 +344,22 @@ def test_crop_region_renders_the_same_pixels():
     from psychopath_amd._lib import PsyError
     with pytest.raises(PsyError):
         r.render(sc.camera, dict(sc.settings, crop=(40, 0, 10, 5)))
+
+
+def test_nested_assemblies_render_like_the_oracle():
+    """A scene file with two levels of sub-assemblies (one instanced twice,
+    once under motion): the reader flattens it with the transform stack's
+    arithmetic; the oracle renders it nested, walking the sub-assemblies as
+    tracer.rs does."""
+    import importlib
+    from oracle import psyfile as opf
+    from psychopath_amd import psyfile
+    tp = importlib.import_module("tests.test_psy_file")
+    text = tp.nested_cornell_text(48, 36, 8, 1)
+    sc = psyfile.read(text)[0]
+    ref, info = psyref.render(opf.to_scene(opf.parse_file(text)))
+    img, stats, _ = gpu_render(sc)
+    frac = matching_fraction(img, ref)
+    print(f"nested cornell 48x36x8: {frac * 100:.3f}% pixels within {REL}; rays {stats['rays']} vs {info['rays']}",
+          flush=True)
+    assert frac >= FRACTION

@@ -318,7 +318,9 @@ SCENE_ERRORS = [
     (lambda t: t.replace("Type [Lambert]", "Type [Glass]", 1), "UnknownVariant"),
     (lambda t: t.replace("FaceVertCounts [3 3]", "FaceVertCounts [2 4]", 1), "IncorrectLeafData"),
     (lambda t: t.replace("FaceVertCounts [3 3]", "FaceVertCounts [3 3 3]", 1), "IncorrectLeafData"),
-    (lambda t: t.replace("Assembly {", "Assembly {\n        Assembly $sub {\n        }", 1), "UnknownVariant"),
+    (lambda t: t.replace("Assembly {", "Assembly {\n        Assembly {\n        }", 1), "UnknownError"),
+    (lambda t: t.replace("Assembly {", "Assembly {\n        Instance {\n            Data [$later]\n        }\n"
+                         "        Assembly $later {\n        }", 1), "InstancedMissingData"),
 ]
 
 
@@ -394,3 +396,169 @@ def test_deep_nesting_is_refused_not_crashed():
         psyfile.tree(text)
     assert e.value.kind == "NestingTooDeep"
     assert psyfile.tree("A { " * 100 + "} " * 100).count("\n") == 101
+
+
+# ------------------------------------------------------- nested assemblies
+
+def _xf_text(xf):
+    from psychopath_amd.psyfile import _matrix
+    return _matrix(np.asarray(xf, np.float32))
+
+
+def nested_cornell_text(w=32, h=24, spp=2, seed=0):
+    """The Cornell box with its two boxes in a sub-assembly (own shader), the
+    sub-assembly instanced twice (once plain, once moved with two time
+    samples) and a second level holding one more box under a rotation."""
+    from psychopath_amd.psyfile import write
+    sc = scenes.cornell_box(w, h, spp, 2, seed)
+    boxes = sc.meshes[5:7]
+    sc.meshes = sc.meshes[:5]
+    sc.instances = [i for i in sc.instances if not (i["kind"] == "mesh" and i["index"] >= 5)]
+    text = write(sc, "nested", "nested.png")
+    from psychopath_amd.psyfile import _g
+    box_txt = []
+    for k, (v, f) in enumerate(boxes):
+        box_txt.append(f"            MeshSurface $box{k} {{\n                Vertices [" +
+                       " ".join(_g(x) for x in np.asarray(v, np.float32).ravel()) + "]\n"
+                       "                FaceVertCounts [" + " ".join("3" for _ in range(len(f))) + "]\n"
+                       "                FaceVertIndices [" + " ".join(str(int(x)) for x in np.asarray(f).ravel()) +
+                       "]\n            }\n")
+    ident = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    move0 = [1, 0, 0, 0.4, 0, 1, 0, -0.2, 0, 0, 1, 0]
+    move1 = [1, 0, 0, 0.7, 0, 1, 0, -0.5, 0, 0, 1, 0.1]
+    c, s_ = np.cos(0.3), np.sin(0.3)
+    rot = [c, -s_, 0, 2.8, s_, c, 0, -1.0, 0, 0, 1, 0]
+    sub = ("        Assembly $boxes {\n"
+           "            SurfaceShader $white2 {\n                Type [Lambert]\n"
+           "                Color [rec709, 0.6 0.65 0.7]\n            }\n"
+           "            SurfaceShader $gloss {\n                Type [GGX]\n"
+           "                Color [rec709, 0.9 0.8 0.5]\n                Roughness [0.3]\n"
+           "                Fresnel [0.8]\n            }\n"
+           + "".join(box_txt) +
+           "            Assembly $inner {\n"
+           "                SurfaceShader $red2 {\n                    Type [Lambert]\n"
+           "                    Color [rec709, 0.7 0.2 0.1]\n                }\n"
+           + box_txt[1].replace("$box1", "$tiny").replace("            ", "                ", 1) +
+           "                Instance {\n                    Data [$tiny]\n"
+           "                    SurfaceShaderBind [$red2]\n"
+           f"                    Transform [{_xf_text(rot)}]\n                }}\n"
+           "            }\n"
+           "            Instance {\n                Data [$box0]\n                SurfaceShaderBind [$white2]\n"
+           "            }\n"
+           "            Instance {\n                Data [$box1]\n                SurfaceShaderBind [$gloss]\n"
+           f"                Transform [{_xf_text(ident)}]\n            }}\n"
+           "            Instance {\n                Data [$inner]\n"
+           f"                Transform [{_xf_text(move0)}]\n            }}\n"
+           "        }\n"
+           "        Instance {\n            Data [$boxes]\n        }\n"
+           "        Instance {\n            Data [$boxes]\n"
+           f"            Transform [{_xf_text(move0)}]\n            Transform [{_xf_text(move1)}]\n        }}\n")
+    at = text.index("        Instance {")
+    return text[:at] + sub + text[at:]
+
+
+def _oracle_math():
+    from oracle.psyref import lib
+    import ctypes as C
+    L = lib()
+    fp = C.POINTER(C.c_float)
+
+    def mul(a, b):
+        a = np.asarray(a, np.float32)
+        b = np.asarray(b, np.float32)
+        out = np.zeros(12, np.float32)
+        L.psyref_xform_mul(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp))
+        return out
+
+    def lerp(a, b, t):
+        a = np.asarray(a, np.float32)
+        b = np.asarray(b, np.float32)
+        out = np.zeros(12, np.float32)
+        L.psyref_xform_lerp(a.ctypes.data_as(fp), b.ctypes.data_as(fp), C.c_float(t), out.ctypes.data_as(fp))
+        return out
+    return mul, lerp
+
+
+def flatten_oracle(sc):
+    """The oracle's nested scene flattened with the oracle's transform
+    arithmetic, in the reader's order: instances depth first, transforms
+    composed as TransformStack::push does (transform_stack.rs:33-60,
+    algorithm.rs:266-305)."""
+    mul, lerp = _oracle_math()
+
+    def lerp_slice(s, alpha):
+        n = len(s)
+        if n == 1:
+            return s[0]
+        if alpha == 1.0:
+            return s[-1]
+        tmp = np.float32(alpha) * np.float32(n - 1)
+        i1 = int(tmp)
+        a2 = float(np.float32(tmp) - np.float32(i1))
+        return lerp(s[i1], s[i1 + 1], a2)
+
+    def push(top, own):
+        if not own:
+            return top
+        if not top:
+            return own
+        if len(top) == len(own):
+            return [mul(a, b) for a, b in zip(top, own)]
+        if len(top) > len(own):
+            s = np.float32(len(top) - 1)
+            return [mul(a, lerp_slice(own, float(np.float32(i) / s))) for i, a in enumerate(top)]
+        s = np.float32(len(own) - 1)
+        return [mul(lerp_slice(top, float(np.float32(i) / s)), b) for i, b in enumerate(own)]
+
+    meshes, insts = [], []
+
+    def place(ns):
+        ns.mesh_base = len(meshes)
+        meshes.extend(ns.meshes)
+        for sub in ns.sub_assemblies:
+            place(sub)
+
+    def walk(ns, top):
+        for i in ns.instances:
+            own = [] if i["xforms"] is None else [np.asarray(x, np.float32) for x in i["xforms"]]
+            xf = push(top, own)
+            if i["kind"] == "assembly":
+                walk(ns.sub_assemblies[i["index"]], xf)
+            else:
+                insts.append((i["kind"], ns.mesh_base + i["index"] if i["kind"] == "mesh" else i["index"],
+                              i["shader"], np.asarray(xf, np.float32).reshape(-1, 12)))
+    place(sc)
+    walk(sc, [])
+    return meshes, insts
+
+
+def test_nested_assemblies_flatten_like_the_transform_stack():
+    text = nested_cornell_text()
+    got = psyfile.read(text)[0]
+    sc = opf.to_scene(opf.parse_file(text))
+    meshes, insts = flatten_oracle(sc)
+    assert len(got.meshes) == len(meshes) == 8
+    for a, b in zip(got.meshes, meshes):
+        assert f32eq(a[0], np.asarray(b[0], np.float32).reshape(a[0].shape)) and np.array_equal(a[1], b[1])
+    assert len(got.instances) == len(insts)
+    for a, (kind, idx, shader, xf) in zip(got.instances, insts):
+        assert (a["kind"], a["index"], a["shader"]) == (kind, idx, shader)
+        if xf.size == 0:
+            assert a["xforms"] is None
+        else:
+            assert f32eq(a["xforms"], xf), (a["xforms"], xf)
+    assert len(got.shaders) == len(sc.shaders) == 3 + 2 + 1
+    # the moved copy composes two samples; its inner box a rotation and two moves
+    assert max(0 if i["xforms"] is None else len(i["xforms"]) for i in got.instances) == 2
+
+
+def test_lights_inside_nested_assemblies_are_refused():
+    text = nested_cornell_text().replace(
+        "        Assembly $boxes {\n",
+        "        Assembly $boxes {\n            SphereLight $bulb {\n                Radius [0.1]\n"
+        "                Color [rec709, 1 1 1]\n            }\n", 1)
+    with pytest.raises(psyfile.PsyFileError) as ce:
+        psyfile.read(text)
+    assert ce.value.kind == "UnknownVariant" and "nested" in str(ce.value)
+    with pytest.raises(opf.PsyParseError):
+        opf.to_scene(opf.parse_file(text))
