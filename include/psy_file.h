@@ -29,8 +29,9 @@
  * shaders and meshes are appended after the root's, depth first in
  * definition order.  Not supported (PSY_E_PARSE with a message saying so):
  * lights inside nested assemblies (the flattened scene has one light tree),
- * and more than one time sample of camera field of view, aperture or focus
- * distance or of a light's size, direction or colour.  Files carry no
+ * more than 64 time samples of a camera lens or light parameter, and light
+ * colour samples that mix representations (the reference panics when it
+ * lerps them).  Every other time sample is kept.  Files carry no
  * bounce limit: settings.max_bounces is the reference renderer's fixed 2
  * (src/renderer.rs:599).
  */

@@ -44,11 +44,18 @@ typedef struct {
 
 /* Distant disk light (DistantDiskLight::new,
  * src/light/distant_disk_light.rs:24-36): angular radius in radians, the
- * direction the light travels, colour. */
+ * direction the light travels, colour.  Each may be time samples, as for
+ * the lights of psy_rt.h: a non-zero count replaces the value before it. */
 typedef struct {
     float radius;
     float direction[3];
     psy_color color;
+    uint32_t radius_count;       /* 0 = radius above */
+    const float* radii;
+    uint32_t direction_count;    /* 0 = direction above */
+    const float* directions;     /* xyz per sample */
+    uint32_t color_count;        /* 0 = color above */
+    const psy_color* colors;
 } psy_distant_light_desc;
 
 /* Shaders (indexed by psy_instance_desc.shader) and world. */
@@ -63,13 +70,27 @@ typedef struct {
 /* Camera (Camera::new, src/camera.rs:22-72): camera-to-world transforms
  * (12 floats each, as in psy_scene_desc), horizontal field of view in
  * radians, thin-lens aperture radius and focus distance (aperture 0 gives a
- * pinhole; focus distance 0 disables the lens as in the reference). */
+ * pinhole; focus distance 0 disables the lens as in the reference).
+ *
+ * The lens parameters may be time samples too (Camera::generate_ray,
+ * camera.rs:74-79): a non-zero count replaces the scalar before it by that
+ * many samples, interpolated over the shutter like the transforms
+ * (lerp_slice, src/lerp.rs:20-35), each list on its own; the field of view
+ * is interpolated in its linear form tan(fov/2).  A zero focus distance
+ * among the samples disables the lens for the whole render.  At most 64
+ * samples per list. */
 typedef struct {
     uint32_t xform_count;
     const float* xforms;
     float fov;
     float aperture_radius;
     float focus_distance;
+    uint32_t fov_count;             /* 0 = the scalar fov */
+    const float* fovs;
+    uint32_t aperture_count;        /* 0 = the scalar aperture_radius */
+    const float* aperture_radii;
+    uint32_t focus_count;           /* 0 = the scalar focus_distance */
+    const float* focus_distances;
 } psy_camera_desc;
 
 /* One render (Renderer, src/renderer.rs:32-38): resolution, samples per

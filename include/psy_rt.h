@@ -80,15 +80,29 @@ typedef struct {
 
 /* Light objects of an assembly (RectangleLight::new,
  * src/light/rectangle_light.rs:31-48; SphereLight::new,
- * src/light/sphere_light.rs:34-48), one time sample each.  Both emit from
- * both sides; the rectangle lies in its local xy plane, centred. */
+ * src/light/sphere_light.rs:34-48).  Both emit from both sides; the
+ * rectangle lies in its local xy plane, centred.  Sizes and colours may be
+ * time samples: a non-zero count replaces the value before it by that many
+ * samples, lerped at each ray's time (lerp_slice, src/lerp.rs:20-35), each
+ * list on its own; at most 64 per list, and the colours of one list must
+ * share a representation (the reference cannot lerp between them).  The
+ * light's bounds are one box per size sample and its energy, for light
+ * selection, the mean over its colours. */
 typedef struct {
     float width, height;
     psy_color color;
+    uint32_t dims_count;      /* 0 = width, height above */
+    const float* dims;        /* dims_count (width, height) pairs */
+    uint32_t color_count;     /* 0 = color above */
+    const psy_color* colors;
 } psy_rect_light_desc;
 typedef struct {
     float radius;
     psy_color color;
+    uint32_t radius_count;    /* 0 = radius above */
+    const float* radii;
+    uint32_t color_count;     /* 0 = color above */
+    const psy_color* colors;
 } psy_sphere_light_desc;
 
 /* Object kinds of an instance (Object, src/scene/assembly.rs:399-402). */
@@ -180,8 +194,10 @@ typedef struct {
  * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
  * live in LDS (8, 16 or 24) and the rest in private memory; `simd_waves`
  * is the register budget, as resident waves per SIMD (4: up to 128 VGPRs,
- * 8: at most 64).  The simple kernel runs one lane per ray over a
- * launch-wide grid. */
+ * 8: at most 64); a wave descends internal nodes until no lane stands on
+ * one or `leaf_min` of its lanes wait at a leaf (1..64, 0 = 64), then runs
+ * the leaf tests.  The simple kernel runs one lane per ray over a
+ * launch-wide grid.  No field changes the results. */
 #define PSY_KERNEL_PERSISTENT 0u
 #define PSY_KERNEL_SIMPLE 1u
 typedef struct {
@@ -190,6 +206,7 @@ typedef struct {
     uint32_t refill_min;
     uint32_t stack_lds;
     uint32_t simd_waves;
+    uint32_t leaf_min;
 } psy_trace_tuning;
 
 /* ---- errors ---------------------------------------------------------- */

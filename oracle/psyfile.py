@@ -739,6 +739,12 @@ def normalized(v):
     return [float(x * inv), float(y * inv), float(z * inv)]
 
 
+def _one(v):
+    """A light parameter's time samples: the value itself when there is one,
+    else the list (the light lerps it over the shutter)."""
+    return v[0] if len(v) == 1 else list(v)
+
+
 def to_scene(parsed: dict):
     """The parse as the attributes the oracle renderer reads (meshes,
     lights, instances, nested `sub_assemblies`, shaders, world, camera,
@@ -770,10 +776,11 @@ def to_scene(parsed: dict):
                     raise PsyParseError("UnknownVariant", 0, "lights inside nested assemblies are not supported")
                 if kind == "rect":
                     where.append(("rect", len(ns.rect_lights)))
-                    ns.rect_lights.append((o["dims"][0][0], o["dims"][0][1], color_desc(o["color"][0])))
+                    ns.rect_lights.append((_one([d[0] for d in o["dims"]]), _one([d[1] for d in o["dims"]]),
+                                           _one([color_desc(c) for c in o["color"]])))
                 else:
                     where.append(("sphere", len(ns.sphere_lights)))
-                    ns.sphere_lights.append((o["radius"][0], color_desc(o["color"][0])))
+                    ns.sphere_lights.append((_one(o["radius"]), _one([color_desc(c) for c in o["color"]])))
         for i in asm["instances"]:
             kind, idx = where[i["object"]]
             xf = i["xforms"]
@@ -788,11 +795,13 @@ def to_scene(parsed: dict):
     sc.shaders = shaders
     w = parsed["world"]
     sc.background = color_desc(w["background"])
-    sc.distant_lights = [(d["radius"][0], tuple(d["direction"][0]), color_desc(d["color"][0]))
+    sc.distant_lights = [(_one(d["radius"]), _one([tuple(v) for v in d["direction"]]),
+                          _one([color_desc(c) for c in d["color"]]))
                          for d in w["distant_lights"]]
     cam = parsed["camera"]
-    sc.camera = dict(xforms=np.asarray(cam["xforms"], np.float32), fov=cam["fovs"][0], aperture=cam["apertures"][0],
-                     focus=cam["focus"][0])
+    one = lambda v: v[0] if len(v) == 1 else list(v)  # noqa: E731
+    sc.camera = dict(xforms=np.asarray(cam["xforms"], np.float32), fov=one(cam["fovs"]), aperture=one(cam["apertures"]),
+                     focus=one(cam["focus"]))
     sc.settings = dict(width=parsed["width"], height=parsed["height"], spp=parsed["spp"], max_bounces=2,
                        seed=parsed["seed"])
     return sc

@@ -193,14 +193,25 @@ class _Distant(C.Structure):
     _fields_ = [("radius", C.c_float), ("dir", C.c_float * 3), ("color", _Color)]
 
 
+class _DistantSamples(C.Structure):
+    _fields_ = [("n_radius", C.c_uint32), ("radii", _fp), ("n_direction", C.c_uint32), ("directions", _fp),
+                ("n_color", C.c_uint32), ("colors", C.POINTER(_Color))]
+
+
 _SIGS.update({
     "psyref_scene_add_rect_light": (C.c_int, [_vp, C.c_float, C.c_float, C.POINTER(_Color)]),
     "psyref_scene_add_sphere_light": (C.c_int, [_vp, C.c_float, C.POINTER(_Color)]),
     "psyref_scene_add_light_instance": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _fp, C.c_uint32]),
+    "psyref_color_lerp_slice": (None, [C.c_void_p, C.c_uint32, C.c_float, C.c_void_p]),
+    "psyref_scene_set_light_samples": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _fp, C.c_uint32,
+                                                 C.c_void_p]),
     "psyref_render": (C.c_int, [_vp, C.POINTER(_Shader), C.c_uint32, C.POINTER(_Color), C.POINTER(_Distant),
-                                C.c_uint32, _fp, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                C.c_uint32, C.c_void_p, _fp, C.c_uint32, _fp, C.c_uint32, _fp, C.c_uint32, _fp, C.c_uint32,
+                                C.c_uint32,
                                 C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _fp,
                                 C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "psyref_camera_ray": (C.c_int, [_fp, C.c_uint32, _fp, C.c_uint32, _fp, C.c_uint32, _fp, C.c_uint32, C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, _fp]),
     "psyref_halton": (C.c_float, [C.c_uint32, C.c_uint32]),
     "psyref_sample_4d": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _fp]),
     "psyref_probit": (C.c_float, [C.c_float, C.c_float]),
@@ -225,10 +236,21 @@ def _assembly(asm) -> OracleScene:
     for m in asm.meshes:
         o.add_mesh(m[0], m[1], m[2] if len(m) > 2 else None)
     L = lib()
+    # Lists are time samples (numbers and colour tuples are single values).
     for (w, h, col) in getattr(asm, "rect_lights", []):
-        L.psyref_scene_add_rect_light(o.h, w, h, C.byref(_color(col)))
+        cs = col if isinstance(col, list) else [col]
+        wh = f32(np.stack([np.atleast_1d(np.asarray(w, np.float32)), np.atleast_1d(np.asarray(h, np.float32))], 1))
+        k = L.psyref_scene_add_rect_light(o.h, float(wh[0, 0]), float(wh[0, 1]), C.byref(_color(cs[0])))
+        if len(wh) > 1 or len(cs) > 1:
+            ca = (_Color * len(cs))(*[_color(c) for c in cs])
+            L.psyref_scene_set_light_samples(o.h, 1, k, len(wh), fptr(wh), len(cs), ca)
     for (r, col) in getattr(asm, "sphere_lights", []):
-        L.psyref_scene_add_sphere_light(o.h, r, C.byref(_color(col)))
+        cs = col if isinstance(col, list) else [col]
+        ra = f32(np.atleast_1d(np.asarray(r, np.float32)))
+        k = L.psyref_scene_add_sphere_light(o.h, float(ra[0]), C.byref(_color(cs[0])))
+        if len(ra) > 1 or len(cs) > 1:
+            ca = (_Color * len(cs))(*[_color(c) for c in cs])
+            L.psyref_scene_set_light_samples(o.h, 2, k, len(ra), fptr(ra), len(cs), ca)
     subs = [o.add_assembly(_assembly(sub)) for sub in getattr(asm, "sub_assemblies", [])]
     for inst in asm.instances:
         xf = inst.get("xforms")
@@ -245,6 +267,30 @@ def _assembly(asm) -> OracleScene:
     return o.build()
 
 
+def color_lerp_slice(colours, time: float) -> tuple:
+    """The colour a light's colour list gives at `time` (lerp_slice with
+    Lerp for Color, color.rs:284-333)."""
+    arr = (_Color * len(colours))(*[_color(c) for c in colours])
+    out = _Color()
+    lib().psyref_color_lerp_slice(arr, len(colours), time, C.byref(out))
+    return (out.kind, out.a, out.b, out.c)
+
+
+def camera_ray(xforms, fovs, apertures, focus, x: float, y: float, time: float, u: float, v: float):
+    """Camera::new then Camera::generate_ray (camera.rs:22-102) for the
+    image-plane point (x, y): (origin, direction) in world space.  The lens
+    arguments are lists of time samples (an empty aperture or focus list
+    disables the lens)."""
+    xf = f32(xforms).reshape(-1, 12)
+    fv, ap, fd = (f32(np.atleast_1d(a)) for a in (fovs, apertures, focus))
+    out = np.zeros(6, np.float32)
+    rc = lib().psyref_camera_ray(fptr(xf), xf.shape[0], fptr(fv), fv.size, fptr(ap), ap.size, fptr(fd), fd.size, x, y,
+                                 time, u, v, fptr(out))
+    if rc != 0:
+        raise ValueError("camera needs a transform and a fov")
+    return out[:3], out[3:]
+
+
 def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
     """Renders a psychopath_amd.scenes.RenderScene with the oracle.
     Returns the XYZ image [H, W, 3] and timing figures."""
@@ -254,10 +300,23 @@ def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
     for i, (k, col, rough, fres) in enumerate(scene.shaders):
         sh[i] = _Shader(k, _color(col), rough, fres)
     dl = (_Distant * max(1, len(scene.distant_lights)))()
+    dls = (_DistantSamples * max(1, len(scene.distant_lights)))()
+    keep = []
     for i, (r, d, col) in enumerate(scene.distant_lights):
-        dl[i] = _Distant(r, (C.c_float * 3)(*d), _color(col))
+        ra = f32(np.atleast_1d(np.asarray(r, np.float32)))
+        da = f32(np.asarray(d, np.float32).reshape(-1, 3))
+        cs = col if isinstance(col, list) else [col]
+        ca = (_Color * len(cs))(*[_color(c) for c in cs])
+        dl[i] = _Distant(float(ra[0]), (C.c_float * 3)(*da[0]), _color(cs[0]))
+        dls[i] = _DistantSamples(len(ra) if len(ra) > 1 else 0, fptr(ra), len(da) if len(da) > 1 else 0, fptr(da),
+                                 len(cs) if len(cs) > 1 else 0, ca)
+        keep += [ra, da, ca]
     cam = scene.camera
     cx = f32(cam["xforms"]).reshape(-1, 12)
+    # Lens time samples: a scalar is a one-sample list (camera.rs:12-19).
+    fovs = f32(np.atleast_1d(cam["fov"]))
+    aps = f32(np.atleast_1d(cam.get("aperture", 0.0)))
+    fds = f32(np.atleast_1d(cam.get("focus", 1.0)))
     st = scene.settings
     img = np.zeros((st["height"], st["width"], 3), np.float32)
     ts = C.c_double()
@@ -265,7 +324,8 @@ def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
     threads = threads or min(16, os.cpu_count() or 1)
     t0 = __import__("time").time()
     rc = L.psyref_render(o.h, sh, len(scene.shaders), C.byref(_color(scene.background)), dl,
-                         len(scene.distant_lights), fptr(cx), cx.shape[0], cam["fov"], cam["aperture"], cam["focus"],
+                         len(scene.distant_lights), dls, fptr(cx), cx.shape[0], fptr(fovs), fovs.size, fptr(aps), aps.size,
+                         fptr(fds), fds.size,
                          st["width"], st["height"], st["spp"], st["max_bounces"], st["seed"], threads, fptr(img),
                          C.byref(ts), C.byref(nr))
     wall = __import__("time").time() - t0

@@ -250,6 +250,12 @@ namespace {
 struct PsyrefColor { uint32_t kind; float a, b, c; };
 struct PsyrefShader { uint32_t kind; PsyrefColor color; float roughness, fresnel; };
 struct PsyrefDistant { float radius; float dir[3]; PsyrefColor color; };
+// Time samples of a distant light (counts 0: the PsyrefDistant values).
+struct PsyrefDistantSamples {
+    uint32_t n_radius; const float* radii;
+    uint32_t n_direction; const float* directions;  // xyz per sample
+    uint32_t n_color; const PsyrefColor* colors;
+};
 Color to_color(const PsyrefColor& c) {
     Color r;
     r.kind = c.kind;
@@ -274,6 +280,37 @@ int psyref_scene_add_sphere_light(void* s, float radius, const PsyrefColor* colo
     sc->root.sphere_lights.push_back(l);
     return (int)sc->root.sphere_lights.size() - 1;
 }
+// lerp_slice over a colour list (the lights' colour at a time).
+void psyref_color_lerp_slice(const PsyrefColor* cs, uint32_t n, float time, PsyrefColor* out) {
+    std::vector<Color> v;
+    for (uint32_t i = 0; i < n; i++) v.push_back(to_color(cs[i]));
+    const Color c = lerp_slice(v.data(), v.size(), time);
+    out->kind = c.kind;
+    out->a = c.a;
+    out->b = c.b;
+    out->c = c.c;
+}
+
+// Time samples of light `light` of `kind` (1 rectangle: n_a (w, h) pairs;
+// 2 sphere: n_a radii) and its colours; a zero count keeps the single
+// value.  Call before psyref_scene_build.
+int psyref_scene_set_light_samples(void* s, uint32_t kind, uint32_t light, uint32_t n_a, const float* a,
+                                   uint32_t n_color, const PsyrefColor* colors) {
+    OracleScene* sc = static_cast<OracleScene*>(s);
+    std::vector<Color> cs;
+    for (uint32_t i = 0; i < n_color; i++) cs.push_back(to_color(colors[i]));
+    if (kind == OBJ_RECT_LIGHT && light < sc->root.rect_lights.size()) {
+        sc->root.rect_lights[light].dims_t.assign(a, a + 2 * n_a);
+        sc->root.rect_lights[light].colors_t = cs;
+        return 0;
+    }
+    if (kind == OBJ_SPHERE_LIGHT && light < sc->root.sphere_lights.size()) {
+        sc->root.sphere_lights[light].radii_t.assign(a, a + n_a);
+        sc->root.sphere_lights[light].colors_t = cs;
+        return 0;
+    }
+    return -1;
+}
 // kind: 1 rectangle light, 2 sphere light.  Lights need transforms
 // (the reference lerps the transform list, rectangle_light.rs:270).
 int psyref_scene_add_light_instance(void* s, uint32_t kind, uint32_t light, const float* xforms, uint32_t n_xf) {
@@ -285,14 +322,19 @@ int psyref_scene_add_light_instance(void* s, uint32_t kind, uint32_t light, cons
     return (int)sc->root.instances.size() - 1;
 }
 
-// Renders the built scene.  camera_xforms: n_cam x 12 (camera to world).
+// Renders the built scene.  camera_xforms: n_cam x 12 (camera to world);
+// fovs (radians), apertures and focus distances: time samples of the lens,
+// as parse_camera collects them (an empty aperture or focus list disables
+// the lens).
 int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, const PsyrefColor* background,
-                  const PsyrefDistant* distant, uint32_t n_distant, const float* camera_xforms, uint32_t n_cam,
-                  float fov, float aperture, float focus, uint32_t width, uint32_t height, uint32_t spp,
+                  const PsyrefDistant* distant, uint32_t n_distant, const PsyrefDistantSamples* distant_samples,
+                  const float* camera_xforms, uint32_t n_cam,
+                  const float* fovs, uint32_t n_fov, const float* apertures, uint32_t n_ap, const float* focus,
+                  uint32_t n_fd, uint32_t width, uint32_t height, uint32_t spp,
                   uint32_t max_bounces, uint32_t seed, uint32_t threads, float* xyz, double* trace_seconds,
                   uint64_t* rays) {
     OracleScene* sc = static_cast<OracleScene*>(s);
-    if (!sc->built || n_cam == 0 || width == 0 || height == 0 || spp == 0) return -1;
+    if (!sc->built || n_cam == 0 || n_fov == 0 || width == 0 || height == 0 || spp == 0) return -1;
     RenderScene rsc;
     rsc.root = &sc->root;
     for (uint32_t i = 0; i < n_shaders; i++) {
@@ -311,12 +353,19 @@ int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, cons
         l.radius = distant[i].radius;
         l.direction = Vec3(distant[i].dir[0], distant[i].dir[1], distant[i].dir[2]);
         l.color = to_color(distant[i].color);
+        if (distant_samples) {
+            const PsyrefDistantSamples& t = distant_samples[i];
+            l.radii_t.assign(t.radii, t.radii + t.n_radius);
+            for (uint32_t k = 0; k < t.n_direction; k++)
+                l.directions_t.push_back(Vec3(t.directions[3 * k], t.directions[3 * k + 1], t.directions[3 * k + 2]));
+            for (uint32_t k = 0; k < t.n_color; k++) l.colors_t.push_back(to_color(t.colors[k]));
+        }
         rsc.world.lights.push_back(l);
     }
     for (uint32_t i = 0; i < n_cam; i++) rsc.camera.transforms.push_back(xf_from(camera_xforms + 12 * i));
-    rsc.camera.fov = fov;
-    rsc.camera.aperture_radius = aperture;
-    rsc.camera.focus_distance = focus;
+    rsc.camera.fovs.assign(fovs, fovs + n_fov);
+    rsc.camera.aperture_radii.assign(apertures, apertures + n_ap);
+    rsc.camera.focus_distances.assign(focus, focus + n_fd);
     rsc.build_lights();
     RenderSettings rs;
     rs.width = width; rs.height = height; rs.spp = spp; rs.max_bounces = max_bounces; rs.seed = seed;
@@ -325,6 +374,23 @@ int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, cons
     render(rsc, rs, xyz, threads ? threads : 1, ts, nr);
     if (trace_seconds) *trace_seconds = ts;
     if (rays) *rays = nr;
+    return 0;
+}
+
+// Camera::new then Camera::generate_ray for image-plane point (x, y):
+// out = origin xyz, direction xyz (world space).
+int psyref_camera_ray(const float* xforms, uint32_t n_xf, const float* fovs, uint32_t n_fov, const float* apertures,
+                      uint32_t n_ap, const float* focus, uint32_t n_fd, float x, float y, float time, float u, float v,
+                      float* out) {
+    if (n_xf == 0 || n_fov == 0) return -1;
+    CameraDesc d;
+    for (uint32_t i = 0; i < n_xf; i++) d.transforms.push_back(xf_from(xforms + 12 * i));
+    d.fovs.assign(fovs, fovs + n_fov);
+    d.aperture_radii.assign(apertures, apertures + n_ap);
+    d.focus_distances.assign(focus, focus + n_fd);
+    const Ray r = generate_ray(camera_new(d), x, y, time, 550.0f, u, v);
+    out[0] = r.orig.x; out[1] = r.orig.y; out[2] = r.orig.z;
+    out[3] = r.dir.x; out[4] = r.dir.y; out[5] = r.dir.z;
     return 0;
 }
 

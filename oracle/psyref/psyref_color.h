@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #ifndef PSY_MENG_STORAGE
 #define PSY_MENG_STORAGE static
@@ -208,6 +209,25 @@ struct Color {
         return r;
     }
 };
+
+// Lerp for Color (color.rs:284-333): component-wise within one
+// representation; the reader refuses lists that mix representations.
+static inline Color lerp(const Color& x, const Color& y, float alpha) {
+    const float inv = 1.0f - alpha;
+    Color r = x;
+    r.a = (x.a * inv) + (y.a * alpha);
+    r.b = (x.b * inv) + (y.b * alpha);
+    if (x.kind == COLOR_XYZ) r.c = (x.c * inv) + (y.c * alpha);
+    return r;
+}
+
+// Light::approximate_energy over a colour list: the mean of the colours'
+// estimates, summed from 0 in list order (rectangle_light.rs:249-254).
+static inline float mean_energy(const std::vector<Color>& cs) {
+    float e = 0.0f;
+    for (const Color& c : cs) e = e + c.approximate_energy();
+    return e / (float)cs.size();
+}
 
 // color.rs:610-628
 static inline float x_1931(float w) {

@@ -24,11 +24,44 @@ struct LightSample {
 struct RectangleLight {
     float w = 1, h = 1;
     Color color;
+    // Time samples (rectangle_light.rs:24-29): (w, h) pairs and colours,
+    // each list lerped on its own at the ray's time; empty = the one value
+    // above.
+    std::vector<float> dims_t;
+    std::vector<Color> colors_t;
     static const float SIMPLE_SAMPLING_THRESHOLD;
 
+    // The light at `time`, with one value per field.
+    RectangleLight at(float time) const {
+        RectangleLight r;
+        r.w = w;
+        r.h = h;
+        r.color = color;
+        const size_t nd = dims_t.size() / 2;
+        if (nd > 0 && !(nd == 1 || time == 1.0f)) {
+            const float tmp = time * (float)(nd - 1);
+            const size_t i1 = (size_t)tmp;
+            const float a2 = tmp - (float)i1;
+            r.w = lerp(dims_t[2 * i1], dims_t[2 * i1 + 2], a2);
+            r.h = lerp(dims_t[2 * i1 + 1], dims_t[2 * i1 + 3], a2);
+        } else if (nd > 0) {
+            r.w = dims_t[2 * nd - 2];
+            r.h = dims_t[2 * nd - 1];
+        }
+        if (!colors_t.empty()) r.color = lerp_slice(colors_t.data(), colors_t.size(), time);
+        return r;
+    }
     BBox bounds() const { return BBox(Vec3(w * -0.5f, h * -0.5f, 0.0f), Vec3(w * 0.5f, h * 0.5f, 0.0f)); }
-    // fold(0.0, +) over the single colour, divided by the colour count
-    float approximate_energy() const { return (0.0f + color.approximate_energy()) / 1.0f; }
+    // One box per dimensions sample (rectangle_light.rs:36-42).
+    std::vector<BBox> bounds_list() const {
+        if (dims_t.empty()) return {bounds()};
+        std::vector<BBox> out;
+        for (size_t i = 0; i + 1 < dims_t.size(); i += 2)
+            out.push_back(BBox(Vec3(dims_t[i] * -0.5f, dims_t[i + 1] * -0.5f, 0.0f),
+                               Vec3(dims_t[i] * 0.5f, dims_t[i + 1] * 0.5f, 0.0f)));
+        return out;
+    }
+    float approximate_energy() const { return colors_t.empty() ? mean_energy({color}) : mean_energy(colors_t); }
 
     void corners(const Transform& space_inv, Vec3 p[4]) const {
         p[0] = xform_point(Vec3(w * 0.5f, h * 0.5f, 0.0f), space_inv);
@@ -135,10 +168,25 @@ struct RectangleLight {
 struct SphereLight {
     float radius = 1;
     Color color;
+    // Time samples (sphere_light.rs:25-31); empty = the one value above.
+    std::vector<float> radii_t;
+    std::vector<Color> colors_t;
 
+    SphereLight at(float time) const {
+        SphereLight r;
+        r.radius = radii_t.empty() ? radius : lerp_slice(radii_t.data(), radii_t.size(), time);
+        r.color = colors_t.empty() ? color : lerp_slice(colors_t.data(), colors_t.size(), time);
+        return r;
+    }
     BBox bounds() const { return BBox(Vec3(-radius, -radius, -radius), Vec3(radius, radius, radius)); }
-    // fold(0.0, +) over the single colour, divided by the colour count
-    float approximate_energy() const { return (0.0f + color.approximate_energy()) / 1.0f; }
+    // One box per radius sample (sphere_light.rs:35-40).
+    std::vector<BBox> bounds_list() const {
+        if (radii_t.empty()) return {bounds()};
+        std::vector<BBox> out;
+        for (float r : radii_t) out.push_back(BBox(Vec3(-r, -r, -r), Vec3(r, r, r)));
+        return out;
+    }
+    float approximate_energy() const { return colors_t.empty() ? mean_energy({color}) : mean_energy(colors_t); }
 
     // sphere_light.rs:51-81 (arr in world space)
     float sample_pdf(const Transform& space, Vec3 arr_world) const {
@@ -254,8 +302,19 @@ struct DistantDiskLight {
     float radius = 0.1f;
     Vec3 direction{0, 0, -1};
     Color color;
-    // fold(0.0, +) over the single colour, divided by the colour count
-    float approximate_energy() const { return (0.0f + color.approximate_energy()) / 1.0f; }
+    // Time samples (distant_disk_light.rs:17-22); empty = the one value.
+    std::vector<float> radii_t;
+    std::vector<Vec3> directions_t;
+    std::vector<Color> colors_t;
+
+    DistantDiskLight at(float time) const {
+        DistantDiskLight r;
+        r.radius = radii_t.empty() ? radius : lerp_slice(radii_t.data(), radii_t.size(), time);
+        r.direction = directions_t.empty() ? direction : lerp_slice(directions_t.data(), directions_t.size(), time);
+        r.color = colors_t.empty() ? color : lerp_slice(colors_t.data(), colors_t.size(), time);
+        return r;
+    }
+    float approximate_energy() const { return colors_t.empty() ? mean_energy({color}) : mean_energy(colors_t); }
     // distant_disk_light.rs:59-85: (color, shadow direction, pdf)
     void sample_from_point(float u, float v, float wavelength, Vec4& col, Vec3& shadow_vec, float& pdf) const {
         double r = (double)radius;

@@ -266,7 +266,7 @@ SceneLightSample sample_lights(const RenderScene& sc, float n, float u, float v,
         Vec4 col;
         Vec3 dir;
         float pdf;
-        sc.world.lights[i].sample_from_point(u, v, wavelength, col, dir, pdf);
+        sc.world.lights[i].at(time).sample_from_point(u, v, wavelength, col, dir, pdf);
         out.kind = LS_DISTANT;
         out.color = col;
         out.direction = dir;
@@ -288,8 +288,9 @@ SceneLightSample sample_lights(const RenderScene& sc, float n, float u, float v,
     Transform xform;
     if (inst.has_xforms) xform = lerp_slice(a->xforms.data() + inst.xf_start, inst.xf_end - inst.xf_start, time);
     LightSample s = inst.obj_kind == OBJ_RECT_LIGHT
-                        ? a->rect_lights[inst.data_index].sample_from_point(xform, is.pos, u, v, wavelength)
-                        : a->sphere_lights[inst.data_index].sample_from_point(xform, is.pos, u, v, wavelength);
+                        ? a->rect_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v, wavelength)
+                        : a->sphere_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v,
+                                                                                      wavelength);
     out.kind = LS_SURFACE;
     out.color = s.color;
     out.point = s.point;
@@ -485,6 +486,38 @@ bool path_next(LightPath& p, const RenderScene& sc, const Isect& is, RayBatch& r
 
 }  // namespace
 
+// Camera::new (camera.rs:22-72): aperture and focus lists fall back to [0]
+// and [1] when either is empty or any focus distance is zero; each fov
+// becomes its linear form tan(fov/2), computed as sin/cos.
+Camera camera_new(const CameraDesc& d) {
+    Camera c;
+    c.transforms = d.transforms;
+    c.apertures = d.aperture_radii;
+    c.focuses = d.focus_distances;
+    bool no_lens = c.apertures.empty() || c.focuses.empty();
+    for (float f : c.focuses) no_lens = no_lens || f == 0.0f;
+    if (no_lens) {
+        c.apertures.assign(1, 0.0f);
+        c.focuses.assign(1, 1.0f);
+    }
+    for (float f : d.fovs) c.tfovs.push_back(std::sin(f / 2.0f) / std::cos(f / 2.0f));
+    return c;
+}
+
+// Camera::generate_ray (camera.rs:74-102): every list is lerped over the
+// shutter on its own; (u, v) in [0, 1)^2 pick the point on the lens.
+Ray generate_ray(const Camera& c, float x, float y, float time, float wavelength, float u, float v) {
+    const Transform xf = lerp_slice(c.transforms.data(), c.transforms.size(), time);
+    const float tfov = lerp_slice(c.tfovs.data(), c.tfovs.size(), time);
+    const float aperture = lerp_slice(c.apertures.data(), c.apertures.size(), time);
+    const float focus = lerp_slice(c.focuses.data(), c.focuses.size(), time);
+    float lu, lv;
+    square_to_circle((u * 2.0f) - 1.0f, (v * 2.0f) - 1.0f, lu, lv);
+    const Vec3 orig(aperture * lu, aperture * lv, 0.0f);
+    const Vec3 dir = normalized(Vec3((x * tfov) - (orig.x / focus), (y * tfov) - (orig.y / focus), 1.0f));
+    return Ray{xform_point(orig, xf), xform_vector(dir, xf), time, wavelength, INFINITY};
+}
+
 // Renderer::render_job (renderer.rs:194-310): buckets of consecutive pixels
 // in scanline order, at most 65,536 paths each, one tracer per thread.
 void render(const RenderScene& sc, const RenderSettings& rs, float* xyz, uint32_t threads, double& trace_seconds,
@@ -495,12 +528,7 @@ void render(const RenderScene& sc, const RenderSettings& rs, float* xyz, uint32_
     const float min_y = -((float)H / (float)W), max_y = (float)H / (float)W;
     const float x_extent = max_x - min_x, y_extent = max_y - min_y;
 
-    // Camera::new (camera.rs:22-72)
-    const CameraDesc& cam = sc.camera;
-    float half = cam.fov / 2.0f;
-    const float tfov = std::sin(half) / std::cos(half);
-    float aperture = cam.aperture_radius, focus = cam.focus_distance;
-    if (focus == 0.0f) { aperture = 0.0f; focus = 1.0f; }
+    const Camera cam = camera_new(sc.camera);
 
     for (uint64_t i = 0; i < (uint64_t)W * H * 3; i++) xyz[i] = 0.0f;
     const uint64_t pixels = (uint64_t)W * H;
@@ -537,13 +565,7 @@ void render(const RenderScene& sc, const RenderSettings& rs, float* xyz, uint32_
                     p.sample_number = si;
                     p.time = d0[0];
                     p.wavelength = map_0_1_to_wavelength(golden_ratio_sample(si, hash_u32((x << 16) ^ y, rs.seed)));
-                    // Camera::generate_ray (camera.rs:74-102)
-                    Transform xf = lerp_slice(cam.transforms.data(), cam.transforms.size(), p.time);
-                    float lu, lv;
-                    square_to_circle((d0[1] * 2.0f) - 1.0f, (d0[2] * 2.0f) - 1.0f, lu, lv);
-                    Vec3 orig(aperture * lu, aperture * lv, 0.0f);
-                    Vec3 dir = normalized(Vec3((img_x * tfov) - (orig.x / focus), (img_y * tfov) - (orig.y / focus), 1.0f));
-                    Ray ray{xform_point(orig, xf), xform_vector(dir, xf), p.time, p.wavelength, INFINITY};
+                    const Ray ray = generate_ray(cam, img_x, img_y, p.time, p.wavelength, d0[1], d0[2]);
                     paths.push_back(p);
                     rays.push(ray, false);
                 }

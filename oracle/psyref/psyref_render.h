@@ -41,12 +41,22 @@ struct World {
     std::vector<DistantDiskLight> lights;
 };
 
+// Camera (camera.rs:12-19): time samples of each lens parameter, each list
+// interpolated over the shutter on its own.
 struct CameraDesc {
-    std::vector<Transform> transforms;  // camera to world
-    float fov = 1.0f;                   // radians
-    float aperture_radius = 0.0f;
-    float focus_distance = 1.0f;
+    std::vector<Transform> transforms;         // camera to world
+    std::vector<float> fovs{1.0f};             // radians
+    std::vector<float> aperture_radii{0.0f};
+    std::vector<float> focus_distances{1.0f};
 };
+
+// The camera after Camera::new's checks, with linear fovs.
+struct Camera {
+    std::vector<Transform> transforms;
+    std::vector<float> tfovs, apertures, focuses;
+};
+Camera camera_new(const CameraDesc& d);
+Ray generate_ray(const Camera& c, float x, float y, float time, float wavelength, float u, float v);
 
 struct RenderSettings {
     uint32_t width = 64, height = 64, spp = 1, max_bounces = 2, seed = 0;

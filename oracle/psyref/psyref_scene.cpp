@@ -257,8 +257,8 @@ void Assembly::build() {
 std::vector<BBox> Assembly::object_bounds(const Instance& inst) const {
     if (inst.is_assembly) return assemblies[inst.data_index]->bounds();
     switch (inst.obj_kind) {
-        case OBJ_RECT_LIGHT: return {rect_lights[inst.data_index].bounds()};
-        case OBJ_SPHERE_LIGHT: return {sphere_lights[inst.data_index].bounds()};
+        case OBJ_RECT_LIGHT: return rect_lights[inst.data_index].bounds_list();
+        case OBJ_SPHERE_LIGHT: return sphere_lights[inst.data_index].bounds_list();
         default: return objects[inst.data_index]->bounds();
     }
 }
@@ -284,11 +284,11 @@ static void intersect_light_rays(const Assembly* a, const Instance& inst, RayBat
         bool hit;
         Color emit;
         if (inst.obj_kind == OBJ_RECT_LIGHT) {
-            const RectangleLight& l = a->rect_lights[inst.data_index];
+            const RectangleLight l = a->rect_lights[inst.data_index].at(time);
             hit = l.intersect(xform, rays.orig(ri), rays.dir(ri), rays.max_t(ri), t, pos, pos_err, nrm, pdf);
             emit = l.emit_color();
         } else {
-            const SphereLight& l = a->sphere_lights[inst.data_index];
+            const SphereLight l = a->sphere_lights[inst.data_index].at(time);
             hit = l.intersect(xform, rays.orig(ri), rays.orig_local(ri), rays.dir(ri), rays.max_t(ri), t, pos,
                               pos_err, nrm, pdf);
             emit = l.emit_color();

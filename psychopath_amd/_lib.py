@@ -48,11 +48,13 @@ class Color(C.Structure):
 
 
 class RectLightDesc(C.Structure):
-    _fields_ = [("width", C.c_float), ("height", C.c_float), ("color", Color)]
+    _fields_ = [("width", C.c_float), ("height", C.c_float), ("color", Color),
+                ("dims_count", C.c_uint32), ("dims", _fp), ("color_count", C.c_uint32), ("colors", C.POINTER(Color))]
 
 
 class SphereLightDesc(C.Structure):
-    _fields_ = [("radius", C.c_float), ("color", Color)]
+    _fields_ = [("radius", C.c_float), ("color", Color),
+                ("radius_count", C.c_uint32), ("radii", _fp), ("color_count", C.c_uint32), ("colors", C.POINTER(Color))]
 
 
 class InstanceDesc(C.Structure):
@@ -85,7 +87,9 @@ class ShaderDesc(C.Structure):
 
 
 class DistantLightDesc(C.Structure):
-    _fields_ = [("radius", C.c_float), ("direction", C.c_float * 3), ("color", Color)]
+    _fields_ = [("radius", C.c_float), ("direction", C.c_float * 3), ("color", Color),
+                ("radius_count", C.c_uint32), ("radii", _fp), ("direction_count", C.c_uint32), ("directions", _fp),
+                ("color_count", C.c_uint32), ("colors", C.POINTER(Color))]
 
 
 class WorldDesc(C.Structure):
@@ -105,6 +109,12 @@ class CameraDesc(C.Structure):
         ("fov", C.c_float),
         ("aperture_radius", C.c_float),
         ("focus_distance", C.c_float),
+        ("fov_count", C.c_uint32),
+        ("fovs", _fp),
+        ("aperture_count", C.c_uint32),
+        ("aperture_radii", _fp),
+        ("focus_count", C.c_uint32),
+        ("focus_distances", _fp),
     ]
 
 
@@ -181,6 +191,7 @@ class TraceTuning(C.Structure):
         ("refill_min", C.c_uint32),
         ("stack_lds", C.c_uint32),
         ("simd_waves", C.c_uint32),
+        ("leaf_min", C.c_uint32),
     ]
 
 

@@ -35,6 +35,7 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.inv_static, &s->inv_static);
     if (!r) r = upload(h.rect_lights, &s->rect_lights);
     if (!r) r = upload(h.sphere_lights, &s->sphere_lights);
+    if (!r) r = upload(h.light_samples, &s->light_samples);
     if (!r) r = upload(h.inst_pos, &s->inst_pos);
     if (!r) r = upload(h.tri_rec, &s->tri_rec);
     if (!r) r = upload(h.light_nodes, &s->light_nodes);
@@ -57,6 +58,7 @@ TraceArgs scene_args(const psy_scene* s) {
     a.inv_static = s->inv_static;
     a.rect_lights = s->rect_lights;
     a.sphere_lights = s->sphere_lights;
+    a.light_samples = s->light_samples;
     a.top_root = s->host.top_root;
     return a;
 }
@@ -72,7 +74,8 @@ int device_cu_count(uint32_t& cus_out) {
 hipError_t launch_tuned(const psy_scene* scene, const psy_trace_tuning& tuning, uint32_t cu_count, unsigned int* work,
                         const TraceArgs& a, bool counted, hipStream_t stream) {
     if (tuning.kernel == PSY_KERNEL_SIMPLE) return launch_trace(a, counted, stream);
-    TraceTuning tt{cu_count * tuning.waves_per_cu, tuning.refill_min, tuning.stack_lds, tuning.simd_waves};
+    TraceTuning tt{cu_count * tuning.waves_per_cu, tuning.refill_min, tuning.stack_lds, tuning.simd_waves,
+                   tuning.leaf_min == 0 ? 64u : tuning.leaf_min};
     return launch_trace_persistent(a, counted, scene->host.has_xf || scene->host.has_lights, scene->host.has_motion,
                                    work, tt, stream);
 }
@@ -85,6 +88,7 @@ int check_tuning(const psy_trace_tuning* t) {
     if (t->stack_lds != 8 && t->stack_lds != 16 && t->stack_lds != 24)
         return fail(PSY_E_RANGE, "stack_lds must be 8, 16 or 24");
     if (t->simd_waves != 4 && t->simd_waves != 8) return fail(PSY_E_RANGE, "simd_waves must be 4 or 8");
+    if (t->leaf_min > 64) return fail(PSY_E_RANGE, "leaf_min must be in [0, 64]");
     return PSY_OK;
 }
 
@@ -205,6 +209,7 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
                       h.meshes.size() * sizeof(GpuMesh) + h.instances.size() * sizeof(GpuInstance) +
                       (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform) +
                       h.rect_lights.size() * sizeof(GpuRectLight) + h.sphere_lights.size() * sizeof(GpuSphereLight) +
+                      h.light_samples.size() * sizeof(float) +
                       (h.inst_pos.size() + h.tri_rec.size() + h.light_list.size()) * 4 +
                       h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox) +
                       (h.normals.size() + h.nidx.size()) * 4;
@@ -324,34 +329,56 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
     if (rc) return rc;
     if (n == 0) return fail(PSY_E_INVALID, "autotune needs rays");
     std::vector<psy_trace_tuning> cands;
-    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8, 4});
+    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8, 4, 64});
     const uint32_t wpc[] = {8, 12, 16, 24, 32};
     const uint32_t refill[] = {1, 8, 16, 32};
     const uint32_t depth[] = {8, 16, 24};
     for (uint32_t w : wpc)
         for (uint32_t r : refill)
             for (uint32_t dd : depth)
-                for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ});
+                for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ, 64});
     const psy_trace_tuning saved = tr->tuning;
     float best_ms = 1e30f;
     psy_trace_tuning best_t = saved;
-    for (const psy_trace_tuning& c : cands) {
-        tr->tuning = c;
-        float ms_min = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {
-            rc = enqueue(tr, rays, n, hits, false);
-            if (rc) { tr->tuning = saved; return rc; }
-            float ms = 0;
-            PSY_HIP(hipEventSynchronize(tr->ev1));
-            PSY_HIP(hipEventElapsedTime(&ms, tr->ev0, tr->ev1));
-            if (rep > 0) ms_min = std::min(ms_min, ms);
+    uint64_t runs = 0;
+    auto time_all = [&]() -> int {
+        for (const psy_trace_tuning& c : cands) {
+            tr->tuning = c;
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                const int e = enqueue(tr, rays, n, hits, false);
+                if (e) {
+                    tr->tuning = saved;
+                    return e;
+                }
+                float ms = 0;
+                PSY_HIP(hipEventSynchronize(tr->ev1));
+                PSY_HIP(hipEventElapsedTime(&ms, tr->ev0, tr->ev1));
+                if (rep > 0) ms_min = std::min(ms_min, ms);
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best_t = c;
+            }
         }
-        if (ms_min < best_ms) {
-            best_ms = ms_min;
-            best_t = c;
-        }
+        runs += 3 * cands.size();
+        return PSY_OK;
+    };
+    // Stage 1: kernel, waves, refill, LDS depth and register budget with
+    // whole-wave descents; stage 2: the descent threshold around the best.
+    rc = time_all();
+    if (rc) return rc;
+    if (best_t.kernel == PSY_KERNEL_PERSISTENT) {
+        const psy_trace_tuning s1 = best_t;
+        cands.clear();
+        for (uint32_t lm : {8u, 16u, 24u, 32u, 48u})
+            for (uint32_t r : {8u, 16u, 32u})
+                for (uint32_t w : {16u, 24u, 32u})
+                    cands.push_back({PSY_KERNEL_PERSISTENT, w, r, s1.stack_lds, s1.simd_waves, lm});
+        rc = time_all();
+        if (rc) return rc;
     }
-    tr->rays_traced -= 3 * n * cands.size();
+    tr->rays_traced -= runs * n;
     tr->tuning = best_t;
     if (best) *best = best_t;
     return PSY_OK;

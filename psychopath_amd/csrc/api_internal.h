@@ -47,6 +47,7 @@ struct psy_scene {
     psy::GpuXform* inv_static = nullptr;
     psy::GpuRectLight* rect_lights = nullptr;
     psy::GpuSphereLight* sphere_lights = nullptr;
+    float* light_samples = nullptr;
     uint32_t* inst_pos = nullptr;
     uint32_t* tri_rec = nullptr;
     psy::GpuLightNode* light_nodes = nullptr;
@@ -59,8 +60,8 @@ struct psy_scene {
 
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,
-                        sphere_lights, inst_pos, tri_rec, light_nodes, light_boxes, light_list,
-                        normals,   nidx};
+                        sphere_lights, light_samples, inst_pos, tri_rec, light_nodes, light_boxes,
+                        light_list, normals, nidx};
         for (void* b : bufs) (void)hipFree(b);
     }
 };

@@ -104,20 +104,30 @@ struct GpuColor {
     float a, b, c;
 };
 
+// Lights.  A light with time samples keeps them in a float array beside the
+// records: `samples` is the offset of its lists there and `counts` their
+// lengths (one byte each, 0 = the value in the record); each list is lerped
+// on its own at the ray's time.  Rectangles: (w, h) pairs, then colours;
+// spheres: radii, then colours; distant lights: radii, directions (xyz),
+// then colours.  A colour sample is four floats: kind (0, 1 or 2), a, b, c.
 struct GpuRectLight {
     float w, h;
-    float pad[2];
+    uint32_t samples, counts;  // counts: dims | colours << 8
     GpuColor color;
 };
 struct GpuSphereLight {
     float radius;
-    float pad[3];
+    uint32_t samples, counts;  // counts: radii | colours << 8
+    float pad;
     GpuColor color;
 };
 struct GpuDistantLight {
     float radius;
     float dir[3];
     GpuColor color;
+    uint32_t samples, counts;  // counts: radii | directions << 8 | colours << 16
+    float energy;              // approximate_energy, the mean over the colours
+    float pad;
 };
 
 // A surface closure as shaded (SimpleSurfaceShader::shade, src/shading/mod.rs:

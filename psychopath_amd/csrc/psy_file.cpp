@@ -10,6 +10,7 @@
 // sections follow src/parse/psy.rs, psy_assembly.rs, psy_mesh_surface.rs,
 // psy_surface_shader.rs and psy_light.rs.
 #include <algorithm>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -403,6 +404,14 @@ struct MeshData {
 
 }  // namespace
 
+// Storage for light time samples: the descriptors point into it, and a
+// deque never moves its elements, so the pointers stay valid as it grows
+// and when its vectors are moved into the file's scene data.
+struct SamplePool {
+    std::deque<std::vector<float>> floats;
+    std::deque<std::vector<psy_color>> colours;
+};
+
 struct psy_file_scene_data {
     std::string name, output;
     std::vector<MeshData> mesh_data;
@@ -415,7 +424,8 @@ struct psy_file_scene_data {
     std::vector<psy_distant_light_desc> distant;
     psy_color background{};
     std::vector<float> cam_xforms;
-    float fov = 0.0f, aperture = 0.0f, focus = 1.0f;
+    std::vector<float> fovs, apertures, focuses;  // lens time samples
+    SamplePool pool;                              // light time samples
     psy_render_settings settings{};
 };
 
@@ -495,35 +505,56 @@ void parse_camera(const Node& n, psy_file_scene_data& sd) {
             sd.cam_xforms.insert(sd.cam_xforms.end(), m, m + 12);
         }
     }
-    // Camera::new (camera.rs:31-58)
+    // Camera::new (camera.rs:29-57): no aperture, no focus distance or a
+    // zero focus distance among the samples disables the lens.
     if (sd.cam_xforms.empty()) raise("MissingNode", n.offset, "Camera has no transform(s).");
     if (fovs.empty()) raise("MissingNode", n.offset, "Camera has no fov(s).");
-    if (fovs.size() > 1 || fds.size() > 1 || ars.size() > 1)
-        raise("UnknownVariant", n.offset, "time-sampled camera fov, aperture or focus distance is not supported");
-    sd.fov = fovs[0];
-    if (ars.empty() || fds.empty() || fds[0] == 0.0f) {
-        sd.aperture = 0.0f;
-        sd.focus = 1.0f;
+    if (fovs.size() > 64 || fds.size() > 64 || ars.size() > 64)
+        raise("UnknownVariant", n.offset, "more than 64 time samples of a camera lens parameter are not supported");
+    sd.fovs = fovs;
+    if (ars.empty() || fds.empty() || std::find(fds.begin(), fds.end(), 0.0f) != fds.end()) {
+        sd.apertures.assign(1, 0.0f);
+        sd.focuses.assign(1, 1.0f);
     } else {
-        sd.aperture = ars[0];
-        sd.focus = fds[0];
+        sd.apertures = ars;
+        sd.focuses = fds;
     }
 }
 
-// The one sample of a light property.
-template <typename T>
-const T& single(const std::vector<T>& v, const Node& n, const char* what) {
+// A light property's time samples (each `width` floats): the first goes to
+// `one`; two or more are kept in the pool and returned as (count, pointer).
+// No sample is an error, as a light without the property cannot render.
+uint32_t light_samples(const std::vector<std::vector<float>>& v, uint32_t width, const Node& n, const char* what,
+                       float* one, SamplePool& pool, const float** list) {
     if (v.empty()) raise("MissingNode", n.offset, std::string("light has no ") + what);
-    if (v.size() > 1) raise("UnknownVariant", n.offset, std::string("time-sampled light ") + what + " is not supported");
-    return v[0];
+    if (v.size() > 64) raise("UnknownVariant", n.offset, std::string("more than 64 time samples of light ") + what);
+    for (uint32_t k = 0; k < width; k++) one[k] = v[0][k];
+    *list = nullptr;
+    if (v.size() == 1) return 0;
+    pool.floats.emplace_back();
+    for (const auto& s : v) pool.floats.back().insert(pool.floats.back().end(), s.begin(), s.end());
+    *list = pool.floats.back().data();
+    return (uint32_t)v.size();
+}
+uint32_t light_colours(const std::vector<psy_color>& v, const Node& n, psy_color* one, SamplePool& pool,
+                       const psy_color** list) {
+    if (v.empty()) raise("MissingNode", n.offset, "light has no colour");
+    if (v.size() > 64) raise("UnknownVariant", n.offset, "more than 64 time samples of light colour");
+    for (const psy_color& c : v)
+        if (c.kind != v[0].kind) raise("UnknownVariant", n.offset, "Cannot lerp colors with different representations.");
+    *one = v[0];
+    *list = nullptr;
+    if (v.size() == 1) return 0;
+    pool.colours.push_back(v);
+    *list = pool.colours.back().data();
+    return (uint32_t)v.size();
 }
 
 // parse_distant_disk_light / parse_sphere_light / parse_rectangle_light
 // (psy_light.rs:16-184): Radius, Direction, Dimensions and Color leaves,
 // each possibly repeated as time samples.
 struct LightFields {
-    std::vector<float> radius;
-    std::vector<std::vector<float>> direction, dims;
+    std::vector<std::vector<float>> radius, direction, dims;
     std::vector<psy_color> color;
 };
 
@@ -536,7 +567,7 @@ LightFields parse_light(const Node& n, const char* which) {
         float v[3];
         if (c.type == "Radius" && (w == "DistantDiskLight" || w == "SphereLight")) {
             if (!all_f32(c.contents, 1, v)) raise("UnknownError", c.offset, "Radius should be one number");
-            f.radius.push_back(v[0]);
+            f.radius.push_back({v[0]});
         } else if (c.type == "Direction" && w == "DistantDiskLight") {
             if (!all_f32(c.contents, 3, v)) raise("UnknownError", c.offset, "Direction should be three numbers");
             f.direction.push_back({v[0], v[1], v[2]});
@@ -570,12 +601,9 @@ void parse_world(const Node& n, psy_file_scene_data& sd) {
         if (!(c.internal && c.type == "DistantDiskLight")) continue;
         const LightFields f = parse_light(c, "DistantDiskLight");
         psy_distant_light_desc d{};
-        d.radius = single(f.radius, c, "radius");
-        const auto& dir = single(f.direction, c, "direction");
-        d.direction[0] = dir[0];
-        d.direction[1] = dir[1];
-        d.direction[2] = dir[2];
-        d.color = single(f.color, c, "colour");
+        d.radius_count = light_samples(f.radius, 1, c, "radius", &d.radius, sd.pool, &d.radii);
+        d.direction_count = light_samples(f.direction, 3, c, "direction", d.direction, sd.pool, &d.directions);
+        d.color_count = light_colours(f.color, c, &d.color, sd.pool, &d.colors);
         sd.distant.push_back(d);
     }
 }
@@ -692,6 +720,7 @@ struct AsmDef {
     std::vector<MeshData> meshes;
     std::vector<psy_rect_light_desc> rects;
     std::vector<psy_sphere_light_desc> spheres;
+    SamplePool pool;  // light time samples
     std::vector<std::unique_ptr<AsmDef>> subs;
     struct Inst {
         uint32_t kind;    // PSY_OBJECT_* or KIND_ASSEMBLY
@@ -765,12 +794,20 @@ std::unique_ptr<AsmDef> parse_assembly(const Node& n, bool root) {
             } else if (c.type == "SphereLight") {
                 const LightFields f = parse_light(c, "SphereLight");
                 ref = ObjectRef{PSY_OBJECT_SPHERE_LIGHT, (uint32_t)def->spheres.size()};
-                def->spheres.push_back(psy_sphere_light_desc{single(f.radius, c, "radius"), single(f.color, c, "colour")});
+                psy_sphere_light_desc l{};
+                l.radius_count = light_samples(f.radius, 1, c, "radius", &l.radius, def->pool, &l.radii);
+                l.color_count = light_colours(f.color, c, &l.color, def->pool, &l.colors);
+                def->spheres.push_back(l);
             } else {
                 const LightFields f = parse_light(c, "RectangleLight");
-                const auto& d = single(f.dims, c, "dimensions");
                 ref = ObjectRef{PSY_OBJECT_RECT_LIGHT, (uint32_t)def->rects.size()};
-                def->rects.push_back(psy_rect_light_desc{d[0], d[1], single(f.color, c, "colour")});
+                psy_rect_light_desc l{};
+                float wh[2];
+                l.dims_count = light_samples(f.dims, 2, c, "dimensions", wh, def->pool, &l.dims);
+                l.width = wh[0];
+                l.height = wh[1];
+                l.color_count = light_colours(f.color, c, &l.color, def->pool, &l.colors);
+                def->rects.push_back(l);
             }
             object_map[c.ident] = ref;
         }
@@ -859,6 +896,8 @@ void read_assembly(const Node& n, psy_file_scene_data& sd) {
     std::unique_ptr<AsmDef> root = parse_assembly(n, true);
     sd.rects = root->rects;
     sd.spheres = root->spheres;
+    for (auto& v : root->pool.floats) sd.pool.floats.push_back(std::move(v));
+    for (auto& v : root->pool.colours) sd.pool.colours.push_back(std::move(v));
     place_definitions(*root, sd);
     flatten(*root, {}, sd);
 }
@@ -958,9 +997,15 @@ int psy_file_get_scene(const psy_file* f, uint32_t index, psy_file_scene* out) {
     s.world.distant_lights = sd.distant.data();
     s.camera.xform_count = (uint32_t)(sd.cam_xforms.size() / 12);
     s.camera.xforms = sd.cam_xforms.data();
-    s.camera.fov = sd.fov;
-    s.camera.aperture_radius = sd.aperture;
-    s.camera.focus_distance = sd.focus;
+    s.camera.fov = sd.fovs[0];
+    s.camera.aperture_radius = sd.apertures[0];
+    s.camera.focus_distance = sd.focuses[0];
+    s.camera.fov_count = (uint32_t)sd.fovs.size();
+    s.camera.fovs = sd.fovs.data();
+    s.camera.aperture_count = (uint32_t)sd.apertures.size();
+    s.camera.aperture_radii = sd.apertures.data();
+    s.camera.focus_count = (uint32_t)sd.focuses.size();
+    s.camera.focus_distances = sd.focuses.data();
     s.settings = sd.settings;
     *out = s;
     return PSY_OK;

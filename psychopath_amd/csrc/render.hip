@@ -106,6 +106,14 @@ struct PathRegs {
     S4 next_fac;
 };
 
+// lerp_slice of a lens list (camera.rs:76-79).
+__device__ __forceinline__ float lens_lerp(const float* s, uint32_t n, float time) {
+    uint32_t i1;
+    float a2;
+    if (!lerp_slice_pos(n, time, i1, a2)) return s[i1];
+    return lerpf(s[i1], s[i1 + 1], a2);
+}
+
 // LightPath::new and Camera::generate_ray for path g of a request
 // (renderer.rs:233-289, camera.rs:74-102).
 __device__ __forceinline__ void camera_path(const ShadeScene& sc, const ReqParams& rp, uint64_t g, PathRegs& ps,
@@ -130,11 +138,17 @@ __device__ __forceinline__ void camera_path(const ShadeScene& sc, const ReqParam
         if (!lerp_slice_pos(rp.cam_xf_count, time, i1, a2)) xf = load_xf(rp.cam_xf + i1);
         else xf = xf_lerp(load_xf(rp.cam_xf + i1), load_xf(rp.cam_xf + i1 + 1), a2);
     }
+    float tfov = rp.tfov, aperture = rp.aperture, focus = rp.focus;
+    if (rp.n_tfov != 0) {
+        const float* lens = reinterpret_cast<const float*>(rp.cam_xf + rp.cam_xf_count);
+        tfov = lens_lerp(lens, rp.n_tfov, time);
+        aperture = lens_lerp(lens + rp.n_tfov, rp.n_aperture, time);
+        focus = lens_lerp(lens + rp.n_tfov + rp.n_aperture, rp.n_focus, time);
+    }
     float lu, lv;
     square_to_circle((d0.v[1] * 2.0f) - 1.0f, (d0.v[2] * 2.0f) - 1.0f, lu, lv);
-    const V3 orig = v3(rp.aperture * lu, rp.aperture * lv, 0.0f);
-    const V3 dir =
-        normalize(v3((img_x * rp.tfov) - (orig.x / rp.focus), (img_y * rp.tfov) - (orig.y / rp.focus), 1.0f));
+    const V3 orig = v3(aperture * lu, aperture * lv, 0.0f);
+    const V3 dir = normalize(v3((img_x * tfov) - (orig.x / focus), (img_y * tfov) - (orig.y / focus), 1.0f));
     ps.color = s4_zero();
     ps.atten = s4_splat(1.0f);
     ps.pending = s4_zero();
@@ -329,7 +343,7 @@ __device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float
     }
     const Xf xform = instance_xf(sc.xforms, inst, time);
     if (inst.kind == PSY_OBJECT_RECT_LIGHT) {
-        const GpuRectLight L = sc.rects[inst.mesh];
+        const GpuRectLight L = rect_at(sc.rects[inst.mesh], sc.light_samples, time);
         V3 p[4];
         rect_corners(L, xf_inverse(xform), p);
         const V3 t0 = prim == 0 ? p[0] : p[2], t1 = prim == 0 ? p[1] : p[3], t2 = prim == 0 ? p[2] : p[0];
@@ -339,7 +353,7 @@ __device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float
         is.sample_pdf = rect_sample_pdf(L, xform, orig, dir, is.pos);
         is.emit = color_scaled(L.color, (float)(1.0 / ((double)L.w * (double)L.h)));
     } else {
-        const GpuSphereLight L = sc.spheres[inst.mesh];
+        const GpuSphereLight L = sphere_at(sc.spheres[inst.mesh], sc.light_samples, time);
         const V3 lo = xf_point(xform, orig);
         const V3 ld = xf_vector(xform, dir);
         const Xf inv = xf_inverse(xform);
@@ -428,7 +442,7 @@ __device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float
     LightPick out;
     out.found = false;
     float wl_sum = 0.0f;
-    for (uint32_t i = 0; i < sc.distant_count; i++) wl_sum = wl_sum + ((0.0f + color_energy(sc.distant[i].color)) / 1.0f);
+    for (uint32_t i = 0; i < sc.distant_count; i++) wl_sum = wl_sum + sc.distant[i].energy;
     const float wl_energy = wl_sum <= 0.0f ? 0.0f : 1.0f;
     const float ll_energy = (sc.has_ltree ? sc.lnodes[0].energy : 0.0f) <= 0.0f ? 0.0f : 1.0f;
     const float tot = wl_energy + ll_energy;
@@ -440,16 +454,16 @@ __device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float
         uint32_t pick = sc.distant_count - 1;
         float w = 0.0f;
         for (uint32_t i = 0; i < sc.distant_count; i++) {
-            w = (0.0f + color_energy(sc.distant[i].color)) / 1.0f;
+            w = sc.distant[i].energy;
             x += w;
             if (x > nn) {
                 pick = i;
                 break;
             }
         }
-        w = (0.0f + color_energy(sc.distant[pick].color)) / 1.0f;
+        w = sc.distant[pick].energy;
         const float p = w / wl_sum;
-        distant_sample(sc.distant[pick], u, v, wl, out.color, out.dir, out.pdf);
+        distant_sample(distant_at(sc.distant[pick], sc.distant_samples, time), u, v, wl, out.color, out.dir, out.pdf);
         out.distant = true;
         out.selection_pdf = p * wl_prob;
         out.found = true;
@@ -466,8 +480,10 @@ __device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float
     const GpuInstance inst = sc.instances[sc.llist[li]];
     const Xf xform = instance_xf(sc.xforms, inst, time);
     const LightSample s = inst.kind == PSY_OBJECT_RECT_LIGHT
-                              ? rect_sample_from_point(sc.rects[inst.mesh], xform, is.pos, u, v, wl)
-                              : sphere_sample_from_point(sc.spheres[inst.mesh], xform, is.pos, u, v, wl);
+                              ? rect_sample_from_point(rect_at(sc.rects[inst.mesh], sc.light_samples, time), xform,
+                                                       is.pos, u, v, wl)
+                              : sphere_sample_from_point(sphere_at(sc.spheres[inst.mesh], sc.light_samples, time),
+                                                         xform, is.pos, u, v, wl);
     out.found = true;
     out.distant = false;
     out.color = s.color;
@@ -911,7 +927,8 @@ int launch_zero(float* p, uint64_t n, hipStream_t s) {
 
 // Camera::new (camera.rs:22-72) and the image plane of Renderer::render_job
 // (renderer.rs:214-231).
-int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp) {
+int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp,
+                std::vector<GpuXform>& cam_slots) {
     if (!cam || !st) return fail(PSY_E_INVALID, "NULL argument");
     if (st->width == 0 || st->height == 0 || st->spp == 0) return fail(PSY_E_INVALID, "empty image or zero spp");
     if (st->width > 65535 || st->height > 65535) return fail(PSY_E_RANGE, "width and height must be below 65536");
@@ -919,7 +936,13 @@ int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqPa
     if (pixels * st->spp >= (1ull << 48)) return fail(PSY_E_RANGE, "image too large");
     if (st->max_bounces > 64) return fail(PSY_E_RANGE, "max_bounces must be at most 64");
     if (cam->xform_count == 0 || !cam->xforms) return fail(PSY_E_INVALID, "camera needs at least one transform");
-    if (cam->xform_count > 64) return fail(PSY_E_RANGE, "at most 64 camera transforms");
+    if (cam->xform_count > CAM_XFORMS) return fail(PSY_E_RANGE, "at most 64 camera transforms");
+    if ((cam->fov_count && !cam->fovs) || (cam->aperture_count && !cam->aperture_radii) ||
+        (cam->focus_count && !cam->focus_distances))
+        return fail(PSY_E_INVALID, "camera lens list is NULL");
+    if (cam->fov_count > CAM_LENS_SAMPLES || cam->aperture_count > CAM_LENS_SAMPLES ||
+        cam->focus_count > CAM_LENS_SAMPLES)
+        return fail(PSY_E_RANGE, "at most 64 time samples per camera lens parameter");
     std::memset(&rp, 0, sizeof(rp));
     rp.width = st->width;
     rp.height = st->height;
@@ -932,15 +955,37 @@ int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqPa
     const float max_y = (float)st->height / (float)st->width, min_y = -max_y;
     rp.x_extent = max_x - min_x;
     rp.y_extent = max_y - min_y;
-    const float half = cam->fov / 2.0f;
-    rp.tfov = std::sin(half) / std::cos(half);
-    rp.aperture = cam->aperture_radius;
-    rp.focus = cam->focus_distance;
-    if (rp.focus == 0.0f) {
-        rp.aperture = 0.0f;
-        rp.focus = 1.0f;
+    // Camera::new (camera.rs:31-64): a zero focus distance anywhere disables
+    // the lens; fovs become tan(fov/2) as sin/cos.
+    std::vector<float> tfovs, apertures, focuses;
+    if (cam->fov_count) tfovs.assign(cam->fovs, cam->fovs + cam->fov_count);
+    else tfovs.assign(1, cam->fov);
+    for (float& f : tfovs) f = std::sin(f / 2.0f) / std::cos(f / 2.0f);
+    if (cam->aperture_count) apertures.assign(cam->aperture_radii, cam->aperture_radii + cam->aperture_count);
+    else apertures.assign(1, cam->aperture_radius);
+    if (cam->focus_count) focuses.assign(cam->focus_distances, cam->focus_distances + cam->focus_count);
+    else focuses.assign(1, cam->focus_distance);
+    if (std::find(focuses.begin(), focuses.end(), 0.0f) != focuses.end()) {
+        apertures.assign(1, 0.0f);
+        focuses.assign(1, 1.0f);
     }
+    rp.tfov = tfovs[0];
+    rp.aperture = apertures[0];
+    rp.focus = focuses[0];
     rp.cam_xf_count = cam->xform_count;
+    cam_slots.assign(cam->xform_count, GpuXform{});
+    for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(cam_slots[i].r, cam->xforms + 12ull * i, 48);
+    if (tfovs.size() > 1 || apertures.size() > 1 || focuses.size() > 1) {
+        rp.n_tfov = (uint32_t)tfovs.size();
+        rp.n_aperture = (uint32_t)apertures.size();
+        rp.n_focus = (uint32_t)focuses.size();
+        std::vector<float> lens(tfovs);
+        lens.insert(lens.end(), apertures.begin(), apertures.end());
+        lens.insert(lens.end(), focuses.begin(), focuses.end());
+        const size_t extra = (lens.size() * sizeof(float) + sizeof(GpuXform) - 1) / sizeof(GpuXform);
+        cam_slots.resize(cam->xform_count + extra, GpuXform{});
+        std::memcpy(cam_slots.data() + cam->xform_count, lens.data(), lens.size() * sizeof(float));
+    }
     rp.rx = 0;
     rp.ry = 0;
     rp.rw = st->width;
@@ -1015,13 +1060,30 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
         if (rc) return rc;
         shaders.push_back(GpuClosure{d.kind, d.roughness, d.fresnel, 0, GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
     }
+    // Distant lights (DistantDiskLight::new, distant_disk_light.rs:24-36),
+    // with their time samples after the records and the mean colour energy
+    // precomputed (approximate_energy, distant_disk_light.rs:91-96).
     std::vector<GpuDistantLight> distant;
+    std::vector<float> distant_samples;
     for (uint32_t i = 0; i < world->distant_light_count; i++) {
         const psy_distant_light_desc& d = world->distant_lights[i];
-        int rc = color_check(d.color);
-        if (rc) return rc;
-        distant.push_back(GpuDistantLight{d.radius, {d.direction[0], d.direction[1], d.direction[2]},
-                                          GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
+        const std::string who = "distant light " + std::to_string(i) + ": ";
+        if ((d.radius_count && !d.radii) || (d.direction_count && !d.directions) || (d.color_count && !d.colors))
+            return fail(PSY_E_INVALID, who + "sample list is NULL");
+        if (d.radius_count > 64 || d.direction_count > 64 || d.color_count > 64)
+            return fail(PSY_E_RANGE, who + "too many time samples");
+        std::vector<psy_color> cs;
+        std::string why;
+        if (!colour_list(d.color, d.color_count, d.colors, cs, why)) return fail(PSY_E_INVALID, who + why);
+        GpuDistantLight g{d.radius, {d.direction[0], d.direction[1], d.direction[2]},
+                          GpuColor{cs[0].kind, cs[0].a, cs[0].b, cs[0].c}, (uint32_t)distant_samples.size(), 0u,
+                          light_energy(cs), 0.0f};
+        const uint32_t nr = pack_samples(d.radius_count, d.radii, 1, &g.radius, distant_samples);
+        const uint32_t nd = pack_samples(d.direction_count, d.directions, 3, g.dir, distant_samples);
+        const uint32_t nc = pack_colours(cs, distant_samples);
+        g.counts = nr | (nd << 8) | (nc << 16);
+        if (g.counts == 0) g.samples = 0;
+        distant.push_back(g);
     }
     int rc = color_check(world->background);
     if (rc) return rc;
@@ -1080,11 +1142,11 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!rc) rc = r->alloc(&r->work, 64);
     if (!rc) rc = r->alloc(&r->reqs, MAX_SEGS);
     if (!rc) rc = r->alloc(&r->plans, 2);
-    if (!rc) rc = r->alloc(&r->cam, (uint64_t)MAX_SEGS * 64);
+    if (!rc) rc = r->alloc(&r->cam, (uint64_t)MAX_SEGS * CAM_SLOTS);
     if (rc) return bail(rc);
     r->req_cap = MAX_SEGS;
     r->plan_cap = 2;
-    r->cam_cap = 64;
+    r->cam_cap = CAM_SLOTS;
     r->qa[0].count = r->counts + 0;
     r->qa[1].count = r->counts + 1;
     r->qs[0].count = r->counts + 2;
@@ -1104,6 +1166,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.inv_static = scene->inv_static;
     sc.rects = scene->rect_lights;
     sc.spheres = scene->sphere_lights;
+    sc.light_samples = scene->light_samples;
     sc.lnodes = scene->light_nodes;
     sc.lboxes = scene->light_boxes;
     sc.llist = scene->light_list;
@@ -1120,6 +1183,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.background = GpuColor{world->background.kind, world->background.a, world->background.b, world->background.c};
     GpuClosure* dsh = nullptr;
     GpuDistantLight* ddl = nullptr;
+    float* dls = nullptr;
     HaltonDim* hd = nullptr;
     uint16_t* hp = nullptr;
     std::vector<HaltonDim> hdims;
@@ -1127,6 +1191,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     halton_tables(hdims, hperms);
     rc = r->alloc(&dsh, shaders.size());
     if (!rc) rc = r->alloc(&ddl, distant.size());
+    if (!rc) rc = r->alloc(&dls, distant_samples.size());
     if (!rc) rc = r->alloc(&hd, hdims.size());
     if (!rc) rc = r->alloc(&hp, hperms.size());
     if (rc) return bail(rc);
@@ -1134,11 +1199,14 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!shaders.empty()) e = hipMemcpy(dsh, shaders.data(), shaders.size() * sizeof(GpuClosure), hipMemcpyHostToDevice);
     if (e == hipSuccess && !distant.empty())
         e = hipMemcpy(ddl, distant.data(), distant.size() * sizeof(GpuDistantLight), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !distant_samples.empty())
+        e = hipMemcpy(dls, distant_samples.data(), distant_samples.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(hd, hdims.data(), hdims.size() * sizeof(HaltonDim), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(hp, hperms.data(), hperms.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bail(fail(PSY_E_DEVICE, std::string("renderer upload: ") + hipGetErrorString(e)));
     sc.shaders = dsh;
     sc.distant = ddl;
+    sc.distant_samples = dls;
     sc.hdims = hd;
     sc.hperms = hp;
     *out = r;
@@ -1156,13 +1224,11 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     if (!r || !cam || !st || !xyz) return fail(PSY_E_INVALID, "NULL argument");
     if (mem != PSY_MEM_HOST && mem != PSY_MEM_DEVICE) return fail(PSY_E_INVALID, "mem must be PSY_MEM_HOST or PSY_MEM_DEVICE");
     ReqParams rp;
-    int rc = make_params(cam, st, rp);
+    std::vector<GpuXform> cx;
+    int rc = make_params(cam, st, rp, cx);
     if (rc) return rc;
     const uint64_t pixels = (uint64_t)st->width * st->height;
     const uint64_t total = (uint64_t)rp.rw * rp.rh * st->spp;
-
-    std::vector<GpuXform> cx(cam->xform_count);
-    for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(cx[i].r, cam->xforms + 12ull * i, 48);
     rp.cam_xf = r->cam;
 
     float* film = xyz;
@@ -1266,7 +1332,8 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
                           psy_trace_tuning* best_tuning, uint32_t* best_flags) {
     if (!r || !cam || !trial) return fail(PSY_E_INVALID, "NULL argument");
     ReqParams rp;
-    int rc = make_params(cam, trial, rp);
+    std::vector<GpuXform> cx;
+    int rc = make_params(cam, trial, rp, cx);
     if (rc) return rc;
     const uint64_t pixels = (uint64_t)trial->width * trial->height;
     float* film = nullptr;
@@ -1308,13 +1375,22 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
             for (uint32_t refill : {8u, 16u, 32u}) {
                 for (uint32_t lds : {16u, 24u}) {
                     if (rc) break;
-                    r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw};
+                    r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw, 64};
                     float ms;
                     rc = time_trial(ms);
                     if (!rc) consider(ms);
                 }
             }
         }
+    }
+    // Stage 1b: the traversal's descent threshold.
+    for (uint32_t lm : {16u, 32u, 48u}) {
+        if (rc) break;
+        r->tuning = best_t;
+        r->tuning.leaf_min = lm;
+        float ms;
+        rc = time_trial(ms);
+        if (!rc) consider(ms);
     }
     // Stage 2: the shading kernels' register budget.
     for (uint32_t sw = 1; sw <= 4 && !rc; sw++) {

@@ -24,6 +24,14 @@ constexpr int KBLOCK = 256;
 constexpr uint32_t MAX_SEGS = 64;  // requests interleaved in one wave
 constexpr uint32_t NUM_BINS = 5;   // miss, light, Lambert, GGX, emit
 
+// A request's camera on the device: its transforms, then (when any lens
+// parameter has several time samples) the lens lists as floats, linear
+// fovs first, then aperture radii, then focus distances.
+constexpr uint32_t CAM_XFORMS = 64;       // transforms per camera
+constexpr uint32_t CAM_LENS_SAMPLES = 64;  // samples per lens list
+constexpr uint32_t CAM_SLOTS =            // GpuXform-sized slots per camera
+    CAM_XFORMS + (3 * CAM_LENS_SAMPLES * 4 + sizeof(GpuXform) - 1) / sizeof(GpuXform);
+
 // One request's render parameters (device copy).
 struct ReqParams {
     uint32_t width, height, spp, max_bounces, seed;
@@ -32,7 +40,9 @@ struct ReqParams {
     // Rendered region (the whole image unless cropped): path g belongs to
     // region pixel g / spp, row-major in the region.
     uint32_t rx, ry, rw, rh;
-    uint32_t pad;
+    // Lengths of the lens lists after the transforms; all 0 when every lens
+    // parameter has one sample (then tfov, aperture and focus hold them).
+    uint32_t n_tfov, n_aperture, n_focus;
     const GpuXform* cam_xf;
     float* film;
 };
@@ -72,6 +82,7 @@ struct ShadeScene {
     const GpuXform* inv_static;
     const GpuRectLight* rects;
     const GpuSphereLight* spheres;
+    const float* light_samples;  // time samples of the lights (layout.h)
     const GpuLightNode* lnodes;
     const GpuBox* lboxes;
     const uint32_t* llist;
@@ -80,6 +91,7 @@ struct ShadeScene {
     const GpuClosure* shaders;
     GpuClosure unassigned;  // mesh surfaces without a shader (tracer.rs:161-164)
     const GpuDistantLight* distant;
+    const float* distant_samples;  // time samples of the distant lights
     uint32_t distant_count;
     uint32_t pad;
     GpuColor background;
@@ -227,8 +239,10 @@ int run_wave(psy_renderer* r, uint32_t steps);
 // Kernels the server launches around waves.
 int launch_set_wave(psy_renderer* r, uint32_t value);
 int launch_zero(float* p, uint64_t n, hipStream_t s);
-// Fills a request's parameters from camera and settings (validated).
-int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp);
+// Checks the camera and settings and fills `rp` (except cam_xf and film)
+// and `cam_slots`, the camera's device image (at most CAM_SLOTS entries).
+int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqParams& rp,
+                std::vector<GpuXform>& cam_slots);
 // Drops captured graphs (after a buffer they name was replaced).
 void drop_graphs(psy_renderer* r);
 

@@ -79,18 +79,74 @@ bool color_ok(const psy_color& c) {
 }
 GpuColor to_gpu(const psy_color& c) { return GpuColor{c.kind, c.a, c.b, c.c}; }
 
-// Color::approximate_energy (src/color.rs:139-152), and the light objects'
-// approximate_energy: the fold over their single colour divided by the
-// colour count (rectangle_light.rs:246-252, sphere_light.rs:200-204).
-float light_energy(const psy_color& c) {
-    float e;
-    if (c.kind == PSY_COLOR_XYZ) e = c.b;
-    else if (c.kind == PSY_COLOR_BLACKBODY) {
-        float t2 = c.a * c.a;
-        e = (t2 * t2) * c.b;
-    } else e = c.b;
-    return (0.0f + e) / 1.0f;
+}  // namespace
+
+// Color::approximate_energy (src/color.rs:127-142), and the light objects'
+// approximate_energy: the fold of their colours' estimates from 0 in list
+// order, divided by the colour count (rectangle_light.rs:249-254,
+// sphere_light.rs:198-203, distant_disk_light.rs:91-96).
+float light_energy(const std::vector<psy_color>& cs) {
+    float sum = 0.0f;
+    for (const psy_color& c : cs) {
+        float e;
+        if (c.kind == PSY_COLOR_XYZ) e = c.b;
+        else if (c.kind == PSY_COLOR_BLACKBODY) {
+            float t2 = c.a * c.a;
+            e = (t2 * t2) * c.b;
+        } else e = c.b;
+        sum = sum + e;
+    }
+    return sum / (float)cs.size();
 }
+
+// A light parameter's time samples: `n` values of `width` floats at `list`,
+// or the one value at `one`.  Appends them to `samples` when there are
+// several and returns the count stored there (0 for a single value, which
+// is written to `one`).
+uint32_t pack_samples(uint32_t n, const float* list, uint32_t width, float* one, std::vector<float>& samples) {
+    if (n == 0) return 0;
+    if (n == 1) {
+        for (uint32_t k = 0; k < width; k++) one[k] = list[k];
+        return 0;
+    }
+    samples.insert(samples.end(), list, list + (size_t)n * width);
+    return n;
+}
+
+// A colour list: checks every colour and that they share a representation
+// (Lerp for Color panics on mixed ones, color.rs:331).
+bool colour_list(const psy_color& one, uint32_t n, const psy_color* cs, std::vector<psy_color>& out,
+                 std::string& why) {
+    out.clear();
+    if (n == 0) out.push_back(one);
+    else out.assign(cs, cs + n);
+    for (const psy_color& c : out) {
+        if (!color_ok(c)) {
+            why = "bad colour";
+            return false;
+        }
+        if (c.kind != out[0].kind) {
+            why = "colour samples of different representations cannot be interpolated";
+            return false;
+        }
+    }
+    return true;
+}
+
+// Appends a checked colour list to `samples` (kind, a, b, c per colour) when
+// it has several entries; returns the count stored (0 for one colour).
+uint32_t pack_colours(const std::vector<psy_color>& cs, std::vector<float>& samples) {
+    if (cs.size() < 2) return 0;
+    for (const psy_color& c : cs) {
+        samples.push_back((float)c.kind);
+        samples.push_back(c.a);
+        samples.push_back(c.b);
+        samples.push_back(c.c);
+    }
+    return (uint32_t)cs.size();
+}
+
+namespace {
 
 // merge_slices_append with union (src/algorithm.rs:233-262).
 std::vector<Box> merge_union(const std::vector<Box>& a, const std::vector<Box>& b) {
@@ -179,22 +235,49 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     if (d->instance_count > MAX_LEAF_FIRST) { err = "too many instances"; return PSY_E_RANGE; }
     if (d->rect_light_count > 0 && !d->rect_lights) { err = "rect_lights is NULL"; return PSY_E_INVALID; }
     if (d->sphere_light_count > 0 && !d->sphere_lights) { err = "sphere_lights is NULL"; return PSY_E_INVALID; }
+    auto size_ok = [](float v) { return v > 0.0f && std::isfinite(v); };
+    std::vector<psy_color> cs;
+    std::string why;
     for (uint32_t i = 0; i < d->rect_light_count; i++) {
         const psy_rect_light_desc& l = d->rect_lights[i];
-        if (!(l.width > 0.0f) || !(l.height > 0.0f) || !std::isfinite(l.width) || !std::isfinite(l.height) ||
-            !color_ok(l.color)) {
-            err = "rect light " + std::to_string(i) + ": bad size or colour";
-            return PSY_E_INVALID;
+        const std::string who = "rect light " + std::to_string(i) + ": ";
+        if ((l.dims_count && !l.dims) || (l.color_count && !l.colors)) { err = who + "sample list is NULL"; return PSY_E_INVALID; }
+        if (l.dims_count > MAX_TIME_SAMPLES || l.color_count > MAX_TIME_SAMPLES) {
+            err = who + "too many time samples";
+            return PSY_E_RANGE;
         }
-        s.rect_lights.push_back(GpuRectLight{l.width, l.height, {0.0f, 0.0f}, to_gpu(l.color)});
+        bool ok = l.dims_count ? true : size_ok(l.width) && size_ok(l.height);
+        for (uint32_t k = 0; k < 2 * l.dims_count; k++) ok = ok && size_ok(l.dims[k]);
+        if (!ok) { err = who + "bad size"; return PSY_E_INVALID; }
+        if (!colour_list(l.color, l.color_count, l.colors, cs, why)) { err = who + why; return PSY_E_INVALID; }
+        GpuRectLight g{l.width, l.height, (uint32_t)s.light_samples.size(), 0u, to_gpu(cs[0])};
+        float wh[2] = {l.width, l.height};
+        const uint32_t nd = pack_samples(l.dims_count, l.dims, 2, wh, s.light_samples);
+        g.w = wh[0];
+        g.h = wh[1];
+        const uint32_t nc = pack_colours(cs, s.light_samples);
+        g.counts = nd | (nc << 8);
+        if (g.counts == 0) g.samples = 0;
+        s.rect_lights.push_back(g);
     }
     for (uint32_t i = 0; i < d->sphere_light_count; i++) {
         const psy_sphere_light_desc& l = d->sphere_lights[i];
-        if (!(l.radius > 0.0f) || !std::isfinite(l.radius) || !color_ok(l.color)) {
-            err = "sphere light " + std::to_string(i) + ": bad radius or colour";
-            return PSY_E_INVALID;
+        const std::string who = "sphere light " + std::to_string(i) + ": ";
+        if ((l.radius_count && !l.radii) || (l.color_count && !l.colors)) { err = who + "sample list is NULL"; return PSY_E_INVALID; }
+        if (l.radius_count > MAX_TIME_SAMPLES || l.color_count > MAX_TIME_SAMPLES) {
+            err = who + "too many time samples";
+            return PSY_E_RANGE;
         }
-        s.sphere_lights.push_back(GpuSphereLight{l.radius, {0.0f, 0.0f, 0.0f}, to_gpu(l.color)});
+        bool ok = l.radius_count ? true : size_ok(l.radius);
+        for (uint32_t k = 0; k < l.radius_count; k++) ok = ok && size_ok(l.radii[k]);
+        if (!ok) { err = who + "bad radius"; return PSY_E_INVALID; }
+        if (!colour_list(l.color, l.color_count, l.colors, cs, why)) { err = who + why; return PSY_E_INVALID; }
+        GpuSphereLight g{l.radius, (uint32_t)s.light_samples.size(), 0u, 0.0f, to_gpu(cs[0])};
+        const uint32_t nr = pack_samples(l.radius_count, l.radii, 1, &g.radius, s.light_samples);
+        const uint32_t nc = pack_colours(cs, s.light_samples);
+        g.counts = nr | (nc << 8);
+        if (g.counts == 0) g.samples = 0;
+        s.sphere_lights.push_back(g);
     }
 
     s.bvhs.resize(d->mesh_count + 1);
@@ -319,13 +402,21 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         if (id.kind == PSY_OBJECT_MESH) {
             ob = s.bvhs[id.mesh].root_bounds;
         } else if (id.kind == PSY_OBJECT_RECT_LIGHT) {
-            // RectangleLight bounds (rectangle_light.rs:36-42)
+            // RectangleLight bounds, one box per dimensions sample
+            // (rectangle_light.rs:36-42)
             const psy_rect_light_desc& l = d->rect_lights[id.mesh];
-            ob.push_back(Box{V3{l.width * -0.5f, l.height * -0.5f, 0.0f}, V3{l.width * 0.5f, l.height * 0.5f, 0.0f}});
+            for (uint32_t k = 0; k < std::max(1u, l.dims_count); k++) {
+                const float w = l.dims_count ? l.dims[2 * k] : l.width, h = l.dims_count ? l.dims[2 * k + 1] : l.height;
+                ob.push_back(Box{V3{w * -0.5f, h * -0.5f, 0.0f}, V3{w * 0.5f, h * 0.5f, 0.0f}});
+            }
         } else {
-            // SphereLight bounds (sphere_light.rs:35-40)
-            const float r = d->sphere_lights[id.mesh].radius;
-            ob.push_back(Box{V3{-r, -r, -r}, V3{r, r, r}});
+            // SphereLight bounds, one box per radius sample
+            // (sphere_light.rs:35-40)
+            const psy_sphere_light_desc& l = d->sphere_lights[id.mesh];
+            for (uint32_t k = 0; k < std::max(1u, l.radius_count); k++) {
+                const float r = l.radius_count ? l.radii[k] : l.radius;
+                ob.push_back(Box{V3{-r, -r, -r}, V3{r, r, r}});
+            }
         }
         std::vector<Box> wb = transform_bounds(ob, xf.data(), id.xform_count);
         ibounds.insert(ibounds.end(), wb.begin(), wb.end());
@@ -373,9 +464,15 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         light_pos.push_back(k);
         lbounds.insert(lbounds.end(), ibounds.begin() + ioffs[gi.id], ibounds.begin() + ioffs[gi.id + 1]);
         loffs.push_back((uint32_t)lbounds.size());
-        const psy_color& c = gi.kind == PSY_OBJECT_RECT_LIGHT ? d->rect_lights[gi.mesh].color
-                                                               : d->sphere_lights[gi.mesh].color;
-        lenergy.push_back(light_energy(c));
+        std::vector<psy_color> lc;
+        if (gi.kind == PSY_OBJECT_RECT_LIGHT) {
+            const psy_rect_light_desc& l = d->rect_lights[gi.mesh];
+            (void)colour_list(l.color, l.color_count, l.colors, lc, why);
+        } else {
+            const psy_sphere_light_desc& l = d->sphere_lights[gi.mesh];
+            (void)colour_list(l.color, l.color_count, l.colors, lc, why);
+        }
+        lenergy.push_back(light_energy(lc));
     }
     if (!light_pos.empty()) {
         std::vector<uint32_t> keys(light_pos.size());

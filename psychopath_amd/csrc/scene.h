@@ -29,6 +29,7 @@ struct PackedScene {
     // Lights and shading lookups.
     std::vector<GpuRectLight> rect_lights;
     std::vector<GpuSphereLight> sphere_lights;
+    std::vector<float> light_samples;       // time samples of the lights, see GpuRectLight
     std::vector<uint32_t> inst_pos;         // description index -> leaf position
     std::vector<uint32_t> tri_rec;          // per mesh at GpuMesh::rec_map: triangle -> record
     std::vector<GpuLightNode> light_nodes;  // light tree, root first; empty without lights
@@ -51,5 +52,14 @@ struct PackedScene {
 // Validates `desc` and builds every BVH.  Returns PSY_OK or an error code
 // with a message in `err`.
 int pack_scene(const psy_scene_desc* desc, PackedScene& out, std::string& err);
+
+// Light time samples (shared with the renderer's distant lights): the mean
+// colour energy, a checked colour list, and the packing of several samples
+// into a light sample array (see GpuRectLight).
+float light_energy(const std::vector<psy_color>& colours);
+bool colour_list(const psy_color& one, uint32_t n, const psy_color* cs, std::vector<psy_color>& out,
+                 std::string& why);
+uint32_t pack_samples(uint32_t n, const float* list, uint32_t width, float* one, std::vector<float>& samples);
+uint32_t pack_colours(const std::vector<psy_color>& cs, std::vector<float>& samples);
 
 }  // namespace psy

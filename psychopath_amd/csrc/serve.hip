@@ -36,8 +36,6 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr uint32_t CAM_SLOTS = 64;  // camera transforms per request slot
-
 struct Request {
     uint64_t id = 0;
     uint32_t ren = 0;
@@ -62,7 +60,7 @@ struct InFlight {
 
 struct RenState {
     psy_renderer* r = nullptr;
-    GpuXform* cams = nullptr;        // CAM_SLOTS transforms per request slot
+    GpuXform* cams = nullptr;        // CAM_SLOTS camera entries per request slot
     WavePlan* host_plans = nullptr;  // pinned, two buffers
     ReqParams* host_params = nullptr;  // pinned staging, one per request slot
     GpuXform* host_cams = nullptr;     // pinned staging, CAM_SLOTS per slot
@@ -297,7 +295,7 @@ int psy_server_create(psy_renderer* const* renderers, uint32_t count, const psy_
             if (renderers[j] == r) return bail(fail(PSY_E_INVALID, "a renderer is listed twice"));
         RenState& rs = sv->rens[i];
         rs.r = r;
-        rs.cams = r->cam;  // CAM_SLOTS transforms for each of the MAX_SEGS request slots
+        rs.cams = r->cam;  // CAM_SLOTS camera entries for each of the MAX_SEGS request slots
         hipError_t e = hipHostMalloc((void**)&rs.host_plans, 2 * sizeof(WavePlan), hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void**)&rs.host_params, MAX_SEGS * sizeof(ReqParams), hipHostMallocDefault);
         if (e == hipSuccess)
@@ -349,14 +347,12 @@ int psy_server_submit(psy_server* sv, uint32_t renderer, const psy_camera_desc* 
     if (renderer >= sv->rens.size()) return fail(PSY_E_INVALID, "renderer index out of range");
     Request* q = new (std::nothrow) Request();
     if (!q) return fail(PSY_E_NOMEM, "out of host memory");
-    int rc = make_params(cam, st, q->rp);
+    int rc = make_params(cam, st, q->rp, q->cam);
     if (rc) {
         delete q;
         return rc;
     }
     q->ren = renderer;
-    q->cam.resize(cam->xform_count);
-    for (uint32_t i = 0; i < cam->xform_count; i++) std::memcpy(q->cam[i].r, cam->xforms + 12ull * i, 48);
     q->total = (uint64_t)q->rp.rw * q->rp.rh * st->spp;
     q->film_bytes = (uint64_t)st->width * st->height * 3 * sizeof(float);
     q->t_submit = Clock::now();

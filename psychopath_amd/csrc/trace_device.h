@@ -6,6 +6,7 @@
 
 #include "../../include/psy_rt.h"
 #include "layout.h"
+#include "light_samples.h"
 #include "psymath.h"
 
 namespace psy {
@@ -140,11 +141,11 @@ __device__ __forceinline__ Xf load_xf(const GpuXform* x) {
 // quadratic in object space with the object-space origin `lo` and direction
 // `ld`.  Returns true on a hit.
 __device__ __forceinline__ bool light_leaf_test(const GpuRectLight* rects, const GpuSphereLight* spheres,
-                                                const GpuInstance& inst, const Xf& x, V3 o, const TriPre& pre,
-                                                V3 lo, V3 ld, float tmax, float& t_out, float& b0, float& b1,
-                                                float& b2, uint32_t& prim) {
+                                                const float* light_samples, float time, const GpuInstance& inst,
+                                                const Xf& x, V3 o, const TriPre& pre, V3 lo, V3 ld, float tmax,
+                                                float& t_out, float& b0, float& b1, float& b2, uint32_t& prim) {
     if (inst.kind == PSY_OBJECT_RECT_LIGHT) {
-        const GpuRectLight L = rects[inst.mesh];
+        const GpuRectLight L = rect_at(rects[inst.mesh], light_samples, time);
         const Xf inv = xf_inverse(x);
         const float hw = L.w * 0.5f, hh = L.h * 0.5f, nw = L.w * -0.5f, nh = L.h * -0.5f;
         const V3 p1 = xf_point(inv, v3(hw, hh, 0.0f));
@@ -162,7 +163,7 @@ __device__ __forceinline__ bool light_leaf_test(const GpuRectLight* rects, const
         }
         return false;
     }
-    const float r = spheres[inst.mesh].radius;
+    const float r = sphere_at(spheres[inst.mesh], light_samples, time).radius;
     const float qa = dot(ld, ld);
     const float qb = 2.0f * dot(ld, lo);
     const float qc = dot(lo, lo) - (r * r);

@@ -142,8 +142,8 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
             const Xf x = instance_xf(a.xforms, inst, time);
             float lt, l0, l1, l2;
             uint32_t lp;
-            if (light_leaf_test(a.rect_lights, a.sphere_lights, inst, x, o, pre, xf_point(x, o), xf_vector(x, d),
-                                tmax, lt, l0, l1, l2, lp)) {
+            if (light_leaf_test(a.rect_lights, a.sphere_lights, a.light_samples, time, inst, x, o, pre,
+                                xf_point(x, o), xf_vector(x, d), tmax, lt, l0, l1, l2, lp)) {
                 if (occl) {
                     h.kind = PSY_HIT_OCCLUDE;
                     break;

@@ -8,8 +8,9 @@
 //   * lanes whose ray finished are refilled in bulk once at least
 //     `refill_min` lanes of the wave are idle (one atomic per refill, the
 //     new rays spread over the idle lanes by a ballot prefix count);
-//   * the traversal is "while-while": lanes descend internal nodes until the
-//     whole wave stands on leaves, then run the leaf tests together;
+//   * the traversal is "while-while": lanes descend internal nodes (popping
+//     past dead ends) until the whole wave stands on leaves, then run the
+//     leaf tests together;
 //   * kernels are specialised on whether the scene has instance transforms
 //     (XF) and time-sampled bounds or vertices (MOTION), so the common static
 //     case keeps no transform or lerp state in registers.
@@ -55,7 +56,7 @@ constexpr int WAVE = 64;
 
 template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
-void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min) {
+void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
     const int lane = threadIdx.x;
     uint32_t* const lds_col = lds_stack + lane;
@@ -126,8 +127,22 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         if (__ballot(has) == 0) break;
         if (!has) continue;
 
-        // ---- descend internal nodes until this lane stands on a leaf ------
-        while (cur != REF_EMPTY && !(cur & REF_LEAF)) {
+        // ---- descend internal nodes ---------------------------------------
+        // A node whose children all miss pops the next entry inside the loop,
+        // so the lane keeps working while others descend.  The loop runs while
+        // some lane stands on an internal node and fewer than `leaf_min`
+        // lanes wait (at a leaf, or at the end of their level: the instance's
+        // subtree or the whole stack); lanes still on internal nodes when it
+        // stops carry on after the leaf round.  Every lane takes its own
+        // depth-first steps in the same order either way; only their timing
+        // changes, so results do not depend on leaf_min.
+        const uint32_t n_active = (uint32_t)__popcll(__ballot(1));
+        for (;;) {
+            if (cur == REF_EMPTY && sp > (in_inst ? inst_base : 0)) SSTACK_POP(cur);
+            const bool inner = cur != REF_EMPTY && !(cur & REF_LEAF);
+            const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
+            if (n_inner == 0 || n_active - n_inner >= leaf_min) break;
+            if (!inner) continue;
             const GpuNode* node = a.nodes + cur;
             const float4* nf = reinterpret_cast<const float4*>(node);
             const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
@@ -175,7 +190,7 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
 
         // ---- leaf --------------------------------------------------------
         bool finished = false;
-        if (cur != REF_EMPTY) {
+        if (cur != REF_EMPTY && (cur & REF_LEAF)) {
             const uint32_t first = (cur & ~REF_LEAF) >> 2;
             const uint32_t count = cur & 3u;
             if (!in_inst && XF && a.instances[first].kind != PSY_OBJECT_MESH) {
@@ -188,8 +203,8 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                 const V3 lld = xf_vector(x, d);
                 float lt, l0, l1, l2;
                 uint32_t lp;
-                if (light_leaf_test(a.rect_lights, a.sphere_lights, inst, x, o, pre, llo, lld, tmax, lt, l0, l1, l2,
-                                    lp)) {
+                if (light_leaf_test(a.rect_lights, a.sphere_lights, a.light_samples, time, inst, x, o, pre, llo, lld,
+                                    tmax, lt, l0, l1, l2, lp)) {
                     if (occl) {
                         kind = PSY_HIT_OCCLUDE;
                         finished = true;
@@ -335,10 +350,10 @@ hipError_t launch_one(const TraceArgs& a, unsigned int* work, const TraceTuning&
     // compiler keeps the rest in private memory), 4 lets it use up to 128.
     if (!COUNT && tune.occupancy >= 8)
         hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8>), dim3(tune.waves), dim3(WAVE), 0, s, a,
-                           work, tune.refill_min);
+                           work, tune.refill_min, tune.leaf_min);
     else
         hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4>), dim3(tune.waves), dim3(WAVE), 0, s, a,
-                           work, tune.refill_min);
+                           work, tune.refill_min, tune.leaf_min);
     return hipGetLastError();
 }
 
@@ -370,7 +385,8 @@ __global__ void reset_work_kernel(unsigned int* work) { *work = 0u; }
 hipError_t launch_trace_persistent(const TraceArgs& args, bool counted, bool has_xf, bool has_motion,
                                    unsigned int* work, const TraceTuning& tune, hipStream_t stream) {
     if (args.n == 0) return hipSuccess;
-    if (args.n >= 0x80000000ull || tune.waves == 0 || tune.refill_min == 0 || tune.refill_min > 64)
+    if (args.n >= 0x80000000ull || tune.waves == 0 || tune.refill_min == 0 || tune.refill_min > 64 ||
+        tune.leaf_min == 0 || tune.leaf_min > 64)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(1), 0, stream, work);
     hipError_t e = hipGetLastError();

@@ -60,6 +60,11 @@ def _color(c: _lib.Color) -> tuple:
     return (int(c.kind), float(c.a), float(c.b), float(c.c))
 
 
+def _colours(n, ptr, one) -> tuple | list:
+    """One colour as a tuple, time samples as a list of tuples."""
+    return [_color(ptr[i]) for i in range(n)] if n else _color(one)
+
+
 def _array(ptr, n, dtype):
     if n == 0:
         return np.zeros(0, dtype)
@@ -103,10 +108,16 @@ def _scene_from(fs: _lib.FileScene) -> RenderScene:
             sc.meshes.append((v, t))
     for i in range(d.rect_light_count):
         r = d.rect_lights[i]
-        sc.rect_lights.append((float(r.width), float(r.height), _color(r.color)))
+        if r.dims_count:
+            wh = _array(r.dims, 2 * r.dims_count, np.float32).reshape(-1, 2)
+            w, h = [float(x) for x in wh[:, 0]], [float(x) for x in wh[:, 1]]
+        else:
+            w, h = float(r.width), float(r.height)
+        sc.rect_lights.append((w, h, _colours(r.color_count, r.colors, r.color)))
     for i in range(d.sphere_light_count):
         s = d.sphere_lights[i]
-        sc.sphere_lights.append((float(s.radius), _color(s.color)))
+        rad = [float(x) for x in _array(s.radii, s.radius_count, np.float32)] if s.radius_count else float(s.radius)
+        sc.sphere_lights.append((rad, _colours(s.color_count, s.colors, s.color)))
     xf = _array(d.xforms, 12 * d.xform_count, np.float32).reshape(d.xform_count, 12)
     kinds = {_lib.PSY_OBJECT_MESH: "mesh", _lib.PSY_OBJECT_RECT_LIGHT: "rect", _lib.PSY_OBJECT_SPHERE_LIGHT: "sphere"}
     for i in range(d.instance_count):
@@ -119,11 +130,26 @@ def _scene_from(fs: _lib.FileScene) -> RenderScene:
     sc.shaders = [(int(w.shaders[i].kind), _color(w.shaders[i].color), float(w.shaders[i].roughness),
                    float(w.shaders[i].fresnel)) for i in range(w.shader_count)]
     sc.background = _color(w.background)
-    sc.distant_lights = [(float(w.distant_lights[i].radius), tuple(float(x) for x in w.distant_lights[i].direction),
-                          _color(w.distant_lights[i].color)) for i in range(w.distant_light_count)]
+    sc.distant_lights = []
+    for i in range(w.distant_light_count):
+        dl = w.distant_lights[i]
+        rad = [float(x) for x in _array(dl.radii, dl.radius_count, np.float32)] if dl.radius_count else float(dl.radius)
+        if dl.direction_count:
+            dirs = [tuple(float(x) for x in v) for v in _array(dl.directions, 3 * dl.direction_count,
+                                                                np.float32).reshape(-1, 3)]
+        else:
+            dirs = tuple(float(x) for x in dl.direction)
+        sc.distant_lights.append((rad, dirs, _colours(dl.color_count, dl.colors, dl.color)))
     cam = fs.camera
+
+    def lens(n, p, scalar):
+        # One sample as a number, time samples as a list.
+        return float(scalar) if n <= 1 else [float(v) for v in _array(p, n, np.float32)]
+
     sc.camera = dict(xforms=_array(cam.xforms, 12 * cam.xform_count, np.float32).reshape(cam.xform_count, 12),
-                     fov=float(cam.fov), aperture=float(cam.aperture_radius), focus=float(cam.focus_distance))
+                     fov=lens(cam.fov_count, cam.fovs, cam.fov),
+                     aperture=lens(cam.aperture_count, cam.aperture_radii, cam.aperture_radius),
+                     focus=lens(cam.focus_count, cam.focus_distances, cam.focus_distance))
     st = fs.settings
     sc.settings = dict(width=int(st.width), height=int(st.height), spp=int(st.spp), max_bounces=int(st.max_bounces),
                        seed=int(st.seed))
@@ -181,19 +207,24 @@ def write(sc: RenderScene, name: str = "Scene", output: str = "render.png") -> s
                f"        Seed [{st['seed']}]\n    }}\n")
     cam = sc.camera
     out.append("    Camera {\n")
-    out.append(f"        Fov [{_g(math.degrees(float(cam['fov'])))}]\n")
-    out.append(f"        FocalDistance [{_g(cam.get('focus', 1.0))}]\n")
-    out.append(f"        ApertureRadius [{_g(cam.get('aperture', 0.0))}]\n")
+    for v in np.atleast_1d(cam["fov"]):
+        out.append(f"        Fov [{_g(math.degrees(float(v)))}]\n")
+    for v in np.atleast_1d(cam.get("focus", 1.0)):
+        out.append(f"        FocalDistance [{_g(v)}]\n")
+    for v in np.atleast_1d(cam.get("aperture", 0.0)):
+        out.append(f"        ApertureRadius [{_g(v)}]\n")
     for xf in np.asarray(cam["xforms"], np.float32).reshape(-1, 12):
         out.append(f"        Transform [{_matrix(xf)}]\n")
     out.append("    }\n")
     out.append("    World {\n        BackgroundShader {\n            Type [Color]\n"
                f"            Color [{_color_text(sc.background)}]\n        }}\n")
     for radius, direction, col in sc.distant_lights:
-        out.append("        DistantDiskLight {\n"
-                   f"            Radius [{_g(radius)}]\n"
-                   f"            Direction [{' '.join(_g(v) for v in direction)}]\n"
-                   f"            Color [{_color_text(col)}]\n        }}\n")
+        out.append("        DistantDiskLight {\n")
+        out += [f"            Radius [{_g(r)}]\n" for r in np.atleast_1d(radius)]
+        dirs = direction if isinstance(direction, list) else [direction]
+        out += [f"            Direction [{' '.join(_g(v) for v in d)}]\n" for d in dirs]
+        out += [f"            Color [{_color_text(c)}]\n" for c in (col if isinstance(col, list) else [col])]
+        out.append("        }\n")
     out.append("    }\n    Assembly {\n")
     for i, (kind, col, rough, fres) in enumerate(sc.shaders):
         t = {SHADER_LAMBERT: "Lambert", SHADER_GGX: "GGX", SHADER_EMIT: "Emit"}[kind]
@@ -217,11 +248,15 @@ def write(sc: RenderScene, name: str = "Scene", output: str = "render.png") -> s
         out.append("            FaceVertCounts [" + " ".join("3" for _ in range(len(t))) + "]\n")
         out.append("            FaceVertIndices [" + " ".join(str(int(x)) for x in t.ravel()) + "]\n        }\n")
     for i, (w, h, col) in enumerate(sc.rect_lights):
-        out.append(f"        RectangleLight $rect{i} {{\n            Dimensions [{_g(w)} {_g(h)}]\n"
-                   f"            Color [{_color_text(col)}]\n        }}\n")
+        out.append(f"        RectangleLight $rect{i} {{\n")
+        out += [f"            Dimensions [{_g(a)} {_g(b)}]\n" for a, b in zip(np.atleast_1d(w), np.atleast_1d(h))]
+        out += [f"            Color [{_color_text(c)}]\n" for c in (col if isinstance(col, list) else [col])]
+        out.append("        }\n")
     for i, (r, col) in enumerate(sc.sphere_lights):
-        out.append(f"        SphereLight $sphere{i} {{\n            Radius [{_g(r)}]\n"
-                   f"            Color [{_color_text(col)}]\n        }}\n")
+        out.append(f"        SphereLight $sphere{i} {{\n")
+        out += [f"            Radius [{_g(x)}]\n" for x in np.atleast_1d(r)]
+        out += [f"            Color [{_color_text(c)}]\n" for c in (col if isinstance(col, list) else [col])]
+        out.append("        }\n")
     for inst in sc.instances:
         data = {"mesh": "mesh", "rect": "rect", "sphere": "sphere"}[inst["kind"]] + str(inst["index"])
         out.append(f"        Instance {{\n            Data [${data}]\n")

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .tracer import Assembly, Mesh, RectLight, SphereLight, color_tuple
+from .tracer import Assembly, Mesh, RectLight, SphereLight, color_tuple, colour_samples, time_samples
 
 
 def _color(c) -> _lib.Color:
@@ -35,8 +35,15 @@ class Renderer:
         for i, (kind, col, rough, fres) in enumerate(shaders):
             sh[i] = _lib.ShaderDesc(int(kind), _color(col), float(rough), float(fres))
         dl = (_lib.DistantLightDesc * max(1, len(distant_lights)))()
+        keep = []
         for i, (radius, direction, col) in enumerate(distant_lights):
-            dl[i] = _lib.DistantLightDesc(float(radius), (C.c_float * 3)(*[float(x) for x in direction]), _color(col))
+            # Numbers and tuples are one sample; lists are time samples.
+            r0, nr, ra = time_samples(radius, 1)
+            d0, nd, da = time_samples(direction, 3)
+            c0, nc, ca = colour_samples(col)
+            dl[i] = _lib.DistantLightDesc(float(r0[0]), (C.c_float * 3)(*[float(x) for x in d0]), c0, nr,
+                                          ra.ctypes.data_as(_lib._fp), nd, da.ctypes.data_as(_lib._fp), nc, ca)
+            keep += [ra, da, ca]
         world = _lib.WorldDesc(len(shaders), sh, _color(background), len(distant_lights), dl)
         h = C.c_void_p()
         check("psy_renderer_create", lib().psy_renderer_create(scene.handle, C.byref(world), int(max_paths),
@@ -83,10 +90,19 @@ class Renderer:
 
     @staticmethod
     def _camera(camera: dict):
+        """The camera descriptor and the arrays it points into.  `fov`,
+        `aperture` and `focus` are numbers or lists of time samples."""
         xf = np.ascontiguousarray(camera["xforms"], np.float32).reshape(-1, 12)
-        cam = _lib.CameraDesc(xf.shape[0], xf.ctypes.data_as(_lib._fp), float(camera["fov"]),
-                              float(camera.get("aperture", 0.0)), float(camera.get("focus", 1.0)))
-        return cam, xf
+        lens = [np.ascontiguousarray(np.atleast_1d(camera.get(k, d)), np.float32).ravel()
+                for k, d in (("fov", None), ("aperture", 0.0), ("focus", 1.0))]
+        cam = _lib.CameraDesc(xf.shape[0], xf.ctypes.data_as(_lib._fp), float(lens[0][0]), float(lens[1][0]),
+                              float(lens[2][0]))
+        for name, a in zip(("fov", "aperture", "focus"), lens):
+            if a.size > 1:
+                field = {"fov": "fovs", "aperture": "aperture_radii", "focus": "focus_distances"}[name]
+                setattr(cam, name + "_count", a.size)
+                setattr(cam, field, a.ctypes.data_as(_lib._fp))
+        return cam, (xf, lens)
 
     @staticmethod
     def _settings(settings: dict):
@@ -98,7 +114,8 @@ class Renderer:
 
     def render(self, camera: dict, settings: dict, device_out=None) -> tuple[np.ndarray | None, dict]:
         """Renders one image.  `camera`: xforms [N,12] (camera to world), fov
-        (radians), aperture, focus.  `settings`: width, height, spp,
+        (radians), aperture, focus (each a number or a list of time
+        samples).  `settings`: width, height, spp,
         max_bounces, seed.  With `device_out` (a device pointer of
         width*height*3 floats) the image stays on the device."""
         cam, _xf = self._camera(camera)
@@ -120,9 +137,9 @@ def build(rs, max_paths: int = 0) -> Renderer:
     for m in rs.meshes:
         asm.add_mesh(Mesh(m[0], m[1], m[2] if len(m) > 2 else None))
     for (w, h, col) in rs.rect_lights:
-        asm.add_rect_light(RectLight(float(w), float(h), col))
+        asm.add_rect_light(RectLight(w, h, col))
     for (r, col) in rs.sphere_lights:
-        asm.add_sphere_light(SphereLight(float(r), col))
+        asm.add_sphere_light(SphereLight(r, col))
     for inst in rs.instances:
         kind = {"mesh": _lib.PSY_OBJECT_MESH, "rect": _lib.PSY_OBJECT_RECT_LIGHT,
                 "sphere": _lib.PSY_OBJECT_SPHERE_LIGHT}[inst["kind"]]

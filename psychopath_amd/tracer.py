@@ -60,9 +60,28 @@ def color_tuple(c) -> tuple:
     return (int(k), float(a), float(b), float(cc))
 
 
+def time_samples(v, width: int):
+    """A light parameter as (first value, count, array): a number or tuple
+    is one sample (count 0, as the descriptors take it), a list holds time
+    samples of `width` floats each."""
+    a = np.ascontiguousarray(np.asarray(v, np.float32).reshape(-1, width))
+    return a[0], (a.shape[0] if a.shape[0] > 1 else 0), a
+
+
+def colour_samples(c):
+    """A colour or a list of colour time samples as (first, count,
+    Color array or None)."""
+    cs = [color_tuple(x) for x in c] if isinstance(c, list) else [color_tuple(c)]
+    if len(cs) == 1:
+        return _lib.Color(*cs[0]), 0, None
+    return _lib.Color(*cs[0]), len(cs), (_lib.Color * len(cs))(*[_lib.Color(*x) for x in cs])
+
+
 @dataclass
 class RectLight:
-    """RectangleLight (src/light/rectangle_light.rs:24-48)."""
+    """RectangleLight (src/light/rectangle_light.rs:24-48).  `width` and
+    `height` may be equal-length lists of time samples and `color` a list of
+    colours."""
     width: float
     height: float
     color: tuple
@@ -70,7 +89,8 @@ class RectLight:
 
 @dataclass
 class SphereLight:
-    """SphereLight (src/light/sphere_light.rs:27-48)."""
+    """SphereLight (src/light/sphere_light.rs:27-48).  `radius` and `color`
+    may be lists of time samples."""
     radius: float
     color: tuple
 
@@ -156,10 +176,18 @@ class Scene:
         self._keep.append(xfa)
         rls = (_lib.RectLightDesc * max(1, len(asm.rect_lights)))()
         for i, l in enumerate(asm.rect_lights):
-            rls[i] = _lib.RectLightDesc(l.width, l.height, _lib.Color(*color_tuple(l.color)))
+            wh = np.stack([np.atleast_1d(np.asarray(l.width, np.float32)),
+                           np.atleast_1d(np.asarray(l.height, np.float32))], axis=1)
+            d0, nd, da = time_samples(wh, 2)
+            c0, nc, ca = colour_samples(l.color)
+            rls[i] = _lib.RectLightDesc(float(d0[0]), float(d0[1]), c0, nd, da.ctypes.data_as(_lib._fp), nc, ca)
+            self._keep += [da, ca]
         sls = (_lib.SphereLightDesc * max(1, len(asm.sphere_lights)))()
         for i, l in enumerate(asm.sphere_lights):
-            sls[i] = _lib.SphereLightDesc(l.radius, _lib.Color(*color_tuple(l.color)))
+            r0, nr, ra = time_samples(l.radius, 1)
+            c0, nc, ca = colour_samples(l.color)
+            sls[i] = _lib.SphereLightDesc(float(r0[0]), c0, nr, ra.ctypes.data_as(_lib._fp), nc, ca)
+            self._keep += [ra, ca]
         self._keep += [rls, sls]
         desc = _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp),
                               len(asm.rect_lights), rls, len(asm.sphere_lights), sls)
@@ -363,10 +391,10 @@ class Tracer:
         return {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
 
     def set_tuning(self, kernel=None, waves_per_cu=None, refill_min=None, stack_lds=None,
-                   simd_waves=None) -> None:
+                   simd_waves=None, leaf_min=None) -> None:
         cur = self.tuning()
         for k, v in dict(kernel=kernel, waves_per_cu=waves_per_cu, refill_min=refill_min,
-                         stack_lds=stack_lds, simd_waves=simd_waves).items():
+                         stack_lds=stack_lds, simd_waves=simd_waves, leaf_min=leaf_min).items():
             if v is not None:
                 cur[k] = v
         t = _lib.TraceTuning(**cur)

@@ -114,7 +114,7 @@ int64_t rc_run(int which, int64_t n, uint64_t seed, int64_t* first) {
             lo.w = uf(0.05f, 3.0f);
             lo.h = uf(0.05f, 3.0f);
             lo.color = rcolor_o(0, ca * 10, cb * 10, cc * 10);
-            psy::GpuRectLight lp{lo.w, lo.h, {0, 0}, rcolor_p(0, ca * 10, cb * 10, cc * 10)};
+            psy::GpuRectLight lp{lo.w, lo.h, 0u, 0u, rcolor_p(0, ca * 10, cb * 10, cc * 10)};
             const psyref::Vec3 arr = rvec(urand() < 0.5 ? 2.0f : 40.0f);
             const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
             psyref::LightSample so = lo.sample_from_point(rxf_o(xv), arr, u, v, wl);
@@ -130,7 +130,7 @@ int64_t rc_run(int which, int64_t n, uint64_t seed, int64_t* first) {
             psyref::SphereLight lo;
             lo.radius = uf(0.05f, 1.0f);
             lo.color = rcolor_o(2, uf(2500.0f, 9000.0f), 40.0f, 0.0f);
-            psy::GpuSphereLight lp{lo.radius, {0, 0, 0}, rcolor_p(2, lo.color.a, 40.0f, 0.0f)};
+            psy::GpuSphereLight lp{lo.radius, 0u, 0u, 0.0f, rcolor_p(2, lo.color.a, 40.0f, 0.0f)};
             const psyref::Vec3 arr = rvec(urand() < 0.2 ? 0.3f : 5.0f);
             const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
             psyref::LightSample so = lo.sample_from_point(rxf_o(xv), arr, u, v, wl);

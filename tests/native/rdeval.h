@@ -59,7 +59,7 @@ RD void eval(int which, const float* in, float* o) {
     } else if (which == 2) {
         o[0] = closure_estimate_sphere_light(clo(in), in3(in + 6), in3(in + 9), in[12], in3(in + 13), in3(in + 16));
     } else if (which == 3) {
-        GpuRectLight L{in[0], in[1], {0.0f, 0.0f}, GpuColor{PSY_COLOR_XYZ, in[2], in[3], in[4]}};
+        GpuRectLight L{in[0], in[1], 0u, 0u, GpuColor{PSY_COLOR_XYZ, in[2], in[3], in[4]}};
         const Xf x = xf_from_values(in + 5);
         const LightSample s = rect_sample_from_point(L, x, in3(in + 17), in[20], in[21], in[22]);
         out4(o, s.color);
@@ -69,7 +69,7 @@ RD void eval(int which, const float* in, float* o) {
         o[11] = s.pdf;
         o[12] = rect_sample_pdf(L, x, in3(in + 17), in3(in + 23), in3(in + 26));
     } else if (which == 4) {
-        GpuSphereLight L{in[0], {0.0f, 0.0f, 0.0f}, GpuColor{PSY_COLOR_TEMPERATURE, in[1], 40.0f, 0.0f}};
+        GpuSphereLight L{in[0], 0u, 0u, 0.0f, GpuColor{PSY_COLOR_TEMPERATURE, in[1], 40.0f, 0.0f}};
         const Xf x = xf_from_values(in + 2);
         const LightSample s = sphere_sample_from_point(L, x, in3(in + 14), in[17], in[18], in[19]);
         out4(o, s.color);

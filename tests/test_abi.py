@@ -148,9 +148,27 @@ def test_light_objects_validate():
     rc, msg = light_scene([_lib.RectLightDesc(0.0, 1.0, white)], [], [rect_inst], ident)
     assert rc == -1 and "bad size" in msg
     rc, msg = light_scene([], [_lib.SphereLightDesc(1.0, _lib.Color(7, 1, 1, 1))], [sph_inst], ident)
-    assert rc == -1 and "bad radius or colour" in msg
+    assert rc == -1 and "bad colour" in msg
     rc, msg = light_scene([rect], [], [_lib.InstanceDesc(0, _lib.PSY_NONE, 0, 1, 9)], ident)
     assert rc == -1 and "unknown object kind" in msg
+    # Time samples: sizes must be positive, colours of one representation,
+    # lists at most 64 long and not NULL.
+    radii = np.array([0.5, 0.0], np.float32)
+    rc, msg = light_scene([], [_lib.SphereLightDesc(1.0, white, 2, radii.ctypes.data_as(_lib._fp))], [sph_inst], ident)
+    assert rc == -1 and "bad radius" in msg
+    mixed = (_lib.Color * 2)(white, _lib.Color(_lib.PSY_COLOR_BLACKBODY, 5000.0, 1.0, 0.0))
+    rc, msg = light_scene([_lib.RectLightDesc(1.0, 1.0, white, 0, None, 2, mixed)], [], [rect_inst], ident)
+    assert rc == -1 and "different representations" in msg
+    rc, msg = light_scene([_lib.RectLightDesc(1.0, 1.0, white, 2, None)], [], [rect_inst], ident)
+    assert rc == -1 and "NULL" in msg
+    many = np.ones(65 * 2, np.float32)
+    rc, msg = light_scene([_lib.RectLightDesc(1.0, 1.0, white, 65, many.ctypes.data_as(_lib._fp))], [], [rect_inst],
+                          ident)
+    assert rc == -2 and "too many" in msg
+    dims = np.array([1.0, 2.0, 3.0, 0.5], np.float32)
+    ok = _lib.RectLightDesc(9.0, 9.0, white, 2, dims.ctypes.data_as(_lib._fp), 2,
+                            (_lib.Color * 2)(white, _lib.Color(_lib.PSY_COLOR_XYZ, 2.0, 2.0, 2.0)))
+    assert light_scene([ok], [], [rect_inst], ident)[0] == 0
 
 
 def test_null_renderer_calls_fail_cleanly():

@@ -363,3 +363,46 @@ def test_nested_assemblies_render_like_the_oracle():
     print(f"nested cornell 48x36x8: {frac * 100:.3f}% pixels within {REL}; rays {stats['rays']} vs {info['rays']}",
           flush=True)
     assert frac >= FRACTION
+
+
+def test_camera_lens_time_samples_match_oracle():
+    """Field of view, aperture and focus distance as time samples of
+    different lengths (Camera::generate_ray lerps each list on its own,
+    camera.rs:74-79): the wavefront, the one-launch form and a served
+    request agree with the oracle."""
+    sc = scenes.cornell_box(64, 48, 8, 3, 5)
+    f = sc.camera["fov"]
+    sc.camera = dict(sc.camera, fov=[f * 0.8, f, f * 1.3], aperture=[0.0, 0.35], focus=[7.0, 9.0, 8.0, 10.5])
+    img, ref = check_parity(sc, "cornell 64x48x8 lens time samples")
+    r = render.build(sc)
+    r.set_options(plain=True)
+    plain, _ = r.render(sc.camera, sc.settings)
+    assert np.array_equal(plain, img)
+    sv = render.Server([r])
+    served, _ = sv.wait(sv.submit(0, sc.camera, sc.settings))
+    sv.close()
+    assert np.array_equal(served, img)
+    # One sample in a list is the scalar.
+    one = dict(sc.camera, fov=[f], aperture=[0.0], focus=[1.0])
+    a, _ = r.render(one, sc.settings)
+    b, _ = r.render(dict(sc.camera, fov=f, aperture=0.0, focus=1.0), sc.settings)
+    assert np.array_equal(a, b)
+
+
+def test_time_sampled_lights_match_oracle():
+    """Light sizes, colours and directions as time samples (PsychoBlend
+    writes one per motion-blur step): a Cornell box whose light grows and
+    changes colour over the shutter, with a distant light that turns, and
+    the motion workload with pulsing blackbody spheres, against the oracle."""
+    sc = scenes.cornell_box(64, 48, 8, 3, 2)
+    w, h, c = sc.rect_lights[0]
+    sc.rect_lights[0] = ([w, 2 * w, 1.5 * w], [h, 0.5 * h, h], [c, (c[0], 0.5 * c[1], 1.5 * c[2], c[3])])
+    sc.distant_lights.append(([0.05, 0.1], [(-0.4, 0.3, -0.6), (0.4, 0.3, -0.6)],
+                              [(2, 5800.0, 3.0, 0.0), (2, 7000.0, 5.0, 0.0)]))
+    check_parity(sc, "cornell 64x48x8 time-sampled lights")
+    mo = scenes.motion_instances(grid=3, levels=3, lights=3, width=64, height=36, spp=8, max_bounces=4)
+    r0, c0 = mo.sphere_lights[0]
+    mo.sphere_lights[0] = ([r0, 2.5 * r0], [c0, (c0[0], c0[1] * 1.3, c0[2] * 0.4, c0[3])])
+    r1, c1 = mo.sphere_lights[1]
+    mo.sphere_lights[1] = (r1, [c1, (c1[0], c1[1] * 0.7, c1[2] * 2.0, c1[3]), c1])
+    check_parity(mo, "motion 64x36x8 time-sampled sphere lights")

@@ -278,14 +278,14 @@ def test_launch_shapes_give_identical_hits(small, instanced):
     for tracer, rays in ((tr, r), (tri, ri)):
         base = None
         for shape in ((1, 1, 1, 8, 4), (0, 1, 1, 8, 4), (0, 8, 16, 16, 8), (0, 16, 64, 24, 4), (0, 32, 32, 8, 8),
-                      (0, 4, 2, 8, 8)):
+                      (0, 4, 2, 8, 8), (0, 16, 16, 16, 4, 1), (0, 24, 32, 24, 8, 8), (0, 8, 8, 8, 4, 32)):
             tracer.set_tuning(*shape)
             g = gpu_trace(tracer, rays)
             if base is None:
                 base = g
             else:
                 assert_exact(g, base, f"shape {shape}")
-        tracer.set_tuning(0, 16, 16, 16, 4)
+        tracer.set_tuning(0, 16, 16, 16, 4, 0)
 
 
 def test_deep_stack_spills_to_private_memory():
@@ -303,7 +303,7 @@ def test_deep_stack_spills_to_private_memory():
     r = scenes.incoherent_rays(np.array([-0.5, -0.5, -1], np.float32), np.array([1.5, 1.5, 5], np.float32), 20_000,
                                seeds=(161, 162, 163))
     want = o.trace(r)
-    for shape in ((0, 8, 8, 8, 8), (0, 8, 8, 8, 4), (1, 1, 1, 8, 4)):
+    for shape in ((0, 8, 8, 8, 8), (0, 8, 8, 8, 4), (1, 1, 1, 8, 4), (0, 8, 8, 8, 4, 4)):
         tr.set_tuning(*shape)
         assert_exact(gpu_trace(tr, r), want, f"deep {shape}")
 
@@ -317,6 +317,49 @@ def test_autotune_picks_a_valid_shape(small):
     best = tr.autotune(rays, hits)
     assert best == tr.tuning()
     assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24) and best["simd_waves"] in (4, 8)
+    assert 1 <= best["leaf_min"] <= 64
     assert tr.rays_traced() == before
     assert_exact(gpu_trace(tr, r), o.trace(r), "after autotune")
-    tr.set_tuning(0, 16, 16, 16, 4)
+    tr.set_tuning(0, 16, 16, 16, 4, 0)
+
+
+def test_time_sampled_light_sizes_match_oracle():
+    """Sphere radii and rectangle dimensions as time samples: the trace
+    kernels lerp them at each ray's time like the oracle, on rays grazing the
+    light at every time between the samples, bit for bit."""
+    from types import SimpleNamespace
+    from oracle import psyref
+    from psychopath_amd.tracer import RectLight, SphereLight
+    import psychopath_amd._lib as _lib
+    white = (0, 1.0, 1.0, 1.0)
+    ident = np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]], np.float32)
+    n = 4096
+    times = (np.arange(n, dtype=np.float32) + np.float32(0.5)) / np.float32(n)
+    offs = np.linspace(0.0, 3.5, n, dtype=np.float32)[::-1].copy()
+    for kind, light in (("sphere", SphereLight([1.0, 3.0, 2.0], [white, (0, 2.0, 2.0, 2.0), white])),
+                        ("rect", RectLight([1.0, 3.0, 2.0], [1.0, 2.0, 4.0], [white, (0, 3.0, 1.0, 1.0)]))):
+        asm = Assembly()
+        if kind == "sphere":
+            asm.add_sphere_light(light)
+            k = _lib.PSY_OBJECT_SPHERE_LIGHT
+            r = dict(ox=np.full(n, -10.0, np.float32), oy=offs, oz=np.zeros(n, np.float32),
+                     dx=np.ones(n, np.float32), dy=np.zeros(n, np.float32), dz=np.zeros(n, np.float32))
+            ns = SimpleNamespace(meshes=[], rect_lights=[], sphere_lights=[(light.radius, light.color)],
+                                 instances=[dict(kind="sphere", index=0, shader=None, xforms=ident)])
+        else:
+            asm.add_rect_light(light)
+            k = _lib.PSY_OBJECT_RECT_LIGHT
+            r = dict(ox=offs * np.float32(0.5), oy=offs * np.float32(0.7), oz=np.full(n, 5.0, np.float32),
+                     dx=np.zeros(n, np.float32), dy=np.zeros(n, np.float32), dz=np.full(n, -1.0, np.float32))
+            ns = SimpleNamespace(meshes=[], rect_lights=[(light.width, light.height, light.color)], sphere_lights=[],
+                                 instances=[dict(kind="rect", index=0, shader=None, xforms=ident)])
+        asm.add_instance(0, ident, _lib.PSY_NONE, k)
+        r["time"] = times
+        r["max_t"] = np.full(n, np.inf, np.float32)
+        tr = Tracer(asm.build(), max_batch=n)
+        want = psyref._assembly(ns).trace(r)
+        got = gpu_trace(tr, r)
+        hits = int((want["kind"] != 0).sum())
+        print(f"{kind} light with time samples: {hits} of {n} rays hit", flush=True)
+        assert 0.1 * n < hits < 0.9 * n
+        assert_exact(got, want, f"time-sampled {kind} light")

@@ -303,3 +303,157 @@ def test_cornell_render_is_deterministic_and_threads_agree():
     sc2 = scenes.cornell_box(32, 32, 4, 4, 1)
     c, _ = psyref.render(sc2, threads=4)
     assert not np.array_equal(a, c)
+
+
+# ---- camera (camera.rs:22-102) -----------------------------------------------
+_ID = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+
+
+def _rad(deg):
+    return np.float32(np.radians(deg))
+
+
+def test_pinhole_direction_is_the_closed_form():
+    """Without a lens every ray leaves the camera origin along
+    normalize(x tan(fov/2), y tan(fov/2), 1), whatever the lens sample."""
+    for deg in (10, 45, 90, 130):
+        t = np.tan(np.float64(_rad(deg)) / 2)
+        for x, y in ((0.3, -0.2), (-1.0, 0.5625), (0.0, 0.0)):
+            for u, v in ((0.7, 0.2), (0.05, 0.95)):
+                o, d = psyref.camera_ray(_ID, [_rad(deg)], [0.0], [1.0], x, y, 0.4, u, v)
+                want = np.array([x * t, y * t, 1.0])
+                assert np.array_equal(o, np.zeros(3, np.float32))
+                assert np.allclose(d, want / np.linalg.norm(want), rtol=0, atol=3e-7), (deg, x, y, d)
+
+
+def test_thin_lens_rays_meet_on_the_focus_plane():
+    """Rays through different points of the lens for one image point cross
+    the plane z = focus at the pinhole ray's point (the thin-lens property);
+    their origins lie on the lens disc of the aperture radius."""
+    f, ap, fd, x, y = _rad(50), 0.4, 6.0, 0.25, -0.4
+    t = np.tan(np.float64(f) / 2)
+    want = np.array([x * t * fd, y * t * fd, fd])
+    for u, v in ((0.1, 0.2), (0.9, 0.5), (0.5, 0.95), (0.3, 0.6), (0.5, 0.5)):
+        o, d = psyref.camera_ray(_ID, [f], [ap], [fd], x, y, 0.3, u, v)
+        assert o[2] == 0 and np.hypot(o[0], o[1]) <= ap * (1 + 1e-6)
+        p = o.astype(np.float64) + d * (fd / np.float64(d[2]))
+        assert np.allclose(p, want, rtol=0, atol=2e-5 * fd), (u, v, p, want)
+    o, _ = psyref.camera_ray(_ID, [f], [ap], [fd], x, y, 0.3, 0.9, 0.5)
+    assert np.hypot(o[0], o[1]) > 0.5 * ap
+
+
+def test_lens_time_samples_interpolate_each_list():
+    """Each lens list is lerp_slice'd over the shutter on its own: the ends of
+    the shutter give the first and last samples, times on a sample give that
+    sample exactly, and the fov is interpolated in its linear form
+    tan(fov/2), not as an angle."""
+    fovs = [_rad(30), _rad(50), _rad(80)]
+    aps, fds = [0.1, 0.5], [3.0, 5.0, 4.0, 9.0]
+    x, y, u, v = 0.4, -0.3, 0.8, 0.3
+
+    def ray(fv, ap, fd, time):
+        return np.concatenate(psyref.camera_ray(_ID, fv, ap, fd, x, y, time, u, v))
+
+    assert np.array_equal(ray(fovs, aps, fds, 0.0), ray(fovs[:1], aps[:1], fds[:1], 0.0))
+    assert np.array_equal(ray(fovs, aps, fds, 1.0), ray(fovs[-1:], aps[-1:], fds[-1:], 1.0))
+    assert np.array_equal(ray(fovs, [0.2], [3.0], 0.5), ray(fovs[1:2], [0.2], [3.0], 0.5))
+    assert np.array_equal(ray([fovs[0]], [0.2], fds, np.float32(1 / 3)), ray([fovs[0]], [0.2], [5.0], np.float32(1 / 3)))
+    assert np.array_equal(ray([fovs[0]], [0.2], fds, np.float32(2 / 3)), ray([fovs[0]], [0.2], [4.0], np.float32(2 / 3)))
+    # Aperture halfway between its samples.
+    a = ray([fovs[0]], aps, [3.0], 0.5)
+    b = ray([fovs[0]], [0.3], [3.0], 0.5)
+    assert np.allclose(a, b, rtol=2e-6, atol=1e-7)
+    # Linear fov at a quarter of the shutter (pinhole: d.x / d.z = x tfov).
+    t0, t1 = np.tan(np.float64(fovs[0]) / 2), np.tan(np.float64(fovs[2]) / 2)
+    d = ray([fovs[0], fovs[2]], [0.0], [1.0], 0.25)[3:]
+    assert abs(d[0] / d[2] / x - (0.75 * t0 + 0.25 * t1)) < 1e-6
+    assert abs(d[0] / d[2] / x - np.tan((0.75 * fovs[0] + 0.25 * fovs[2]) / 2)) > 1e-2
+
+
+def test_zero_or_missing_focus_disables_the_lens():
+    """A zero focus distance among the samples, or an empty aperture or focus
+    list, turns the camera into a pinhole (camera.rs:32-57)."""
+    f = [_rad(40)]
+    pin = np.concatenate(psyref.camera_ray(_ID, f, [0.0], [1.0], 0.2, 0.1, 0.5, 0.9, 0.1))
+    for ap, fd in (([0.5, 0.5], [3.0, 0.0]), ([], [3.0]), ([0.5], [])):
+        got = np.concatenate(psyref.camera_ray(_ID, f, ap, fd, 0.2, 0.1, 0.5, 0.9, 0.1))
+        assert np.array_equal(got, pin), (ap, fd)
+    lens = np.concatenate(psyref.camera_ray(_ID, f, [0.5], [3.0], 0.2, 0.1, 0.5, 0.9, 0.1))
+    assert not np.array_equal(lens, pin)
+
+
+# ---- time-sampled lights (light/*.rs: every list lerped at the ray's time) ----
+def _light_scene(rect=None, sphere=None):
+    from types import SimpleNamespace
+    ns = SimpleNamespace(meshes=[], rect_lights=[], sphere_lights=[], instances=[], sub_assemblies=[])
+    ident = np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]], np.float32)
+    if rect is not None:
+        ns.rect_lights.append(rect)
+        ns.instances.append(dict(kind="rect", index=0, shader=None, xforms=ident))
+    if sphere is not None:
+        ns.sphere_lights.append(sphere)
+        ns.instances.append(dict(kind="sphere", index=0, shader=None, xforms=ident))
+    return psyref._assembly(ns)
+
+
+def _rays(o, d, times):
+    n = len(times)
+    r = {k: np.full(n, v, np.float32) for k, v in zip(("ox", "oy", "oz", "dx", "dy", "dz"), (*o, *d))}
+    r["time"] = np.asarray(times, np.float32)
+    r["max_t"] = np.full(n, np.inf, np.float32)
+    return r
+
+
+LIGHT_TIMES = [0.0, 0.3, 0.45, 0.55, 0.7, 1.0]
+
+
+def test_sphere_light_radius_samples_are_lerped_at_the_ray_time():
+    """Radii [1, 3]: a ray passing 2 from the centre hits once the lerped
+    radius 1 + 2t exceeds 2, at the distance the circle gives."""
+    o = _light_scene(sphere=([1.0, 3.0], (0, 1.0, 1.0, 1.0)))
+    h = o.trace(_rays((-10.0, 2.0, 0.0), (1.0, 0.0, 0.0), LIGHT_TIMES))
+    assert list(h["kind"] != 0) == [t > 0.5 for t in LIGHT_TIMES]
+    for t, kind, dist in zip(LIGHT_TIMES, h["kind"], h["t"]):
+        if kind:
+            r = 1.0 + 2.0 * t
+            assert abs(dist - (10.0 - np.sqrt(r * r - 4.0))) < 1e-4
+
+
+def test_rect_light_dimension_samples_are_lerped_at_the_ray_time():
+    """Widths [1, 3] (height 1): a ray down onto x = 1 hits once the half
+    width (1 + 2t) / 2 exceeds 1; the height samples are lerped separately."""
+    o = _light_scene(rect=([1.0, 3.0], [1.0, 1.0], (0, 1.0, 1.0, 1.0)))
+    h = o.trace(_rays((1.0, 0.0, 5.0), (0.0, 0.0, -1.0), LIGHT_TIMES))
+    assert list(h["kind"] != 0) == [t > 0.5 for t in LIGHT_TIMES]
+    assert np.all(h["t"][h["kind"] != 0] == 5.0)
+    o = _light_scene(rect=([4.0, 4.0], [1.0, 3.0], (0, 1.0, 1.0, 1.0)))
+    h = o.trace(_rays((0.0, 1.0, 5.0), (0.0, 0.0, -1.0), LIGHT_TIMES))
+    assert list(h["kind"] != 0) == [t > 0.5 for t in LIGHT_TIMES]
+
+
+def test_light_colour_samples_are_lerped_at_the_path_time():
+    """A colour list gives its first colour at the shutter's start, its last
+    at the end, a sample exactly on its time, and leans toward the nearer
+    sample between; blackbody lists lerp temperature and factor.  In a
+    render, emission is linear in the colour: ramps 0 -> 2c and 2c -> 0
+    average to c in every pixel, and [2c, 2c] is twice c."""
+    a, b, m = (0, 1.0, 2.0, 3.0), (0, 5.0, 4.0, 1.0), (0, 2.0, 2.0, 2.0)
+    assert psyref.color_lerp_slice([a, b], 0.0) == a and psyref.color_lerp_slice([a, b], 1.0) == b
+    assert psyref.color_lerp_slice([a, m, b], 0.5) == m
+    q = np.array(psyref.color_lerp_slice([a, b], 0.25)[1:])
+    assert np.linalg.norm(q - a[1:]) < np.linalg.norm(q - b[1:]) and np.allclose(q, [2.0, 2.5, 2.5])
+    bb = psyref.color_lerp_slice([(1, 3000.0, 2.0, 0.0), (1, 5000.0, 4.0, 0.0)], 0.5)
+    assert bb[0] == 1 and np.allclose(bb[1:3], [4000.0, 3.0])
+    base = scenes.cornell_box(48, 48, 16, 2, 0)
+    w, h, c = base.rect_lights[0]
+    img_c, _ = psyref.render(base)
+    two = (0, 2 * c[1], 2 * c[2], 2 * c[3])
+    imgs = []
+    for cols in ([(0, 0.0, 0.0, 0.0), two], [two, (0, 0.0, 0.0, 0.0)], [two, two]):
+        sc = scenes.cornell_box(48, 48, 16, 2, 0)
+        sc.rect_lights[0] = (w, h, cols)
+        imgs.append(psyref.render(sc)[0])
+    up, down, dbl = imgs
+    assert np.allclose((up + down) / 2, img_c, rtol=1e-4, atol=1e-6)
+    assert not np.allclose(up, img_c, rtol=1e-2)
+    assert np.allclose(dbl, 2 * img_c, rtol=1e-5, atol=1e-6)

@@ -193,10 +193,11 @@ def oracle_scene(parsed):
             meshes.append(o)
         elif kind == "rect":
             where.append(("rect", len(rects)))
-            rects.append((o["dims"][0][0], o["dims"][0][1], opf.color_desc(o["color"][0])))
+            rects.append(([d[0] for d in o["dims"]], [d[1] for d in o["dims"]],
+                          [opf.color_desc(c) for c in o["color"]]))
         else:
             where.append(("sphere", len(spheres)))
-            spheres.append((o["radius"][0], opf.color_desc(o["color"][0])))
+            spheres.append((o["radius"], [opf.color_desc(c) for c in o["color"]]))
     insts = []
     for i in asm["instances"]:
         kind, idx = where[i["object"]]
@@ -206,10 +207,10 @@ def oracle_scene(parsed):
         insts.append((kind, idx, i["shader"], np.asarray(xf, np.float32).reshape(-1, 12)))
     cam = parsed["camera"]
     world = parsed["world"]
-    distant = [(d["radius"][0], tuple(d["direction"][0]), opf.color_desc(d["color"][0]))
+    distant = [(d["radius"], [tuple(v) for v in d["direction"]], [opf.color_desc(c) for c in d["color"]])
                for d in world["distant_lights"]]
     return dict(shaders=shaders, meshes=meshes, rects=rects, spheres=spheres, instances=insts,
-                camera=(np.asarray(cam["xforms"], np.float32), cam["fovs"][0], cam["apertures"][0], cam["focus"][0]),
+                camera=(np.asarray(cam["xforms"], np.float32), cam["fovs"], cam["apertures"], cam["focus"]),
                 background=opf.color_desc(world["background"]), distant=distant,
                 settings=(parsed["width"], parsed["height"], parsed["spp"], parsed["seed"]))
 
@@ -222,7 +223,8 @@ def assert_same_scene(sc, o):
     assert (sc.settings["width"], sc.settings["height"], sc.settings["spp"], sc.settings["seed"]) == o["settings"]
     assert sc.settings["max_bounces"] == 2
     assert f32eq(sc.camera["xforms"], o["camera"][0])
-    assert f32eq([sc.camera["fov"], sc.camera["aperture"], sc.camera["focus"]], list(o["camera"][1:]))
+    for k, want in zip(("fov", "aperture", "focus"), o["camera"][1:]):
+        assert f32eq(np.atleast_1d(sc.camera[k]), want)
     assert sc.background[0] == o["background"][0] and f32eq(sc.background[1:], o["background"][1:])
     assert len(sc.shaders) == len(o["shaders"])
     for a, b in zip(sc.shaders, o["shaders"]):
@@ -235,11 +237,19 @@ def assert_same_scene(sc, o):
             assert len(m) == 2
         else:
             assert f32eq(m[2], np.asarray(om["normals"], np.float32))
+    def colours(c):  # a colour or a list of time samples, as kinds and values
+        cs = c if isinstance(c, list) else [c]
+        return [x[0] for x in cs], [x[1:] for x in cs]
+
+    def same_colours(a, b):
+        (ka, va), (kb, vb) = colours(a), colours(b)
+        return ka == kb and f32eq(va, vb)
+
     assert len(sc.rect_lights) == len(o["rects"]) and len(sc.sphere_lights) == len(o["spheres"])
     for a, b in zip(sc.rect_lights, o["rects"]):
-        assert f32eq(a[:2], b[:2]) and a[2][0] == b[2][0] and f32eq(a[2][1:], b[2][1:])
+        assert f32eq(np.atleast_1d(a[0]), b[0]) and f32eq(np.atleast_1d(a[1]), b[1]) and same_colours(a[2], b[2])
     for a, b in zip(sc.sphere_lights, o["spheres"]):
-        assert f32eq(a[:1], b[:1]) and a[1][0] == b[1][0] and f32eq(a[1][1:], b[1][1:])
+        assert f32eq(np.atleast_1d(a[0]), b[0]) and same_colours(a[1], b[1])
     assert len(sc.instances) == len(o["instances"])
     for a, (kind, idx, shader, xf) in zip(sc.instances, o["instances"]):
         assert (a["kind"], a["index"], a["shader"]) == (kind, idx, shader)
@@ -249,7 +259,8 @@ def assert_same_scene(sc, o):
             assert f32eq(a["xforms"], xf)
     assert len(sc.distant_lights) == len(o["distant"])
     for a, b in zip(sc.distant_lights, o["distant"]):
-        assert f32eq([a[0], *a[1]], [b[0], *b[1]]) and a[2][0] == b[2][0] and f32eq(a[2][1:], b[2][1:])
+        dirs = a[1] if isinstance(a[1], list) else [a[1]]
+        assert f32eq(np.atleast_1d(a[0]), b[0]) and f32eq(dirs, b[1]) and same_colours(a[2], b[2])
 
 
 def generated_scenes():
@@ -377,17 +388,59 @@ Scene $b {
     assert (b.camera["aperture"], b.camera["focus"]) == (0.0, 1.0) and b.settings["seed"] == 7
 
 
-def test_unsupported_time_samples_are_rejected():
-    """The reference lerps several camera fov/aperture/focus samples and
-    light samples over time; the descriptors hold one, so the reader refuses
-    such files rather than dropping samples."""
+def test_light_time_samples_are_read():
+    """Repeated Dimensions, Radius, Direction and Color leaves are time
+    samples (psy_light.rs:16-184), kept in file order; colour samples must
+    share one representation, as Lerp for Color requires (color.rs:331)."""
+    sc = scenes.motion_instances(grid=2, levels=2, lights=2, width=16, height=16, spp=1)
+    sc.rect_lights.append(([1.0, 2.0, 0.5], [1.5, 1.0, 2.5], [(0, 1.0, 2.0, 3.0), (0, 3.0, 2.0, 1.0)]))
+    sc.sphere_lights[0] = ([0.25, 0.5], sc.sphere_lights[0][1])
+    sc.sphere_lights[1] = (0.3, [(2, 4000.0, 30.0, 0.0), (2, 6500.0, 60.0, 0.0), (2, 5000.0, 45.0, 0.0)])
+    sc.distant_lights.append(([0.02, 0.04], [(-0.4, 0.3, -0.6), (0.3, -0.4, -0.7)],
+                              [(2, 5800.0, 100.0, 0.0), (2, 5900.0, 120.0, 0.0)]))
+    sc.instances.append(dict(kind="rect", index=len(sc.rect_lights) - 1, shader=None,
+                             xforms=np.array([[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, -3]], np.float32)))
+    text = psyfile.write(sc)
+    got = psyfile.read(text)[0]
+    assert_same_scene(got, oracle_scene(opf.parse_file(text)))
+    assert len(got.rect_lights[-1][0]) == 3 and len(got.rect_lights[-1][2]) == 2
+    assert len(got.sphere_lights[0][0]) == 2 and len(got.sphere_lights[1][1]) == 3
+    assert len(got.distant_lights[0][1]) == 2
+    mixed = text.replace("Color [color_temperature, 6500", "Color [blackbody, 6500", 1)
+    assert mixed != text
+    with pytest.raises(psyfile.PsyFileError) as e:
+        psyfile.read(mixed)
+    assert e.value.kind == "UnknownVariant" and "different representations" in str(e.value)
+
+
+def test_camera_lens_time_samples_are_read():
+    """Several Fov, FocalDistance and ApertureRadius leaves are time
+    samples (parse_camera, psy.rs:355-420), kept in file order; a zero focus
+    distance among them, or a missing list, disables the lens (camera.rs:
+    32-57)."""
     base = psyfile.write(scenes.cornell_box(8, 8, 1, 1, 0))
-    for text in (base.replace("Fov [", "Fov [1]\n        Fov [", 1),
-                 base.replace("Dimensions [", "Dimensions [1 1]\n            Dimensions [", 1)):
-        opf.parse_file(text)  # accepted by the reference's rules
-        with pytest.raises(psyfile.PsyFileError) as ce:
-            psyfile.read(text)
-        assert ce.value.kind == "UnknownVariant" and "not supported" in str(ce.value)
+    head, tail = base.split("        Fov [", 1)
+    tail = tail.split("\n", 1)[1]
+    cam = ("        Fov [30]\n        Fov [45]\n        Fov [60]\n"
+           "        FocalDistance [2]\n        FocalDistance [3]\n"
+           "        ApertureRadius [0.5]\n")
+    lines = [ln for ln in tail.split("\n", 2)[:2]]
+    assert lines[0].lstrip().startswith("FocalDistance") and lines[1].lstrip().startswith("ApertureRadius")
+    text = head + cam + tail.split("\n", 2)[2]
+    sc = psyfile.read(text)[0]
+    k = np.float32(np.float32(np.pi) / np.float32(180.0))
+    assert f32eq(sc.camera["fov"], [np.float32(30) * k, np.float32(45) * k, np.float32(60) * k])
+    assert f32eq(sc.camera["focus"], [2.0, 3.0]) and sc.camera["aperture"] == 0.5
+    assert_same_scene(sc, oracle_scene(opf.parse_file(text)))
+    # A zero focus sample disables the lens; so does a missing aperture.
+    for t2 in (text.replace("FocalDistance [3]", "FocalDistance [0]"),
+               text.replace("        ApertureRadius [0.5]\n", "")):
+        sc2 = psyfile.read(t2)[0]
+        assert (sc2.camera["aperture"], sc2.camera["focus"]) == (0.0, 1.0)
+        assert_same_scene(sc2, oracle_scene(opf.parse_file(t2)))
+    # The writer keeps every sample.
+    again = psyfile.read(psyfile.write(sc))[0]
+    assert len(again.camera["fov"]) == 3 and f32eq(again.camera["focus"], [2.0, 3.0])
 
 
 def test_deep_nesting_is_refused_not_crashed():
