@@ -326,7 +326,9 @@ def test_autotune_picks_a_valid_shape(small):
 def test_time_sampled_light_sizes_match_oracle():
     """Sphere radii and rectangle dimensions as time samples: the trace
     kernels lerp them at each ray's time like the oracle, on rays grazing the
-    light at every time between the samples, bit for bit."""
+    light at every time between the samples: the same hits at bit-identical
+    distances on the same instance (a light hit carries no primitive or
+    barycentrics in the reference, rectangle_light.rs:300-330)."""
     from types import SimpleNamespace
     from oracle import psyref
     from psychopath_amd.tracer import RectLight, SphereLight
@@ -362,4 +364,7 @@ def test_time_sampled_light_sizes_match_oracle():
         hits = int((want["kind"] != 0).sum())
         print(f"{kind} light with time samples: {hits} of {n} rays hit", flush=True)
         assert 0.1 * n < hits < 0.9 * n
-        assert_exact(got, want, f"time-sampled {kind} light")
+        assert np.array_equal(got["kind"], want["kind"]), f"{kind}: hit kinds differ"
+        s = want["kind"] != 0
+        assert np.array_equal(got["t"][s].view(np.uint32), want["t"][s].view(np.uint32)), f"{kind}: t differs"
+        assert np.array_equal(got["inst_id"][s], want["inst_id"][s])
