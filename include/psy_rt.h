@@ -194,10 +194,11 @@ typedef struct {
  * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
  * live in LDS (8, 16 or 24) and the rest in private memory; `simd_waves`
  * is the register budget, as resident waves per SIMD (4: up to 128 VGPRs,
- * 8: at most 64); a wave descends internal nodes until no lane stands on
- * one or `leaf_min` of its lanes wait at a leaf (1..64, 0 = 64), then runs
- * the leaf tests.  The simple kernel runs one lane per ray over a
- * launch-wide grid.  No field changes the results. */
+ * 8: at most 64); `leaf_min` 0 lets each lane descend until it reaches a
+ * leaf or a node whose children all miss, 1..64 pops past such nodes inside
+ * the descent and ends a wave's descent round once no lane stands on an
+ * internal node or `leaf_min` lanes wait at a leaf.  The simple kernel runs
+ * one lane per ray over a launch-wide grid.  No field changes the results. */
 #define PSY_KERNEL_PERSISTENT 0u
 #define PSY_KERNEL_SIMPLE 1u
 typedef struct {

@@ -60,6 +60,8 @@ TraceArgs scene_args(const psy_scene* s) {
     a.sphere_lights = s->sphere_lights;
     a.light_samples = s->light_samples;
     a.top_root = s->host.top_root;
+    a.hot_base = s->host.hot_base;
+    a.hot_count = s->host.hot_count;
     return a;
 }
 
@@ -75,7 +77,7 @@ hipError_t launch_tuned(const psy_scene* scene, const psy_trace_tuning& tuning, 
                         const TraceArgs& a, bool counted, hipStream_t stream) {
     if (tuning.kernel == PSY_KERNEL_SIMPLE) return launch_trace(a, counted, stream);
     TraceTuning tt{cu_count * tuning.waves_per_cu, tuning.refill_min, tuning.stack_lds, tuning.simd_waves,
-                   tuning.leaf_min == 0 ? 64u : tuning.leaf_min};
+                   tuning.leaf_min};
     return launch_trace_persistent(a, counted, scene->host.has_xf || scene->host.has_lights, scene->host.has_motion,
                                    work, tt, stream);
 }
@@ -329,14 +331,14 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
     if (rc) return rc;
     if (n == 0) return fail(PSY_E_INVALID, "autotune needs rays");
     std::vector<psy_trace_tuning> cands;
-    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8, 4, 64});
+    cands.push_back({PSY_KERNEL_SIMPLE, 1, 1, 8, 4, 0});
     const uint32_t wpc[] = {8, 12, 16, 24, 32};
     const uint32_t refill[] = {1, 8, 16, 32};
     const uint32_t depth[] = {8, 16, 24};
     for (uint32_t w : wpc)
         for (uint32_t r : refill)
             for (uint32_t dd : depth)
-                for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ, 64});
+                for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ, 0});
     const psy_trace_tuning saved = tr->tuning;
     float best_ms = 1e30f;
     psy_trace_tuning best_t = saved;
@@ -365,13 +367,14 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
         return PSY_OK;
     };
     // Stage 1: kernel, waves, refill, LDS depth and register budget with
-    // whole-wave descents; stage 2: the descent threshold around the best.
+    // per-lane descents; stage 2: the in-loop descents and their threshold
+    // around the best.
     rc = time_all();
     if (rc) return rc;
     if (best_t.kernel == PSY_KERNEL_PERSISTENT) {
         const psy_trace_tuning s1 = best_t;
         cands.clear();
-        for (uint32_t lm : {8u, 16u, 24u, 32u, 48u})
+        for (uint32_t lm : {8u, 16u, 24u, 32u, 48u, 64u})
             for (uint32_t r : {8u, 16u, 32u})
                 for (uint32_t w : {16u, 24u, 32u})
                     cands.push_back({PSY_KERNEL_PERSISTENT, w, r, s1.stack_lds, s1.simd_waves, lm});

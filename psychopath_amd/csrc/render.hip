@@ -1375,7 +1375,7 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
             for (uint32_t refill : {8u, 16u, 32u}) {
                 for (uint32_t lds : {16u, 24u}) {
                     if (rc) break;
-                    r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw, 64};
+                    r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw, 0};
                     float ms;
                     rc = time_trial(ms);
                     if (!rc) consider(ms);
@@ -1384,7 +1384,7 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
         }
     }
     // Stage 1b: the traversal's descent threshold.
-    for (uint32_t lm : {16u, 32u, 48u}) {
+    for (uint32_t lm : {8u, 16u, 32u, 64u}) {
         if (rc) break;
         r->tuning = best_t;
         r->tuning.leaf_min = lm;

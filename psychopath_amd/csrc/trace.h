@@ -26,6 +26,7 @@ struct TraceArgs {
     const GpuSphereLight* sphere_lights;
     const float* light_samples;  // time samples of the lights (layout.h)
     uint32_t top_root;
+    uint32_t hot_base, hot_count;  // nodes a kernel may keep in LDS (HOT_NODES_MAX at most)
     // rays (device)
     const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
     const uint8_t* flags;
@@ -46,7 +47,7 @@ struct TraceTuning {
     uint32_t refill_min;  // idle lanes that trigger a refill (1..64)
     uint32_t lds_depth;   // stack entries per lane kept in LDS (8, 16, 24)
     uint32_t occupancy;   // register budget as waves per SIMD (4 or 8)
-    uint32_t leaf_min;    // waiting lanes that end a descent round (1..64)
+    uint32_t leaf_min;    // waiting lanes that end a descent round (1..64), 0: per-lane descents
 };
 
 // One lane per ray, one launch-wide grid (trace.hip).

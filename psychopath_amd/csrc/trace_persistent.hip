@@ -58,7 +58,17 @@ template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
+    // The scene's hottest nodes (one tree's top levels, laid out first by
+    // the packer): every ray starts there, so the wave reads them from LDS
+    // instead of through the L1, whose tag lookups bound divergent fetches.
+    __shared__ float4 lds_hot[HOT_NODES_MAX * 8];
     const int lane = threadIdx.x;
+    const uint32_t hot_count = a.hot_count < HOT_NODES_MAX ? a.hot_count : HOT_NODES_MAX;
+    {
+        const float4* src = reinterpret_cast<const float4*>(a.nodes + a.hot_base);
+        for (uint32_t i = (uint32_t)lane; i < hot_count * 8u; i += WAVE) lds_hot[i] = src[i];
+    }
+    __syncthreads();
     uint32_t* const lds_col = lds_stack + lane;
     uint32_t spill[TRACE_STACK_CAPACITY];
     int sp = 0, bot = 0;
@@ -128,64 +138,84 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         if (!has) continue;
 
         // ---- descend internal nodes ---------------------------------------
-        // A node whose children all miss pops the next entry inside the loop,
-        // so the lane keeps working while others descend.  The loop runs while
+        // leaf_min 0: each lane descends until it stands on a leaf or on a
+        // dead end (a node whose children all miss), which pops after the
+        // leaf round.  leaf_min 1..64: dead ends pop inside the loop, so the
+        // lane keeps working while others descend, and the loop runs while
         // some lane stands on an internal node and fewer than `leaf_min`
-        // lanes wait (at a leaf, or at the end of their level: the instance's
-        // subtree or the whole stack); lanes still on internal nodes when it
-        // stops carry on after the leaf round.  Every lane takes its own
+        // lanes wait (at a leaf, or at the end of their level: the
+        // instance's subtree or the whole stack); lanes still on internal
+        // nodes then carry on after the leaf round.  Every lane takes its own
         // depth-first steps in the same order either way; only their timing
         // changes, so results do not depend on leaf_min.
-        const uint32_t n_active = (uint32_t)__popcll(__ballot(1));
-        for (;;) {
-            if (cur == REF_EMPTY && sp > (in_inst ? inst_base : 0)) SSTACK_POP(cur);
-            const bool inner = cur != REF_EMPTY && !(cur & REF_LEAF);
-            const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
-            if (n_inner == 0 || n_active - n_inner >= leaf_min) break;
-            if (!inner) continue;
-            const GpuNode* node = a.nodes + cur;
-            const float4* nf = reinterpret_cast<const float4*>(node);
-            const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
-            const uint4 tail = *reinterpret_cast<const uint4*>(nf + 7);
-            const uint32_t cnt = tail.z & 7u;
-            float4 lx, hx, ly, hy, lz, hz;
-            const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
-            if (!MOTION || blen <= 1) {
-                lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
-                if (COUNT) c_nodes++;
-            } else {
-                uint32_t i1;
-                float a2;
-                const float4* m0 = reinterpret_cast<const float4*>(a.motion + tail.w);
-                if (!lerp_slice_pos(blen, time, i1, a2)) {
-                    m0 += 6 * i1;
-                    lx = m0[0]; hx = m0[1]; ly = m0[2]; hy = m0[3]; lz = m0[4]; hz = m0[5];
+        //
+        // One descent step of a lane standing on an internal node: its four
+        // child boxes, then the hit children pushed in visit order.
+        auto node_step = [&]() {
+                const uint32_t hot = cur - a.hot_base;
+                const float4* nf = reinterpret_cast<const float4*>(a.nodes + cur);
+                float4 lx, hx, ly, hy, lz, hz;
+                uint4 refs, tail;
+                if (hot < hot_count) {
+                    const float4* h = lds_hot + 8u * hot;
+                    lx = h[0]; hx = h[1]; ly = h[2]; hy = h[3]; lz = h[4]; hz = h[5];
+                    refs = __builtin_bit_cast(uint4, h[6]);
+                    tail = __builtin_bit_cast(uint4, h[7]);
                 } else {
-                    m0 += 6 * i1;
-                    const float4* m1 = m0 + 6;
-                    lx = lerp4(m0[0], m1[0], a2); hx = lerp4(m0[1], m1[1], a2);
-                    ly = lerp4(m0[2], m1[2], a2); hy = lerp4(m0[3], m1[3], a2);
-                    lz = lerp4(m0[4], m1[4], a2); hz = lerp4(m0[5], m1[5], a2);
+                    lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
+                    refs = *reinterpret_cast<const uint4*>(nf + 6);
+                    tail = *reinterpret_cast<const uint4*>(nf + 7);
                 }
-                if (COUNT) c_mnodes++;
-            }
-            uint32_t mask = 0;
-            mask |= slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, lo, di, tmax) ? 1u : 0u;
-            mask |= slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, lo, di, tmax) ? 2u : 0u;
-            mask |= slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, lo, di, tmax) ? 4u : 0u;
-            mask |= slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, lo, di, tmax) ? 8u : 0u;
-            mask &= (1u << cnt) - 1u;
-            const uint32_t ord = ((oct < 4 ? tail.x : tail.y) >> (8u * (oct & 3u))) & 0xffu;
-            uint32_t next = REF_EMPTY;
-            for (int k = 3; k >= 0; --k) {
-                if (k >= (int)cnt) continue;
-                const uint32_t c = (ord >> (2 * k)) & 3u;
-                if (mask & (1u << c)) {
-                    if (next != REF_EMPTY) SSTACK_PUSH(next);
-                    next = c == 0 ? refs.x : (c == 1 ? refs.y : (c == 2 ? refs.z : refs.w));
+                const uint32_t cnt = tail.z & 7u;
+                const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
+                if (!MOTION || blen <= 1) {
+                    if (COUNT) c_nodes++;
+                } else {
+                    uint32_t i1;
+                    float a2;
+                    const float4* m0 = reinterpret_cast<const float4*>(a.motion + tail.w);
+                    if (!lerp_slice_pos(blen, time, i1, a2)) {
+                        m0 += 6 * i1;
+                        lx = m0[0]; hx = m0[1]; ly = m0[2]; hy = m0[3]; lz = m0[4]; hz = m0[5];
+                    } else {
+                        m0 += 6 * i1;
+                        const float4* m1 = m0 + 6;
+                        lx = lerp4(m0[0], m1[0], a2); hx = lerp4(m0[1], m1[1], a2);
+                        ly = lerp4(m0[2], m1[2], a2); hy = lerp4(m0[3], m1[3], a2);
+                        lz = lerp4(m0[4], m1[4], a2); hz = lerp4(m0[5], m1[5], a2);
+                    }
+                    if (COUNT) c_mnodes++;
                 }
+                uint32_t mask = 0;
+                mask |= slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, lo, di, tmax) ? 1u : 0u;
+                mask |= slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, lo, di, tmax) ? 2u : 0u;
+                mask |= slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, lo, di, tmax) ? 4u : 0u;
+                mask |= slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, lo, di, tmax) ? 8u : 0u;
+                mask &= (1u << cnt) - 1u;
+                const uint32_t ord = ((oct < 4 ? tail.x : tail.y) >> (8u * (oct & 3u))) & 0xffu;
+                uint32_t next = REF_EMPTY;
+                for (int k = 3; k >= 0; --k) {
+                    if (k >= (int)cnt) continue;
+                    const uint32_t c = (ord >> (2 * k)) & 3u;
+                    if (mask & (1u << c)) {
+                        if (next != REF_EMPTY) SSTACK_PUSH(next);
+                        next = c == 0 ? refs.x : (c == 1 ? refs.y : (c == 2 ? refs.z : refs.w));
+                    }
+                }
+                cur = next;
+        };
+        if (leaf_min == 0) {
+            // Each lane descends until it stands on a leaf or a dead end.
+            while (cur != REF_EMPTY && !(cur & REF_LEAF)) node_step();
+        } else {
+            const uint32_t n_active = (uint32_t)__popcll(__ballot(1));
+            for (;;) {
+                if (cur == REF_EMPTY && sp > (in_inst ? inst_base : 0)) SSTACK_POP(cur);
+                const bool inner = cur != REF_EMPTY && !(cur & REF_LEAF);
+                const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
+                if (n_inner == 0 || n_active - n_inner >= leaf_min) break;
+                if (inner) node_step();
             }
-            cur = next;
         }
 
         // ---- leaf --------------------------------------------------------
@@ -386,7 +416,7 @@ hipError_t launch_trace_persistent(const TraceArgs& args, bool counted, bool has
                                    unsigned int* work, const TraceTuning& tune, hipStream_t stream) {
     if (args.n == 0) return hipSuccess;
     if (args.n >= 0x80000000ull || tune.waves == 0 || tune.refill_min == 0 || tune.refill_min > 64 ||
-        tune.leaf_min == 0 || tune.leaf_min > 64)
+        tune.leaf_min > 64)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_work_kernel, dim3(1), dim3(1), 0, stream, work);
     hipError_t e = hipGetLastError();

@@ -317,7 +317,7 @@ def test_autotune_picks_a_valid_shape(small):
     best = tr.autotune(rays, hits)
     assert best == tr.tuning()
     assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24) and best["simd_waves"] in (4, 8)
-    assert 1 <= best["leaf_min"] <= 64
+    assert 0 <= best["leaf_min"] <= 64
     assert tr.rays_traced() == before
     assert_exact(gpu_trace(tr, r), o.trace(r), "after autotune")
     tr.set_tuning(0, 16, 16, 16, 4, 0)
