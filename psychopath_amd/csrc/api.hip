@@ -60,8 +60,6 @@ TraceArgs scene_args(const psy_scene* s) {
     a.sphere_lights = s->sphere_lights;
     a.light_samples = s->light_samples;
     a.top_root = s->host.top_root;
-    a.hot_base = s->host.hot_base;
-    a.hot_count = s->host.hot_count;
     return a;
 }
 

@@ -18,11 +18,6 @@ constexpr uint32_t MAX_LEAF_FIRST = (1u << 29) - 1;
 
 static inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) { return REF_LEAF | (first << 2) | count; }
 
-// Levels of one tree laid out first, breadth first, for kernels that keep
-// them in LDS: at most 1 + 4 + 16 internal nodes.
-constexpr uint32_t HOT_LEVELS = 3;
-constexpr uint32_t HOT_NODES_MAX = 21;
-
 // One BVH4 node: a 64-B-aligned 128-B record.  The first 96 B are the four
 // child boxes as SoA (BBox4, src/bbox4.rs:16-21) for time sample 0; nodes with
 // time-sampled bounds keep all samples in the motion-bounds array instead.

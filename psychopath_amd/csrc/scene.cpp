@@ -59,46 +59,19 @@ GpuXform to_gpu(const Xf& x) {
 }
 
 // Appends `bvh` to the scene arrays, rebasing node and motion indices.
-// Appends a tree's nodes.  The internal nodes of its first HOT_LEVELS
-// levels come first, breadth first, so a kernel can hold them in LDS as one
-// index range (`hot` = their count); the rest keep the builder's order.
-// Only indices change: every child reference is remapped, so traversal
-// visits the same boxes in the same order.
-uint32_t append_bvh(PackedScene& s, const HostBvh4& bvh, uint32_t* hot = nullptr) {
-    const uint32_t node_base = (uint32_t)s.nodes.size();
-    const uint32_t motion_base = (uint32_t)s.motion.size();
-    const uint32_t n_nodes = (uint32_t)bvh.nodes.size();
-    std::vector<uint32_t> perm(n_nodes, REF_EMPTY);  // builder index -> new index
-    uint32_t next = 0;
-    const bool internal_root = !(bvh.root == REF_EMPTY || (bvh.root & REF_LEAF));
-    if (internal_root) {
-        std::vector<uint32_t> level{bvh.root}, below;
-        for (uint32_t l = 0; l < HOT_LEVELS && !level.empty(); l++) {
-            below.clear();
-            for (uint32_t n : level) {
-                perm[n] = next++;
-                const GpuNode& g = bvh.nodes[n];
-                for (uint32_t c = 0; c < (g.meta & 7u); c++)
-                    if (!(g.child[c] & REF_LEAF)) below.push_back(g.child[c]);
-            }
-            level.swap(below);
-        }
-    }
-    if (hot) *hot = next;
-    for (uint32_t i = 0; i < n_nodes; i++)
-        if (perm[i] == REF_EMPTY) perm[i] = next++;
-    s.nodes.resize(node_base + n_nodes);
-    for (uint32_t i = 0; i < n_nodes; i++) {
-        GpuNode n = bvh.nodes[i];
-        const uint32_t cnt = n.meta & 7u;
+uint32_t append_bvh(PackedScene& s, const HostBvh4& bvh) {
+    uint32_t node_base = (uint32_t)s.nodes.size();
+    uint32_t motion_base = (uint32_t)s.motion.size();
+    for (GpuNode n : bvh.nodes) {
+        uint32_t cnt = n.meta & 7u;
         for (uint32_t c = 0; c < cnt; c++)
-            if (!(n.child[c] & REF_LEAF)) n.child[c] = perm[n.child[c]] + node_base;
+            if (!(n.child[c] & REF_LEAF)) n.child[c] += node_base;
         if (n.motion != REF_EMPTY) n.motion += motion_base;
-        s.nodes[node_base + perm[i]] = n;
+        s.nodes.push_back(n);
     }
     s.motion.insert(s.motion.end(), bvh.motion.begin(), bvh.motion.end());
-    if (!internal_root) return bvh.root;
-    return perm[bvh.root] + node_base;
+    if (bvh.root == REF_EMPTY || (bvh.root & REF_LEAF)) return bvh.root;
+    return bvh.root + node_base;
 }
 
 bool color_ok(const psy_color& c) {
@@ -309,7 +282,6 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
 
     s.bvhs.resize(d->mesh_count + 1);
     s.meshes.resize(d->mesh_count);
-    uint32_t mesh_hot_tris = 0, mesh_hot_base = 0, mesh_hot = 0;
 
     // ---- meshes -----------------------------------------------------------
     for (uint32_t m = 0; m < d->mesh_count; m++) {
@@ -352,13 +324,7 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         const HostBvh4& bvh = s.bvhs[m];
 
         GpuMesh gm{};
-        uint32_t hot = 0;
-        gm.root = append_bvh(s, bvh, &hot);
-        if (hot > 0 && T > mesh_hot_tris) {  // the largest mesh's top levels
-            mesh_hot_tris = T;
-            mesh_hot_base = gm.root;
-            mesh_hot = hot;
-        }
+        gm.root = append_bvh(s, bvh);
         gm.time_samples = S;
         gm.tri_stride = tri_record_bytes(S);
         gm.tri_offset = s.tris.size();
@@ -459,17 +425,7 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     BuildInput bi{d->instance_count, ibounds.data(), ioffs.data()};
     HostBvh4& top = s.bvhs[d->mesh_count];
     top = build_bvh4(bi, 1, 0);
-    uint32_t top_hot = 0;
-    s.top_root = append_bvh(s, top, &top_hot);
-    // Nodes a trace kernel may keep in LDS: the instance tree's top levels
-    // when it has more than a root, else the largest mesh's.
-    if (top_hot > 1) {
-        s.hot_base = s.top_root;
-        s.hot_count = top_hot;
-    } else {
-        s.hot_base = mesh_hot_base;
-        s.hot_count = mesh_hot;
-    }
+    s.top_root = append_bvh(s, top);
     s.max_depth = std::max(s.max_depth, top.depth);
     s.leaf_count += top.leaf_count;
 

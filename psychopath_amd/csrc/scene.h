@@ -25,9 +25,6 @@ struct PackedScene {
     std::vector<float> normals;        // xyz per (vertex, time sample), see GpuMesh
     std::vector<uint32_t> nidx;        // stored-order vertex triples per triangle
     uint32_t top_root = REF_EMPTY;
-    // The nodes [hot_base, hot_base + hot_count) are one tree's first
-    // HOT_LEVELS levels, breadth first (see append_bvh).
-    uint32_t hot_base = 0, hot_count = 0;
 
     // Lights and shading lookups.
     std::vector<GpuRectLight> rect_lights;

@@ -26,7 +26,6 @@ struct TraceArgs {
     const GpuSphereLight* sphere_lights;
     const float* light_samples;  // time samples of the lights (layout.h)
     uint32_t top_root;
-    uint32_t hot_base, hot_count;  // nodes a kernel may keep in LDS (HOT_NODES_MAX at most)
     // rays (device)
     const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
     const uint8_t* flags;

@@ -58,17 +58,7 @@ template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
-    // The scene's hottest nodes (one tree's top levels, laid out first by
-    // the packer): every ray starts there, so the wave reads them from LDS
-    // instead of through the L1, whose tag lookups bound divergent fetches.
-    __shared__ float4 lds_hot[HOT_NODES_MAX * 8];
     const int lane = threadIdx.x;
-    const uint32_t hot_count = a.hot_count < HOT_NODES_MAX ? a.hot_count : HOT_NODES_MAX;
-    {
-        const float4* src = reinterpret_cast<const float4*>(a.nodes + a.hot_base);
-        for (uint32_t i = (uint32_t)lane; i < hot_count * 8u; i += WAVE) lds_hot[i] = src[i];
-    }
-    __syncthreads();
     uint32_t* const lds_col = lds_stack + lane;
     uint32_t spill[TRACE_STACK_CAPACITY];
     int sp = 0, bot = 0;
@@ -152,23 +142,14 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         // One descent step of a lane standing on an internal node: its four
         // child boxes, then the hit children pushed in visit order.
         auto node_step = [&]() {
-                const uint32_t hot = cur - a.hot_base;
                 const float4* nf = reinterpret_cast<const float4*>(a.nodes + cur);
-                float4 lx, hx, ly, hy, lz, hz;
-                uint4 refs, tail;
-                if (hot < hot_count) {
-                    const float4* h = lds_hot + 8u * hot;
-                    lx = h[0]; hx = h[1]; ly = h[2]; hy = h[3]; lz = h[4]; hz = h[5];
-                    refs = __builtin_bit_cast(uint4, h[6]);
-                    tail = __builtin_bit_cast(uint4, h[7]);
-                } else {
-                    lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
-                    refs = *reinterpret_cast<const uint4*>(nf + 6);
-                    tail = *reinterpret_cast<const uint4*>(nf + 7);
-                }
+                const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
+                const uint4 tail = *reinterpret_cast<const uint4*>(nf + 7);
                 const uint32_t cnt = tail.z & 7u;
+                float4 lx, hx, ly, hy, lz, hz;
                 const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
                 if (!MOTION || blen <= 1) {
+                    lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
                     if (COUNT) c_nodes++;
                 } else {
                     uint32_t i1;
