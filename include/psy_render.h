@@ -160,8 +160,8 @@ int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
 /* Autotuner: renders `trial` (a reduced version of the intended renders)
  * with every candidate launch shape of the traversal kernel (waves per CU,
- * refill threshold, LDS stack depth, register budget), then every shading
- * register budget,
+ * refill threshold, LDS stack depth, register budget), then its descent
+ * threshold (leaf_min), then every shading register budget,
  * then with and without material sorting, keeps the fastest on the renderer
  * and returns it (either output may be NULL; `best_flags` holds
  * PSY_RENDER_NO_SORT if sorting lost and PSY_RENDER_SHADE_WAVES(n) of the
