@@ -98,17 +98,14 @@ def main():
         del src, dst
 
     # Launch shape: the simple grid kernel for reference, then the autotuner
-    # over persistent shapes on a 2M-ray sample of the same workload.
+    # over persistent shapes on the workload's own rays (a smaller sample
+    # ranks the close candidates less reliably).
     tracer.set_tuning(kernel=1)
     tracer.trace_device(d_rays, d_hits)
     tracer.trace_device(d_rays, d_hits)
     simple_ms = tracer.last_kernel_ms()
-    ns = min(args.rays, 1 << 21)
-    tune_rays = DeviceRays(RayBatch(*(r[k][:ns] for k in ("ox", "oy", "oz", "dx", "dy", "dz", "time", "max_t"))))
-    tune_hits = DeviceHits(ns)
     t1 = time.time()
-    tuning = tracer.autotune(tune_rays, tune_hits)
-    del tune_rays, tune_hits
+    tuning = tracer.autotune(d_rays, d_hits)
     log(f"simple kernel {simple_ms:.2f} ms; autotune {time.time() - t1:.1f}s -> {tuning}; warmup {args.warmup}")
 
     for _ in range(args.warmup):
