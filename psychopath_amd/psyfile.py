@@ -249,7 +249,8 @@ def write(sc: RenderScene, name: str = "Scene", output: str = "render.png") -> s
         out.append("            FaceVertIndices [" + " ".join(str(int(x)) for x in t.ravel()) + "]\n        }\n")
     for i, (w, h, col) in enumerate(sc.rect_lights):
         out.append(f"        RectangleLight $rect{i} {{\n")
-        out += [f"            Dimensions [{_g(a)} {_g(b)}]\n" for a, b in zip(np.atleast_1d(w), np.atleast_1d(h))]
+        ws, hs = np.broadcast_arrays(np.atleast_1d(w), np.atleast_1d(h))
+        out += [f"            Dimensions [{_g(a)} {_g(b)}]\n" for a, b in zip(ws, hs)]
         out += [f"            Color [{_color_text(c)}]\n" for c in (col if isinstance(col, list) else [col])]
         out.append("        }\n")
     for i, (r, col) in enumerate(sc.sphere_lights):

@@ -176,8 +176,10 @@ class Scene:
         self._keep.append(xfa)
         rls = (_lib.RectLightDesc * max(1, len(asm.rect_lights)))()
         for i, l in enumerate(asm.rect_lights):
-            wh = np.stack([np.atleast_1d(np.asarray(l.width, np.float32)),
-                           np.atleast_1d(np.asarray(l.height, np.float32))], axis=1)
+            # A number broadcasts against the other list (Dimensions leaves
+            # always pair them in the file).
+            wh = np.stack(np.broadcast_arrays(np.atleast_1d(np.asarray(l.width, np.float32)),
+                                              np.atleast_1d(np.asarray(l.height, np.float32))), axis=1)
             d0, nd, da = time_samples(wh, 2)
             c0, nc, ca = colour_samples(l.color)
             rls[i] = _lib.RectLightDesc(float(d0[0]), float(d0[1]), c0, nd, da.ctypes.data_as(_lib._fp), nc, ca)
