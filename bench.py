@@ -412,6 +412,9 @@ def large_bench(args) -> dict:
     t1 = time.time()
     choice = r.autotune(sc.camera, trial)
     log(f"large scene autotune {time.time() - t1:.1f}s -> {choice}")
+    # The wavefront form with the tuned traversal, shading budget and
+    # sorting (the autotuner may also have preferred the one-launch form).
+    r.set_options(sort=choice["sort"], shade_waves=choice["shade_waves"])
     t = time.perf_counter()
     img, st = r.render(sc.camera, sc.settings)
     wave_s = time.perf_counter() - t

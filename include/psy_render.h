@@ -162,10 +162,12 @@ int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
  * with every candidate launch shape of the traversal kernel (waves per CU,
  * refill threshold, LDS stack depth, register budget), then its descent
  * threshold (leaf_min), then every shading register budget,
- * then with and without material sorting, keeps the fastest on the renderer
- * and returns it (either output may be NULL; `best_flags` holds
- * PSY_RENDER_NO_SORT if sorting lost and PSY_RENDER_SHADE_WAVES(n) of the
- * chosen budget).  Images do not depend on the choice. */
+ * then with and without material sorting, then the one-launch form, keeps
+ * the fastest on the renderer and returns it (either output may be NULL;
+ * `best_flags` holds PSY_RENDER_NO_SORT if sorting lost, PSY_RENDER_PLAIN if
+ * the one-launch form won, and PSY_RENDER_SHADE_WAVES(n) of the chosen
+ * budget).  Images do not depend on the choice; the request server always
+ * batches in waves. */
 int psy_renderer_autotune(psy_renderer* renderer, const psy_camera_desc* camera, const psy_render_settings* trial,
                           psy_trace_tuning* best_tuning, uint32_t* best_flags);
 

@@ -1409,15 +1409,33 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
         rc = time_trial(ms);
         if (!rc) consider(ms);
     }
+    // Stage 4: the one-launch form, which wins on small closed scenes where
+    // path state never needs to leave registers.
+    bool best_plain = false;
+    if (!rc) {
+        r->tuning = best_t;
+        r->shade_waves = best_sw;
+        r->sort_materials = best_sort;
+        r->plain = true;
+        float ms;
+        rc = time_trial(ms);
+        r->plain = false;
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best_plain = true;
+        }
+    }
     r->release(film);
     r->tuning = rc ? saved_t : best_t;
     r->sort_materials = rc ? saved_sort : best_sort;
     r->shade_waves = rc ? saved_sw : best_sw;
-    r->plain = saved_plain;
+    r->plain = rc ? saved_plain : best_plain;
     drop_graphs(r);
     if (rc) return rc;
     if (best_tuning) *best_tuning = best_t;
-    if (best_flags) *best_flags = (best_sort ? 0u : PSY_RENDER_NO_SORT) | PSY_RENDER_SHADE_WAVES(best_sw);
+    if (best_flags)
+        *best_flags = (best_sort ? 0u : PSY_RENDER_NO_SORT) | (best_plain ? PSY_RENDER_PLAIN : 0u) |
+                      PSY_RENDER_SHADE_WAVES(best_sw);
     return PSY_OK;
 }
 

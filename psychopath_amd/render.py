@@ -76,6 +76,7 @@ class Renderer:
                                                                     C.byref(t), C.byref(flags)))
         out = {f: getattr(t, f) for f, _ in _lib.TraceTuning._fields_}
         out["sort"] = not (flags.value & _lib.PSY_RENDER_NO_SORT)
+        out["plain"] = bool(flags.value & _lib.PSY_RENDER_PLAIN)
         out["shade_waves"] = (flags.value >> _lib.PSY_RENDER_SHADE_WAVES_SHIFT) & 7
         return out
 
