@@ -406,3 +406,38 @@ def test_time_sampled_lights_match_oracle():
     r1, c1 = mo.sphere_lights[1]
     mo.sphere_lights[1] = (r1, [c1, (c1[0], c1[1] * 0.7, c1[2] * 2.0, c1[3]), c1])
     check_parity(mo, "motion 64x36x8 time-sampled sphere lights")
+
+
+def test_motion_blurred_export_renders_like_the_oracle(tmp_path):
+    """A scene file shaped like a motion-blurred PsychoBlend export (one
+    Transform, Fov, FocalDistance and ApertureRadius per blur step, lights
+    with per-step sizes, colours and directions) read by the product's and
+    the oracle's readers and rendered by both, then written by the command
+    line exactly as the rendered image encodes."""
+    from oracle import psyfile as opf
+    from psychopath_amd import __main__ as cli
+    from psychopath_amd import film, psyfile
+    sc = scenes.cornell_box(48, 36, 8, 3, 4)
+    xf = np.asarray(sc.camera["xforms"], np.float32).reshape(-1, 12)
+    steps = np.stack([xf[0] + np.float32(k) * np.array([0, 0, 0, 0.05, 0, 0, 0, 0.0, 0, 0, 0, 0.02], np.float32)
+                      for k in range(3)])
+    f = sc.camera["fov"]
+    sc.camera = dict(xforms=steps, fov=[f, f * 1.05, f * 1.1], aperture=[0.02, 0.03], focus=[8.0, 8.5])
+    w, h, c = sc.rect_lights[0]
+    sc.rect_lights[0] = ([w, 1.2 * w, 1.4 * w], [h, h, 1.3 * h], [c, (c[0], 1.1 * c[1], c[2], 0.9 * c[3])])
+    sc.distant_lights.append(([0.03, 0.04, 0.05], [(-0.4, 0.3, -0.6), (-0.3, 0.35, -0.6), (-0.2, 0.4, -0.6)],
+                              (2, 6000.0, 2.0, 0.0)))
+    out = tmp_path / "blur.png"
+    text = psyfile.write(sc, "blur", str(out))
+    got = psyfile.read(text)[0]
+    assert len(got.camera["fov"]) == 3 and len(got.distant_lights[0][1]) == 3
+    img, stats, _ = gpu_render(got)
+    ref, info = psyref.render(opf.to_scene(opf.parse_file(text)))
+    frac = matching_fraction(img, ref)
+    print(f"motion-blurred export 48x36x8: {frac * 100:.3f}% pixels within {REL}; rays {stats['rays']} vs "
+          f"{info['rays']}", flush=True)
+    assert frac >= FRACTION and stats["rays"] == info["rays"]
+    src = tmp_path / "blur.psy"
+    src.write_text(text)
+    assert cli.main(["-i", str(src)]) == 0
+    assert out.read_bytes() == film.encode(img, film.FORMATS[".png"])
