@@ -9,6 +9,7 @@
 #include "../../oracle/psyref/psyref_lights.h"
 #include "../../oracle/psyref/psyref_shading.h"
 #define PSY_RD_HOST 1
+#include "../../psychopath_amd/csrc/light_samples.h"
 #include "../../psychopath_amd/csrc/render_device.h"
 
 // Defined with the oracle's renderer, which this harness does not link.
@@ -177,6 +178,95 @@ int64_t rc_run(int which, int64_t n, uint64_t seed, int64_t* first) {
     }
     return bad;
 }
+}
+
+// Time-sampled lights: the product's resolvers (light_samples.h) on the
+// packed sample layout of layout.h against the oracle's Light::at, on random
+// lists of one to five samples and random times including the shutter's
+// ends.  Returns the number of (light, time) cases that differ in any bit.
+extern "C" int64_t rc_light_samples(int64_t n, uint64_t seed) {
+    g_state = seed * 2654435761ull + 29;
+    int64_t bad = 0;
+    auto colour = [](int kind, std::vector<float>& ls, psyref::Color& o, psy::GpuColor& p) {
+        const float a = kind == 0 ? uf(0.0f, 5.0f) : uf(1000.0f, 9000.0f), b = uf(0.0f, 5.0f), c = uf(0.0f, 5.0f);
+        o = rcolor_o(kind, a, b, c);
+        p = rcolor_p(kind, a, b, c);
+        ls.push_back((float)kind);
+        ls.push_back(a);
+        ls.push_back(b);
+        ls.push_back(c);
+    };
+    auto same_colour = [](const psyref::Color& o, const psy::GpuColor& p) {
+        return o.kind == p.kind && same(o.a, p.a) && same(o.b, p.b) && (o.kind != 0 || same(o.c, p.c));
+    };
+    for (int64_t i = 0; i < n; i++) {
+        const int na = 1 + (int)(urand() * 5), nb = 1 + (int)(urand() * 5), nc = 1 + (int)(urand() * 5);
+        const int kind = (int)(urand() * 3);
+        std::vector<float> ls(3, 0.0f);  // an offset, as lights share the array
+        const uint32_t base = (uint32_t)ls.size();
+        // Rectangle: (w, h) pairs, then colours.
+        psyref::RectangleLight ro;
+        psy::GpuRectLight rp{0.0f, 0.0f, base, 0u, psy::GpuColor{}};
+        for (int k = 0; k < na; k++) {
+            const float w = uf(0.1f, 3.0f), h = uf(0.1f, 3.0f);
+            ro.dims_t.push_back(w);
+            ro.dims_t.push_back(h);
+            ls.push_back(w);
+            ls.push_back(h);
+        }
+        for (int k = 0; k < nc; k++) {
+            psyref::Color co;
+            psy::GpuColor cp;
+            colour(kind, ls, co, cp);
+            ro.colors_t.push_back(co);
+            if (k == 0) rp.color = cp;
+        }
+        rp.counts = (uint32_t)na | ((uint32_t)nc << 8);
+        // Sphere: radii, then colours.
+        const uint32_t sbase = (uint32_t)ls.size();
+        psyref::SphereLight so;
+        psy::GpuSphereLight sp{0.0f, sbase, (uint32_t)nb | ((uint32_t)nc << 8), 0.0f, rp.color};
+        for (int k = 0; k < nb; k++) {
+            const float r = uf(0.05f, 2.0f);
+            so.radii_t.push_back(r);
+            ls.push_back(r);
+        }
+        so.colors_t = ro.colors_t;
+        ls.insert(ls.end(), ls.begin() + base + 2 * na, ls.begin() + base + 2 * na + 4 * nc);
+        // Distant: radii, directions, then colours.
+        const uint32_t dbase = (uint32_t)ls.size();
+        psyref::DistantDiskLight dO;
+        psy::GpuDistantLight dp{0.0f, {0.0f, 0.0f, 0.0f}, rp.color, dbase,
+                                (uint32_t)nb | ((uint32_t)na << 8) | ((uint32_t)nc << 16), 0.0f, 0.0f};
+        for (int k = 0; k < nb; k++) {
+            dO.radii_t.push_back(so.radii_t[k]);
+            ls.push_back(so.radii_t[k]);
+        }
+        for (int k = 0; k < na; k++) {
+            const psyref::Vec3 v = rvec(1.0f);
+            dO.directions_t.push_back(v);
+            ls.push_back(v.x);
+            ls.push_back(v.y);
+            ls.push_back(v.z);
+        }
+        dO.colors_t = ro.colors_t;
+        ls.insert(ls.end(), ls.begin() + base + 2 * na, ls.begin() + base + 2 * na + 4 * nc);
+        for (int t = 0; t < 6; t++) {
+            const float time = t == 0 ? 0.0f : t == 1 ? 1.0f : uf(0.0f, 1.0f);
+            const psyref::RectangleLight a = ro.at(time);
+            const psy::GpuRectLight b = psy::rect_at(rp, ls.data(), time);
+            const psyref::SphereLight c = so.at(time);
+            const psy::GpuSphereLight d = psy::sphere_at(sp, ls.data(), time);
+            const psyref::DistantDiskLight e = dO.at(time);
+            const psy::GpuDistantLight f = psy::distant_at(dp, ls.data(), time);
+            const bool ok = same(a.w, b.w) && same(a.h, b.h) && same_colour(a.color, b.color) &&
+                            same(c.radius, d.radius) && same_colour(c.color, d.color) && same(e.radius, f.radius) &&
+                            same(e.direction.x, f.dir[0]) && same(e.direction.y, f.dir[1]) &&
+                            same(e.direction.z, f.dir[2]) && same_colour(e.color, f.color);
+            if (!ok) bad++;
+        }
+    }
+    return bad;
 }
 
 #include "rdeval.h"
