@@ -16,7 +16,7 @@
 #ifndef PSY_MENG_STORAGE
 #define PSY_MENG_STORAGE static
 #endif
-#include "../../psychopath_amd/csrc/_gen/meng_tables.h"
+#include "../../psychopath_amd/csrc/meng_tables.h"
 
 namespace psyref {
 

@@ -27,7 +27,7 @@
 #define PSY_MENG_STORAGE static __device__
 #define RD __device__ __forceinline__
 #endif
-#include "_gen/meng_tables.h"
+#include "meng_tables.h"
 
 namespace psy {
 namespace rd {
