@@ -220,7 +220,140 @@ _SIGS.update({
     "psyref_spectrum_to_xyz": (None, [_fp, C.c_float, _fp]),
     "psyref_planck": (C.c_float, [C.c_float, C.c_float]),
     "psyref_meng_sha256": (C.c_char_p, []),
+    "psyref_closure_evaluate_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, C.c_float, _fp, _fp]),
+    "psyref_closure_sample_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, C.c_float, _fp, _fp,
+                                       _fp]),
+    "psyref_closure_estimate_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, _fp, _fp]),
+    "psyref_sphere_light_sample_n": (None, [C.c_float, C.POINTER(_Color), _fp, C.c_uint32, _fp, _fp, C.c_float,
+                                            _fp, _fp, _fp, _fp, _fp]),
+    "psyref_sphere_light_pdf_n": (None, [C.c_float, _fp, C.c_uint32, _fp, _fp]),
+    "psyref_distant_light_sample_n": (None, [C.c_float, _fp, C.POINTER(_Color), C.c_uint32, _fp, C.c_float, _fp,
+                                             _fp, _fp]),
+    "psyref_light_tree_select_n": (C.c_int, [C.c_uint32, _fp, _fp, C.POINTER(_Shader), _fp, _fp, _fp, _fp,
+                                             C.c_float, C.c_uint32, _fp, _u32p, _fp, _fp]),
+    "psyref_light_tree_dump": (C.c_uint32, [C.c_uint32, _fp, _fp, _u32p, C.c_uint32]),
 })
+
+
+# ---- single functions, vectorised over numpy arrays (for the mathematical
+# checks of the closures, light samplers and light tree) --------------------
+
+def _v3(a, n):
+    a = f32(np.broadcast_to(np.asarray(a, np.float32), (n, 3)))
+    return a
+
+
+def _shader(kind, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
+    return _Shader(int(kind), _color(color), float(roughness), float(fresnel))
+
+
+def closure_evaluate(kind, inc, out, nor, nor_g, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
+    """SurfaceClosure::evaluate over rows: returns (filter [n,4], pdf [n])."""
+    n = max(np.asarray(a).reshape(-1, 3).shape[0] for a in (inc, out, nor, nor_g))
+    inc, out, nor, nor_g = (_v3(a, n) for a in (inc, out, nor, nor_g))
+    filt = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    sh = _shader(kind, color, roughness, fresnel)
+    lib().psyref_closure_evaluate_n(C.byref(sh), n, fptr(inc), fptr(out), fptr(nor), fptr(nor_g), wl, fptr(filt),
+                                    fptr(pdf))
+    return filt, pdf
+
+
+def closure_sample(kind, inc, nor, nor_g, uv, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
+    """SurfaceClosure::sample over rows of (u, v): returns (out [n,3], filter [n,4], pdf [n])."""
+    uv = f32(uv).reshape(-1, 2)
+    n = uv.shape[0]
+    inc, nor, nor_g = (_v3(a, n) for a in (inc, nor, nor_g))
+    out = np.zeros((n, 3), np.float32)
+    filt = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    sh = _shader(kind, color, roughness, fresnel)
+    lib().psyref_closure_sample_n(C.byref(sh), n, fptr(inc), fptr(nor), fptr(nor_g), fptr(uv), wl, fptr(out),
+                                  fptr(filt), fptr(pdf))
+    return out, filt, pdf
+
+
+def closure_estimate(kind, inc, to_light, r2, nor, nor_g, roughness=0.0):
+    """SurfaceClosure::estimate_eval_over_sphere_light over rows."""
+    r2 = f32(np.atleast_1d(r2))
+    n = max(r2.shape[0], *(np.asarray(a).reshape(-1, 3).shape[0] for a in (inc, to_light, nor, nor_g)))
+    r2 = f32(np.broadcast_to(r2, (n,)))
+    inc, to_light, nor, nor_g = (_v3(a, n) for a in (inc, to_light, nor, nor_g))
+    out = np.zeros(n, np.float32)
+    sh = _shader(kind, roughness=roughness)
+    lib().psyref_closure_estimate_n(C.byref(sh), n, fptr(inc), fptr(to_light), fptr(r2), fptr(nor), fptr(nor_g),
+                                    fptr(out))
+    return out
+
+
+IDENTITY_XF = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+
+
+def sphere_light_sample(radius, arr, uv, xform=IDENTITY_XF, color=(0, 1.0, 1.0, 1.0), wl=550.0):
+    """SphereLight::sample_from_point: returns dict(point, normal, pdf, err, color)."""
+    uv = f32(uv).reshape(-1, 2)
+    n = uv.shape[0]
+    arr = _v3(arr, n)
+    r = {k: np.zeros((n, 3), np.float32) for k in ("point", "normal")}
+    r["pdf"] = np.zeros(n, np.float32)
+    r["err"] = np.zeros(n, np.float32)
+    r["color"] = np.zeros((n, 4), np.float32)
+    c = _color(color)
+    lib().psyref_sphere_light_sample_n(radius, C.byref(c), fptr(f32(xform)), n, fptr(arr), fptr(uv), wl,
+                                       fptr(r["point"]), fptr(r["normal"]), fptr(r["pdf"]), fptr(r["err"]),
+                                       fptr(r["color"]))
+    return r
+
+
+def sphere_light_pdf(radius, arr, xform=IDENTITY_XF):
+    arr = f32(arr).reshape(-1, 3)
+    pdf = np.zeros(arr.shape[0], np.float32)
+    lib().psyref_sphere_light_pdf_n(radius, fptr(f32(xform)), arr.shape[0], fptr(arr), fptr(pdf))
+    return pdf
+
+
+def distant_light_sample(radius, direction, uv, color=(0, 1.0, 1.0, 1.0), wl=550.0):
+    """DistantDiskLight::sample_from_point: returns (color [n,4], shadow dir [n,3], pdf [n])."""
+    uv = f32(uv).reshape(-1, 2)
+    n = uv.shape[0]
+    col = np.zeros((n, 4), np.float32)
+    sv = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    c = _color(color)
+    lib().psyref_distant_light_sample_n(radius, fptr(f32(direction)), C.byref(c), n, fptr(uv), wl, fptr(col),
+                                        fptr(sv), fptr(pdf))
+    return col, sv, pdf
+
+
+def light_tree_select(boxes, energies, us, inc, pos, nor, nor_g=None, kind=0, roughness=0.0, time=0.0):
+    """Builds a light tree over boxes [n,6] with energies [n] and selects at
+    one shading point for each number in `us`: (index, prob, whittled)."""
+    boxes = f32(boxes).reshape(-1, 6)
+    energies = f32(energies)
+    us = f32(us)
+    m = us.shape[0]
+    nor_g = nor if nor_g is None else nor_g
+    idx = np.zeros(m, np.uint32)
+    prob = np.zeros(m, np.float32)
+    wh = np.zeros(m, np.float32)
+    sh = _shader(kind, roughness=roughness)
+    rc = lib().psyref_light_tree_select_n(boxes.shape[0], fptr(boxes), fptr(energies), C.byref(sh),
+                                          fptr(f32(inc)), fptr(f32(pos)), fptr(f32(nor)), fptr(f32(nor_g)), time, m,
+                                          fptr(us), idx.ctypes.data_as(_u32p), fptr(prob), fptr(wh))
+    if rc:
+        raise ValueError("no lights")
+    return idx, prob, wh
+
+
+def light_tree_dump(boxes, energies):
+    """Pre-order nodes of the light tree: rows (leaf, light|child count,
+    energy bits, min xyz bits, max xyz bits)."""
+    boxes = f32(boxes).reshape(-1, 6)
+    energies = f32(energies)
+    n = lib().psyref_light_tree_dump(boxes.shape[0], fptr(boxes), fptr(energies), None, 0)
+    out = np.zeros((n, 9), np.uint32)
+    lib().psyref_light_tree_dump(boxes.shape[0], fptr(boxes), fptr(energies), out.ctypes.data_as(_u32p), n)
+    return out
 
 
 def _color(c):

@@ -412,3 +412,169 @@ float psyref_planck(float temperature, float wavelength) { return plancks_law(te
 const char* psyref_meng_sha256() { return PSY_MENG_TABLES_SHA256; }
 
 }  // extern "C"
+
+// ---- single-function hooks that the mathematical checks of the closures,
+// light samplers and light tree run against --------------------------------
+namespace {
+Closure to_closure(const PsyrefShader* sh) {
+    Closure c;
+    c.kind = sh->kind;
+    c.color = to_color(sh->color);
+    c.roughness = sh->roughness;
+    c.fresnel = sh->fresnel;
+    return c;
+}
+Vec3 v3at(const float* p, uint32_t i) { return Vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+void put3(float* p, uint32_t i, Vec3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+void put4(float* p, uint32_t i, Vec4 v) { for (int k = 0; k < 4; k++) p[4 * i + k] = v[k]; }
+}  // namespace
+
+extern "C" {
+
+// SurfaceClosure::evaluate (surface_closure.rs:57-155) for n (inc, out, nor,
+// nor_g) quadruples, 3 floats each; filter is 4 floats per item.
+void psyref_closure_evaluate_n(const PsyrefShader* sh, uint32_t n, const float* inc, const float* out,
+                               const float* nor, const float* nor_g, float wl, float* filter, float* pdf) {
+    const Closure c = to_closure(sh);
+    for (uint32_t i = 0; i < n; i++) {
+        Vec4 f;
+        float p;
+        closure_evaluate(c, v3at(inc, i), v3at(out, i), v3at(nor, i), v3at(nor_g, i), wl, f, p);
+        put4(filter, i, f);
+        pdf[i] = p;
+    }
+}
+
+// SurfaceClosure::sample for n (inc, nor, nor_g, (u, v)) items.
+void psyref_closure_sample_n(const PsyrefShader* sh, uint32_t n, const float* inc, const float* nor,
+                             const float* nor_g, const float* uv, float wl, float* out, float* filter, float* pdf) {
+    const Closure c = to_closure(sh);
+    for (uint32_t i = 0; i < n; i++) {
+        Vec3 o;
+        Vec4 f;
+        float p;
+        closure_sample(c, v3at(inc, i), v3at(nor, i), v3at(nor_g, i), uv[2 * i], uv[2 * i + 1], wl, o, f, p);
+        put3(out, i, o);
+        put4(filter, i, f);
+        pdf[i] = p;
+    }
+}
+
+// SurfaceClosure::estimate_eval_over_sphere_light for n items.
+void psyref_closure_estimate_n(const PsyrefShader* sh, uint32_t n, const float* inc, const float* to_light,
+                               const float* r2, const float* nor, const float* nor_g, float* out) {
+    const Closure c = to_closure(sh);
+    for (uint32_t i = 0; i < n; i++)
+        out[i] = closure_estimate_sphere_light(c, v3at(inc, i), v3at(to_light, i), r2[i], v3at(nor, i),
+                                               v3at(nor_g, i));
+}
+
+// SphereLight::sample_from_point (sphere_light.rs:85-192) with the light's
+// world-to-object transform `xform` (12 floats), for n points and (u, v).
+void psyref_sphere_light_sample_n(float radius, const PsyrefColor* color, const float* xform, uint32_t n,
+                                  const float* arr, const float* uv, float wl, float* point, float* normal,
+                                  float* pdf, float* err, float* col) {
+    SphereLight l;
+    l.radius = radius;
+    l.color = to_color(*color);
+    const Transform x = xf_from(xform);
+    for (uint32_t i = 0; i < n; i++) {
+        const LightSample s = l.sample_from_point(x, v3at(arr, i), uv[2 * i], uv[2 * i + 1], wl);
+        put3(point, i, s.point);
+        put3(normal, i, s.normal);
+        pdf[i] = s.pdf;
+        err[i] = s.point_err;
+        put4(col, i, s.color);
+    }
+}
+
+// SphereLight::sample_pdf (sphere_light.rs:51-81) for n points.
+void psyref_sphere_light_pdf_n(float radius, const float* xform, uint32_t n, const float* arr, float* pdf) {
+    SphereLight l;
+    l.radius = radius;
+    const Transform x = xf_from(xform);
+    for (uint32_t i = 0; i < n; i++) pdf[i] = l.sample_pdf(x, v3at(arr, i));
+}
+
+// DistantDiskLight::sample_from_point (distant_disk_light.rs:59-85) for n
+// (u, v) pairs: colour (4 floats), shadow direction, pdf.
+void psyref_distant_light_sample_n(float radius, const float* dir, const PsyrefColor* color, uint32_t n,
+                                   const float* uv, float wl, float* col, float* shadow, float* pdf) {
+    DistantDiskLight l;
+    l.radius = radius;
+    l.direction = Vec3(dir[0], dir[1], dir[2]);
+    l.color = to_color(*color);
+    for (uint32_t i = 0; i < n; i++) {
+        Vec4 c;
+        Vec3 sv;
+        float p;
+        l.sample_from_point(uv[2 * i], uv[2 * i + 1], wl, c, sv, p);
+        put4(col, i, c);
+        put3(shadow, i, sv);
+        pdf[i] = p;
+    }
+}
+
+// A light tree (light_tree.rs:211-372) over n lights with one box each
+// (6 floats: min xyz, max xyz) and an energy each, then LightTree::select
+// (light_tree.rs:135-201) at one shading point for m numbers `us`: the
+// chosen light (its index in the input), the selection probability and the
+// whittled number.  Returns 0, or -1 when n is 0.
+int psyref_light_tree_select_n(uint32_t n, const float* boxes, const float* energies, const PsyrefShader* sh,
+                               const float* inc, const float* pos, const float* nor, const float* nor_g, float time,
+                               uint32_t m, const float* us, uint32_t* index, float* prob, float* whittled) {
+    if (n == 0) return -1;
+    std::vector<BBox> bb(n);
+    for (uint32_t i = 0; i < n; i++)
+        bb[i] = BBox(Vec3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]),
+                     Vec3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]));
+    std::vector<uint32_t> keys(n);
+    for (uint32_t i = 0; i < n; i++) keys[i] = i;
+    LightTree t;
+    t.build(keys.data(), n, [&](uint32_t k) { return BoundsSlice{&bb[k], 1}; },
+            [&](uint32_t k) { return energies[k]; });
+    const Closure c = to_closure(sh);
+    for (uint32_t i = 0; i < m; i++) {
+        size_t idx = 0;
+        float p = 0, w = 0;
+        t.select(v3at(inc, 0), v3at(pos, 0), v3at(nor, 0), v3at(nor_g, 0), c, time, us[i], idx, p, w);
+        index[i] = keys[idx];
+        prob[i] = p;
+        whittled[i] = w;
+    }
+    return 0;
+}
+
+// Pre-order dump of the same light tree: per node (leaf, light or child
+// count, energy bits, box: 6 floats as bits), 9 u32 each.  Returns the node
+// count; writes at most `cap` nodes.
+uint32_t psyref_light_tree_dump(uint32_t n, const float* boxes, const float* energies, uint32_t* out, uint32_t cap) {
+    if (n == 0) return 0;
+    std::vector<BBox> bb(n);
+    for (uint32_t i = 0; i < n; i++)
+        bb[i] = BBox(Vec3(boxes[6 * i], boxes[6 * i + 1], boxes[6 * i + 2]),
+                     Vec3(boxes[6 * i + 3], boxes[6 * i + 4], boxes[6 * i + 5]));
+    std::vector<uint32_t> keys(n);
+    for (uint32_t i = 0; i < n; i++) keys[i] = i;
+    LightTree t;
+    t.build(keys.data(), n, [&](uint32_t k) { return BoundsSlice{&bb[k], 1}; },
+            [&](uint32_t k) { return energies[k]; });
+    uint32_t count = 0;
+    std::function<void(const LightTree::Node&)> walk = [&](const LightTree::Node& nd) {
+        if (count < cap) {
+            uint32_t* o = out + 9 * count;
+            o[0] = nd.leaf ? 1u : 0u;
+            o[1] = nd.leaf ? keys[nd.light_index] : (uint32_t)nd.children.size();
+            o[2] = f2u(nd.energy);
+            const BBox b = nd.bounds.front();
+            o[3] = f2u(b.min.x); o[4] = f2u(b.min.y); o[5] = f2u(b.min.z);
+            o[6] = f2u(b.max.x); o[7] = f2u(b.max.y); o[8] = f2u(b.max.z);
+        }
+        count++;
+        for (const LightTree::Node& ch : nd.children) walk(ch);
+    };
+    walk(t.root);
+    return count;
+}
+
+}  // extern "C"

@@ -211,7 +211,43 @@ def test_meng_upsampling_round_trips(rgb):
 
 
 def test_meng_tables_are_the_generated_ones():
-    assert len(L.psyref_meng_sha256()) == 64
+    """The committed Meng header's values hash to the checksum it records
+    (taken by its generator over the values parsed from the reference's
+    meng_spectra_tables.rs), and the oracle was built from it."""
+    import hashlib
+    import os
+    import re
+    import struct
+    path = os.path.join(os.path.dirname(__file__), "..", "psychopath_amd", "csrc", "meng_tables.h")
+    text = open(path).read()
+    stated = re.search(r'PSY_MENG_TABLES_SHA256 "([0-9a-f]{64})"', text).group(1)
+    assert L.psyref_meng_sha256().decode() == stated
+
+    def floats(block):
+        return [float(x) for x in re.findall(r"-?\d+\.\d+(?:e[-+]?\d+)?", block)]
+
+    def section(name):
+        return text[text.index(name):].split("};", 1)[0].split("{", 1)[1]
+
+    h = hashlib.sha256()
+    eer = float(re.search(r"EQUAL_ENERGY_REFLECTANCE = ([-0-9.e+]+)f", text).group(1))
+    for x in [eer] + floats(section("MAT_XY_TO_UV")):
+        h.update(struct.pack("<f", x))
+    grid = section("GRID[168][8]")
+    rows = re.findall(r"\{([-0-9, ]+)\}", grid)
+    assert len(rows) == 168
+    for row in rows:
+        vals = [int(x) for x in row.split(",")]
+        h.update(struct.pack("<ii", vals[0], vals[1]))
+        for i in vals[2:]:
+            h.update(struct.pack("<i", i))
+    uv = floats(section("POINT_UV"))
+    spec = floats(section("POINT_SPECTRUM"))
+    assert len(uv) == 186 * 2 and len(spec) == 186 * 95
+    for k in range(186):
+        for x in uv[2 * k:2 * k + 2] + spec[95 * k:95 * k + 95]:
+            h.update(struct.pack("<f", x))
+    assert h.hexdigest() == stated
 
 
 # ---- integrator ------------------------------------------------------------
