@@ -207,13 +207,18 @@ typedef struct {
 
 int psy_server_create(psy_renderer* const* renderers, uint32_t count, const psy_server_config* config,
                       psy_server** out);
-/* Waits for every submitted request to finish, then releases the server. */
+/* Releases the server: requests not yet admitted are dropped, no further
+ * waves start, and the waves already on the device finish first.  Results
+ * not collected by psy_server_wait are discarded. */
 int psy_server_destroy(psy_server* server);
 /* Queues a render of `renderer` (an index into the creation list). */
 int psy_server_submit(psy_server* server, uint32_t renderer, const psy_camera_desc* camera,
                       const psy_render_settings* settings, uint64_t* request_id);
 /* Blocks until the request is complete, copies its XYZ image to host memory
- * (`xyz` may be NULL to discard it) and releases the request. */
+ * (`xyz` may be NULL to discard it) and releases the request.  A request
+ * whose film cannot be allocated while no other request runs (so no memory
+ * will free up) completes with PSY_E_NOMEM.  A psy_server_read of the same
+ * request may run concurrently: the film is released only after it. */
 int psy_server_wait(psy_server* server, uint64_t request_id, float* xyz, psy_request_stats* stats);
 /* Incremental delivery: the number of leading pixels of request `id`
  * (row-major from the top row of its crop region, or of the image) that are
@@ -227,6 +232,11 @@ int psy_server_progress(psy_server* server, uint64_t id, uint64_t* pixels_done);
 int psy_server_read(psy_server* server, uint64_t id, uint64_t pixel_begin, uint64_t pixel_end, float* xyz);
 
 int psy_server_get_stats(psy_server* server, psy_server_stats* stats);
+
+/* Test hook (no counterpart in the reference): the next `count` film
+ * allocations of the server fail as if device memory were exhausted, so the
+ * error path above can be exercised without exhausting the device. */
+int psy_server_fail_film_allocs(psy_server* server, uint32_t count);
 
 #ifdef __cplusplus
 }

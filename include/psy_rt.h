@@ -149,18 +149,34 @@ typedef struct {
     const uint8_t* flags;
 } psy_ray_soa;
 
-/* Hit records as structure-of-arrays.  Barycentrics refer to the stored
- * (winding-swapped) vertex order (src/surface/triangle_mesh.rs:80): b0 to the
- * first vertex, b1 to the file's third, b2 to the file's second.  prim_id is
- * the triangle's index in the mesh description, inst_id the instance's index
- * in psy_scene_desc.instances.  On a light instance prim_id is the
- * rectangle's triangle (0: corners 1,2,3; 1: corners 3,4,1,
- * rectangle_light.rs:288) or 0 for a sphere, whose barycentrics are 0.
- * Fields other than `kind` are written for PSY_HIT_SURFACE only. */
+/* One hit record, 32 bytes: the array-of-records form of the output, one
+ * record per ray like the slice of SurfaceIntersection that Tracer::trace
+ * returns (src/tracer.rs:34-37, src/surface/mod.rs:72-92).  Every field is
+ * written: for a miss or an occlusion t and the barycentrics are 0 and the
+ * ids PSY_NONE. */
+typedef struct {
+    float t, b0, b1, b2;
+    uint32_t prim_id, inst_id;
+    uint32_t kind; /* PSY_HIT_* */
+    uint32_t reserved;
+} psy_hit_record;
+
+/* Hit output.  Barycentrics refer to the stored (winding-swapped) vertex
+ * order (src/surface/triangle_mesh.rs:80): b0 to the first vertex, b1 to the
+ * file's third, b2 to the file's second.  prim_id is the triangle's index in
+ * the mesh description, inst_id the instance's index in
+ * psy_scene_desc.instances.  On a light instance prim_id is the rectangle's
+ * triangle (0: corners 1,2,3; 1: corners 3,4,1, rectangle_light.rs:288) or 0
+ * for a sphere, whose barycentrics are 0.
+ * When `records` is set the hits are written there (16-byte aligned, one
+ * psy_hit_record per ray) and the arrays may be NULL; otherwise they go to
+ * the arrays as structure-of-arrays, where fields other than `kind` are
+ * written for PSY_HIT_SURFACE only. */
 typedef struct {
     uint8_t* kind;
     float *t, *b0, *b1, *b2;
     uint32_t *prim_id, *inst_id;
+    psy_hit_record* records;
 } psy_hit_soa;
 
 /* Per-scene figures, including the record sizes used for the algorithmic-byte

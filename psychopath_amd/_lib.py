@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpsy_rt.so")
 
 PSY_OK = 0
+PSY_E_INVALID, PSY_E_RANGE, PSY_E_DEVICE, PSY_E_NOMEM, PSY_E_STATE = -1, -2, -3, -4, -5
 PSY_HIT_MISS = 0
 PSY_HIT_OCCLUDE = 1
 PSY_HIT_SURFACE = 2
@@ -155,6 +156,7 @@ class HitSoA(C.Structure):
         ("kind", C.c_void_p),
         ("t", C.c_void_p), ("b0", C.c_void_p), ("b1", C.c_void_p), ("b2", C.c_void_p),
         ("prim_id", C.c_void_p), ("inst_id", C.c_void_p),
+        ("records", C.c_void_p),
     ]
 
 
@@ -266,6 +268,7 @@ _SIGNATURES.update({
     "psy_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "psy_server_progress": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "psy_server_read": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "psy_server_fail_film_allocs": (C.c_int, [C.c_void_p, C.c_uint32]),
 })
 
 

@@ -43,6 +43,7 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.light_list, &s->light_list);
     if (!r) r = upload(h.normals, &s->normals);
     if (!r) r = upload(h.nidx, &s->nidx);
+    if (!r) r = upload(h.order_table, &s->order_table);
     if (r == PSY_OK) s->uploaded = true;
     return r;
 }
@@ -59,6 +60,7 @@ TraceArgs scene_args(const psy_scene* s) {
     a.rect_lights = s->rect_lights;
     a.sphere_lights = s->sphere_lights;
     a.light_samples = s->light_samples;
+    a.order_table = s->order_table;
     a.top_root = s->host.top_root;
     return a;
 }
@@ -103,6 +105,7 @@ void set_rays(TraceArgs& a, const psy_ray_soa* r) {
     a.time = r->time; a.max_t = r->max_t; a.flags = r->flags;
 }
 void set_hits(TraceArgs& a, const psy_hit_soa* h) {
+    a.rec = reinterpret_cast<uint4*>(h->records);
     a.kind = h->kind;
     a.t = h->t; a.b0 = h->b0; a.b1 = h->b1; a.b2 = h->b2;
     a.prim_id = h->prim_id; a.inst_id = h->inst_id;
@@ -111,6 +114,11 @@ int check_soa(const psy_ray_soa* r, const psy_hit_soa* h) {
     if (!r || !h) return fail(PSY_E_INVALID, "rays or hits is NULL");
     if (!r->ox || !r->oy || !r->oz || !r->dx || !r->dy || !r->dz || !r->time || !r->max_t)
         return fail(PSY_E_INVALID, "a ray array is NULL");
+    if (h->records) {
+        if (reinterpret_cast<uintptr_t>(h->records) % 16 != 0)
+            return fail(PSY_E_INVALID, "hit records must be 16-byte aligned");
+        return PSY_OK;
+    }
     if (!h->kind || !h->t || !h->b0 || !h->b1 || !h->b2 || !h->prim_id || !h->inst_id)
         return fail(PSY_E_INVALID, "a hit array is NULL");
     return PSY_OK;
@@ -136,8 +144,9 @@ int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hi
 int trace_host(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits) {
     const uint64_t B = tr->max_batch;
     if (!tr->staging) {
-        // 8 f32 + 1 u8 per ray in, 1 u8 + 4 f32 + 2 u32 per ray out.
-        PSY_HIP(hipMalloc(&tr->staging, B * (8 * 4 + 1 + 1 + 6 * 4) + 256));
+        // 8 f32 + 1 u8 per ray in; out 1 u8 + 4 f32 + 2 u32 per ray, or a
+        // 32-byte record (placed after the SoA part, 16-byte aligned).
+        PSY_HIP(hipMalloc(&tr->staging, B * (8 * 4 + 1 + 1 + 6 * 4) + B * sizeof(psy_hit_record) + 256));
     }
     uint8_t* base = static_cast<uint8_t*>(tr->staging);
     float* f = reinterpret_cast<float*>(base);
@@ -146,7 +155,9 @@ int trace_host(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa*
     uint32_t* hu = reinterpret_cast<uint32_t*>(hf + 4 * B);
     uint8_t* flags = reinterpret_cast<uint8_t*>(hu + 2 * B);
     uint8_t* kind = flags + B;
-    psy_hit_soa dh{kind, hf, hf + B, hf + 2 * B, hf + 3 * B, hu, hu + B};
+    const uint64_t rec_off = (B * (8 * 4 + 1 + 1 + 6 * 4) + 15) & ~uint64_t(15);
+    psy_hit_record* drec = hits->records ? reinterpret_cast<psy_hit_record*>(base + rec_off) : nullptr;
+    psy_hit_soa dh{kind, hf, hf + B, hf + 2 * B, hf + 3 * B, hu, hu + B, drec};
 
     for (uint64_t s = 0; s < n; s += B) {
         uint64_t m = std::min(B, n - s);
@@ -162,6 +173,12 @@ int trace_host(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa*
         }
         int rc = enqueue(tr, &dr, m, &dh, false);
         if (rc) return rc;
+        if (drec) {
+            PSY_HIP(hipMemcpyAsync(hits->records + s, drec, m * sizeof(psy_hit_record), hipMemcpyDeviceToHost,
+                                   tr->stream));
+            PSY_HIP(hipStreamSynchronize(tr->stream));
+            continue;
+        }
         PSY_HIP(hipMemcpyAsync(hits->kind + s, dh.kind, m, hipMemcpyDeviceToHost, tr->stream));
         float* hsrc[4] = {dh.t, dh.b0, dh.b1, dh.b2};
         float* hdst[4] = {hits->t, hits->b0, hits->b1, hits->b2};
@@ -212,7 +229,7 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
                       h.light_samples.size() * sizeof(float) +
                       (h.inst_pos.size() + h.tri_rec.size() + h.light_list.size()) * 4 +
                       h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox) +
-                      (h.normals.size() + h.nidx.size()) * 4;
+                      (h.normals.size() + h.nidx.size()) * 4 + h.order_table.size();
     *out = s;
     return PSY_OK;
 }

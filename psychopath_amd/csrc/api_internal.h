@@ -55,13 +55,14 @@ struct psy_scene {
     uint32_t* light_list = nullptr;
     float* normals = nullptr;
     uint32_t* nidx = nullptr;
+    uint8_t* order_table = nullptr;
     uint64_t device_bytes = 0;
     bool uploaded = false;
 
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,
                         sphere_lights, light_samples, inst_pos, tri_rec, light_nodes, light_boxes,
-                        light_list, normals, nidx};
+                        light_list, normals, nidx, order_table};
         for (void* b : bufs) (void)hipFree(b);
     }
 };

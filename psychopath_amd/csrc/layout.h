@@ -8,13 +8,22 @@
 namespace psy {
 
 // Child references inside a BVH4 node.
-//   internal child: node index (bit 31 clear)
-//   leaf child:     bit 31 set, bits 2..30 first object, bits 0..1 count
+//   internal child: node index (bit 31 clear, bits 0..27)
+//   leaf child:     bit 31 set, bits 2..27 first object, bits 0..1 count
+//   absent child:   REF_EMPTY
 // Mesh leaves hold 1..3 triangles (src/surface/triangle_mesh.rs:17); instance
 // leaves hold exactly one instance (src/scene/assembly.rs:293).
+// In the device node array bits 28..30 of the first two references (a node
+// always has at least two children) carry the node's 6-bit traversal code,
+// low three bits in child 0, so a static node's visit order comes from the
+// order table without loading the node's last 16 bytes; device code masks
+// them off (child_ref) before following a reference.
 constexpr uint32_t REF_LEAF = 0x80000000u;
 constexpr uint32_t REF_EMPTY = 0xffffffffu;
-constexpr uint32_t MAX_LEAF_FIRST = (1u << 29) - 1;
+constexpr uint32_t REF_CODE_MASK = 0x70000000u;
+constexpr uint32_t REF_CODE_SHIFT = 28;
+constexpr uint32_t MAX_LEAF_FIRST = (1u << 26) - 1;
+constexpr uint32_t MAX_NODES = 1u << 28;
 
 static inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) { return REF_LEAF | (first << 2) | count; }
 

@@ -493,6 +493,20 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     uint32_t mesh_depth = 0;
     for (uint32_t m = 0; m < d->mesh_count; m++) mesh_depth = std::max(mesh_depth, s.bvhs[m].depth);
     s.max_stack = 3 * (top.depth + mesh_depth) + 2;
+
+    // Device form of the node array: each node's traversal code also rides
+    // in bits 28..30 of its first two child references (layout.h), and the
+    // order table it indexes travels with the scene.
+    if (s.nodes.size() > MAX_NODES) {
+        err = "the scene's BVHs have more than 2^28 nodes";
+        return PSY_E_RANGE;
+    }
+    for (GpuNode& n : s.nodes) {
+        const uint32_t code = (n.meta >> 16) & 0x3fu;
+        n.child[0] |= (code & 7u) << REF_CODE_SHIFT;
+        n.child[1] |= (code >> 3) << REF_CODE_SHIFT;
+    }
+    s.order_table.assign(traversal_order_table(), traversal_order_table() + 8 * 48);
     s.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PSY_OK;
 }

@@ -24,6 +24,7 @@ struct PackedScene {
     std::vector<GpuXform> inv_static;
     std::vector<float> normals;        // xyz per (vertex, time sample), see GpuMesh
     std::vector<uint32_t> nidx;        // stored-order vertex triples per triangle
+    std::vector<uint8_t> order_table;  // TRAVERSAL_TABLE[octant][code], 8 x 48 bytes
     uint32_t top_root = REF_EMPTY;
 
     // Lights and shading lookups.

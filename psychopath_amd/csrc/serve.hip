@@ -47,8 +47,10 @@ struct Request {
     float* film = nullptr;
     uint32_t slot = 0xffffffffu;
     uint64_t waves = 0;
+    uint32_t readers = 0;  // psy_server_read copies running from the film
     bool admitted = false, done = false;
     int error = PSY_OK;
+    std::string msg;       // why the request failed, when it did
     Clock::time_point t_submit, t_admit, t_done;
 };
 
@@ -91,6 +93,7 @@ struct psy_server {
     int worker_error = PSY_OK;
     std::string worker_msg;
     std::thread worker;
+    uint32_t fail_allocs = 0;  // film allocations to fail on purpose (psy_server_fail_film_allocs)
 
     uint64_t waves = 0, paths = 0;
 };
@@ -130,8 +133,25 @@ float* film_alloc(psy_server* sv, uint64_t bytes) {
         sv->pool.erase(victim);
     }
     float* p = nullptr;
-    if (hipMalloc((void**)&p, bytes) != hipSuccess) return nullptr;
-    return p;
+    if (sv->fail_allocs > 0) {
+        sv->fail_allocs--;
+        return nullptr;
+    }
+    if (hipMalloc((void**)&p, bytes) == hipSuccess) return p;
+    // Device memory is short: give the pooled films back and try once more.
+    for (auto& kv : sv->pool) (void)hipFree(kv.second);
+    sv->pool.clear();
+    sv->pool_bytes = 0;
+    if (hipMalloc((void**)&p, bytes) == hipSuccess) return p;
+    return nullptr;
+}
+
+// True while some renderer has admitted requests or waves in flight, whose
+// completion can still free device memory.
+bool work_pending(const psy_server* sv) {
+    for (const RenState& rs : sv->rens)
+        if (!rs.active.empty() || !rs.inflight.empty()) return true;
+    return false;
 }
 
 // Admits queued requests in order while budget and slots allow (called with
@@ -143,7 +163,18 @@ int admit(psy_server* sv) {
         if (rs.free_slots.empty()) break;
         if (sv->budget && sv->fixed_bytes + sv->film_bytes + q->film_bytes > sv->budget) break;
         q->film = film_alloc(sv, q->film_bytes);
-        if (!q->film) break;  // retry when memory frees up
+        if (!q->film) {
+            // Retry once running requests free memory; with nothing running
+            // nothing will, so the request fails instead of waiting forever.
+            if (work_pending(sv)) break;
+            sv->queued.pop_front();
+            q->done = true;
+            q->error = PSY_E_NOMEM;
+            q->msg = "no device memory for the request's film";
+            q->t_admit = q->t_done = Clock::now();
+            sv->cv_done.notify_all();
+            continue;
+        }
         sv->queued.pop_front();
         q->slot = rs.free_slots.back();
         rs.free_slots.pop_back();
@@ -220,6 +251,18 @@ int launch(psy_server* sv, RenState& rs, bool& launched) {
 void worker_main(psy_server* sv) {
     std::unique_lock<std::mutex> lk(sv->mu);
     for (;;) {
+        if (sv->stop) {
+            // psy_server_destroy: requests not yet admitted are dropped and
+            // no further waves start; the waves in flight still finish
+            // before the worker returns.
+            for (Request* q : sv->queued) {
+                q->done = true;
+                q->error = PSY_E_STATE;
+                q->msg = "server destroyed";
+            }
+            sv->queued.clear();
+            for (RenState& rs : sv->rens) rs.active.clear();
+        }
         bool busy = false;
         int rc = admit(sv);
         bool finished = false;
@@ -245,11 +288,7 @@ void worker_main(psy_server* sv) {
             sv->cv_done.notify_all();
             return;
         }
-        if (sv->stop && !busy && sv->queued.empty()) {
-            bool pending = false;
-            for (RenState& rs : sv->rens) pending = pending || !rs.active.empty();
-            if (!pending) return;
-        }
+        if (sv->stop && !busy) return;
         if (busy) {
             // Waves are running: poll their events without holding the lock
             // for long.
@@ -387,6 +426,7 @@ int psy_server_wait(psy_server* sv, uint64_t id, float* xyz, psy_request_stats* 
         sv->reqs.erase(it);
     }
     int rc = q->error;
+    const std::string msg = q->msg.empty() ? sv->worker_msg : q->msg;
     hipError_t e = hipSuccess;
     if (!rc && xyz) e = hipMemcpy(xyz, q->film, q->film_bytes, hipMemcpyDeviceToHost);
     if (stats) {
@@ -397,8 +437,10 @@ int psy_server_wait(psy_server* sv, uint64_t id, float* xyz, psy_request_stats* 
     }
     {
         // The film goes back to the pool (hipFree would synchronise the
-        // device under the other requests' waves).
-        std::lock_guard<std::mutex> lk(sv->mu);
+        // device under the other requests' waves) once no psy_server_read
+        // copies from it.
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_done.wait(lk, [&] { return q->readers == 0; });
         if (q->admitted) {
             sv->film_bytes -= q->film_bytes;
             sv->pool.emplace(q->film_bytes, q->film);
@@ -407,7 +449,7 @@ int psy_server_wait(psy_server* sv, uint64_t id, float* xyz, psy_request_stats* 
     }
     sv->cv_work.notify_all();
     delete q;
-    if (rc) return fail(rc, "request failed: " + sv->worker_msg);
+    if (rc) return fail(rc, "request failed: " + msg);
     if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("image copy: ") + hipGetErrorString(e));
     return PSY_OK;
 }
@@ -429,28 +471,45 @@ int psy_server_read(psy_server* sv, uint64_t id, uint64_t pixel_begin, uint64_t 
     if (!sv || (!xyz && pixel_end > pixel_begin)) return fail(PSY_E_INVALID, "NULL argument");
     const float* film;
     ReqParams rp;
+    Request* q;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         auto it = sv->reqs.find(id);
         if (it == sv->reqs.end()) return fail(PSY_E_INVALID, "unknown request id");
-        const Request* q = it->second;
-        if (q->error) return fail(q->error, "request failed: " + sv->worker_msg);
+        q = it->second;
+        if (q->error) return fail(q->error, "request failed: " + (q->msg.empty() ? sv->worker_msg : q->msg));
         const uint64_t done = q->done ? q->total / q->rp.spp : q->retired / q->rp.spp;
         if (pixel_begin > pixel_end || pixel_end > done) return fail(PSY_E_RANGE, "pixels not finished yet");
         film = q->film;
         rp = q->rp;
+        q->readers++;
     }
-    // The film stays allocated until psy_server_wait collects the request;
-    // the finished pixels are no longer written, so the copy can run beside
-    // the waves still rendering the rest.  Region pixels are contiguous in
-    // the film within a row of the region.
+    // The film stays with the request while this copy runs (the reader
+    // count holds psy_server_wait from pooling it); the finished pixels are
+    // no longer written, so the copy can run beside the waves still
+    // rendering the rest.  Region pixels are contiguous in the film within a
+    // row of the region.
+    hipError_t e = hipSuccess;
     uint64_t p = pixel_begin;
-    while (p < pixel_end) {
+    while (p < pixel_end && e == hipSuccess) {
         const uint64_t row_end = std::min<uint64_t>(pixel_end, (p / rp.rw + 1) * rp.rw);
-        PSY_HIP(hipMemcpy(xyz + 3 * (p - pixel_begin), film + 3 * region_pixel(rp, p), (row_end - p) * 3 * sizeof(float),
-                          hipMemcpyDeviceToHost));
+        e = hipMemcpy(xyz + 3 * (p - pixel_begin), film + 3 * region_pixel(rp, p), (row_end - p) * 3 * sizeof(float),
+                      hipMemcpyDeviceToHost);
         p = row_end;
     }
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        q->readers--;
+    }
+    sv->cv_done.notify_all();
+    if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("film copy: ") + hipGetErrorString(e));
+    return PSY_OK;
+}
+
+int psy_server_fail_film_allocs(psy_server* sv, uint32_t count) {
+    if (!sv) return fail(PSY_E_INVALID, "server is NULL");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->fail_allocs = count;
     return PSY_OK;
 }
 

@@ -11,7 +11,10 @@ namespace psy {
 
 constexpr int TRACE_BLOCK = 128;           // lanes per workgroup (two wave64s)
 constexpr int TRACE_LDS_DEPTH = 24;        // stack entries per lane held in LDS
-constexpr int TRACE_STACK_CAPACITY = 96;   // total stack entries per lane
+// Total stack entries per lane: the bound 3 * (instance-BVH depth + mesh-BVH
+// depth) + 2 for two BVH4s of the deepest the builder makes (depth 22, from
+// a binary depth of at most 42: src/accel/bvh_base.rs:7, bvh4.rs:87).
+constexpr int TRACE_STACK_CAPACITY = 3 * (22 + 22) + 2;
 
 struct TraceArgs {
     // scene
@@ -25,6 +28,7 @@ struct TraceArgs {
     const GpuRectLight* rect_lights;
     const GpuSphereLight* sphere_lights;
     const float* light_samples;  // time samples of the lights (layout.h)
+    const uint8_t* order_table;  // TRAVERSAL_TABLE[octant][code], 8 x 48 bytes
     uint32_t top_root;
     // rays (device)
     const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;
@@ -32,7 +36,9 @@ struct TraceArgs {
     uint32_t all_occlusion;    // every ray is an occlusion ray (flags ignored)
     uint64_t n;                // ray count, or the capacity when n_dev is set
     const uint32_t* n_dev;     // device-resident ray count (persistent kernel), or NULL
-    // hits (device)
+    // hits (device): 32-byte records when `rec` is set, else structure of
+    // arrays
+    uint4* rec;
     uint8_t* kind;
     float *t, *b0, *b1, *b2;
     uint32_t *prim_id, *inst_id;

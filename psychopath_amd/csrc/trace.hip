@@ -35,14 +35,22 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     TraceCount cnt;
     const TraceHit h = trace_one<COUNT, LDS_DEPTH, SPILL_DEPTH>(a, o, d, a.time[ray], a.max_t[ray], occl,
                                                                  lds_stack + threadIdx.x, BLOCK, cnt);
-    a.kind[ray] = h.kind;
-    if (h.kind == PSY_HIT_SURFACE) {
-        a.t[ray] = h.t;
-        a.b0[ray] = h.b0;
-        a.b1[ray] = h.b1;
-        a.b2[ray] = h.b2;
-        a.prim_id[ray] = h.prim;
-        a.inst_id[ray] = h.inst;
+    if (a.rec != nullptr) {
+        const bool s = h.kind == PSY_HIT_SURFACE;
+        uint4* r = a.rec + 2ull * ray;
+        r[0] = make_uint4(s ? __float_as_uint(h.t) : 0u, s ? __float_as_uint(h.b0) : 0u,
+                          s ? __float_as_uint(h.b1) : 0u, s ? __float_as_uint(h.b2) : 0u);
+        r[1] = make_uint4(s ? h.prim : PSY_NONE, s ? h.inst : PSY_NONE, (uint32_t)h.kind, 0u);
+    } else {
+        a.kind[ray] = h.kind;
+        if (h.kind == PSY_HIT_SURFACE) {
+            a.t[ray] = h.t;
+            a.b0[ray] = h.b0;
+            a.b1[ray] = h.b1;
+            a.b2[ray] = h.b2;
+            a.prim_id[ray] = h.prim;
+            a.inst_id[ray] = h.inst;
+        }
     }
     if (COUNT) {
         atomicAdd(a.counters + 0, (unsigned long long)cnt.nodes);

@@ -85,8 +85,10 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
             // Internal node: test the four child boxes (bvh4.rs:120-169).
             const GpuNode* node = a.nodes + cur;
             const float4* nf = reinterpret_cast<const float4*>(node);
-            const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
+            uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
             const uint4 tail = *reinterpret_cast<const uint4*>(nf + 7);
+            refs.x &= ~REF_CODE_MASK;  // the traversal code's bits (layout.h)
+            refs.y &= ~REF_CODE_MASK;
             const uint32_t meta = tail.z;
             const uint32_t cnt = meta & 7u;
             const uint32_t blen = (meta >> 8) & 0xffu;

@@ -58,8 +58,17 @@ template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
+    // TRAVERSAL_TABLE (8 octants x 48 codes): static nodes carry only their
+    // code, in spare bits of their child references (layout.h).
+    __shared__ uint32_t lds_order[8 * 48 / 4];
     const int lane = threadIdx.x;
     uint32_t* const lds_col = lds_stack + lane;
+    if (!MOTION) {
+        const uint32_t* tab = reinterpret_cast<const uint32_t*>(a.order_table);
+        for (int i = lane; i < 8 * 48 / 4; i += WAVE) lds_order[i] = tab[i];
+        __syncthreads();
+    }
+    const uint8_t* const order_lds = reinterpret_cast<const uint8_t*>(lds_order);
     uint32_t spill[TRACE_STACK_CAPACITY];
     int sp = 0, bot = 0;
 
@@ -143,9 +152,19 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         // child boxes, then the hit children pushed in visit order.
         auto node_step = [&]() {
                 const float4* nf = reinterpret_cast<const float4*>(a.nodes + cur);
-                const uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
-                const uint4 tail = *reinterpret_cast<const uint4*>(nf + 7);
-                const uint32_t cnt = tail.z & 7u;
+                uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
+                // Static scenes read seven of the node's eight 16-byte parts:
+                // the child count follows from the absent references and the
+                // visit order from the traversal code in the references' spare
+                // bits; time-sampled scenes read the last part for the bounds'
+                // sample count and location.
+                uint4 tail = make_uint4(0u, 0u, 0u, 0u);
+                if (MOTION) tail = *reinterpret_cast<const uint4*>(nf + 7);
+                const uint32_t code = ((refs.x >> REF_CODE_SHIFT) & 7u) | (((refs.y >> REF_CODE_SHIFT) & 7u) << 3);
+                refs.x &= ~REF_CODE_MASK;
+                refs.y &= ~REF_CODE_MASK;
+                const uint32_t cnt =
+                    MOTION ? (tail.z & 7u) : 2u + (refs.z != REF_EMPTY ? 1u : 0u) + (refs.w != REF_EMPTY ? 1u : 0u);
                 float4 lx, hx, ly, hy, lz, hz;
                 const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
                 if (!MOTION || blen <= 1) {
@@ -173,7 +192,8 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                 mask |= slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, lo, di, tmax) ? 4u : 0u;
                 mask |= slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, lo, di, tmax) ? 8u : 0u;
                 mask &= (1u << cnt) - 1u;
-                const uint32_t ord = ((oct < 4 ? tail.x : tail.y) >> (8u * (oct & 3u))) & 0xffu;
+                const uint32_t ord = MOTION ? ((oct < 4 ? tail.x : tail.y) >> (8u * (oct & 3u))) & 0xffu
+                                            : (uint32_t)order_lds[oct * 48u + code];
                 uint32_t next = REF_EMPTY;
                 for (int k = 3; k >= 0; --k) {
                     if (k >= (int)cnt) continue;
@@ -334,14 +354,26 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         }
 
         if (finished) {
-            a.kind[ray] = kind;
-            if (kind == PSY_HIT_SURFACE) {
-                a.t[ray] = ht;
-                a.b0[ray] = hb0;
-                a.b1[ray] = hb1;
-                a.b2[ray] = hb2;
-                a.prim_id[ray] = hprim;
-                a.inst_id[ray] = hinst;
+            if (a.rec != nullptr) {
+                // One 32-byte record per ray, two 16-byte stores: rays finish
+                // out of order, and a whole record fills a memory sector
+                // where the seven arrays of the SoA form leave seven partial
+                // lines.
+                const bool s = kind == PSY_HIT_SURFACE;
+                uint4* r = a.rec + 2ull * ray;
+                r[0] = make_uint4(s ? __float_as_uint(ht) : 0u, s ? __float_as_uint(hb0) : 0u,
+                                  s ? __float_as_uint(hb1) : 0u, s ? __float_as_uint(hb2) : 0u);
+                r[1] = make_uint4(s ? hprim : PSY_NONE, s ? hinst : PSY_NONE, (uint32_t)kind, 0u);
+            } else {
+                a.kind[ray] = kind;
+                if (kind == PSY_HIT_SURFACE) {
+                    a.t[ray] = ht;
+                    a.b0[ray] = hb0;
+                    a.b1[ray] = hb1;
+                    a.b2[ray] = hb2;
+                    a.prim_id[ray] = hprim;
+                    a.inst_id[ray] = hinst;
+                }
             }
             has = false;
         }

@@ -223,6 +223,11 @@ class Server:
                 time.sleep(poll_s)
         self.wait(request_id, keep_image=False)
 
+    def fail_film_allocs(self, count: int) -> None:
+        """Test hook: the next `count` film allocations fail as if device
+        memory were exhausted."""
+        check("psy_server_fail_film_allocs", lib().psy_server_fail_film_allocs(self.handle, int(count)))
+
     def stats(self) -> dict:
         st = _lib.ServerStats()
         check("psy_server_get_stats", lib().psy_server_get_stats(self.handle, C.byref(st)))

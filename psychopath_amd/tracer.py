@@ -22,6 +22,10 @@ from ._lib import check, lib
 RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "time", "max_t")
 HIT_FLOATS = ("t", "b0", "b1", "b2")
 HIT_U32 = ("prim_id", "inst_id")
+# psy_hit_record: one 32-byte record per ray.
+HIT_RECORD = np.dtype([("t", "<f4"), ("b0", "<f4"), ("b1", "<f4"), ("b2", "<f4"), ("prim_id", "<u4"),
+                       ("inst_id", "<u4"), ("kind", "<u4"), ("reserved", "<u4")])
+assert HIT_RECORD.itemsize == 32
 
 
 def _f32(a) -> np.ndarray:
@@ -278,6 +282,15 @@ class Hits:
             setattr(h, f, getattr(self, f).ctypes.data)
         return h
 
+    @classmethod
+    def from_records(cls, rec: np.ndarray) -> "Hits":
+        """The SoA view of an array of HIT_RECORD."""
+        out = cls.__new__(cls)
+        out.kind = rec["kind"].astype(np.uint8)
+        for f in HIT_FLOATS + HIT_U32:
+            setattr(out, f, np.ascontiguousarray(rec[f]))
+        return out
+
 
 class DeviceBuffer:
     """A raw HBM allocation (psy_device_alloc)."""
@@ -355,6 +368,27 @@ class DeviceHits:
         return out
 
 
+class DeviceHitRecords:
+    """Hit records (psy_hit_record, 32 bytes per ray) resident in HBM."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.buf = DeviceBuffer(n * HIT_RECORD.itemsize + 64)
+
+    def soa(self) -> _lib.HitSoA:
+        h = _lib.HitSoA()
+        h.records = self.buf.ptr
+        return h
+
+    def records(self) -> np.ndarray:
+        out = np.zeros(self.n, HIT_RECORD)
+        self.buf.download(out)
+        return out
+
+    def to_host(self) -> Hits:
+        return Hits.from_records(self.records())
+
+
 class Tracer:
     """Tracer::from_assembly / trace / rays_traced (tracer.rs:15-42)."""
 
@@ -368,7 +402,17 @@ class Tracer:
     def from_assembly(cls, scene: Scene, max_batch: int = 1 << 22) -> "Tracer":
         return cls(scene, max_batch)
 
-    def trace(self, rays: RayBatch) -> Hits:
+    def trace(self, rays: RayBatch, records: bool = False) -> Hits:
+        """Traces host rays; `records` has the library write 32-byte hit
+        records (returned as their SoA view) instead of arrays."""
+        if records:
+            rec = np.zeros(len(rays), HIT_RECORD)
+            hs = _lib.HitSoA()
+            hs.records = rec.ctypes.data
+            rs = rays.soa()
+            check("psy_trace", lib().psy_trace(self.handle, C.byref(rs), len(rays), C.byref(hs),
+                                               _lib.PSY_MEM_HOST))
+            return Hits.from_records(rec)
         hits = Hits(len(rays))
         rs, hs = rays.soa(), hits.soa()
         check("psy_trace", lib().psy_trace(self.handle, C.byref(rs), len(rays), C.byref(hs), _lib.PSY_MEM_HOST))

@@ -96,8 +96,9 @@ def test_scene_info_and_dump_on_host():
     assert info["tri_count"] == 256
     assert info["node_bytes"] == 128 and info["tri_bytes"] == 48 and info["motion_bounds_bytes"] == 96
     # nodes, triangle records, mesh and instance records, the instance
-    # position map and the triangle -> record map
-    assert info["device_bytes"] == info["node_count"] * 128 + 256 * 48 + 32 + 32 + 4 + 256 * 4
+    # position map, the triangle -> record map and the 8 x 48 traversal
+    # order table
+    assert info["device_bytes"] == info["node_count"] * 128 + 256 * 48 + 32 + 32 + 4 + 256 * 4 + 384
     d, order = s.dump_bvh(0)
     assert sorted(order.tolist()) == list(range(256))
     with pytest.raises(_lib.PsyError):

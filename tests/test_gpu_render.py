@@ -197,6 +197,35 @@ def test_server_budget_admission():
     server.close()
 
 
+def test_server_film_allocation_failure_fails_the_request():
+    """A film allocation that fails while nothing else runs ends the request
+    with PSY_E_NOMEM instead of waiting for memory that cannot free up; one
+    that fails while another request renders is retried and succeeds; the
+    server still serves afterwards and closes with requests queued."""
+    from psychopath_amd._lib import PSY_E_NOMEM, PsyError
+    sc = scenes.cornell_box(32, 32, 2, 2, 0)
+    r = render.build(sc, max_paths=1024)
+    ref, _ = r.render(sc.camera, sc.settings)
+    server = render.Server([r])
+    server.fail_film_allocs(1)
+    rid = server.submit(0, sc.camera, sc.settings)
+    with pytest.raises(PsyError) as e:
+        server.wait(rid)
+    assert e.value.code == PSY_E_NOMEM
+    big = dict(sc.settings, width=96, height=96, spp=16)
+    first = server.submit(0, sc.camera, big)
+    while server.progress(first) == 0:
+        pass
+    server.fail_film_allocs(1)
+    second = server.submit(0, sc.camera, sc.settings)
+    img, _ = server.wait(second)
+    assert np.array_equal(img, ref)
+    server.wait(first, keep_image=False)
+    for s in range(3):
+        server.submit(0, sc.camera, dict(sc.settings, seed=s))
+    server.close()
+
+
 def test_autotune_keeps_images():
     sc = scenes.cornell_box(48, 48, 4, 3, 0)
     r = render.build(sc)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from psychopath_amd import scenes
-from psychopath_amd.tracer import Assembly, DeviceHits, DeviceRays, Mesh, RayBatch, Tracer
+from psychopath_amd.tracer import Assembly, DeviceHitRecords, DeviceHits, DeviceRays, Mesh, RayBatch, Tracer
 
 pytestmark = pytest.mark.gpu
 
@@ -25,8 +25,8 @@ def ray_batch(r):
                     r.get("occlusion"))
 
 
-def gpu_trace(tracer, r):
-    h = tracer.trace(ray_batch(r))
+def gpu_trace(tracer, r, records=False):
+    h = tracer.trace(ray_batch(r), records=records)
     return dict(kind=h.kind, t=h.t, b0=h.b0, b1=h.b1, b2=h.b2, prim_id=h.prim_id, inst_id=h.inst_id)
 
 
@@ -124,6 +124,57 @@ def test_big_mesh_exact_on_sample(big):
     v, f, s, o, tr = big
     r = scenes.incoherent_rays(v.min(0), v.max(0), 200_000, seeds=(2, 3, 4))
     assert_exact(gpu_trace(tr, r), o.trace(r), "1M closest")
+
+
+def test_big_mesh_anyhit_shadow_exact(big):
+    """Any-hit shadow segments on the 1M-triangle mesh (origin and end
+    uniform in the bounds, max_t 1, the benchmark's generator and seeds):
+    occlusion bits identical to the oracle on a 200,000-ray sample."""
+    v, f, s, o, tr = big
+    r = scenes.shadow_rays(v.min(0), v.max(0), 200_000)
+    g, w = gpu_trace(tr, r), o.trace(r)
+    assert np.array_equal(g["kind"], w["kind"])
+    assert set(np.unique(g["kind"])) <= {0, 1}
+    assert 0.05 < (w["kind"] == 1).mean() < 0.95
+
+
+def test_big_mesh_device_records_exact(big):
+    """32-byte hit records written on the device (the benchmark's output
+    form) against the oracle, and a miss record holds zeros and PSY_NONE."""
+    v, f, s, o, tr = big
+    r = scenes.incoherent_rays(v.min(0), v.max(0), 200_000, seeds=(12, 13, 14))
+    rays = DeviceRays(ray_batch(r))
+    out = DeviceHitRecords(rays.n)
+    tr.trace_device(rays, out)
+    rec = out.records()
+    h = Hits_view(rec)
+    assert_exact(h, o.trace(r), "1M closest, records")
+    miss = rec["kind"] == 0
+    assert miss.any()
+    assert np.all(rec["t"][miss] == 0) and np.all(rec["prim_id"][miss] == 0xFFFFFFFF)
+    assert np.all(rec["reserved"] == 0)
+
+
+def Hits_view(rec):
+    return dict(kind=rec["kind"].astype(np.uint8), t=rec["t"], b0=rec["b0"], b1=rec["b1"], b2=rec["b2"],
+                prim_id=rec["prim_id"], inst_id=rec["inst_id"])
+
+
+def test_hit_records_equal_arrays(small, instanced):
+    """Records and arrays carry the same hits for closest, any-hit and
+    instanced/motion rays, through the simple and persistent kernels and
+    the host-memory path's staging."""
+    v, f, s, o, tr = small
+    si, oi, tri = instanced
+    lo, hi = np.array([-10, -10, -2], np.float32), np.array([10, 10, 2], np.float32)
+    cases = ((tr, scenes.incoherent_rays(v.min(0), v.max(0), 70_000, seeds=(171, 172, 173))),
+             (tr, scenes.shadow_rays(v.min(0), v.max(0), 30_000, seeds=(174, 175))),
+             (tri, scenes.incoherent_rays(lo, hi, 30_000, seeds=(176, 177, 178))))
+    for tracer, r in cases:
+        for shape in ((1, 1, 1, 8, 4), (0, 16, 16, 16, 4, 0), (0, 16, 32, 24, 4, 24)):
+            tracer.set_tuning(*shape)
+            assert_exact(gpu_trace(tracer, r, records=True), gpu_trace(tracer, r), f"records {shape}")
+        tracer.set_tuning(0, 16, 16, 16, 4, 0)
 
 
 def test_big_mesh_coherent_exact(big):
