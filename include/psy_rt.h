@@ -105,16 +105,20 @@ typedef struct {
     const psy_color* colors;
 } psy_sphere_light_desc;
 
-/* Object kinds of an instance (Object, src/scene/assembly.rs:399-402). */
+/* Object kinds of an instance (Object, src/scene/assembly.rs:399-402, and
+ * InstanceType::Assembly, :413-417). */
 #define PSY_OBJECT_MESH 0u
 #define PSY_OBJECT_RECT_LIGHT 1u
 #define PSY_OBJECT_SPHERE_LIGHT 2u
+#define PSY_OBJECT_ASSEMBLY 3u
 
 /* One instance of an object (Instance, src/scene/assembly.rs:404-411).
- * `mesh` indexes meshes, rect_lights or sphere_lights according to `kind`.
- * `xform_count` transforms starting at `xform_start` in psy_scene_desc.xforms
- * are its world-to-object time samples (0 = no transform; light instances
- * need at least one). */
+ * `mesh` indexes meshes, rect_lights, sphere_lights or assemblies according
+ * to `kind`.  `xform_count` transforms starting at `xform_start` in
+ * psy_scene_desc.xforms are its time samples, from the enclosing assembly's
+ * space to the object's (0 = no transform; a light instance of the root
+ * assembly needs at least one).  Shader indices refer to the one shader
+ * list the renderer takes, in every assembly. */
 typedef struct {
     uint32_t mesh;
     uint32_t shader;      /* surface shader index, PSY_NONE = unassigned */
@@ -123,10 +127,24 @@ typedef struct {
     uint32_t kind;        /* PSY_OBJECT_* */
 } psy_instance_desc;
 
-/* A single-level assembly: meshes, light objects and the instances that
- * place them.  Transforms are 12 floats each, the arguments a..l of
- * Transform::new_from_values (sub_crates/math3d/src/transform.rs:29-47). */
-typedef struct {
+/* An assembly (Assembly, src/scene/assembly.rs:18-40): meshes, light
+ * objects, sub-assemblies and the instances that place them.  Transforms are
+ * 12 floats each, the arguments a..l of Transform::new_from_values
+ * (sub_crates/math3d/src/transform.rs:29-47).  A sub-assembly is instanced
+ * by its parent's instances of kind PSY_OBJECT_ASSEMBLY; its indices refer
+ * to its own arrays.  At most 64 levels of nesting.
+ *
+ * Nested assemblies are flattened for traversal: each mesh and light
+ * instance gets the transform list TransformStack::push composes along its
+ * path (src/transform_stack.rs:33-60), and one instance BVH covers them all
+ * (the reference walks the sub-assemblies' BVHs in turn, which orders
+ * equal-distance hits differently at most).  Hit records number the
+ * flattened instances depth first, in instance order (so a single-level
+ * scene keeps its instance indices).  Light selection keeps the reference's
+ * hierarchy: one light tree per assembly over its lights and its light-
+ * holding sub-assemblies (assembly.rs:297-338), selected recursively as
+ * Assembly::sample_lights does (assembly.rs:46-141). */
+typedef struct psy_scene_desc {
     uint32_t mesh_count;
     const psy_mesh_desc* meshes;
     uint32_t instance_count;
@@ -137,6 +155,8 @@ typedef struct {
     const psy_rect_light_desc* rect_lights;
     uint32_t sphere_light_count;
     const psy_sphere_light_desc* sphere_lights;
+    uint32_t assembly_count;
+    const struct psy_scene_desc* assemblies;
 } psy_scene_desc;
 
 /* Rays as structure-of-arrays (RayBatch, src/ray.rs:24-44).  `flags` may be
@@ -179,6 +199,21 @@ typedef struct {
     psy_hit_record* records;
 } psy_hit_soa;
 
+/* The geometry of one hit, 96 bytes: the remaining fields of the
+ * reference's SurfaceIntersectionData (src/surface/mod.rs:82-92; `incoming`
+ * is the ray's direction and `t` the hit record's), as the mesh and light
+ * intersection code builds them (triangle_mesh.rs:264-317,
+ * rectangle_light.rs:288-330, sphere_light.rs:283-330). */
+typedef struct {
+    float pos[3];          /* hit point, world space (triangle::surface_point) */
+    float pos_err;         /* its error magnitude */
+    float nor[3];          /* shading normal (interpolated vertex normals, else nor_g) */
+    float nor_g[3];        /* geometric normal, cross(p0 - p1, p0 - p2), unnormalised */
+    float local_space[12]; /* local_space as the 12 values a..l of Transform::new_from_values */
+    float sample_pdf;      /* light hits: the light's pdf of sampling this point from the ray's origin; else 0 */
+    float reserved;
+} psy_hit_geometry;
+
 /* Per-scene figures, including the record sizes used for the algorithmic-byte
  * roofline model. */
 typedef struct {
@@ -204,7 +239,12 @@ typedef struct {
     uint64_t rays;
 } psy_trace_counters;
 
-/* Launch shape of the traversal kernel.  The persistent kernel keeps
+/* Launch shape of the traversal kernel.  PSY_KERNEL_PERSISTENT_COOP is the
+ * persistent kernel with a wave-cooperative node fetch: the nodes the lanes
+ * of a wave stand on are loaded by the wave together, consecutive lanes
+ * reading consecutive parts of one node, staged in LDS and read back, for
+ * scenes without time samples (elsewhere it runs as the persistent kernel).
+ * The persistent kernel keeps
  * `waves_per_cu` wave64 workgroups per compute unit resident, each taking
  * rays from a global counter; lanes whose ray finished are refilled once
  * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
@@ -217,6 +257,7 @@ typedef struct {
  * one lane per ray over a launch-wide grid.  No field changes the results. */
 #define PSY_KERNEL_PERSISTENT 0u
 #define PSY_KERNEL_SIMPLE 1u
+#define PSY_KERNEL_PERSISTENT_COOP 2u
 typedef struct {
     uint32_t kernel;
     uint32_t waves_per_cu;

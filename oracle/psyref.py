@@ -52,7 +52,7 @@ _SIGS = {
     "psyref_scene_add_assembly_instance": (C.c_int, [_vp, C.c_uint32, _fp, C.c_uint32]),
     "psyref_scene_dump_bvh": (C.c_uint64, [_vp, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_uint64]),
     "psyref_trace": (C.c_int, [_vp] + [_fp] * 8 + [_u8p, C.c_uint64, C.c_uint32, C.c_uint32, _u8p] + [_fp] * 4
-                     + [_u32p, _u32p, _fp, _fp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+                     + [_u32p, _u32p, _fp, _fp, _fp, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }
 
 _lib = None
@@ -153,6 +153,7 @@ class OracleScene:
                    prim_id=np.zeros(n, np.uint32), inst_id=np.zeros(n, np.uint32))
         nor_g = np.zeros((n, 3), np.float32) if want_geometry else None
         pos = np.zeros((n, 4), np.float32) if want_geometry else None
+        geom = np.zeros((n, 24), np.float32) if want_geometry else None
         tests = C.c_uint64()
         secs = C.c_double()
         if threads <= 0:
@@ -163,7 +164,7 @@ class OracleScene:
                                 fptr(out["b1"]), fptr(out["b2"]), out["prim_id"].ctypes.data_as(_u32p),
                                 out["inst_id"].ctypes.data_as(_u32p),
                                 None if nor_g is None else fptr(nor_g), None if pos is None else fptr(pos),
-                                C.byref(tests), C.byref(secs))
+                                None if geom is None else fptr(geom), C.byref(tests), C.byref(secs))
         if rc != 0:
             raise RuntimeError(f"psyref_trace failed ({rc})")
         out["node_tests"] = tests.value
@@ -172,6 +173,8 @@ class OracleScene:
         if want_geometry:
             out["nor_g"] = nor_g
             out["pos"] = pos
+            # psy_hit_geometry rows: pos, pos_err, nor, nor_g, local_space, sample_pdf, 0
+            out["geometry"] = geom
         return out
 
     def __del__(self):

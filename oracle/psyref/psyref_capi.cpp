@@ -191,7 +191,7 @@ int psyref_trace(void* s, const float* ox, const float* oy, const float* oz, con
                  const float* dz, const float* time, const float* max_t, const uint8_t* occl, uint64_t n,
                  uint32_t batch, uint32_t threads, uint8_t* kind, float* t, float* b0, float* b1, float* b2,
                  uint32_t* prim_id, uint32_t* inst_id, float* nor_g /*n*3 or NULL*/, float* pos /*n*4 or NULL*/,
-                 uint64_t* node_tests_out, double* seconds) {
+                 float* geom /*n*24 or NULL*/, uint64_t* node_tests_out, double* seconds) {
     OracleScene* sc = static_cast<OracleScene*>(s);
     if (!sc->built || batch == 0 || batch > 65536) return -1;
     if (threads == 0) threads = 1;
@@ -227,6 +227,19 @@ int psyref_trace(void* s, const float* ox, const float* oy, const float* oz, con
                 inst_id[i] = h.inst_id;
                 if (nor_g) { nor_g[3 * i] = h.nor_g.x; nor_g[3 * i + 1] = h.nor_g.y; nor_g[3 * i + 2] = h.nor_g.z; }
                 if (pos) { pos[4 * i] = h.pos.x; pos[4 * i + 1] = h.pos.y; pos[4 * i + 2] = h.pos.z; pos[4 * i + 3] = h.pos_err; }
+                if (geom) {
+                    // SurfaceIntersectionData: pos, pos_err, nor, nor_g,
+                    // local_space (a..l), sample_pdf, 0 (psy_hit_geometry).
+                    float* g = geom + 24 * i;
+                    for (int k = 0; k < 24; k++) g[k] = 0.0f;
+                    if (h.kind == ISECT_HIT) {
+                        g[0] = h.pos.x; g[1] = h.pos.y; g[2] = h.pos.z; g[3] = h.pos_err;
+                        g[4] = h.nor.x; g[5] = h.nor.y; g[6] = h.nor.z;
+                        g[7] = h.nor_g.x; g[8] = h.nor_g.y; g[9] = h.nor_g.z;
+                        h.local_space.to_values(g + 10);
+                        g[22] = h.sample_pdf;
+                    }
+                }
             }
         }
         total_tests += tracer.node_tests;

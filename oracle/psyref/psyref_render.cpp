@@ -163,24 +163,45 @@ bool LightTree::select(Vec3 inc, Vec3 pos, Vec3 nor, Vec3 nor_g, const Closure& 
     return true;
 }
 
-// AssemblyBuilder::build, light part (assembly.rs:297-338)
-void RenderScene::build_lights() {
-    light_instances.clear();
-    for (const Instance& inst : root->instances)
-        if (!inst.is_assembly && inst.obj_kind != OBJ_MESH) light_instances.push_back(inst);
-    std::vector<uint32_t> keys(light_instances.size());
+// AssemblyBuilder::build, light part (assembly.rs:297-338), for every
+// assembly, sub-assemblies first: the light instances are the instances of
+// light objects and of sub-assemblies whose own light tree has energy, the
+// latter weighted by that energy.
+namespace {
+void build_assembly_lights(const Assembly* a, std::map<const Assembly*, AssemblyLights>& out) {
+    for (const auto& sub : a->assemblies) build_assembly_lights(sub.get(), out);
+    AssemblyLights& al = out[a];
+    al.instances.clear();
+    auto energy_of = [&](const Instance& inst) {
+        if (inst.is_assembly) return out.at(a->assemblies[inst.data_index].get()).tree.approximate_energy();
+        return a->light_energy(inst);
+    };
+    for (const Instance& inst : a->instances) {
+        if (inst.is_assembly) {
+            if (out.at(a->assemblies[inst.data_index].get()).tree.approximate_energy() > 0.0f)
+                al.instances.push_back(inst);
+        } else if (inst.obj_kind != OBJ_MESH) {
+            al.instances.push_back(inst);
+        }
+    }
+    std::vector<uint32_t> keys(al.instances.size());
     for (size_t i = 0; i < keys.size(); i++) keys[i] = (uint32_t)i;
-    const Assembly* a = root;
-    std::vector<Instance> li = light_instances;
-    light_tree.build(
+    const std::vector<Instance> li = al.instances;
+    al.tree.build(
         keys.data(), keys.size(),
         [&](uint32_t k) {
             size_t id = li[k].id;
             return BoundsSlice{a->inst_bounds.data() + a->inst_bound_start[id],
                                a->inst_bound_start[id + 1] - a->inst_bound_start[id]};
         },
-        [&](uint32_t k) { return a->light_energy(li[k]); });
-    for (size_t i = 0; i < keys.size(); i++) light_instances[i] = li[keys[i]];
+        [&](uint32_t k) { return energy_of(li[k]); });
+    for (size_t i = 0; i < keys.size(); i++) al.instances[i] = li[keys[i]];
+}
+}  // namespace
+
+void RenderScene::build_lights() {
+    lights.clear();
+    build_assembly_lights(root, lights);
 }
 
 // ---- math.rs:67-132 ----------------------------------------------------------
@@ -249,13 +270,56 @@ float weighted_choice_energy(const std::vector<DistantDiskLight>& ls, float n, s
     return ls.back().approximate_energy() / total;
 }
 
+// Assembly::sample_lights (assembly.rs:46-141): selects in this assembly's
+// light tree with the shading data taken into its space (the lerped top of
+// the transform stack), then samples the chosen light under the stack's
+// transform times the instance's, or recurses into the chosen sub-assembly
+// with its instance transforms pushed.  Returns false when nothing is
+// selected; `sel_pdf` is the product of the selection probabilities.
+bool assembly_sample_lights(const RenderScene& sc, const Assembly* a, TransformStack& xs, float n, float u, float v,
+                            float wavelength, float time, const Isect& is, const Closure& closure, LightSample& s,
+                            float& sel_pdf) {
+    size_t ntop;
+    const Transform* top = xs.top(ntop);
+    const Transform sel_xform = ntop ? lerp_slice(top, ntop, time) : Transform();
+    const AssemblyLights& al = sc.lights_of(a);
+    size_t li;
+    float p, whittled;
+    if (!al.tree.select(xform_vector(is.incoming, sel_xform), xform_point(is.pos, sel_xform),
+                        xform_normal(is.nor, sel_xform), xform_normal(is.nor_g, sel_xform), closure, time, n, li, p,
+                        whittled))
+        return false;
+    const Instance& inst = al.instances[li];
+    if (!inst.is_assembly) {
+        Transform xform;
+        if (inst.has_xforms) {
+            const Transform x = lerp_slice(a->xforms.data() + inst.xf_start, inst.xf_end - inst.xf_start, time);
+            xform = ntop ? xform_compose(lerp_slice(top, ntop, time), x) : x;
+        } else if (ntop) {
+            xform = lerp_slice(top, ntop, time);
+        }
+        s = inst.obj_kind == OBJ_RECT_LIGHT
+                ? a->rect_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v, wavelength)
+                : a->sphere_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v, wavelength);
+        sel_pdf = p;
+        return true;
+    }
+    if (inst.has_xforms) xs.push(a->xforms.data() + inst.xf_start, inst.xf_end - inst.xf_start);
+    float sub_pdf = 0.0f;
+    const bool ok = assembly_sample_lights(sc, a->assemblies[inst.data_index].get(), xs, whittled, u, v, wavelength,
+                                           time, is, closure, s, sub_pdf);
+    if (inst.has_xforms) xs.pop();
+    sel_pdf = sub_pdf * p;
+    return ok;
+}
+
 SceneLightSample sample_lights(const RenderScene& sc, float n, float u, float v, float wavelength, float time,
                                const Isect& is, const Closure& closure) {
     SceneLightSample out;
     float wl_sum = 0.0f;
     for (const auto& l : sc.world.lights) wl_sum = wl_sum + l.approximate_energy();
     float wl_energy = wl_sum <= 0.0f ? 0.0f : 1.0f;
-    float ll_energy = sc.light_tree.approximate_energy() <= 0.0f ? 0.0f : 1.0f;
+    float ll_energy = sc.lights_of(sc.root).tree.approximate_energy() <= 0.0f ? 0.0f : 1.0f;
     float tot = wl_energy + ll_energy;
     if (tot <= 0.0f) return out;
     float wl_prob = wl_energy / tot;
@@ -275,22 +339,11 @@ SceneLightSample sample_lights(const RenderScene& sc, float n, float u, float v,
         return out;
     }
     float nn = (n - wl_prob) / (1.0f - wl_prob);
-    // Assembly::sample_lights with an empty transform stack.
-    Transform sel_xform;
-    size_t li;
-    float sel_pdf, whittled;
-    if (!sc.light_tree.select(xform_vector(is.incoming, sel_xform), xform_point(is.pos, sel_xform),
-                              xform_normal(is.nor, sel_xform), xform_normal(is.nor_g, sel_xform), closure, time, nn,
-                              li, sel_pdf, whittled))
-        return out;
-    const Instance& inst = sc.light_instances[li];
-    const Assembly* a = sc.root;
-    Transform xform;
-    if (inst.has_xforms) xform = lerp_slice(a->xforms.data() + inst.xf_start, inst.xf_end - inst.xf_start, time);
-    LightSample s = inst.obj_kind == OBJ_RECT_LIGHT
-                        ? a->rect_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v, wavelength)
-                        : a->sphere_lights[inst.data_index].at(time).sample_from_point(xform, is.pos, u, v,
-                                                                                      wavelength);
+    // Assembly::sample_lights from the root with an empty transform stack.
+    TransformStack xs;
+    LightSample s;
+    float sel_pdf = 0.0f;
+    if (!assembly_sample_lights(sc, sc.root, xs, nn, u, v, wavelength, time, is, closure, s, sel_pdf)) return out;
     out.kind = LS_SURFACE;
     out.color = s.color;
     out.point = s.point;

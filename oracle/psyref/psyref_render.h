@@ -6,6 +6,7 @@
 // loop (src/renderer.rs:194-692) and the XYZ film accumulation.
 #pragma once
 
+#include <map>
 #include <vector>
 
 #include "psyref_color.h"
@@ -62,17 +63,25 @@ struct RenderSettings {
     uint32_t width = 64, height = 64, spp = 1, max_bounces = 2, seed = 0;
 };
 
-// A scene ready to render: the root assembly with its light tree, shaders,
-// world and camera.
+// The light accel of one assembly (assembly.rs:297-338): a light tree over
+// its light instances and the instances of sub-assemblies that hold lights,
+// in light-tree order.
+struct AssemblyLights {
+    LightTree tree;
+    std::vector<Instance> instances;
+};
+
+// A scene ready to render: the root assembly with the light trees of it and
+// its sub-assemblies, shaders, world and camera.
 struct RenderScene {
     Assembly* root = nullptr;
     std::vector<Closure> shaders;  // SimpleSurfaceShader -> closure (shading/mod.rs:42-61)
     World world;
     CameraDesc camera;
-    LightTree light_tree;
-    std::vector<Instance> light_instances;  // in light-tree order
+    std::map<const Assembly*, AssemblyLights> lights;
 
     void build_lights();
+    const AssemblyLights& lights_of(const Assembly* a) const { return lights.at(a); }
 };
 
 // Renders into xyz (width*height*3).  Returns the summed time of the trace

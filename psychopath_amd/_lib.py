@@ -40,7 +40,7 @@ class MeshDesc(C.Structure):
 
 
 PSY_COLOR_XYZ, PSY_COLOR_BLACKBODY, PSY_COLOR_TEMPERATURE = 0, 1, 2
-PSY_OBJECT_MESH, PSY_OBJECT_RECT_LIGHT, PSY_OBJECT_SPHERE_LIGHT = 0, 1, 2
+PSY_OBJECT_MESH, PSY_OBJECT_RECT_LIGHT, PSY_OBJECT_SPHERE_LIGHT, PSY_OBJECT_ASSEMBLY = 0, 1, 2, 3
 PSY_SHADER_LAMBERT, PSY_SHADER_GGX, PSY_SHADER_EMIT = 0, 1, 2
 
 
@@ -69,18 +69,23 @@ class InstanceDesc(C.Structure):
 
 
 class SceneDesc(C.Structure):
-    _fields_ = [
-        ("mesh_count", C.c_uint32),
-        ("meshes", C.POINTER(MeshDesc)),
-        ("instance_count", C.c_uint32),
-        ("instances", C.POINTER(InstanceDesc)),
-        ("xform_count", C.c_uint32),
-        ("xforms", _fp),
-        ("rect_light_count", C.c_uint32),
-        ("rect_lights", C.POINTER(RectLightDesc)),
-        ("sphere_light_count", C.c_uint32),
-        ("sphere_lights", C.POINTER(SphereLightDesc)),
-    ]
+    pass
+
+
+SceneDesc._fields_ = [
+    ("mesh_count", C.c_uint32),
+    ("meshes", C.POINTER(MeshDesc)),
+    ("instance_count", C.c_uint32),
+    ("instances", C.POINTER(InstanceDesc)),
+    ("xform_count", C.c_uint32),
+    ("xforms", _fp),
+    ("rect_light_count", C.c_uint32),
+    ("rect_lights", C.POINTER(RectLightDesc)),
+    ("sphere_light_count", C.c_uint32),
+    ("sphere_lights", C.POINTER(SphereLightDesc)),
+    ("assembly_count", C.c_uint32),
+    ("assemblies", C.POINTER(SceneDesc)),
+]
 
 
 class ShaderDesc(C.Structure):
@@ -199,6 +204,7 @@ class TraceTuning(C.Structure):
 
 PSY_KERNEL_PERSISTENT = 0
 PSY_KERNEL_SIMPLE = 1
+PSY_KERNEL_PERSISTENT_COOP = 2
 
 # Every symbol the header declares, with its ctypes signature.
 _SIGNATURES = {
@@ -212,6 +218,7 @@ _SIGNATURES = {
     "psy_tracer_destroy": (C.c_int, [C.c_void_p]),
     "psy_trace": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA), C.c_uint32]),
     "psy_trace_async": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA)]),
+    "psy_finalize_hits": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA), C.c_void_p]),
     "psy_tracer_sync": (C.c_int, [C.c_void_p]),
     "psy_tracer_last_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "psy_trace_counted": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA),

@@ -40,7 +40,8 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.tri_rec, &s->tri_rec);
     if (!r) r = upload(h.light_nodes, &s->light_nodes);
     if (!r) r = upload(h.light_boxes, &s->light_boxes);
-    if (!r) r = upload(h.light_list, &s->light_list);
+    if (!r) r = upload(h.light_leaves, &s->light_leaves);
+    if (!r) r = upload(h.light_xforms, &s->light_xforms);
     if (!r) r = upload(h.normals, &s->normals);
     if (!r) r = upload(h.nidx, &s->nidx);
     if (!r) r = upload(h.order_table, &s->order_table);
@@ -77,13 +78,13 @@ hipError_t launch_tuned(const psy_scene* scene, const psy_trace_tuning& tuning, 
                         const TraceArgs& a, bool counted, hipStream_t stream) {
     if (tuning.kernel == PSY_KERNEL_SIMPLE) return launch_trace(a, counted, stream);
     TraceTuning tt{cu_count * tuning.waves_per_cu, tuning.refill_min, tuning.stack_lds, tuning.simd_waves,
-                   tuning.leaf_min};
+                   tuning.leaf_min, tuning.kernel == PSY_KERNEL_PERSISTENT_COOP ? 1u : 0u};
     return launch_trace_persistent(a, counted, scene->host.has_xf || scene->host.has_lights, scene->host.has_motion,
                                    work, tt, stream);
 }
 
 int check_tuning(const psy_trace_tuning* t) {
-    if (t->kernel != PSY_KERNEL_PERSISTENT && t->kernel != PSY_KERNEL_SIMPLE)
+    if (t->kernel != PSY_KERNEL_PERSISTENT && t->kernel != PSY_KERNEL_SIMPLE && t->kernel != PSY_KERNEL_PERSISTENT_COOP)
         return fail(PSY_E_INVALID, "unknown kernel");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 32) return fail(PSY_E_RANGE, "waves_per_cu must be in [1, 32]");
     if (t->refill_min < 1 || t->refill_min > 64) return fail(PSY_E_RANGE, "refill_min must be in [1, 64]");
@@ -93,11 +94,6 @@ int check_tuning(const psy_trace_tuning* t) {
     if (t->leaf_min > 64) return fail(PSY_E_RANGE, "leaf_min must be in [0, 64]");
     return PSY_OK;
 }
-
-}  // namespace api
-}  // namespace psy
-
-namespace {
 
 void set_rays(TraceArgs& a, const psy_ray_soa* r) {
     a.ox = r->ox; a.oy = r->oy; a.oz = r->oz;
@@ -123,6 +119,11 @@ int check_soa(const psy_ray_soa* r, const psy_hit_soa* h) {
         return fail(PSY_E_INVALID, "a hit array is NULL");
     return PSY_OK;
 }
+
+}  // namespace api
+}  // namespace psy
+
+namespace {
 
 int enqueue(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits, bool counted) {
     TraceArgs a = scene_args(tr->scene);
@@ -227,7 +228,8 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
                       (h.xforms.size() + h.inv_static.size()) * sizeof(GpuXform) +
                       h.rect_lights.size() * sizeof(GpuRectLight) + h.sphere_lights.size() * sizeof(GpuSphereLight) +
                       h.light_samples.size() * sizeof(float) +
-                      (h.inst_pos.size() + h.tri_rec.size() + h.light_list.size()) * 4 +
+                      (h.inst_pos.size() + h.tri_rec.size()) * 4 + h.light_leaves.size() * sizeof(GpuLightLeaf) +
+                      h.light_xforms.size() * sizeof(GpuXform) +
                       h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox) +
                       (h.normals.size() + h.nidx.size()) * 4 + h.order_table.size();
     *out = s;
@@ -354,6 +356,13 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
         for (uint32_t r : refill)
             for (uint32_t dd : depth)
                 for (uint32_t occ : {4u, 8u}) cands.push_back({PSY_KERNEL_PERSISTENT, w, r, dd, occ, 0});
+    // The cooperative node fetch (static scenes only; on scenes with time
+    // samples it runs as the plain persistent kernel).  Its 7 KB of node
+    // staging per wave caps it near 16 resident waves per CU.
+    if (!tr->scene->host.has_motion)
+        for (uint32_t w : {8u, 12u, 16u})
+            for (uint32_t r : {8u, 16u, 32u})
+                for (uint32_t dd : {8u, 16u}) cands.push_back({PSY_KERNEL_PERSISTENT_COOP, w, r, dd, 4u, 0});
     const psy_trace_tuning saved = tr->tuning;
     float best_ms = 1e30f;
     psy_trace_tuning best_t = saved;
@@ -386,13 +395,15 @@ int psy_tracer_autotune(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, psy
     // around the best.
     rc = time_all();
     if (rc) return rc;
-    if (best_t.kernel == PSY_KERNEL_PERSISTENT) {
+    if (best_t.kernel != PSY_KERNEL_SIMPLE) {
         const psy_trace_tuning s1 = best_t;
         cands.clear();
+        const uint32_t k = s1.kernel;
         for (uint32_t lm : {8u, 16u, 24u, 32u, 48u, 64u})
             for (uint32_t r : {8u, 16u, 32u})
-                for (uint32_t w : {16u, 24u, 32u})
-                    cands.push_back({PSY_KERNEL_PERSISTENT, w, r, s1.stack_lds, s1.simd_waves, lm});
+                for (uint32_t w : (k == PSY_KERNEL_PERSISTENT_COOP ? std::vector<uint32_t>{8u, 12u, 16u}
+                                                                   : std::vector<uint32_t>{16u, 24u, 32u}))
+                    cands.push_back({k, w, r, s1.stack_lds, s1.simd_waves, lm});
         rc = time_all();
         if (rc) return rc;
     }

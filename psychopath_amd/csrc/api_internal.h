@@ -52,7 +52,8 @@ struct psy_scene {
     uint32_t* tri_rec = nullptr;
     psy::GpuLightNode* light_nodes = nullptr;
     psy::GpuBox* light_boxes = nullptr;
-    uint32_t* light_list = nullptr;
+    psy::GpuLightLeaf* light_leaves = nullptr;
+    psy::GpuXform* light_xforms = nullptr;
     float* normals = nullptr;
     uint32_t* nidx = nullptr;
     uint8_t* order_table = nullptr;
@@ -62,7 +63,7 @@ struct psy_scene {
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,
                         sphere_lights, light_samples, inst_pos, tri_rec, light_nodes, light_boxes,
-                        light_list, normals, nidx, order_table};
+                        light_leaves, light_xforms, normals, nidx, order_table};
         for (void* b : bufs) (void)hipFree(b);
     }
 };
@@ -107,6 +108,11 @@ hipError_t launch_tuned(const psy_scene* scene, const psy_trace_tuning& tuning, 
                         const TraceArgs& a, bool counted, hipStream_t stream);
 // Validates a launch shape (PSY_OK or an error with message).
 int check_tuning(const psy_trace_tuning* t);
+// Ray and hit pointers of the traversal kernels' arguments, and their
+// validation.
+void set_rays(TraceArgs& a, const psy_ray_soa* r);
+void set_hits(TraceArgs& a, const psy_hit_soa* h);
+int check_soa(const psy_ray_soa* r, const psy_hit_soa* h);
 
 }  // namespace api
 }  // namespace psy

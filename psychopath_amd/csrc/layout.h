@@ -168,4 +168,29 @@ struct GpuBox {
     float lo[3], hi[3];
 };
 
+// A light-tree leaf (GpuLightNode::light of a node without children).  Each
+// assembly on an instance path has its own tree (assembly.rs:297-338); a
+// leaf is a light instance or a sub-assembly instance whose tree continues
+// the selection (assembly.rs:46-141).  Transform lists live in the light
+// transform array:
+//   kind 0 (light): target = the instance's leaf position; top = the
+//     enclosing assemblies' composed transforms (the transform stack's top),
+//     own = the light instance's own transforms; the light is sampled with
+//     lerp(top) * lerp(own).
+//   kind 1 (sub-assembly): target = the root node of its tree; top = the
+//     composed transforms into it, whose lerp takes the shading point into
+//     the sub-assembly's space.
+struct GpuLightLeaf {
+    uint32_t kind;
+    uint32_t target;
+    uint32_t top_start, top_count;
+    uint32_t own_start, own_count;
+    uint32_t pad[2];
+};
+constexpr uint32_t LIGHT_LEAF_LIGHT = 0u;
+constexpr uint32_t LIGHT_LEAF_ASSEMBLY = 1u;
+// Light trees chain through at most this many assemblies (the root's and
+// three levels of sub-assemblies holding lights).
+constexpr uint32_t MAX_LIGHT_LEVELS = 4u;
+
 }  // namespace psy

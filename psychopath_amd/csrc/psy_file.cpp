@@ -25,6 +25,7 @@
 #include "../../include/psy_file.h"
 #include "color_host.h"
 #include "psymath.h"
+#include "scene.h"
 
 namespace psy {
 namespace api {
@@ -826,47 +827,7 @@ void place_definitions(AsmDef& d, psy_file_scene_data& sd) {
     for (auto& sub : d.subs) place_definitions(*sub, sd);
 }
 
-// math3d `a * b` (transform.rs:89-96): Self(b.0 * a.0), glam's Affine3A
-// product, which applies a, then b.
-psy::Xf xf_then(const psy::Xf& a, const psy::Xf& b) {
-    psy::Xf r;
-    for (int j = 0; j < 4; j++) {
-        const psy::V3 c = psy::xf_mat_mul(b, psy::V3{a.m[0][j], a.m[1][j], a.m[2][j]});
-        r.m[0][j] = c.x;
-        r.m[1][j] = c.y;
-        r.m[2][j] = c.z;
-    }
-    r.m[0][3] += b.m[0][3];
-    r.m[1][3] += b.m[1][3];
-    r.m[2][3] += b.m[2][3];
-    return r;
-}
-
-psy::Xf xf_lerp_slice(const std::vector<psy::Xf>& s, float alpha) {
-    uint32_t i1;
-    float a2;
-    if (!psy::lerp_slice_pos((uint32_t)s.size(), alpha, i1, a2)) return s[i1];
-    return psy::xf_lerp(s[i1], s[i1 + 1], a2);
-}
-
-// TransformStack::push (transform_stack.rs:33-60): the parent's and the
-// instance's time samples combined by merge_slices_to (algorithm.rs:
-// 266-305), the shorter list resampled with lerp_slice.
-std::vector<psy::Xf> push_xforms(const std::vector<psy::Xf>& top, const std::vector<psy::Xf>& own) {
-    if (own.empty()) return top;
-    if (top.empty()) return own;
-    std::vector<psy::Xf> out;
-    if (top.size() == own.size()) {
-        for (size_t i = 0; i < own.size(); i++) out.push_back(xf_then(top[i], own[i]));
-    } else if (top.size() > own.size()) {
-        const float s = (float)(top.size() - 1);
-        for (size_t i = 0; i < top.size(); i++) out.push_back(xf_then(top[i], xf_lerp_slice(own, (float)i / s)));
-    } else {
-        const float s = (float)(own.size() - 1);
-        for (size_t i = 0; i < own.size(); i++) out.push_back(xf_then(xf_lerp_slice(top, (float)i / s), own[i]));
-    }
-    return out;
-}
+using psy::push_xforms;
 
 // The instances of an assembly under the transform stack `top`, sub-
 // assemblies expanded in place (tracer.rs:87-150 walks them the same way).

@@ -109,6 +109,22 @@ PSY_HD Xf xf_lerp(const Xf& a, const Xf& b, float alpha) {
         for (int j = 0; j < 4; j++) r.m[i][j] = (a.m[i][j] * ia) + (b.m[i][j] * alpha);
     return r;
 }
+// math3d `a * b` (transform.rs:89-96): Self(b.0 * a.0), glam's Affine3A
+// product, which applies a, then b.
+PSY_HD Xf xf_then(const Xf& a, const Xf& b) {
+    Xf r;
+    for (int j = 0; j < 4; j++) {
+        const V3 c = xf_mat_mul(b, V3{a.m[0][j], a.m[1][j], a.m[2][j]});
+        r.m[0][j] = c.x;
+        r.m[1][j] = c.y;
+        r.m[2][j] = c.z;
+    }
+    r.m[0][3] += b.m[0][3];
+    r.m[1][3] += b.m[1][3];
+    r.m[2][3] += b.m[2][3];
+    return r;
+}
+
 // glam Affine3A::inverse: 3x3 inverse from column cross products scaled by
 // 1/det (det = c2 . (c0 x c1)), then t' = -(M' * t).
 PSY_HD Xf xf_inverse(const Xf& a) {

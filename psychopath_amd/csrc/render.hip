@@ -370,11 +370,19 @@ __device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float
     return is;
 }
 
-// LightTree::select (src/accel/light_tree.rs:135-201)
-__device__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, V3 nor_g, const GpuClosure& clo,
-                                  float time, float n, uint32_t& index, float& prob) {
-    if (!sc.has_ltree) return false;
-    uint32_t node = 0;
+// lerp_slice over a transform list of the light leaves.
+__device__ __forceinline__ Xf light_xf_at(const GpuXform* l, uint32_t n, float time) {
+    uint32_t i1;
+    float a2;
+    if (!lerp_slice_pos(n, time, i1, a2)) return load_xf(l + i1);
+    return xf_lerp(load_xf(l + i1), load_xf(l + i1 + 1), a2);
+}
+
+// LightTree::select (src/accel/light_tree.rs:135-201) in the tree rooted at
+// `node`, for shading data already in that tree's space; returns the leaf
+// node, with the selection probability and the whittled number.
+__device__ uint32_t light_tree_descend(const ShadeScene& sc, uint32_t node, V3 inc, V3 pos, V3 nor, V3 nor_g,
+                                       const GpuClosure& clo, float time, float& n, float& prob) {
     float tot_prob = 1.0f;
     while (sc.lnodes[node].child_count > 0) {
         const GpuLightNode nd = sc.lnodes[node];
@@ -420,9 +428,38 @@ __device__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, 
             base += p;
         }
     }
-    index = sc.lnodes[node].light;
     prob = tot_prob;
-    return true;
+    return node;
+}
+
+// Assembly::sample_lights' selection (src/scene/assembly.rs:46-141): each
+// assembly's tree sees the world-space shading data through the lerped
+// transforms into that assembly (the identity at the root, which the
+// reference applies all the same); a sub-assembly leaf continues in its
+// tree with the whittled number.  The reference multiplies the levels'
+// probabilities as its recursion returns, innermost first.  Returns the
+// light leaf.
+__device__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, V3 nor_g, const GpuClosure& clo,
+                                  float time, float n, uint32_t& leaf, float& prob) {
+    if (!sc.has_ltree) return false;
+    Xf sel = xf_identity();
+    uint32_t node = 0;
+    float pl[MAX_LIGHT_LEVELS];
+    for (uint32_t level = 0; level < MAX_LIGHT_LEVELS; level++) {
+        node = light_tree_descend(sc, node, xf_vector(sel, inc), xf_point(sel, pos), xf_normal(sel, nor),
+                                  xf_normal(sel, nor_g), clo, time, n, pl[level]);
+        const GpuLightLeaf lf = sc.lleaves[sc.lnodes[node].light];
+        if (lf.kind == LIGHT_LEAF_LIGHT) {
+            float p = pl[level];
+            for (uint32_t k = level; k-- > 0;) p = p * pl[k];
+            leaf = sc.lnodes[node].light;
+            prob = p;
+            return true;
+        }
+        sel = lf.top_count ? light_xf_at(sc.lxforms + lf.top_start, lf.top_count, time) : xf_identity();
+        node = lf.target;
+    }
+    return false;
 }
 
 // Scene::sample_lights (src/scene/mod.rs:28-101) and Assembly::sample_lights
@@ -470,15 +507,20 @@ __device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float
         return out;
     }
     const float nn = (n - wl_prob) / (1.0f - wl_prob);
-    // With an empty transform stack the reference still applies the identity.
-    const Xf ident = xf_identity();
     uint32_t li;
     float sel_pdf;
-    if (!light_tree_select(sc, xf_vector(ident, incoming), xf_point(ident, is.pos), xf_normal(ident, is.nor),
-                           xf_normal(ident, is.nor_g), clo, time, nn, li, sel_pdf))
-        return out;
-    const GpuInstance inst = sc.instances[sc.llist[li]];
-    const Xf xform = instance_xf(sc.xforms, inst, time);
+    if (!light_tree_select(sc, incoming, is.pos, is.nor, is.nor_g, clo, time, nn, li, sel_pdf)) return out;
+    // The light is sampled under lerp(stack top) * lerp(its own transforms)
+    // (assembly.rs:77-95).
+    const GpuLightLeaf lf = sc.lleaves[li];
+    const GpuInstance inst = sc.instances[lf.target];
+    Xf xform = xf_identity();
+    if (lf.own_count) {
+        xform = light_xf_at(sc.lxforms + lf.own_start, lf.own_count, time);
+        if (lf.top_count) xform = xf_then(light_xf_at(sc.lxforms + lf.top_start, lf.top_count, time), xform);
+    } else if (lf.top_count) {
+        xform = light_xf_at(sc.lxforms + lf.top_start, lf.top_count, time);
+    }
     const LightSample s = inst.kind == PSY_OBJECT_RECT_LIGHT
                               ? rect_sample_from_point(rect_at(sc.rects[inst.mesh], sc.light_samples, time), xform,
                                                        is.pos, u, v, wl)
@@ -673,54 +715,92 @@ __global__ __launch_bounds__(KBLOCK, MINW) void shade_kernel(ShadeScene sc, cons
 }
 
 // The plain renderer: one launch per render over the whole frame, one lane
-// per pixel running its samples one after another, each path traced and
-// shaded to its end in registers (the reference's per-path logic without
-// wavefronts), then added to the pixel in sample order.
+// per path (pixel * spp + sample), each path traced and shaded to its end in
+// registers (the reference's per-path logic without wavefronts).  A block
+// holds the paths of MEGA_BLOCK / spp whole pixels, or, above MEGA_BLOCK spp,
+// one pixel's paths in chunks of MEGA_BLOCK; the paths' XYZ meet in LDS and
+// one lane per pixel adds them to the pixel in sample order.
 constexpr int MEGA_BLOCK = TRACE_BLOCK;
 constexpr int MEGA_LDS = TRACE_LDS_DEPTH;
 constexpr int MEGA_SPILL = TRACE_STACK_CAPACITY - TRACE_LDS_DEPTH;
 
+__host__ __device__ inline uint32_t mega_pixels_per_block(uint32_t spp) {
+    return spp <= (uint32_t)MEGA_BLOCK ? (uint32_t)MEGA_BLOCK / spp : 1u;
+}
+
 template <int MINW>
 __global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
     __shared__ uint32_t lds_stack[MEGA_LDS * MEGA_BLOCK];
+    __shared__ float lds_xyz[3 * MEGA_BLOCK];
     const ReqParams rp = reqs[0];
     const uint32_t pixels = rp.rw * rp.rh;
-    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t* lds_col = lds_stack + threadIdx.x;
-    const float fspp = (float)rp.spp;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const uint32_t spp = rp.spp;
+    const uint32_t lane = threadIdx.x;
+    const bool small = spp <= (uint32_t)MEGA_BLOCK;
+    const uint32_t per_block = mega_pixels_per_block(spp);
+    const uint32_t pix0 = blockIdx.x * per_block;
+    const uint32_t chunks = small ? 1u : (spp + MEGA_BLOCK - 1) / MEGA_BLOCK;
+    uint32_t* lds_col = lds_stack + lane;
+    const float fspp = (float)spp;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;  // the pixel's sum, on its summing lane
     TraceCount tc;
     uint32_t traced = 0;
-    for (uint32_t si = 0; pixel < pixels && si < rp.spp; si++) {
-        PathRegs ps;
-        V3 o, d;
-        camera_path(sc, rp, (uint64_t)pixel * rp.spp + si, ps, o, d);
-        for (;;) {
-            const TraceHit th = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, o, d, ps.time, __builtin_huge_valf(),
-                                                                       false, lds_col, MEGA_BLOCK, tc);
-            const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
-            ShadeOut so;
-            shade_path(sc, rp, ps, hr, o, d, so);
-            traced += so.shadow ? 2u : 1u;
-            if (so.shadow) {
-                const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, so.so, so.sd, ps.time, so.smax, true,
-                                                                           lds_col, MEGA_BLOCK, tc);
-                if (sh.kind == PSY_HIT_MISS) ps.color = ps.color + ps.pending;
+    for (uint32_t ch = 0; ch < chunks; ch++) {
+        const uint32_t pixel = small ? pix0 + lane / spp : pix0;
+        const uint32_t si = small ? lane % spp : ch * MEGA_BLOCK + lane;
+        const bool active = (!small || lane < per_block * spp) && pixel < pixels && si < spp;
+        V3 c = v3(0.0f, 0.0f, 0.0f);
+        if (active) {
+            PathRegs ps;
+            V3 o, d;
+            camera_path(sc, rp, (uint64_t)pixel * spp + si, ps, o, d);
+            for (;;) {
+                const TraceHit th = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, o, d, ps.time, __builtin_huge_valf(),
+                                                                           false, lds_col, MEGA_BLOCK, tc);
+                const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
+                ShadeOut so;
+                shade_path(sc, rp, ps, hr, o, d, so);
+                traced += so.shadow ? 2u : 1u;
+                if (so.shadow) {
+                    const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, so.so, so.sd, ps.time, so.smax, true,
+                                                                               lds_col, MEGA_BLOCK, tc);
+                    if (sh.kind == PSY_HIT_MISS) ps.color = ps.color + ps.pending;
+                }
+                if (!so.bounce) break;
+                o = so.bo;
+                d = so.bd;
             }
-            if (!so.bounce) break;
-            o = so.bo;
-            d = so.bd;
+            c = xyz_from_spectral(ps.color, ps.wl);
         }
-        const V3 c = xyz_from_spectral(ps.color, ps.wl);
-        fx += c.x / fspp;
-        fy += c.y / fspp;
-        fz += c.z / fspp;
+        lds_xyz[3 * lane] = c.x;
+        lds_xyz[3 * lane + 1] = c.y;
+        lds_xyz[3 * lane + 2] = c.z;
+        __syncthreads();
+        // Film accumulation (renderer.rs:299-310) in sample order.
+        if (small) {
+            if (lane < per_block && pix0 + lane < pixels) {
+                for (uint32_t j = lane * spp; j < (lane + 1) * spp; j++) {
+                    fx += lds_xyz[3 * j] / fspp;
+                    fy += lds_xyz[3 * j + 1] / fspp;
+                    fz += lds_xyz[3 * j + 2] / fspp;
+                }
+            }
+        } else if (lane == 0) {
+            const uint32_t m = min((uint32_t)MEGA_BLOCK, spp - ch * MEGA_BLOCK);
+            for (uint32_t j = 0; j < m; j++) {
+                fx += lds_xyz[3 * j] / fspp;
+                fy += lds_xyz[3 * j + 1] / fspp;
+                fz += lds_xyz[3 * j + 2] / fspp;
+            }
+        }
+        __syncthreads();
     }
     // One atomic per wave for the ray count.
     for (int off = 32; off > 0; off >>= 1) traced += __shfl_xor(traced, off);
-    if ((threadIdx.x & 63u) == 0u && traced) atomicAdd(sc.rays, (unsigned long long)traced);
-    if (pixel >= pixels) return;
-    float* px = rp.film + 3 * region_pixel(rp, pixel);
+    if ((lane & 63u) == 0u && traced) atomicAdd(sc.rays, (unsigned long long)traced);
+    const uint32_t mine = pix0 + lane;
+    if (lane >= (small ? per_block : 1u) || mine >= pixels) return;
+    float* px = rp.film + 3 * region_pixel(rp, mine);
     px[0] = fx;
     px[1] = fy;
     px[2] = fz;
@@ -822,6 +902,61 @@ void halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perms) {
     }
 }
 
+// psy_finalize_hits: the hit geometry of each traced ray, rebuilt from its
+// record as the reference's intersection code computes it (make_isect) plus
+// the local_space it stores: the mesh's mat_space (the inverse of the
+// instance's world-to-object transform, triangle_mesh.rs:134-138, 183-189)
+// or the light instance's transform (rectangle_light.rs:318-328,
+// sphere_light.rs:320-330).
+struct FinalizeArgs {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *time;
+    const uint4* rec;  // hit records, or NULL for the arrays below
+    const uint8_t* kind;
+    const float *t, *b0, *b1, *b2;
+    const uint32_t *prim, *inst;
+    float4* out;       // 6 float4 per ray (psy_hit_geometry)
+    uint64_t n;
+};
+
+__global__ __launch_bounds__(KBLOCK) void finalize_kernel(ShadeScene sc, FinalizeArgs f) {
+    const uint64_t stride = (uint64_t)gridDim.x * KBLOCK;
+    for (uint64_t i = (uint64_t)blockIdx.x * KBLOCK + threadIdx.x; i < f.n; i += stride) {
+        uint32_t kind, prim, inst_id;
+        float t, b0, b1, b2;
+        if (f.rec) {
+            const uint4 r0 = f.rec[2 * i], r1 = f.rec[2 * i + 1];
+            t = __uint_as_float(r0.x); b0 = __uint_as_float(r0.y);
+            b1 = __uint_as_float(r0.z); b2 = __uint_as_float(r0.w);
+            prim = r1.x; inst_id = r1.y; kind = r1.z;
+        } else {
+            kind = f.kind[i];
+            t = f.t[i]; b0 = f.b0[i]; b1 = f.b1[i]; b2 = f.b2[i];
+            prim = f.prim[i]; inst_id = f.inst[i];
+        }
+        float4* o = f.out + 6 * i;
+        if (kind != PSY_HIT_SURFACE) {
+            for (int k = 0; k < 6; k++) o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            continue;
+        }
+        const float time = f.time[i];
+        const V3 orig = v3(f.ox[i], f.oy[i], f.oz[i]);
+        const V3 dir = v3(f.dx[i], f.dy[i], f.dz[i]);
+        const GpuInstance inst = sc.instances[sc.inst_pos[inst_id]];
+        const Isect is = make_isect(sc, inst, t, b0, b1, b2, prim, orig, dir, time);
+        Xf ls = xf_identity();
+        if (inst.kind != PSY_OBJECT_MESH) ls = instance_xf(sc.xforms, inst, time);
+        else if (inst.xf_count > 0)
+            ls = inst.xf_count == 1 ? load_xf(sc.inv_static + inst.inv_static)
+                                    : xf_inverse(instance_xf(sc.xforms, inst, time));
+        o[0] = make_float4(is.pos.x, is.pos.y, is.pos.z, is.pos_err);
+        o[1] = make_float4(is.nor.x, is.nor.y, is.nor.z, is.nor_g.x);
+        o[2] = make_float4(is.nor_g.y, is.nor_g.z, ls.m[0][0], ls.m[0][1]);
+        o[3] = make_float4(ls.m[0][2], ls.m[0][3], ls.m[1][0], ls.m[1][1]);
+        o[4] = make_float4(ls.m[1][2], ls.m[1][3], ls.m[2][0], ls.m[2][1]);
+        o[5] = make_float4(ls.m[2][2], ls.m[2][3], is.sample_pdf, 0.0f);
+    }
+}
+
 }  // namespace
 
 namespace psy {
@@ -859,7 +994,7 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
                        qa_in, qs_in.count);
     PSY_HIP(hipGetLastError());
     psy_trace_tuning tune = r->tuning;
-    tune.kernel = PSY_KERNEL_PERSISTENT;
+    if (tune.kernel == PSY_KERNEL_SIMPLE) tune.kernel = PSY_KERNEL_PERSISTENT;
     for (uint32_t step = 0; step <= steps; step++) {
         const bool last = step == steps;
         if (!last) {
@@ -1169,7 +1304,8 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.light_samples = scene->light_samples;
     sc.lnodes = scene->light_nodes;
     sc.lboxes = scene->light_boxes;
-    sc.llist = scene->light_list;
+    sc.lleaves = scene->light_leaves;
+    sc.lxforms = scene->light_xforms;
     sc.has_ltree = scene->host.light_nodes.empty() ? 0u : 1u;
     sc.shader_count = (uint32_t)shaders.size();
     {
@@ -1273,9 +1409,11 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
                     : r->plain_waves == 3 ? megakernel<3>
                     : r->plain_waves == 2 ? megakernel<2>
                                           : megakernel<1>;
-        hipLaunchKernelGGL(mega, dim3((unsigned)(((uint64_t)rp.rw * rp.rh + MEGA_BLOCK - 1) / MEGA_BLOCK)),
-                           dim3(MEGA_BLOCK), 0, s,
-                           scene_args(r->scene), r->sc, r->reqs);
+        const uint64_t per_block = mega_pixels_per_block(rp.spp);
+        const uint64_t blocks = ((uint64_t)rp.rw * rp.rh + per_block - 1) / per_block;
+        if (blocks > 0x7fffffffull) return fail(PSY_E_RANGE, "image too large for the one-launch renderer");
+        hipLaunchKernelGGL(mega, dim3((unsigned)blocks), dim3(MEGA_BLOCK), 0, s, scene_args(r->scene), r->sc,
+                           r->reqs);
         PSY_HIP(hipGetLastError());
     } else {
         for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
@@ -1439,6 +1577,43 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
     return PSY_OK;
 }
 
+int psy_finalize_hits(psy_tracer* tr, const psy_ray_soa* rays, uint64_t n, const psy_hit_soa* hits,
+                      psy_hit_geometry* geometry) {
+    if (!tr || !geometry) return fail(PSY_E_INVALID, "NULL argument");
+    int rc = check_soa(rays, hits);
+    if (rc) return rc;
+    if (reinterpret_cast<uintptr_t>(geometry) % 16 != 0) return fail(PSY_E_INVALID, "geometry must be 16-byte aligned");
+    if (n == 0) return PSY_OK;
+    const psy_scene* s = tr->scene;
+    ShadeScene sc{};
+    sc.instances = s->instances;
+    sc.inst_pos = s->inst_pos;
+    sc.meshes = s->meshes;
+    sc.tris = s->tris;
+    sc.tri_rec = s->tri_rec;
+    sc.normals = s->normals;
+    sc.nidx = s->nidx;
+    sc.xforms = s->xforms;
+    sc.inv_static = s->inv_static;
+    sc.rects = s->rect_lights;
+    sc.spheres = s->sphere_lights;
+    sc.light_samples = s->light_samples;
+    FinalizeArgs f{};
+    f.ox = rays->ox; f.oy = rays->oy; f.oz = rays->oz;
+    f.dx = rays->dx; f.dy = rays->dy; f.dz = rays->dz;
+    f.time = rays->time;
+    f.rec = reinterpret_cast<const uint4*>(hits->records);
+    f.kind = hits->kind;
+    f.t = hits->t; f.b0 = hits->b0; f.b1 = hits->b1; f.b2 = hits->b2;
+    f.prim = hits->prim_id; f.inst = hits->inst_id;
+    f.out = reinterpret_cast<float4*>(geometry);
+    f.n = n;
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(n)), dim3(KBLOCK), 0, tr->stream, sc, f);
+    PSY_HIP(hipGetLastError());
+    PSY_HIP(hipStreamSynchronize(tr->stream));
+    return PSY_OK;
+}
+
 int psy_renderer_get_tuning(const psy_renderer* r, psy_trace_tuning* t) {
     if (!r || !t) return fail(PSY_E_INVALID, "NULL argument");
     *t = r->tuning;
@@ -1449,7 +1624,7 @@ int psy_renderer_set_tuning(psy_renderer* r, const psy_trace_tuning* t) {
     if (!r || !t) return fail(PSY_E_INVALID, "NULL argument");
     int rc = check_tuning(t);
     if (rc) return rc;
-    if (t->kernel != PSY_KERNEL_PERSISTENT) return fail(PSY_E_INVALID, "the renderer traces with the persistent kernel");
+    if (t->kernel == PSY_KERNEL_SIMPLE) return fail(PSY_E_INVALID, "the renderer traces with a persistent kernel");
     r->tuning = *t;
     for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.second);
     r->graphs.clear();

@@ -85,7 +85,8 @@ struct ShadeScene {
     const float* light_samples;  // time samples of the lights (layout.h)
     const GpuLightNode* lnodes;
     const GpuBox* lboxes;
-    const uint32_t* llist;
+    const GpuLightLeaf* lleaves;
+    const GpuXform* lxforms;
     uint32_t has_ltree;
     uint32_t shader_count;
     const GpuClosure* shaders;

@@ -7,6 +7,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 
 namespace psy {
@@ -80,6 +82,24 @@ bool color_ok(const psy_color& c) {
 GpuColor to_gpu(const psy_color& c) { return GpuColor{c.kind, c.a, c.b, c.c}; }
 
 }  // namespace
+
+std::vector<Xf> push_xforms(const std::vector<Xf>& top, const std::vector<Xf>& own) {
+    if (own.empty()) return top;
+    if (top.empty()) return own;
+    std::vector<Xf> out;
+    if (top.size() == own.size()) {
+        for (size_t i = 0; i < own.size(); i++) out.push_back(xf_then(top[i], own[i]));
+    } else if (top.size() > own.size()) {
+        const float s = (float)(top.size() - 1);
+        for (size_t i = 0; i < top.size(); i++)
+            out.push_back(xf_then(top[i], lerp_slice_xf(own.data(), (uint32_t)own.size(), (float)i / s)));
+    } else {
+        const float s = (float)(own.size() - 1);
+        for (size_t i = 0; i < own.size(); i++)
+            out.push_back(xf_then(lerp_slice_xf(top.data(), (uint32_t)top.size(), (float)i / s), own[i]));
+    }
+    return out;
+}
 
 // Color::approximate_energy (src/color.rs:127-142), and the light objects'
 // approximate_energy: the fold of their colours' estimates from 0 in list
@@ -205,7 +225,10 @@ void lt_collect(const LtNode* n, int level, std::vector<const LtNode*>& out) {
     }
 }
 
-void lt_emit(const LtNode* n, uint32_t slot, PackedScene& s) {
+// Emits the collapsed tree of `n` with its root at light node `slot`;
+// `leaf(light)` returns the light-leaf index of a leaf's light.
+template <typename LeafFn>
+void lt_emit(const LtNode* n, uint32_t slot, PackedScene& s, LeafFn&& leaf) {
     GpuLightNode g{};
     g.bounds_start = (uint32_t)s.light_boxes.size();
     g.bounds_len = (uint32_t)n->bounds.size();
@@ -213,7 +236,7 @@ void lt_emit(const LtNode* n, uint32_t slot, PackedScene& s) {
     g.energy = n->energy;
     std::vector<const LtNode*> kids;
     if (n->leaf) {
-        g.light = n->light;
+        g.light = leaf(n->light);
     } else {
         lt_collect(n, 3, kids);
         g.child_first = (uint32_t)s.light_nodes.size();
@@ -221,12 +244,174 @@ void lt_emit(const LtNode* n, uint32_t slot, PackedScene& s) {
         s.light_nodes.resize(s.light_nodes.size() + kids.size());
     }
     s.light_nodes[slot] = g;
-    for (uint32_t i = 0; i < (uint32_t)kids.size(); i++) lt_emit(kids[i], g.child_first + i, s);
+    for (uint32_t i = 0; i < (uint32_t)kids.size(); i++) lt_emit(kids[i], g.child_first + i, s, leaf);
+}
+
+// ---- nested assemblies --------------------------------------------------
+//
+// A description with sub-assemblies is flattened into one single-level
+// description: every assembly's meshes and lights (definitions placed depth
+// first, the root's first) and, for every instance path, the mesh and light
+// instances with the transforms the transform stack composes along it
+// (tracer.rs:95-148).  The assemblies themselves stay for light selection:
+// each instance path through an assembly is a group with the composed
+// transforms into it and, per instance of the assembly, the flattened
+// instance or sub-group it became.
+constexpr uint32_t MAX_NESTING = 64;
+
+struct DefInfo {
+    uint32_t mesh_base = 0, rect_base = 0, sphere_base = 0;
+    std::vector<Box> inst_bounds;   // per instance, its time samples in the assembly's space
+    std::vector<uint32_t> inst_offs;
+    std::vector<uint32_t> order;    // instance order after the assembly's instance BVH build
+    std::vector<Box> bounds;        // that BVH's root bounds (Assembly::bounds, assembly.rs:144-148)
+};
+struct Group {
+    const psy_scene_desc* def;
+    std::vector<Xf> top;            // composed transforms into the assembly
+    std::vector<uint32_t> target;   // per instance: flattened instance, or sub-group
+    uint32_t depth;                 // 0 for the root assembly
+};
+struct Flat {
+    std::vector<psy_mesh_desc> meshes;
+    std::vector<psy_rect_light_desc> rects;
+    std::vector<psy_sphere_light_desc> spheres;
+    std::vector<psy_instance_desc> instances;
+    std::vector<float> xforms;
+    std::vector<const psy_scene_desc*> defs;  // depth first, the root first
+    std::map<const psy_scene_desc*, uint32_t> def_index;
+    std::vector<DefInfo> info;
+    std::vector<Group> groups;                // [0]: the root assembly
+    psy_scene_desc desc{};
+};
+
+int place_defs(const psy_scene_desc* d, Flat& f, uint32_t depth, std::string& err) {
+    if (depth > MAX_NESTING) { err = "assemblies nest deeper than 64 levels"; return PSY_E_RANGE; }
+    const std::string who = depth ? "sub-assembly: " : "";
+    if (d->mesh_count > 0 && !d->meshes) { err = who + "meshes is NULL"; return PSY_E_INVALID; }
+    if (d->instance_count > 0 && !d->instances) { err = who + "instances is NULL"; return PSY_E_INVALID; }
+    if (d->xform_count > 0 && !d->xforms) { err = who + "xforms is NULL"; return PSY_E_INVALID; }
+    if (d->instance_count > MAX_LEAF_FIRST) { err = who + "too many instances"; return PSY_E_RANGE; }
+    if (d->rect_light_count > 0 && !d->rect_lights) { err = who + "rect_lights is NULL"; return PSY_E_INVALID; }
+    if (d->sphere_light_count > 0 && !d->sphere_lights) { err = who + "sphere_lights is NULL"; return PSY_E_INVALID; }
+    if (d->assembly_count > 0 && !d->assemblies) { err = who + "assemblies is NULL"; return PSY_E_INVALID; }
+    if (f.def_index.count(d)) { err = "an assembly description is listed twice"; return PSY_E_INVALID; }
+    const uint32_t id = (uint32_t)f.defs.size();
+    f.defs.push_back(d);
+    f.def_index[d] = id;
+    f.info.emplace_back();
+    f.info[id].mesh_base = (uint32_t)f.meshes.size();
+    f.info[id].rect_base = (uint32_t)f.rects.size();
+    f.info[id].sphere_base = (uint32_t)f.spheres.size();
+    f.meshes.insert(f.meshes.end(), d->meshes, d->meshes + d->mesh_count);
+    f.rects.insert(f.rects.end(), d->rect_lights, d->rect_lights + d->rect_light_count);
+    f.spheres.insert(f.spheres.end(), d->sphere_lights, d->sphere_lights + d->sphere_light_count);
+    for (uint32_t a = 0; a < d->assembly_count; a++) {
+        const int rc = place_defs(&d->assemblies[a], f, depth + 1, err);
+        if (rc) return rc;
+    }
+    return PSY_OK;
+}
+
+// Expands group `g`'s instances; sub-assembly instances open sub-groups.
+int expand(uint32_t g, uint32_t depth, Flat& f, std::string& err) {
+    const psy_scene_desc* d = f.groups[g].def;
+    const uint32_t di = f.def_index.at(d);
+    const std::vector<Xf> top = f.groups[g].top;
+    std::vector<uint32_t> target(d->instance_count);
+    for (uint32_t i = 0; i < d->instance_count; i++) {
+        const psy_instance_desc& id = d->instances[i];
+        const std::string who = (depth ? "sub-assembly instance " : "instance ") + std::to_string(i);
+        if (id.kind > PSY_OBJECT_ASSEMBLY) { err = who + ": unknown object kind"; return PSY_E_INVALID; }
+        const uint32_t limit = id.kind == PSY_OBJECT_MESH ? d->mesh_count
+                               : id.kind == PSY_OBJECT_RECT_LIGHT ? d->rect_light_count
+                               : id.kind == PSY_OBJECT_SPHERE_LIGHT ? d->sphere_light_count
+                                                                     : d->assembly_count;
+        if (id.mesh >= limit) { err = who + ": object out of range"; return PSY_E_INVALID; }
+        if ((uint64_t)id.xform_start + id.xform_count > d->xform_count) {
+            err = who + ": transform range out of bounds";
+            return PSY_E_INVALID;
+        }
+        if (id.xform_count > MAX_TIME_SAMPLES) { err = who + ": too many transforms"; return PSY_E_RANGE; }
+        std::vector<Xf> own;
+        for (uint32_t k = 0; k < id.xform_count; k++) own.push_back(xf_at(d, id.xform_start + k));
+        const std::vector<Xf> composed = push_xforms(top, own);
+        if (composed.size() > MAX_TIME_SAMPLES) { err = who + ": too many composed transforms"; return PSY_E_RANGE; }
+        if (id.kind == PSY_OBJECT_ASSEMBLY) {
+            const uint32_t cg = (uint32_t)f.groups.size();
+            f.groups.push_back(Group{&d->assemblies[id.mesh], composed, {}, depth + 1});
+            target[i] = cg;
+            const int rc = expand(cg, depth + 1, f, err);
+            if (rc) return rc;
+            continue;
+        }
+        if (id.kind != PSY_OBJECT_MESH && composed.empty()) {
+            err = who + ": light instances need a transform";
+            return PSY_E_INVALID;
+        }
+        psy_instance_desc out{};
+        out.kind = id.kind;
+        out.shader = id.shader;
+        const DefInfo& info = f.info[di];
+        out.mesh = id.mesh + (id.kind == PSY_OBJECT_MESH         ? info.mesh_base
+                              : id.kind == PSY_OBJECT_RECT_LIGHT ? info.rect_base
+                                                                 : info.sphere_base);
+        out.xform_start = (uint32_t)(f.xforms.size() / 12);
+        out.xform_count = (uint32_t)composed.size();
+        for (const Xf& x : composed)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 4; c++) f.xforms.push_back(x.m[r][c]);
+        target[i] = (uint32_t)f.instances.size();
+        f.instances.push_back(out);
+        if (f.instances.size() > MAX_LEAF_FIRST) { err = "too many instances"; return PSY_E_RANGE; }
+    }
+    f.groups[g].target = std::move(target);
+    return PSY_OK;
+}
+
+// Object bounds of instance `id` of an assembly in the object's own space:
+// a mesh's BVH root bounds, a light's bounds list (rectangle_light.rs:36-42,
+// sphere_light.rs:35-40), or a sub-assembly's bounds.
+std::vector<Box> object_bounds(const psy_scene_desc* d, const psy_instance_desc& id, const DefInfo& info,
+                               const PackedScene& s, const Flat& f) {
+    std::vector<Box> ob;
+    if (id.kind == PSY_OBJECT_MESH) {
+        ob = s.bvhs[info.mesh_base + id.mesh].root_bounds;
+    } else if (id.kind == PSY_OBJECT_RECT_LIGHT) {
+        const psy_rect_light_desc& l = d->rect_lights[id.mesh];
+        for (uint32_t k = 0; k < std::max(1u, l.dims_count); k++) {
+            const float w = l.dims_count ? l.dims[2 * k] : l.width, h = l.dims_count ? l.dims[2 * k + 1] : l.height;
+            ob.push_back(Box{V3{w * -0.5f, h * -0.5f, 0.0f}, V3{w * 0.5f, h * 0.5f, 0.0f}});
+        }
+    } else if (id.kind == PSY_OBJECT_SPHERE_LIGHT) {
+        const psy_sphere_light_desc& l = d->sphere_lights[id.mesh];
+        for (uint32_t k = 0; k < std::max(1u, l.radius_count); k++) {
+            const float r = l.radius_count ? l.radii[k] : l.radius;
+            ob.push_back(Box{V3{-r, -r, -r}, V3{r, r, r}});
+        }
+    } else {
+        ob = f.info[f.def_index.at(&d->assemblies[id.mesh])].bounds;
+    }
+    return ob;
+}
+
+float object_energy(const psy_scene_desc* d, const psy_instance_desc& id) {
+    std::vector<psy_color> lc;
+    std::string why;
+    if (id.kind == PSY_OBJECT_RECT_LIGHT) {
+        const psy_rect_light_desc& l = d->rect_lights[id.mesh];
+        (void)colour_list(l.color, l.color_count, l.colors, lc, why);
+    } else {
+        const psy_sphere_light_desc& l = d->sphere_lights[id.mesh];
+        (void)colour_list(l.color, l.color_count, l.colors, lc, why);
+    }
+    return light_energy(lc);
 }
 
 }  // namespace
 
-int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
+// Packs a single-level description (the flattened scene `f`).
+static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::string& err) {
     auto t0 = std::chrono::steady_clock::now();
     if (!d) { err = "scene description is NULL"; return PSY_E_INVALID; }
     if (d->mesh_count > 0 && !d->meshes) { err = "meshes is NULL"; return PSY_E_INVALID; }
@@ -452,36 +637,113 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
         s.instances.push_back(gi);
         if (id.xform_count > 0) s.has_xf = true;
     }
-    // Light tree over the light instances in leaf order
-    // (AssemblyBuilder::build, assembly.rs:297-338).
-    std::vector<uint32_t> light_pos;
-    std::vector<Box> lbounds;
-    std::vector<uint32_t> loffs{0};
-    std::vector<float> lenergy;
-    for (uint32_t k = 0; k < (uint32_t)s.instances.size(); k++) {
-        const GpuInstance& gi = s.instances[k];
-        if (gi.kind == PSY_OBJECT_MESH) continue;
-        light_pos.push_back(k);
-        lbounds.insert(lbounds.end(), ibounds.begin() + ioffs[gi.id], ibounds.begin() + ioffs[gi.id + 1]);
-        loffs.push_back((uint32_t)lbounds.size());
-        std::vector<psy_color> lc;
-        if (gi.kind == PSY_OBJECT_RECT_LIGHT) {
-            const psy_rect_light_desc& l = d->rect_lights[gi.mesh];
-            (void)colour_list(l.color, l.color_count, l.colors, lc, why);
-        } else {
-            const psy_sphere_light_desc& l = d->sphere_lights[gi.mesh];
-            (void)colour_list(l.color, l.color_count, l.colors, lc, why);
+    // Each assembly's instance bounds in its own space and the instance
+    // order its instance BVH leaves (AssemblyBuilder::build, assembly.rs:
+    // 288-295), sub-assemblies (placed after their parents) first.
+    for (uint32_t di = (uint32_t)f.defs.size(); di-- > 0;) {
+        const psy_scene_desc* ad = f.defs[di];
+        DefInfo& info = f.info[di];
+        info.inst_bounds.clear();
+        info.inst_offs.assign(1, 0u);
+        for (uint32_t i = 0; i < ad->instance_count; i++) {
+            const psy_instance_desc& id = ad->instances[i];
+            std::vector<Xf> own;
+            for (uint32_t k = 0; k < id.xform_count; k++) own.push_back(xf_at(ad, id.xform_start + k));
+            const std::vector<Box> wb = transform_bounds(object_bounds(ad, id, info, s, f), own.data(), id.xform_count);
+            info.inst_bounds.insert(info.inst_bounds.end(), wb.begin(), wb.end());
+            info.inst_offs.push_back((uint32_t)info.inst_bounds.size());
         }
-        lenergy.push_back(light_energy(lc));
+        BuildInput abi{ad->instance_count, info.inst_bounds.data(), info.inst_offs.data()};
+        const HostBvh4 ab = build_bvh4(abi, 1, 0);
+        info.order = ab.order;
+        info.bounds = ab.root_bounds;
     }
-    if (!light_pos.empty()) {
-        std::vector<uint32_t> keys(light_pos.size());
+
+    // Light trees (AssemblyBuilder::build, assembly.rs:297-338): per group,
+    // over its light instances and its sub-assembly instances whose trees
+    // have energy, in the assembly's instance-BVH order; sub-groups (which
+    // follow their parents) first, so their energies are known.
+    const uint32_t ng = (uint32_t)f.groups.size();
+    std::vector<std::unique_ptr<LtNode>> gtree(ng);
+    std::vector<std::vector<uint32_t>> gents(ng);  // light order -> instance of the assembly
+    std::vector<float> genergy(ng, 0.0f);
+    for (uint32_t g = ng; g-- > 0;) {
+        const Group& G = f.groups[g];
+        const psy_scene_desc* ad = G.def;
+        const DefInfo& info = f.info[f.def_index.at(ad)];
+        std::vector<uint32_t> ents;
+        std::vector<Box> lb;
+        std::vector<uint32_t> lo{0};
+        std::vector<float> le;
+        for (uint32_t k : info.order) {
+            const psy_instance_desc& id = ad->instances[k];
+            float e;
+            if (id.kind == PSY_OBJECT_MESH) continue;
+            if (id.kind == PSY_OBJECT_ASSEMBLY) {
+                e = genergy[G.target[k]];
+                if (!(e > 0.0f)) continue;
+            } else {
+                e = object_energy(ad, id);
+            }
+            ents.push_back(k);
+            lb.insert(lb.end(), info.inst_bounds.begin() + info.inst_offs[k], info.inst_bounds.begin() + info.inst_offs[k + 1]);
+            lo.push_back((uint32_t)lb.size());
+            le.push_back(e);
+        }
+        if (ents.empty()) continue;
+        if (G.depth >= MAX_LIGHT_LEVELS) {
+            err = "lights nested in more than " + std::to_string(MAX_LIGHT_LEVELS - 1) + " levels of sub-assemblies";
+            return PSY_E_RANGE;
+        }
+        std::vector<uint32_t> keys(ents.size());
         for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) keys[i] = i;
-        BuildInput lin{(uint32_t)keys.size(), lbounds.data(), loffs.data()};
-        std::unique_ptr<LtNode> root = lt_build(lin, lenergy, keys.data(), (uint32_t)keys.size(), 0);
+        BuildInput lin{(uint32_t)keys.size(), lb.data(), lo.data()};
+        gtree[g] = lt_build(lin, le, keys.data(), (uint32_t)keys.size(), 0);
+        genergy[g] = gtree[g]->energy;
+        gents[g].resize(keys.size());
+        for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) gents[g][i] = ents[keys[i]];
+    }
+    if (gtree[0]) {
+        auto push_list = [&](const std::vector<Xf>& l, uint32_t& start, uint32_t& count) {
+            start = (uint32_t)s.light_xforms.size();
+            count = (uint32_t)l.size();
+            for (const Xf& x : l) s.light_xforms.push_back(to_gpu(x));
+        };
+        std::deque<std::pair<uint32_t, uint32_t>> pending;  // (leaf, sub-group) whose tree is not emitted yet
+        auto emit_group = [&](uint32_t g, uint32_t slot) {
+            const Group& G = f.groups[g];
+            const psy_scene_desc* ad = G.def;
+            lt_emit(gtree[g].get(), slot, s, [&](uint32_t light) {
+                const uint32_t k = gents[g][light];
+                const psy_instance_desc& id = ad->instances[k];
+                GpuLightLeaf lf{};
+                const uint32_t leaf = (uint32_t)s.light_leaves.size();
+                if (id.kind == PSY_OBJECT_ASSEMBLY) {
+                    lf.kind = LIGHT_LEAF_ASSEMBLY;
+                    push_list(f.groups[G.target[k]].top, lf.top_start, lf.top_count);
+                    pending.emplace_back(leaf, G.target[k]);
+                } else {
+                    lf.kind = LIGHT_LEAF_LIGHT;
+                    lf.target = s.inst_pos[G.target[k]];
+                    push_list(G.top, lf.top_start, lf.top_count);
+                    std::vector<Xf> own;
+                    for (uint32_t x = 0; x < id.xform_count; x++) own.push_back(xf_at(ad, id.xform_start + x));
+                    push_list(own, lf.own_start, lf.own_count);
+                }
+                s.light_leaves.push_back(lf);
+                return leaf;
+            });
+        };
         s.light_nodes.resize(1);
-        lt_emit(root.get(), 0, s);
-        for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) s.light_list.push_back(light_pos[keys[i]]);
+        emit_group(0, 0);
+        while (!pending.empty()) {
+            const auto [leaf, g] = pending.front();
+            pending.pop_front();
+            const uint32_t slot = (uint32_t)s.light_nodes.size();
+            s.light_nodes.resize(slot + 1);
+            emit_group(g, slot);
+            s.light_leaves[leaf].target = slot;
+        }
     }
 
     s.has_motion = !s.motion.empty();
@@ -509,6 +771,27 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     s.order_table.assign(traversal_order_table(), traversal_order_table() + 8 * 48);
     s.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PSY_OK;
+}
+
+int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
+    if (!d) { err = "scene description is NULL"; return PSY_E_INVALID; }
+    Flat f;
+    int rc = place_defs(d, f, 0, err);
+    if (rc) return rc;
+    f.groups.push_back(Group{d, {}, {}, 0});
+    rc = expand(0, 0, f, err);
+    if (rc) return rc;
+    f.desc.mesh_count = (uint32_t)f.meshes.size();
+    f.desc.meshes = f.meshes.data();
+    f.desc.instance_count = (uint32_t)f.instances.size();
+    f.desc.instances = f.instances.data();
+    f.desc.xform_count = (uint32_t)(f.xforms.size() / 12);
+    f.desc.xforms = f.xforms.data();
+    f.desc.rect_light_count = (uint32_t)f.rects.size();
+    f.desc.rect_lights = f.rects.data();
+    f.desc.sphere_light_count = (uint32_t)f.spheres.size();
+    f.desc.sphere_lights = f.spheres.data();
+    return pack_flat(&f.desc, f, s, err);
 }
 
 }  // namespace psy

@@ -33,9 +33,10 @@ struct PackedScene {
     std::vector<float> light_samples;       // time samples of the lights, see GpuRectLight
     std::vector<uint32_t> inst_pos;         // description index -> leaf position
     std::vector<uint32_t> tri_rec;          // per mesh at GpuMesh::rec_map: triangle -> record
-    std::vector<GpuLightNode> light_nodes;  // light tree, root first; empty without lights
+    std::vector<GpuLightNode> light_nodes;  // light trees, the root assembly's root first; empty without lights
     std::vector<GpuBox> light_boxes;
-    std::vector<uint32_t> light_list;       // light-tree order -> instance leaf position
+    std::vector<GpuLightLeaf> light_leaves; // leaves of the light trees
+    std::vector<GpuXform> light_xforms;     // transform lists of the leaves
 
     // Host builds, kept for parity dumps (index = mesh; the instance BVH last).
     std::vector<HostBvh4> bvhs;
@@ -53,6 +54,12 @@ struct PackedScene {
 // Validates `desc` and builds every BVH.  Returns PSY_OK or an error code
 // with a message in `err`.
 int pack_scene(const psy_scene_desc* desc, PackedScene& out, std::string& err);
+
+// TransformStack::push (src/transform_stack.rs:33-60): the parent's and an
+// instance's time samples combined by merge_slices_to (src/algorithm.rs:
+// 266-305), the shorter list resampled with lerp_slice, each pair composed
+// as math3d's parent * child.
+std::vector<Xf> push_xforms(const std::vector<Xf>& top, const std::vector<Xf>& own);
 
 // Light time samples (shared with the renderer's distant lights): the mean
 // colour energy, a checked colour list, and the packing of several samples

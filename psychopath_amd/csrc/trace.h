@@ -53,6 +53,7 @@ struct TraceTuning {
     uint32_t lds_depth;   // stack entries per lane kept in LDS (8, 16, 24)
     uint32_t occupancy;   // register budget as waves per SIMD (4 or 8)
     uint32_t leaf_min;    // waiting lanes that end a descent round (1..64), 0: per-lane descents
+    uint32_t coop;        // 1: wave-cooperative node fetch (static scenes)
 };
 
 // One lane per ray, one launch-wide grid (trace.hip).

@@ -54,10 +54,20 @@ constexpr int WAVE = 64;
         }                                                                 \
     } while (0)
 
-template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES>
+// Node parts a static descent step reads: the six box rows and the child
+// references (the last 16 bytes are not needed, layout.h).
+constexpr uint32_t NODE_PARTS = 7;
+
+template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES, bool COOP>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
+    // COOP: the nodes the wave's lanes stand on are fetched by the wave
+    // together, consecutive lanes loading consecutive 16-byte parts of one
+    // node (one cache line per node instead of one tag lookup per lane and
+    // part), staged here and read back by their lanes.
+    __shared__ uint4 lds_nodes[COOP ? WAVE * NODE_PARTS : 1];
+    __shared__ uint32_t lds_slot[COOP ? WAVE : 1];
     // TRAVERSAL_TABLE (8 octants x 48 codes): static nodes carry only their
     // code, in spare bits of their child references (layout.h).
     __shared__ uint32_t lds_order[8 * 48 / 4];
@@ -150,9 +160,39 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         //
         // One descent step of a lane standing on an internal node: its four
         // child boxes, then the hit children pushed in visit order.
-        auto node_step = [&]() {
+        // The wave's lanes standing on internal nodes (`inner`, ballot `m`)
+        // have their nodes' parts loaded by all active lanes together:
+        // parts item = slot * 7 + part, item k of a batch by the k-th active
+        // lane, four loads in flight per lane; returns this lane's slot.
+        auto coop_fetch = [&](bool inner, uint64_t m) -> uint32_t {
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t slot = (uint32_t)__popcll(m & below);
+            const uint32_t items = NODE_PARTS * (uint32_t)__popcll(m);
+            const uint64_t act = __ballot(1);
+            const uint32_t na = (uint32_t)__popcll(act), ar = (uint32_t)__popcll(act & below);
+            if (inner) lds_slot[slot] = cur;
+            __syncthreads();
+            for (uint32_t base = 0; base < items; base += 4 * na) {
+                uint4 v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t item = base + (uint32_t)k * na + ar;
+                    if (item < items)
+                        v[k] = reinterpret_cast<const uint4*>(a.nodes + lds_slot[item / NODE_PARTS])[item % NODE_PARTS];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t item = base + (uint32_t)k * na + ar;
+                    if (item < items) lds_nodes[item] = v[k];
+                }
+            }
+            __syncthreads();
+            return slot;
+        };
+        auto node_step = [&](uint32_t slot) {
                 const float4* nf = reinterpret_cast<const float4*>(a.nodes + cur);
-                uint4 refs = *reinterpret_cast<const uint4*>(nf + 6);
+                const uint4* staged = lds_nodes + (COOP ? slot * NODE_PARTS : 0u);
+                uint4 refs = COOP ? staged[6] : *reinterpret_cast<const uint4*>(nf + 6);
                 // Static scenes read seven of the node's eight 16-byte parts:
                 // the child count follows from the absent references and the
                 // visit order from the traversal code in the references' spare
@@ -167,7 +207,10 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                     MOTION ? (tail.z & 7u) : 2u + (refs.z != REF_EMPTY ? 1u : 0u) + (refs.w != REF_EMPTY ? 1u : 0u);
                 float4 lx, hx, ly, hy, lz, hz;
                 const uint32_t blen = MOTION ? ((tail.z >> 8) & 0xffu) : 1u;
-                if (!MOTION || blen <= 1) {
+                if (COOP) {
+                    const float4* sf = reinterpret_cast<const float4*>(staged);
+                    lx = sf[0]; hx = sf[1]; ly = sf[2]; hy = sf[3]; lz = sf[4]; hz = sf[5];
+                } else if (!MOTION || blen <= 1) {
                     lx = nf[0]; hx = nf[1]; ly = nf[2]; hy = nf[3]; lz = nf[4]; hz = nf[5];
                     if (COUNT) c_nodes++;
                 } else {
@@ -207,15 +250,27 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         };
         if (leaf_min == 0) {
             // Each lane descends until it stands on a leaf or a dead end.
-            while (cur != REF_EMPTY && !(cur & REF_LEAF)) node_step();
+            if (COOP) {
+                for (;;) {
+                    const bool inner = cur != REF_EMPTY && !(cur & REF_LEAF);
+                    const uint64_t m = __ballot(inner);
+                    if (m == 0) break;
+                    const uint32_t slot = coop_fetch(inner, m);
+                    if (inner) node_step(slot);
+                }
+            } else {
+                while (cur != REF_EMPTY && !(cur & REF_LEAF)) node_step(0u);
+            }
         } else {
             const uint32_t n_active = (uint32_t)__popcll(__ballot(1));
             for (;;) {
                 if (cur == REF_EMPTY && sp > (in_inst ? inst_base : 0)) SSTACK_POP(cur);
                 const bool inner = cur != REF_EMPTY && !(cur & REF_LEAF);
-                const uint32_t n_inner = (uint32_t)__popcll(__ballot(inner));
+                const uint64_t m = __ballot(inner);
+                const uint32_t n_inner = (uint32_t)__popcll(m);
                 if (n_inner == 0 || n_active - n_inner >= leaf_min) break;
-                if (inner) node_step();
+                const uint32_t slot = COOP ? coop_fetch(inner, m) : 0u;
+                if (inner) node_step(slot);
             }
         }
 
@@ -391,12 +446,24 @@ template <bool COUNT, bool XF, bool MOTION, int D>
 hipError_t launch_one(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
     // Register budget: 8 waves per SIMD caps the kernel at 64 VGPRs (the
     // compiler keeps the rest in private memory), 4 lets it use up to 128.
+    // The cooperative node fetch serves static scenes.
+    if constexpr (!COUNT && !MOTION) {
+        if (tune.coop) {
+            if (tune.occupancy >= 8)
+                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 8, true>), dim3(tune.waves),
+                                   dim3(WAVE), 0, s, a, work, tune.refill_min, tune.leaf_min);
+            else
+                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 4, true>), dim3(tune.waves),
+                                   dim3(WAVE), 0, s, a, work, tune.refill_min, tune.leaf_min);
+            return hipGetLastError();
+        }
+    }
     if (!COUNT && tune.occupancy >= 8)
-        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8>), dim3(tune.waves), dim3(WAVE), 0, s, a,
-                           work, tune.refill_min, tune.leaf_min);
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8, false>), dim3(tune.waves), dim3(WAVE), 0,
+                           s, a, work, tune.refill_min, tune.leaf_min);
     else
-        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4>), dim3(tune.waves), dim3(WAVE), 0, s, a,
-                           work, tune.refill_min, tune.leaf_min);
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4, false>), dim3(tune.waves), dim3(WAVE), 0,
+                           s, a, work, tune.refill_min, tune.leaf_min);
     return hipGetLastError();
 }
 

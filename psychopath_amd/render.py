@@ -132,21 +132,31 @@ class Renderer:
         return img, {f: getattr(stats, f) for f, _ in _lib.RenderStats._fields_}
 
 
-def build(rs, max_paths: int = 0) -> Renderer:
-    """Device scene + renderer for a psychopath_amd.scenes.RenderScene."""
+def assembly_of(rs) -> Assembly:
+    """The tracer Assembly of a RenderScene (or of one of its optional
+    `sub_assemblies`: objects with meshes, rect_lights, sphere_lights,
+    instances and sub_assemblies, instances of kind "assembly" naming a
+    sub-assembly by index)."""
     asm = Assembly()
     for m in rs.meshes:
         asm.add_mesh(Mesh(m[0], m[1], m[2] if len(m) > 2 else None))
-    for (w, h, col) in rs.rect_lights:
+    for (w, h, col) in getattr(rs, "rect_lights", []):
         asm.add_rect_light(RectLight(w, h, col))
-    for (r, col) in rs.sphere_lights:
+    for (r, col) in getattr(rs, "sphere_lights", []):
         asm.add_sphere_light(SphereLight(r, col))
+    for sub in getattr(rs, "sub_assemblies", []):
+        asm.add_assembly(assembly_of(sub))
     for inst in rs.instances:
         kind = {"mesh": _lib.PSY_OBJECT_MESH, "rect": _lib.PSY_OBJECT_RECT_LIGHT,
-                "sphere": _lib.PSY_OBJECT_SPHERE_LIGHT}[inst["kind"]]
+                "sphere": _lib.PSY_OBJECT_SPHERE_LIGHT, "assembly": _lib.PSY_OBJECT_ASSEMBLY}[inst["kind"]]
         shader = _lib.PSY_NONE if inst.get("shader") is None else int(inst["shader"])
         asm.add_instance(inst["index"], inst.get("xforms"), shader, kind)
-    scene = asm.build()
+    return asm
+
+
+def build(rs, max_paths: int = 0) -> Renderer:
+    """Device scene + renderer for a psychopath_amd.scenes.RenderScene."""
+    scene = assembly_of(rs).build()
     return Renderer(scene, rs.shaders, rs.background, rs.distant_lights, max_paths)
 
 

@@ -319,6 +319,54 @@ def cornell_box(width=512, height=512, spp=16, max_bounces=4, seed=0) -> RenderS
     return sc
 
 
+class SubAssembly:
+    """A sub-assembly of a RenderScene: meshes, light objects, instances and
+    its own sub-assemblies, instanced by instances of kind "assembly" (index
+    into the parent's `sub_assemblies`).  Shader indices refer to the
+    scene's one shader list."""
+
+    def __init__(self):
+        self.meshes = []
+        self.rect_lights = []
+        self.sphere_lights = []
+        self.instances = []
+        self.sub_assemblies = []
+
+
+def nested_lights_scene(width=96, height=96, spp=8, max_bounces=3, seed=0) -> RenderScene:
+    """The Cornell box with lamps in nested assemblies: a "lamp" assembly
+    (a small box shade, a sphere light and a rectangle light, and a "bulb"
+    sub-assembly holding a tiny sphere light, instanced twice inside the
+    lamp) placed three times in the room, once under two-sample motion.
+    Light selection then recurses through two levels of sub-assembly light
+    trees (scene/assembly.rs:46-141)."""
+    sc = cornell_box(width, height, spp, max_bounces, seed)
+    bulb = SubAssembly()
+    bulb.sphere_lights.append((0.04, (COLOR_BLACKBODY, 3200.0, 4.0e-10, 0.0)))
+    bulb.instances.append(dict(kind="sphere", index=0, shader=None, xforms=translate_xf((0.0, 0.0, 0.05))[None]))
+    lamp = SubAssembly()
+    v, f = _box((0.0, 0.0, 0.0), (0.3, 0.3, 0.2), 10.0)
+    lamp.meshes.append((v, f))
+    lamp.instances.append(dict(kind="mesh", index=0, shader=0, xforms=None))
+    lamp.sphere_lights.append((0.07, rec709(12.0, 10.0, 7.0)))
+    lamp.instances.append(dict(kind="sphere", index=0, shader=None, xforms=translate_xf((0.0, 0.0, 0.35))[None]))
+    lamp.rect_lights.append((0.12, 0.2, rec709(30.0, 28.0, 20.0)))
+    lamp.instances.append(dict(kind="rect", index=0, shader=None,
+                               xforms=world_to_object(_rotation((1.0, 0.0, 0.0), 2.2), np.array([0.0, -0.2, 0.28]))[None]))
+    lamp.sub_assemblies.append(bulb)
+    lamp.instances.append(dict(kind="assembly", index=0, shader=None, xforms=translate_xf((0.1, 0.0, 0.2))[None]))
+    lamp.instances.append(dict(kind="assembly", index=0, shader=None,
+                               xforms=world_to_object(_rotation((0.0, 0.0, 1.0), 0.7), np.array([-0.1, 0.05, 0.2]))[None]))
+    sc.sub_assemblies = [lamp]
+    places = [(-4.3, 1.2, 0.0), (-1.0, 4.3, 0.0)]
+    for p in places:
+        sc.instances.append(dict(kind="assembly", index=0, shader=None, xforms=translate_xf(p)[None]))
+    moving = np.stack([world_to_object(_rotation((0.0, 0.0, 1.0), a), np.array([-2.7, 2.4, 3.3 + 0.4 * a]))
+                       for a in (0.0, 0.6)])
+    sc.instances.append(dict(kind="assembly", index=0, shader=None, xforms=moving))
+    return sc
+
+
 def normal01(seed: int, n: int) -> np.ndarray:
     """Seeded standard normal samples (Box-Muller over uniform01)."""
     u = uniform01(seed, 2 * n).astype(np.float64).reshape(2, n)

@@ -26,6 +26,10 @@ HIT_U32 = ("prim_id", "inst_id")
 HIT_RECORD = np.dtype([("t", "<f4"), ("b0", "<f4"), ("b1", "<f4"), ("b2", "<f4"), ("prim_id", "<u4"),
                        ("inst_id", "<u4"), ("kind", "<u4"), ("reserved", "<u4")])
 assert HIT_RECORD.itemsize == 32
+# psy_hit_geometry: the rest of SurfaceIntersectionData, 96 bytes per ray.
+HIT_GEOMETRY = np.dtype([("pos", "<f4", 3), ("pos_err", "<f4"), ("nor", "<f4", 3), ("nor_g", "<f4", 3),
+                         ("local_space", "<f4", 12), ("sample_pdf", "<f4"), ("reserved", "<f4")])
+assert HIT_GEOMETRY.itemsize == 96
 
 
 def _f32(a) -> np.ndarray:
@@ -112,12 +116,20 @@ class Instance:
 
 @dataclass
 class Assembly:
-    """Single-level assembly of meshes, light objects and instances
-    (AssemblyBuilder::add_object / add_instance, assembly.rs:197-282)."""
+    """An assembly of meshes, light objects, sub-assemblies and the
+    instances that place them (AssemblyBuilder::add_object / add_assembly /
+    add_instance, assembly.rs:197-282)."""
     meshes: list = field(default_factory=list)
     instances: list = field(default_factory=list)
     rect_lights: list = field(default_factory=list)
     sphere_lights: list = field(default_factory=list)
+    assemblies: list = field(default_factory=list)
+
+    def add_assembly(self, sub: "Assembly") -> int:
+        """A sub-assembly, placed by instances of kind PSY_OBJECT_ASSEMBLY
+        with its index."""
+        self.assemblies.append(sub)
+        return len(self.assemblies) - 1
 
     def add_mesh(self, mesh: Mesh) -> int:
         self.meshes.append(mesh)
@@ -144,59 +156,67 @@ class Assembly:
         return Scene(self)
 
 
+def _scene_desc(asm: Assembly, keep: list) -> _lib.SceneDesc:
+    """psy_scene_desc of `asm` and its sub-assemblies; `keep` holds the
+    arrays the descriptors point into."""
+    mds = (_lib.MeshDesc * max(1, len(asm.meshes)))()
+    for i, m in enumerate(asm.meshes):
+        v = m.verts
+        idx = m.indices
+        keep += [v, idx]
+        mds[i].vert_count = v.shape[1]
+        mds[i].time_samples = v.shape[0]
+        mds[i].verts = v.ctypes.data_as(_lib._fp)
+        mds[i].normals = None
+        if m.normals is not None:
+            keep.append(m.normals)
+            mds[i].normals = m.normals.ctypes.data_as(_lib._fp)
+        mds[i].tri_count = idx.shape[0]
+        mds[i].indices = idx.ctypes.data_as(_lib._u32p)
+    xf_list = []
+    ids = (_lib.InstanceDesc * max(1, len(asm.instances)))()
+    nxf = 0
+    for i, inst in enumerate(asm.instances):
+        ids[i].mesh = inst.mesh
+        ids[i].shader = inst.shader
+        ids[i].kind = inst.kind
+        ids[i].xform_start = nxf
+        cnt = 0 if inst.xforms is None else len(inst.xforms)
+        ids[i].xform_count = cnt
+        if cnt:
+            xf_list.append(inst.xforms)
+        nxf += cnt
+    xfa = _f32(np.concatenate(xf_list)) if xf_list else np.zeros((1, 12), np.float32)
+    keep.append(xfa)
+    rls = (_lib.RectLightDesc * max(1, len(asm.rect_lights)))()
+    for i, l in enumerate(asm.rect_lights):
+        # A number broadcasts against the other list (Dimensions leaves
+        # always pair them in the file).
+        wh = np.stack(np.broadcast_arrays(np.atleast_1d(np.asarray(l.width, np.float32)),
+                                          np.atleast_1d(np.asarray(l.height, np.float32))), axis=1)
+        d0, nd, da = time_samples(wh, 2)
+        c0, nc, ca = colour_samples(l.color)
+        rls[i] = _lib.RectLightDesc(float(d0[0]), float(d0[1]), c0, nd, da.ctypes.data_as(_lib._fp), nc, ca)
+        keep += [da, ca]
+    sls = (_lib.SphereLightDesc * max(1, len(asm.sphere_lights)))()
+    for i, l in enumerate(asm.sphere_lights):
+        r0, nr, ra = time_samples(l.radius, 1)
+        c0, nc, ca = colour_samples(l.color)
+        sls[i] = _lib.SphereLightDesc(float(r0[0]), c0, nr, ra.ctypes.data_as(_lib._fp), nc, ca)
+        keep += [ra, ca]
+    keep += [rls, sls]
+    subs = (_lib.SceneDesc * max(1, len(asm.assemblies)))(*[_scene_desc(a, keep) for a in asm.assemblies])
+    keep.append(subs)
+    return _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp),
+                          len(asm.rect_lights), rls, len(asm.sphere_lights), sls, len(asm.assemblies), subs)
+
+
 class Scene:
     """The assembly uploaded to the GPU (psy_scene_create)."""
 
     def __init__(self, asm: Assembly):
         self._keep = []
-        mds = (_lib.MeshDesc * max(1, len(asm.meshes)))()
-        for i, m in enumerate(asm.meshes):
-            v = m.verts
-            idx = m.indices
-            self._keep += [v, idx]
-            mds[i].vert_count = v.shape[1]
-            mds[i].time_samples = v.shape[0]
-            mds[i].verts = v.ctypes.data_as(_lib._fp)
-            mds[i].normals = None
-            if m.normals is not None:
-                self._keep.append(m.normals)
-                mds[i].normals = m.normals.ctypes.data_as(_lib._fp)
-            mds[i].tri_count = idx.shape[0]
-            mds[i].indices = idx.ctypes.data_as(_lib._u32p)
-        xf_list = []
-        ids = (_lib.InstanceDesc * max(1, len(asm.instances)))()
-        nxf = 0
-        for i, inst in enumerate(asm.instances):
-            ids[i].mesh = inst.mesh
-            ids[i].shader = inst.shader
-            ids[i].kind = inst.kind
-            ids[i].xform_start = nxf
-            cnt = 0 if inst.xforms is None else len(inst.xforms)
-            ids[i].xform_count = cnt
-            if cnt:
-                xf_list.append(inst.xforms)
-            nxf += cnt
-        xfa = _f32(np.concatenate(xf_list)) if xf_list else np.zeros((1, 12), np.float32)
-        self._keep.append(xfa)
-        rls = (_lib.RectLightDesc * max(1, len(asm.rect_lights)))()
-        for i, l in enumerate(asm.rect_lights):
-            # A number broadcasts against the other list (Dimensions leaves
-            # always pair them in the file).
-            wh = np.stack(np.broadcast_arrays(np.atleast_1d(np.asarray(l.width, np.float32)),
-                                              np.atleast_1d(np.asarray(l.height, np.float32))), axis=1)
-            d0, nd, da = time_samples(wh, 2)
-            c0, nc, ca = colour_samples(l.color)
-            rls[i] = _lib.RectLightDesc(float(d0[0]), float(d0[1]), c0, nd, da.ctypes.data_as(_lib._fp), nc, ca)
-            self._keep += [da, ca]
-        sls = (_lib.SphereLightDesc * max(1, len(asm.sphere_lights)))()
-        for i, l in enumerate(asm.sphere_lights):
-            r0, nr, ra = time_samples(l.radius, 1)
-            c0, nc, ca = colour_samples(l.color)
-            sls[i] = _lib.SphereLightDesc(float(r0[0]), c0, nr, ra.ctypes.data_as(_lib._fp), nc, ca)
-            self._keep += [ra, ca]
-        self._keep += [rls, sls]
-        desc = _lib.SceneDesc(len(asm.meshes), mds, len(asm.instances), ids, nxf, xfa.ctypes.data_as(_lib._fp),
-                              len(asm.rect_lights), rls, len(asm.sphere_lights), sls)
+        desc = _scene_desc(asm, self._keep)
         h = C.c_void_p()
         check("psy_scene_create", lib().psy_scene_create(C.byref(desc), C.byref(h)))
         self.handle = h
@@ -424,6 +444,16 @@ class Tracer:
             check("psy_trace", lib().psy_trace(self.handle, C.byref(rs), rays.n, C.byref(hs), _lib.PSY_MEM_DEVICE))
         else:
             check("psy_trace_async", lib().psy_trace_async(self.handle, C.byref(rs), rays.n, C.byref(hs)))
+
+    def finalize(self, rays: DeviceRays, hits) -> np.ndarray:
+        """psy_finalize_hits: the hit geometry (HIT_GEOMETRY rows) of device
+        rays and their device hits (arrays or records)."""
+        out = DeviceBuffer(rays.n * HIT_GEOMETRY.itemsize + 64)
+        rs, hs = rays.soa(), hits.soa()
+        check("psy_finalize_hits", lib().psy_finalize_hits(self.handle, C.byref(rs), rays.n, C.byref(hs), out.ptr))
+        g = np.zeros(rays.n, HIT_GEOMETRY)
+        out.download(g)
+        return g
 
     def trace_counted(self, rays: DeviceRays, hits: DeviceHits) -> dict:
         c = _lib.TraceCounters()

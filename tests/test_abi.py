@@ -185,3 +185,54 @@ def test_null_server_calls_fail_cleanly():
     assert L.psy_server_submit(None, 0, None, None, None) == -1
     assert L.psy_server_wait(None, 1, None, None) == -1
     assert L.psy_server_destroy(None) == 0
+
+
+def test_nested_assembly_descriptions():
+    """Sub-assemblies in the scene description: flattened on the host, with
+    bad indices, cycles of depth and over-deep light nesting refused."""
+    from psychopath_amd import render, scenes
+    from psychopath_amd.tracer import Assembly, Mesh
+    sc = scenes.nested_lights_scene(8, 8, 1, 1, 0)
+    info = render.assembly_of(sc).build().info()
+    # 7 room meshes and the lamp's box; every lamp instance path flattened
+    assert info["tri_count"] == 46
+    v, f = scenes.displaced_sphere(1)
+    # an instance naming a missing sub-assembly
+    a = Assembly()
+    a.add_mesh(Mesh(v, f))
+    a.add_instance(0, kind=_lib.PSY_OBJECT_ASSEMBLY)
+    with pytest.raises(_lib.PsyError, match="object out of range"):
+        a.build()
+    # nesting deeper than 64 levels
+    leaf = Assembly()
+    leaf.add_mesh(Mesh(v, f))
+    leaf.add_instance(0)
+    cur = leaf
+    for _ in range(70):
+        up = Assembly()
+        up.add_assembly(cur)
+        up.add_instance(0, kind=_lib.PSY_OBJECT_ASSEMBLY)
+        cur = up
+    with pytest.raises(_lib.PsyError, match="deeper than 64"):
+        cur.build()
+    # lights four sub-assembly levels down
+    from psychopath_amd.tracer import SphereLight
+    lit = Assembly()
+    lit.add_sphere_light(SphereLight(0.1, (0, 1.0, 1.0, 1.0)))
+    lit.add_instance(0, np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32), kind=_lib.PSY_OBJECT_SPHERE_LIGHT)
+    cur = lit
+    for _ in range(4):
+        up = Assembly()
+        up.add_assembly(cur)
+        up.add_instance(0, kind=_lib.PSY_OBJECT_ASSEMBLY)
+        cur = up
+    with pytest.raises(_lib.PsyError, match="lights nested"):
+        cur.build()
+    # three levels are accepted
+    cur = lit
+    for _ in range(3):
+        up = Assembly()
+        up.add_assembly(cur)
+        up.add_instance(0, kind=_lib.PSY_OBJECT_ASSEMBLY)
+        cur = up
+    assert cur.build().info()["tri_count"] == 0

@@ -394,6 +394,28 @@ def test_nested_assemblies_render_like_the_oracle():
     assert frac >= FRACTION
 
 
+def test_nested_assembly_lights_match_oracle():
+    """Lights inside nested assemblies (lamps, each with a bulb sub-assembly,
+    one lamp under motion): the packer flattens the geometry and keeps one
+    light tree per assembly, and light selection recurses through them as
+    Assembly::sample_lights does (scene/assembly.rs:46-141), against the
+    oracle's restatement of that recursion; the one-launch form and a served
+    request give the same image."""
+    sc = scenes.nested_lights_scene(80, 80, 8, 3, 0)
+    img, ref = check_parity(sc, "nested lights 80x80x8")
+    r = render.build(sc)
+    r.set_options(plain=True)
+    plain, _ = r.render(sc.camera, sc.settings)
+    assert np.array_equal(plain, img)
+    sv = render.Server([r])
+    served, _ = sv.wait(sv.submit(0, sc.camera, sc.settings))
+    sv.close()
+    assert np.array_equal(served, img)
+    # The lamps light the room: brighter than the box alone.
+    base, _, _ = gpu_render(scenes.cornell_box(80, 80, 8, 3, 0))
+    assert img.mean() > 1.5 * base.mean()
+
+
 def test_camera_lens_time_samples_match_oracle():
     """Field of view, aperture and focus distance as time samples of
     different lengths (Camera::generate_ray lerps each list on its own,

@@ -171,10 +171,93 @@ def test_hit_records_equal_arrays(small, instanced):
              (tr, scenes.shadow_rays(v.min(0), v.max(0), 30_000, seeds=(174, 175))),
              (tri, scenes.incoherent_rays(lo, hi, 30_000, seeds=(176, 177, 178))))
     for tracer, r in cases:
-        for shape in ((1, 1, 1, 8, 4), (0, 16, 16, 16, 4, 0), (0, 16, 32, 24, 4, 24)):
+        for shape in ((1, 1, 1, 8, 4), (0, 16, 16, 16, 4, 0), (0, 16, 32, 24, 4, 24), (2, 16, 16, 8, 4, 0),
+                      (2, 12, 8, 16, 4, 16)):
             tracer.set_tuning(*shape)
             assert_exact(gpu_trace(tracer, r, records=True), gpu_trace(tracer, r), f"records {shape}")
         tracer.set_tuning(0, 16, 16, 16, 4, 0)
+
+
+def _finalize_case(tracer, oracle_scene, r, label, exact=True, lights=False):
+    """psy_finalize_hits on device records against the oracle's
+    SurfaceIntersectionData for the same rays (light hits carry no primitive
+    in the reference, so only their instance and distance are compared)."""
+    rays = DeviceRays(ray_batch(r))
+    recs = DeviceHitRecords(rays.n)
+    tracer.trace_device(rays, recs)
+    g = tracer.finalize(rays, recs)
+    rec = recs.records()
+    w = oracle_scene.trace(r, want_geometry=True)
+    same = ((rec["kind"] == 2) & (w["kind"] == 2) & (rec["inst_id"] == w["inst_id"])
+            & (rec["t"].view(np.uint32) == w["t"].view(np.uint32)))
+    if not lights:
+        same &= rec["prim_id"] == w["prim_id"]
+        assert same.sum() > 0.2 * len(rec), label
+    if exact:
+        assert same.sum() == (w["kind"] == 2).sum(), label
+    mine = np.concatenate([g["pos"], g["pos_err"][:, None], g["nor"], g["nor_g"], g["local_space"],
+                           g["sample_pdf"][:, None], g["reserved"][:, None]], 1)
+    want = w["geometry"]
+    bad = (mine[same].view(np.uint32) != want[same].view(np.uint32)).any(1)
+    assert not bad.any(), f"{label}: geometry differs on {bad.sum()} of {same.sum()} hits, e.g. " \
+                          f"{mine[same][bad][:1]} vs {want[same][bad][:1]}"
+    assert np.all(mine[rec["kind"] != 2] == 0)
+    return int(same.sum())
+
+
+def test_finalize_hits_match_oracle(small, instanced):
+    """The hit geometry (pos, pos_err, nor, nor_g, local_space, sample_pdf)
+    bit for bit against the oracle, which computes it during intersection as
+    triangle_mesh.rs:264-317 does: a static mesh, instances with static and
+    motion transforms and a deforming mesh, smooth-shaded meshes, and
+    rectangle and sphere light instances."""
+    from oracle.psyref import OracleScene
+    v, f, s, o, tr = small
+    _finalize_case(tr, o, scenes.incoherent_rays(v.min(0), v.max(0), 40_000, seeds=(181, 182, 183)), "static")
+    si, oi, tri = instanced
+    lo, hi = np.array([-10, -10, -2], np.float32), np.array([10, 10, 2], np.float32)
+    _finalize_case(tri, oi, scenes.incoherent_rays(lo, hi, 40_000, seeds=(184, 185, 186)), "instanced", exact=False)
+    # Smooth shading: vertex normals with two time samples under motion.
+    rng = np.random.default_rng(19)
+    vs, fs = scenes.displaced_sphere(4)
+    vd = np.stack([vs, (vs + rng.normal(0, 0.02, size=vs.shape)).astype(np.float32)])
+    nd = np.stack([scenes.vertex_normals(vd[k], fs) for k in range(2)]).astype(np.float32)
+    a = Assembly()
+    a.add_mesh(Mesh(vd, fs, nd))
+    xfs = np.stack([rot_xf(0.1 * k, 2, (0.2 * k, 0.0, 0.0)) for k in range(3)])
+    a.add_instance(0, xfs)
+    os_ = OracleScene()
+    os_.add_mesh(vd, fs, nd)
+    os_.add_instance(0, xfs)
+    os_.build()
+    trs = Tracer(a.build(), max_batch=1 << 16)
+    _finalize_case(trs, os_, scenes.incoherent_rays(vs.min(0) - 0.5, vs.max(0) + 0.5, 40_000, seeds=(187, 188, 189)),
+                   "smooth motion", exact=False)
+
+
+def test_finalize_light_hits_match_oracle():
+    from types import SimpleNamespace
+    from oracle import psyref
+    from psychopath_amd.tracer import RectLight, SphereLight
+    import psychopath_amd._lib as _lib
+    rect = RectLight(1.5, 0.75, (0, 2.0, 2.0, 2.0))
+    sph = SphereLight(0.6, (1, 4000.0, 1.0, 0.0))
+    x_rect = np.stack([rot_xf(0.4, 0, (0.0, 0.0, -1.0))])
+    x_sph = np.stack([rot_xf(0.2, 1, (1.5, 0.5, 0.0)), rot_xf(0.3, 1, (1.7, 0.4, 0.1))])
+    asm = Assembly()
+    asm.add_rect_light(rect)
+    asm.add_sphere_light(sph)
+    asm.add_instance(0, x_rect, _lib.PSY_NONE, _lib.PSY_OBJECT_RECT_LIGHT)
+    asm.add_instance(0, x_sph, _lib.PSY_NONE, _lib.PSY_OBJECT_SPHERE_LIGHT)
+    ns = SimpleNamespace(meshes=[], rect_lights=[(rect.width, rect.height, rect.color)],
+                         sphere_lights=[(sph.radius, sph.color)],
+                         instances=[dict(kind="rect", index=0, shader=None, xforms=x_rect),
+                                    dict(kind="sphere", index=0, shader=None, xforms=x_sph)])
+    tr = Tracer(asm.build(), max_batch=1 << 16)
+    lo, hi = np.array([-3, -3, -3], np.float32), np.array([3, 3, 3], np.float32)
+    n = _finalize_case(tr, psyref._assembly(ns), scenes.incoherent_rays(lo, hi, 60_000, seeds=(190, 191, 192)),
+                       "lights", exact=False, lights=True)
+    assert n > 1000
 
 
 def test_big_mesh_coherent_exact(big):
@@ -326,10 +409,21 @@ def test_launch_shapes_give_identical_hits(small, instanced):
     si, oi, tri = instanced
     lo, hi = np.array([-10, -10, -2], np.float32), np.array([10, 10, 2], np.float32)
     ri = scenes.incoherent_rays(lo, hi, 30_000, seeds=(144, 145, 146))
-    for tracer, rays in ((tr, r), (tri, ri)):
+    # Static instances (transforms without time samples): the cooperative
+    # fetch's instanced specialisation.
+    vs, fs = scenes.displaced_sphere(4)
+    rng = np.random.default_rng(23)
+    sx, ox = build_pair([(vs, fs)], [(0, np.stack([rot_xf(rng.uniform(0, 1), k % 3, rng.uniform(-6, 6, 3))]))
+                                     for k in range(24)])
+    trx = Tracer(sx, max_batch=1 << 16)
+    rx = scenes.incoherent_rays(np.array([-7, -7, -7], np.float32), np.array([7, 7, 7], np.float32), 30_000,
+                                seeds=(147, 148, 149))
+    assert_within_bar(gpu_trace(trx, rx), ox.trace(rx), "static instances")
+    for tracer, rays in ((tr, r), (tri, ri), (trx, rx)):
         base = None
         for shape in ((1, 1, 1, 8, 4), (0, 1, 1, 8, 4), (0, 8, 16, 16, 8), (0, 16, 64, 24, 4), (0, 32, 32, 8, 8),
-                      (0, 4, 2, 8, 8), (0, 16, 16, 16, 4, 1), (0, 24, 32, 24, 8, 8), (0, 8, 8, 8, 4, 32)):
+                      (0, 4, 2, 8, 8), (0, 16, 16, 16, 4, 1), (0, 24, 32, 24, 8, 8), (0, 8, 8, 8, 4, 32),
+                      (2, 16, 16, 8, 4), (2, 12, 1, 16, 4), (2, 16, 32, 8, 4, 24), (2, 8, 64, 24, 4, 1)):
             tracer.set_tuning(*shape)
             g = gpu_trace(tracer, rays)
             if base is None:
@@ -354,7 +448,8 @@ def test_deep_stack_spills_to_private_memory():
     r = scenes.incoherent_rays(np.array([-0.5, -0.5, -1], np.float32), np.array([1.5, 1.5, 5], np.float32), 20_000,
                                seeds=(161, 162, 163))
     want = o.trace(r)
-    for shape in ((0, 8, 8, 8, 8), (0, 8, 8, 8, 4), (1, 1, 1, 8, 4), (0, 8, 8, 8, 4, 4)):
+    for shape in ((0, 8, 8, 8, 8), (0, 8, 8, 8, 4), (1, 1, 1, 8, 4), (0, 8, 8, 8, 4, 4), (2, 8, 8, 8, 4),
+                  (2, 8, 8, 8, 4, 4)):
         tr.set_tuning(*shape)
         assert_exact(gpu_trace(tr, r), want, f"deep {shape}")
 
@@ -367,7 +462,7 @@ def test_autotune_picks_a_valid_shape(small):
     before = tr.rays_traced()
     best = tr.autotune(rays, hits)
     assert best == tr.tuning()
-    assert best["kernel"] in (0, 1) and best["stack_lds"] in (8, 16, 24) and best["simd_waves"] in (4, 8)
+    assert best["kernel"] in (0, 1, 2) and best["stack_lds"] in (8, 16, 24) and best["simd_waves"] in (4, 8)
     assert 0 <= best["leaf_min"] <= 64
     assert tr.rays_traced() == before
     assert_exact(gpu_trace(tr, r), o.trace(r), "after autotune")
