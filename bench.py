@@ -13,10 +13,11 @@ is autotuned on a second ray set from the same generator with other seeds,
 then the benchmark rays are timed.
 
 Prints one JSON line.  `roofline` models the traversal kernel's algorithmic
-bytes: per ray, internal-node visits x 128 B node record + triangle tests x
-48 B triangle record + 65 B of ray and hit I/O (33 B ray in, 32 B record
-out), with visit counts from an instrumented run of the same kernel on the
-same rays.  `cpu_baseline` times the oracle (a C++ restatement of the
+bytes: per ray, internal-node visits x 112 B (the seven 16-B parts of a
+128-B node record that a node without time samples needs: child boxes and
+references) + triangle tests x 48 B triangle record + 65 B of ray and hit
+I/O (33 B ray in, 32 B record out), with visit counts from an instrumented
+run of the same kernel on the same rays.  `cpu_baseline` times the oracle (a C++ restatement of the
 reference's ray-stream tracer, batches of 4096 rays, one tracer per thread)
 on a sample of the same rays, on the host cores allotted to this process.
 """
@@ -34,6 +35,7 @@ METRIC = "incoherent Mrays/s on 1M-tri BVH4; samples/s + p50/p99 latency served 
 HBM_PEAK_GBS = 8000.0
 RAY_IN_BYTES = 8 * 4 + 1     # ox oy oz dx dy dz time max_t + flags
 HIT_OUT_BYTES = 32           # one psy_hit_record: t b0 b1 b2 prim inst kind reserved
+STATIC_NODE_READ_BYTES = 112  # 96 B child boxes + 16 B child references of a 128-B node
 
 
 def log(msg):
@@ -175,7 +177,7 @@ def main():
     counts = tracer.trace_counted(d_rays, d_hits)
     h = d_hits.to_host()
     hit_frac = float((h.kind == 2).mean())
-    node_b = info["node_bytes"]
+    node_b = STATIC_NODE_READ_BYTES
     tri_b = info["tri_bytes"]
     io_b = RAY_IN_BYTES + HIT_OUT_BYTES
     alg_bytes = counts["node_visits"] * node_b + counts["tri_tests"] * tri_b + args.rays * io_b

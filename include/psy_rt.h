@@ -328,6 +328,16 @@ int psy_tracer_set_tuning(psy_tracer* tracer, const psy_trace_tuning* tuning);
 int psy_tracer_autotune(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
                         psy_trace_tuning* best);
 
+/* The rest of each hit's SurfaceIntersectionData for rays traced by this
+ * tracer's scene (device-resident rays, hits and output; synchronous on the
+ * tracer's stream): `geometry[i]` is written for every PSY_HIT_SURFACE record
+ * and zeroed for the others.  The reference computes these during
+ * intersection (triangle_mesh.rs:264-317); here they are derived from the
+ * hit record afterwards, with the same arithmetic, only for callers that
+ * need them. */
+int psy_finalize_hits(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, const psy_hit_soa* hits,
+                      psy_hit_geometry* geometry);
+
 /* Tracer::rays_traced (src/tracer.rs:39-41). */
 uint64_t psy_tracer_rays_traced(const psy_tracer* tracer);
 
