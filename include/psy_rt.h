@@ -338,6 +338,23 @@ int psy_tracer_autotune(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n,
 int psy_finalize_hits(psy_tracer* tracer, const psy_ray_soa* rays, uint64_t n, const psy_hit_soa* hits,
                       psy_hit_geometry* geometry);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc after set-up).  psy_trace_async without a tracer and
+ * without any allocation: device-resident rays and hits, enqueued on
+ * `stream` (a hipStream_t, NULL for the default stream) and not waited for.
+ * The scene must already be resident on the current device (creating a
+ * tracer for it uploads it).  `tuning` may be NULL (the tracer's default
+ * shape); the simple kernel is not offered.  The one scratch word the
+ * persistent kernels need (their ray counter) lives in a fixed
+ * PSY_NOALLOC_SCRATCH_BYTES device array of the library, one per device and
+ * shared by every call on it: calls that overlap on one device race.  At
+ * most PSY_NOALLOC_MAX_RAYS rays per call (PSY_E_RANGE beyond). */
+#define PSY_NOALLOC_SCRATCH_BYTES 256u
+#define PSY_NOALLOC_MAX_RAYS 0x7fffffffull
+int psy_trace_noalloc(const psy_scene* scene, const psy_ray_soa* rays, uint64_t n, psy_hit_soa* hits,
+                      const psy_trace_tuning* tuning, void* stream);
+
 /* Tracer::rays_traced (src/tracer.rs:39-41). */
 uint64_t psy_tracer_rays_traced(const psy_tracer* tracer);
 

@@ -205,6 +205,8 @@ class TraceTuning(C.Structure):
 PSY_KERNEL_PERSISTENT = 0
 PSY_KERNEL_SIMPLE = 1
 PSY_KERNEL_PERSISTENT_COOP = 2
+PSY_NOALLOC_SCRATCH_BYTES = 256
+PSY_NOALLOC_MAX_RAYS = 0x7FFFFFFF
 
 # Every symbol the header declares, with its ctypes signature.
 _SIGNATURES = {
@@ -218,6 +220,8 @@ _SIGNATURES = {
     "psy_tracer_destroy": (C.c_int, [C.c_void_p]),
     "psy_trace": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA), C.c_uint32]),
     "psy_trace_async": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA)]),
+    "psy_trace_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA),
+                                    C.POINTER(TraceTuning), C.c_void_p]),
     "psy_finalize_hits": (C.c_int, [C.c_void_p, C.POINTER(RaySoA), C.c_uint64, C.POINTER(HitSoA), C.c_void_p]),
     "psy_tracer_sync": (C.c_int, [C.c_void_p]),
     "psy_tracer_last_kernel_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),

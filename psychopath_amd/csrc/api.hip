@@ -45,7 +45,10 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.normals, &s->normals);
     if (!r) r = upload(h.nidx, &s->nidx);
     if (!r) r = upload(h.order_table, &s->order_table);
-    if (r == PSY_OK) s->uploaded = true;
+    if (r == PSY_OK) {
+        PSY_HIP(hipGetDevice(&s->device));
+        s->uploaded = true;
+    }
     return r;
 }
 

@@ -59,6 +59,7 @@ struct psy_scene {
     uint8_t* order_table = nullptr;
     uint64_t device_bytes = 0;
     bool uploaded = false;
+    int device = -1;  // device the scene is resident on
 
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,

@@ -236,3 +236,24 @@ def test_nested_assembly_descriptions():
         up.add_instance(0, kind=_lib.PSY_OBJECT_ASSEMBLY)
         cur = up
     assert cur.build().info()["tri_count"] == 0
+
+
+def test_noalloc_entry_point_checks_before_touching_the_device():
+    from psychopath_amd import scenes
+    v, f = scenes.displaced_sphere(2)
+    a = Assembly()
+    a.add_mesh(Mesh(v, f))
+    a.add_instance(0)
+    s = a.build()  # never uploaded: no tracer was created
+    L = _lib.lib()
+    fake = 16  # non-NULL addresses the checks never dereference
+    rs = _lib.RaySoA(fake, fake, fake, fake, fake, fake, fake, fake, None)
+    hs = _lib.HitSoA(None, None, None, None, None, None, None, 16)
+    assert L.psy_trace_noalloc(None, C.byref(rs), 1, C.byref(hs), None, None) == _lib.PSY_E_INVALID
+    assert L.psy_trace_noalloc(s.handle, C.byref(rs), _lib.PSY_NOALLOC_MAX_RAYS + 1, C.byref(hs), None,
+                               None) == _lib.PSY_E_RANGE
+    assert "2^31" in L.psy_last_error().decode()
+    assert L.psy_trace_noalloc(s.handle, C.byref(rs), 1, C.byref(hs), None, None) == _lib.PSY_E_STATE
+    assert "not resident" in L.psy_last_error().decode()
+    bad = _lib.TraceTuning(_lib.PSY_KERNEL_PERSISTENT, 0, 16, 16, 4, 0)
+    assert L.psy_trace_noalloc(s.handle, C.byref(rs), 1, C.byref(hs), C.byref(bad), None) == _lib.PSY_E_RANGE
