@@ -134,13 +134,14 @@ typedef struct {
  * by its parent's instances of kind PSY_OBJECT_ASSEMBLY; its indices refer
  * to its own arrays.  At most 64 levels of nesting.
  *
- * Nested assemblies are flattened for traversal: each mesh and light
- * instance gets the transform list TransformStack::push composes along its
- * path (src/transform_stack.rs:33-60), and one instance BVH covers them all
- * (the reference walks the sub-assemblies' BVHs in turn, which orders
- * equal-distance hits differently at most).  Hit records number the
- * flattened instances depth first, in instance order (so a single-level
- * scene keeps its instance indices).  Light selection keeps the reference's
+ * Nested assemblies are traversed nested, as Tracer::trace_assembly
+ * recurses (src/tracer.rs:87-150): every assembly keeps its own instance BVH
+ * in its own space, a ray entering a sub-assembly instance moves into it by
+ * the lerped transforms TransformStack::push composes along its path
+ * (src/transform_stack.rs:33-60), and leaves it when that BVH is done.  Hit
+ * records number the object instances of all paths depth first, in instance
+ * order (so a single-level scene keeps its instance indices).  Light
+ * selection keeps the reference's
  * hierarchy: one light tree per assembly over its lights and its light-
  * holding sub-assemblies (assembly.rs:297-338), selected recursively as
  * Assembly::sample_lights does (assembly.rs:46-141). */

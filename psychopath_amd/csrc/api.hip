@@ -45,6 +45,7 @@ int ensure_uploaded(psy_scene* s) {
     if (!r) r = upload(h.normals, &s->normals);
     if (!r) r = upload(h.nidx, &s->nidx);
     if (!r) r = upload(h.order_table, &s->order_table);
+    if (!r) r = upload(h.groups, &s->groups);
     if (r == PSY_OK) {
         PSY_HIP(hipGetDevice(&s->device));
         s->uploaded = true;
@@ -65,6 +66,7 @@ TraceArgs scene_args(const psy_scene* s) {
     a.sphere_lights = s->sphere_lights;
     a.light_samples = s->light_samples;
     a.order_table = s->order_table;
+    a.groups = s->host.nested ? s->groups : nullptr;
     a.top_root = s->host.top_root;
     return a;
 }
@@ -234,7 +236,8 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
                       (h.inst_pos.size() + h.tri_rec.size()) * 4 + h.light_leaves.size() * sizeof(GpuLightLeaf) +
                       h.light_xforms.size() * sizeof(GpuXform) +
                       h.light_nodes.size() * sizeof(GpuLightNode) + h.light_boxes.size() * sizeof(GpuBox) +
-                      (h.normals.size() + h.nidx.size()) * 4 + h.order_table.size();
+                      (h.normals.size() + h.nidx.size()) * 4 + h.order_table.size() +
+                      h.groups.size() * sizeof(GpuGroup);
     *out = s;
     return PSY_OK;
 }

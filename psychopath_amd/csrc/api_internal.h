@@ -57,6 +57,7 @@ struct psy_scene {
     float* normals = nullptr;
     uint32_t* nidx = nullptr;
     uint8_t* order_table = nullptr;
+    psy::GpuGroup* groups = nullptr;
     uint64_t device_bytes = 0;
     bool uploaded = false;
     int device = -1;  // device the scene is resident on
@@ -64,7 +65,7 @@ struct psy_scene {
     ~psy_scene() {
         void* bufs[] = {nodes, motion, tris, meshes, instances, xforms, inv_static, rect_lights,
                         sphere_lights, light_samples, inst_pos, tri_rec, light_nodes, light_boxes,
-                        light_leaves, light_xforms, normals, nidx, order_table};
+                        light_leaves, light_xforms, normals, nidx, order_table, groups};
         for (void* b : bufs) (void)hipFree(b);
     }
 };

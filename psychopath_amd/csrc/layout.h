@@ -98,6 +98,24 @@ struct GpuInstance {
     uint32_t pad;
 };
 
+// An instance path into an assembly of a nested scene (the assembly
+// reached through a chain of assembly instances): the root of the
+// assembly's own instance BVH, the first of this path's instance records
+// (one per instance of the assembly, in its BVH's leaf order; sub-assembly
+// instances are records of kind PSY_OBJECT_ASSEMBLY whose `mesh` names the
+// path they open), and the transforms TransformStack::push composes into
+// the assembly along the path (transform_stack.rs:33-60; none at the root
+// assembly).  Traversal enters a sub-assembly by pushing a level marker on
+// its node stack: a leaf reference with count 0 holding the enclosing path,
+// restored when the marker is popped (tracer.rs:95-148).
+struct GpuGroup {
+    uint32_t root;
+    uint32_t base;
+    uint32_t xf_start;
+    uint32_t xf_count;
+};
+constexpr uint32_t make_level_marker(uint32_t group) { return REF_LEAF | (group << 2); }
+
 // Transform: glam Affine3A columns c0, c1, c2 and translation t, stored as
 // three rows (x: c0.x c1.x c2.x t.x, ...) so each row is one float4.
 struct GpuXform {

@@ -561,7 +561,33 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
         s.tri_count += T;
     }
 
+    // Each assembly's instance bounds in its own space and its instance BVH
+    // (AssemblyBuilder::build, assembly.rs:288-295), sub-assemblies (placed
+    // after their parents) first.
+    std::vector<HostBvh4> def_bvh(f.defs.size());
+    for (uint32_t di = (uint32_t)f.defs.size(); di-- > 0;) {
+        const psy_scene_desc* ad = f.defs[di];
+        DefInfo& info = f.info[di];
+        info.inst_bounds.clear();
+        info.inst_offs.assign(1, 0u);
+        for (uint32_t i = 0; i < ad->instance_count; i++) {
+            const psy_instance_desc& id = ad->instances[i];
+            std::vector<Xf> own;
+            for (uint32_t k = 0; k < id.xform_count; k++) own.push_back(xf_at(ad, id.xform_start + k));
+            const std::vector<Box> wb = transform_bounds(object_bounds(ad, id, info, s, f), own.data(), id.xform_count);
+            info.inst_bounds.insert(info.inst_bounds.end(), wb.begin(), wb.end());
+            info.inst_offs.push_back((uint32_t)info.inst_bounds.size());
+        }
+        BuildInput abi{ad->instance_count, info.inst_bounds.data(), info.inst_offs.data()};
+        def_bvh[di] = build_bvh4(abi, 1, 0);
+        info.order = def_bvh[di].order;
+        info.bounds = def_bvh[di].root_bounds;
+    }
+    const bool nested = f.groups.size() > 1;
+    uint32_t top_depth = 0;
+
     // ---- instances and the instance BVH (assembly.rs:288-395) ---------------
+    if (!nested) {
     std::vector<Box> ibounds;
     std::vector<uint32_t> ioffs{0};
     for (uint32_t i = 0; i < d->instance_count; i++) {
@@ -637,28 +663,68 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
         s.instances.push_back(gi);
         if (id.xform_count > 0) s.has_xf = true;
     }
-    // Each assembly's instance bounds in its own space and the instance
-    // order its instance BVH leaves (AssemblyBuilder::build, assembly.rs:
-    // 288-295), sub-assemblies (placed after their parents) first.
-    for (uint32_t di = (uint32_t)f.defs.size(); di-- > 0;) {
-        const psy_scene_desc* ad = f.defs[di];
-        DefInfo& info = f.info[di];
-        info.inst_bounds.clear();
-        info.inst_offs.assign(1, 0u);
-        for (uint32_t i = 0; i < ad->instance_count; i++) {
-            const psy_instance_desc& id = ad->instances[i];
-            std::vector<Xf> own;
-            for (uint32_t k = 0; k < id.xform_count; k++) own.push_back(xf_at(ad, id.xform_start + k));
-            const std::vector<Box> wb = transform_bounds(object_bounds(ad, id, info, s, f), own.data(), id.xform_count);
-            info.inst_bounds.insert(info.inst_bounds.end(), wb.begin(), wb.end());
-            info.inst_offs.push_back((uint32_t)info.inst_bounds.size());
+    top_depth = top.depth;
+    } else {
+        // Nested scene: every assembly keeps its own instance BVH, and every
+        // instance path into an assembly (a group) gets that BVH's leaves as
+        // instance records: the flattened object instances with their
+        // composed transforms, and sub-assembly entries naming the path they
+        // open (tracer.rs:87-150).
+        const uint32_t nd = (uint32_t)f.defs.size();
+        std::vector<uint32_t> def_root(nd);
+        for (uint32_t di = 0; di < nd; di++) {
+            def_root[di] = append_bvh(s, def_bvh[di]);
+            s.leaf_count += def_bvh[di].leaf_count;
         }
-        BuildInput abi{ad->instance_count, info.inst_bounds.data(), info.inst_offs.data()};
-        const HostBvh4 ab = build_bvh4(abi, 1, 0);
-        info.order = ab.order;
-        info.bounds = ab.root_bounds;
+        s.bvhs[d->mesh_count] = def_bvh[0];
+        s.top_root = def_root[0];
+        s.nested = true;
+        s.has_xf = true;
+        s.inst_pos.assign(d->instance_count, PSY_NONE);
+        const uint32_t ng = (uint32_t)f.groups.size();
+        s.groups.resize(ng);
+        for (uint32_t g = 0; g < ng; g++) {
+            const Group& G = f.groups[g];
+            const uint32_t di = f.def_index.at(G.def);
+            GpuGroup gg{def_root[di], (uint32_t)s.instances.size(), (uint32_t)s.xforms.size(), (uint32_t)G.top.size()};
+            for (const Xf& x : G.top) s.xforms.push_back(to_gpu(x));
+            s.groups[g] = gg;
+            for (uint32_t k = 0; k < G.def->instance_count; k++) {
+                const uint32_t orig = def_bvh[di].order[k];
+                GpuInstance gi{};
+                gi.inv_static = REF_EMPTY;
+                if (G.def->instances[orig].kind == PSY_OBJECT_ASSEMBLY) {
+                    gi.kind = PSY_OBJECT_ASSEMBLY;
+                    gi.mesh = G.target[orig];
+                    gi.id = PSY_NONE;
+                    gi.shader = PSY_NONE;
+                    s.instances.push_back(gi);
+                    continue;
+                }
+                const uint32_t fi = G.target[orig];
+                const psy_instance_desc& id = d->instances[fi];
+                gi.mesh = id.mesh;
+                gi.id = fi;
+                gi.shader = id.shader;
+                gi.kind = id.kind;
+                s.inst_pos[fi] = (uint32_t)s.instances.size();
+                if (id.kind != PSY_OBJECT_MESH) s.has_lights = true;
+                gi.xf_start = (uint32_t)s.xforms.size();
+                gi.xf_count = id.xform_count;
+                for (uint32_t x = 0; x < id.xform_count; x++) s.xforms.push_back(to_gpu(xf_at(d, id.xform_start + x)));
+                if (id.xform_count == 1) {
+                    gi.inv_static = (uint32_t)s.inv_static.size();
+                    s.inv_static.push_back(to_gpu(xf_inverse(xf_at(d, id.xform_start))));
+                }
+                s.instances.push_back(gi);
+            }
+        }
+        if (s.instances.size() > MAX_LEAF_FIRST || ng > MAX_LEAF_FIRST) {
+            err = "too many instance records in the nested scene";
+            return PSY_E_RANGE;
+        }
+        for (uint32_t di = 0; di < nd; di++) s.max_depth = std::max(s.max_depth, def_bvh[di].depth);
     }
-
     // Light trees (AssemblyBuilder::build, assembly.rs:297-338): per group,
     // over its light instances and its sub-assembly instances whose trees
     // have energy, in the assembly's instance-BVH order; sub-groups (which
@@ -754,7 +820,24 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
     // internal level along the current path, in both trees.
     uint32_t mesh_depth = 0;
     for (uint32_t m = 0; m < d->mesh_count; m++) mesh_depth = std::max(mesh_depth, s.bvhs[m].depth);
-    s.max_stack = 3 * (top.depth + mesh_depth) + 2;
+    if (!nested) {
+        s.max_stack = 3 * (top_depth + mesh_depth) + 2;
+    } else {
+        // Along every path: the assemblies' BVHs, a level marker per
+        // sub-assembly entered, and the deepest mesh BVH at its end.
+        std::vector<uint32_t> need(f.groups.size(), 0);
+        for (uint32_t g = (uint32_t)f.groups.size(); g-- > 0;) {
+            const Group& G = f.groups[g];
+            uint32_t below = 0;
+            for (uint32_t k = 0; k < G.def->instance_count; k++) {
+                const psy_instance_desc& id = G.def->instances[k];
+                if (id.kind == PSY_OBJECT_ASSEMBLY) below = std::max(below, 1 + need[G.target[k]]);
+                else if (id.kind == PSY_OBJECT_MESH) below = std::max(below, 3 * s.bvhs[d->instances[G.target[k]].mesh].depth);
+            }
+            need[g] = 3 * def_bvh[f.def_index.at(G.def)].depth + below;
+        }
+        s.max_stack = need[0] + 2;
+    }
 
     // Device form of the node array: each node's traversal code also rides
     // in bits 28..30 of its first two child references (layout.h), and the

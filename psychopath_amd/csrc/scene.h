@@ -19,7 +19,8 @@ struct PackedScene {
     std::vector<GpuBox4> motion;
     std::vector<uint8_t> tris;         // triangle records, per-mesh stride
     std::vector<GpuMesh> meshes;
-    std::vector<GpuInstance> instances;  // in instance-BVH leaf order
+    std::vector<GpuInstance> instances;  // in instance-BVH leaf order (per path of a nested scene)
+    std::vector<GpuGroup> groups;        // nested scenes: instance paths into assemblies, the root's first
     std::vector<GpuXform> xforms;
     std::vector<GpuXform> inv_static;
     std::vector<float> normals;        // xyz per (vertex, time sample), see GpuMesh
@@ -44,6 +45,7 @@ struct PackedScene {
     bool has_motion = false;    // time-sampled node bounds or vertices
     bool has_lights = false;    // some instance places a light
     bool has_normals = false;   // some mesh has shading normals
+    bool nested = false;        // sub-assemblies are traversed as nested BVHs (`groups`)
     uint32_t max_stack = 0;     // deepest pending-node count traversal can need
     uint32_t max_depth = 0;
     uint64_t leaf_count = 0;

@@ -29,6 +29,7 @@ struct TraceArgs {
     const GpuSphereLight* sphere_lights;
     const float* light_samples;  // time samples of the lights (layout.h)
     const uint8_t* order_table;  // TRAVERSAL_TABLE[octant][code], 8 x 48 bytes
+    const GpuGroup* groups;      // nested scenes: instance paths (layout.h), else NULL
     uint32_t top_root;
     // rays (device)
     const float *ox, *oy, *oz, *dx, *dy, *dz, *time, *max_t;

@@ -62,6 +62,22 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
     uint32_t inst_id = PSY_NONE;
     uint64_t tri_base = 0;
     uint32_t tri_stride = 0, tri_samples = 1;
+    // Nested scenes (a.groups set): the current assembly path and its first
+    // instance record; level_ray sets the local ray of the path's level.
+    uint32_t grp = 0, gbase = 0;
+    auto level_ray = [&](const GpuGroup& g) {
+        Xf x = xf_identity();
+        if (g.xf_count > 0) {
+            uint32_t i1;
+            float a2;
+            if (!lerp_slice_pos(g.xf_count, time, i1, a2)) x = load_xf(a.xforms + g.xf_start + i1);
+            else x = xf_lerp(load_xf(a.xforms + g.xf_start + i1), load_xf(a.xforms + g.xf_start + i1 + 1), a2);
+        }
+        lo = xf_point(x, o);
+        const V3 ld = xf_vector(x, d);
+        di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+        oct = octant_of(di);
+    };
 
     TraceHit h{PSY_HIT_MISS, 0.0f, 0.0f, 0.0f, 0.0f, PSY_NONE, PSY_NONE};
     bool done = false;
@@ -70,12 +86,16 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
     while (true) {
         if (cur == REF_EMPTY) {
             if (in_inst && sp == inst_base) {
-                // Leave the instance: restore the top-level local ray
-                // (tracer.rs:130-148).
+                // Leave the instance: restore the local ray of its assembly
+                // level (tracer.rs:130-148).
                 in_inst = false;
-                lo = top_lo;
-                di = top_di;
-                oct = top_oct;
+                if (a.groups) {
+                    level_ray(a.groups[grp]);
+                } else {
+                    lo = top_lo;
+                    di = top_di;
+                    oct = top_oct;
+                }
             }
             if (sp == 0) break;
             cur = T1_POP();
@@ -137,9 +157,33 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
 
         const uint32_t first = (cur & ~REF_LEAF) >> 2;
         const uint32_t count = cur & 3u;
-        if (!in_inst && a.instances[first].kind != PSY_OBJECT_MESH) {
+        const uint32_t ii = gbase + first;
+        if (a.groups && !in_inst && count == 0u) {
+            // Level marker: the sub-assembly is done; back to the enclosing
+            // assembly's path and local ray (tracer.rs:131-148).
+            grp = first;
+            const GpuGroup g = a.groups[grp];
+            gbase = g.base;
+            level_ray(g);
+            cur = REF_EMPTY;
+            continue;
+        }
+        if (a.groups && !in_inst && a.instances[ii].kind == PSY_OBJECT_ASSEMBLY) {
+            // Sub-assembly instance (tracer.rs:95-128): remember the
+            // enclosing path on the stack and walk the sub-assembly's
+            // instance BVH in its space.
+            if (COUNT) ctr.inst++;
+            T1_PUSH(make_level_marker(grp));
+            grp = a.instances[ii].mesh;
+            const GpuGroup g = a.groups[grp];
+            gbase = g.base;
+            level_ray(g);
+            cur = g.root;
+            continue;
+        }
+        if (!in_inst && a.instances[ii].kind != PSY_OBJECT_MESH) {
             // Light instance: test the light's surface in place.
-            const GpuInstance inst = a.instances[first];
+            const GpuInstance inst = a.instances[ii];
             if (COUNT) ctr.inst++;
             const Xf x = instance_xf(a.xforms, inst, time);
             float lt, l0, l1, l2;
@@ -159,7 +203,7 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
         if (!in_inst) {
             // Instance leaf (tracer.rs:95-128): move the ray into the
             // instance's space and descend into its mesh.
-            const GpuInstance inst = a.instances[first];
+            const GpuInstance inst = a.instances[ii];
             const GpuMesh mesh = a.meshes[inst.mesh];
             if (COUNT) ctr.inst++;
             if (inst.xf_count > 0) {

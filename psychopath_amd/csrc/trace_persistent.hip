@@ -12,8 +12,10 @@
 //     past dead ends) until the whole wave stands on leaves, then run the
 //     leaf tests together;
 //   * kernels are specialised on whether the scene has instance transforms
-//     (XF) and time-sampled bounds or vertices (MOTION), so the common static
-//     case keeps no transform or lerp state in registers.
+//     (XF), time-sampled bounds or vertices (MOTION) and nested assemblies
+//     (NESTED: each assembly's own instance BVH walked in its space, as
+//     trace_assembly recurses, tracer.rs:87-150), so the common static case
+//     keeps no transform, lerp or nesting state in registers.
 #include <hip/hip_runtime.h>
 
 #include "layout.h"
@@ -58,7 +60,25 @@ constexpr int WAVE = 64;
 // references (the last 16 bytes are not needed, layout.h).
 constexpr uint32_t NODE_PARTS = 7;
 
-template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES, bool COOP>
+// The local ray of an assembly level: the lerped composed transforms into
+// the assembly applied to the world ray, or the identity at the root
+// (update_local, tracer.rs:65-70, 101-109, 131-148).
+__device__ __forceinline__ void level_local_ray(const TraceArgs& a, const GpuGroup& g, float time, V3 o, V3 d, V3& lo,
+                                                V3& di, uint32_t& oct) {
+    Xf x = xf_identity();
+    if (g.xf_count > 0) {
+        uint32_t i1;
+        float a2;
+        if (!lerp_slice_pos(g.xf_count, time, i1, a2)) x = load_xf(a.xforms + g.xf_start + i1);
+        else x = xf_lerp(load_xf(a.xforms + g.xf_start + i1), load_xf(a.xforms + g.xf_start + i1 + 1), a2);
+    }
+    lo = xf_point(x, o);
+    const V3 ld = xf_vector(x, d);
+    di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
+    oct = octant_of(di);
+}
+
+template <bool COUNT, bool XF, bool MOTION, int LDS_DEPTH, int MIN_WAVES, bool COOP, bool NESTED>
 __global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(MIN_WAVES)))
 void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_min, uint32_t leaf_min) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * WAVE];
@@ -100,6 +120,7 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
     bool has_space = false;
     Xf space;
     uint32_t inst_id = PSY_NONE;
+    uint32_t grp = 0, gbase = 0;  // NESTED: the current assembly path and its first instance record
     uint64_t tri_base = 0;
     uint32_t tri_stride = sizeof(GpuTriStatic), tri_samples = 1;
     uint8_t kind = PSY_HIT_MISS;
@@ -139,6 +160,8 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                     sp = 0;
                     bot = 0;
                     in_inst = false;
+                    grp = 0;
+                    gbase = 0;
                     kind = PSY_HIT_MISS;
                 }
             }
@@ -279,10 +302,30 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         if (cur != REF_EMPTY && (cur & REF_LEAF)) {
             const uint32_t first = (cur & ~REF_LEAF) >> 2;
             const uint32_t count = cur & 3u;
-            if (!in_inst && XF && a.instances[first].kind != PSY_OBJECT_MESH) {
+            const uint32_t ii = NESTED ? gbase + first : first;
+            if (NESTED && !in_inst && count == 0u) {
+                // Level marker: the sub-assembly is done; back to the
+                // enclosing assembly's path and local ray (tracer.rs:131-148).
+                grp = first;
+                const GpuGroup g = a.groups[grp];
+                gbase = g.base;
+                level_local_ray(a, g, time, o, d, lo, di, oct);
+                cur = REF_EMPTY;
+            } else if (NESTED && !in_inst && a.instances[ii].kind == PSY_OBJECT_ASSEMBLY) {
+                // Sub-assembly instance (tracer.rs:95-128): remember the
+                // enclosing path on the stack, move the ray into the
+                // sub-assembly's space and walk its instance BVH.
+                if (COUNT) c_inst++;
+                SSTACK_PUSH(make_level_marker(grp));
+                grp = a.instances[ii].mesh;
+                const GpuGroup g = a.groups[grp];
+                gbase = g.base;
+                level_local_ray(a, g, time, o, d, lo, di, oct);
+                cur = g.root;
+            } else if (!in_inst && XF && a.instances[ii].kind != PSY_OBJECT_MESH) {
                 // Light instance (tracer.rs:95-128 with a light object; light
                 // instances always carry transforms).
-                const GpuInstance inst = a.instances[first];
+                const GpuInstance inst = a.instances[ii];
                 if (COUNT) c_inst++;
                 const Xf x = instance_xf(a.xforms, inst, time);
                 const V3 llo = xf_point(x, o);
@@ -305,7 +348,7 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                 cur = REF_EMPTY;
             } else if (!in_inst) {
                 // Instance leaf (tracer.rs:95-128).
-                const GpuInstance inst = a.instances[first];
+                const GpuInstance inst = a.instances[ii];
                 const GpuMesh mesh = a.meshes[inst.mesh];
                 if (COUNT) c_inst++;
                 if (XF) {
@@ -397,7 +440,9 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
         if (!finished && cur == REF_EMPTY) {
             if (in_inst && sp == inst_base) {
                 in_inst = false;
-                if (XF) {
+                if (NESTED) {
+                    level_local_ray(a, a.groups[grp], time, o, d, lo, di, oct);
+                } else if (XF) {
                     lo = xf_point(ident, o);
                     V3 ld = xf_vector(ident, d);
                     di = v3(1.0f / ld.x, 1.0f / ld.y, 1.0f / ld.z);
@@ -442,48 +487,53 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
     }
 }
 
-template <bool COUNT, bool XF, bool MOTION, int D>
+template <bool COUNT, bool XF, bool MOTION, int D, bool NESTED>
 hipError_t launch_one(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
     // Register budget: 8 waves per SIMD caps the kernel at 64 VGPRs (the
     // compiler keeps the rest in private memory), 4 lets it use up to 128.
-    // The cooperative node fetch serves static scenes.
-    if constexpr (!COUNT && !MOTION) {
+    // The cooperative node fetch serves static single-level scenes.
+    if constexpr (!COUNT && !MOTION && !NESTED) {
         if (tune.coop) {
             if (tune.occupancy >= 8)
-                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 8, true>), dim3(tune.waves),
+                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 8, true, false>), dim3(tune.waves),
                                    dim3(WAVE), 0, s, a, work, tune.refill_min, tune.leaf_min);
             else
-                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 4, true>), dim3(tune.waves),
+                hipLaunchKernelGGL((trace_persistent_kernel<false, XF, false, D, 4, true, false>), dim3(tune.waves),
                                    dim3(WAVE), 0, s, a, work, tune.refill_min, tune.leaf_min);
             return hipGetLastError();
         }
     }
     if (!COUNT && tune.occupancy >= 8)
-        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8, false>), dim3(tune.waves), dim3(WAVE), 0,
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 8, false, NESTED>), dim3(tune.waves), dim3(WAVE), 0,
                            s, a, work, tune.refill_min, tune.leaf_min);
     else
-        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4, false>), dim3(tune.waves), dim3(WAVE), 0,
+        hipLaunchKernelGGL((trace_persistent_kernel<COUNT, XF, MOTION, D, 4, false, NESTED>), dim3(tune.waves), dim3(WAVE), 0,
                            s, a, work, tune.refill_min, tune.leaf_min);
     return hipGetLastError();
 }
 
-template <bool COUNT, bool XF, bool MOTION>
+template <bool COUNT, bool XF, bool MOTION, bool NESTED>
 hipError_t launch_depth(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, hipStream_t s) {
-    if (COUNT) return launch_one<COUNT, XF, MOTION, 16>(a, work, tune, s);
+    if (COUNT || NESTED) return launch_one<COUNT, XF, MOTION, 16, NESTED>(a, work, tune, s);
     switch (tune.lds_depth) {
-        case 8: return launch_one<COUNT, XF, MOTION, 8>(a, work, tune, s);
-        case 24: return launch_one<COUNT, XF, MOTION, 24>(a, work, tune, s);
-        default: return launch_one<COUNT, XF, MOTION, 16>(a, work, tune, s);
+        case 8: return launch_one<COUNT, XF, MOTION, 8, false>(a, work, tune, s);
+        case 24: return launch_one<COUNT, XF, MOTION, 24, false>(a, work, tune, s);
+        default: return launch_one<COUNT, XF, MOTION, 16, false>(a, work, tune, s);
     }
 }
 
 template <bool COUNT>
 hipError_t launch_feat(const TraceArgs& a, unsigned int* work, const TraceTuning& tune, bool xf, bool motion,
                        hipStream_t s) {
-    if (xf && motion) return launch_depth<COUNT, true, true>(a, work, tune, s);
-    if (xf) return launch_depth<COUNT, true, false>(a, work, tune, s);
-    if (motion) return launch_depth<COUNT, false, true>(a, work, tune, s);
-    return launch_depth<COUNT, false, false>(a, work, tune, s);
+    // Nested scenes take the transform path, with or without time samples.
+    if (a.groups) {
+        if (motion) return launch_depth<COUNT, true, true, true>(a, work, tune, s);
+        return launch_depth<COUNT, true, false, true>(a, work, tune, s);
+    }
+    if (xf && motion) return launch_depth<COUNT, true, true, false>(a, work, tune, s);
+    if (xf) return launch_depth<COUNT, true, false, false>(a, work, tune, s);
+    if (motion) return launch_depth<COUNT, false, true, false>(a, work, tune, s);
+    return launch_depth<COUNT, false, false, false>(a, work, tune, s);
 }
 
 }  // namespace

@@ -514,3 +514,74 @@ def test_time_sampled_light_sizes_match_oracle():
         s = want["kind"] != 0
         assert np.array_equal(got["t"][s].view(np.uint32), want["t"][s].view(np.uint32)), f"{kind}: t differs"
         assert np.array_equal(got["inst_id"][s], want["inst_id"][s])
+
+
+def test_nested_assemblies_under_rotation_trace_exact():
+    """A sub-assembly of two boxes placed by a rotating, moving instance:
+    lerped boxes are not conservative under rotation, so which rays reach
+    the boxes depends on walking the assembly's own BVH inside the moving
+    instance's box (tracer.rs:87-150); the GPU walks the same nested BVHs as
+    the oracle and returns identical records on every ray.  Also a static
+    second instance of the same assembly and one without transforms."""
+    import psychopath_amd._lib as _lib
+    from oracle.psyref import OracleScene
+    rot, w2o = scenes._rotation, scenes.world_to_object
+    v1, f1 = scenes._box((0.0, 0.0, 0.0), (0.3, 0.3, 0.2), 10.0)
+    v2, f2 = scenes._box((0.6, 0.2, 0.0), (0.2, 0.2, 0.4), 30.0)
+    moving = np.stack([w2o(rot((0.0, 0.0, 1.0), a), np.array([-2.7, 2.4, 3.3 + 0.4 * a])) for a in (0.0, 0.6)])
+    still = w2o(rot((1.0, 0.0, 0.0), 0.3), np.array([-1.5, 2.0, 3.0]))[None]
+    lamp = Assembly()
+    lamp.add_mesh(Mesh(v1, f1))
+    lamp.add_mesh(Mesh(v2, f2))
+    lamp.add_instance(0)
+    lamp.add_instance(1, w2o(rot((0.0, 1.0, 0.0), 0.2), np.array([0.1, 0.0, 0.0]))[None])
+    asm = Assembly()
+    asm.add_mesh(Mesh(v2, f2))
+    asm.add_instance(0, scenes.translate_xf((-2.0, 1.0, 2.5))[None])
+    a = asm.add_assembly(lamp)
+    asm.add_instance(a, moving, kind=_lib.PSY_OBJECT_ASSEMBLY)
+    asm.add_instance(a, still, kind=_lib.PSY_OBJECT_ASSEMBLY)
+    asm.add_instance(a, None, kind=_lib.PSY_OBJECT_ASSEMBLY)
+    scene = asm.build()
+    tracer = Tracer(scene, max_batch=1 << 18)
+
+    o_lamp = OracleScene()
+    o_lamp.add_mesh(v1, f1)
+    o_lamp.add_mesh(v2, f2)
+    o_lamp.add_instance(0)
+    o_lamp.add_instance(1, w2o(rot((0.0, 1.0, 0.0), 0.2), np.array([0.1, 0.0, 0.0]))[None])
+    o_lamp.build()
+    o = OracleScene()
+    o.add_mesh(v2, f2)
+    o.add_instance(0, scenes.translate_xf((-2.0, 1.0, 2.5))[None])
+    oa = o.add_assembly(o_lamp)
+    o.add_assembly_instance(oa, moving)
+    o.add_assembly_instance(oa, still)
+    o.add_assembly_instance(oa, None)
+    o.build()
+
+    rng = np.random.default_rng(7)
+    n = 200_000
+    centers = np.array([[-2.7, 2.4, 3.4], [-1.5, 2.0, 3.1], [0.2, 0.1, 0.2], [-2.0, 1.0, 2.6]])
+    tgt = centers[rng.integers(0, 4, n)] + rng.uniform(-0.9, 0.9, (n, 3))
+    org = tgt + rng.normal(0, 1, (n, 3)) * 3
+    dd = tgt - org
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    r = {k: v.astype(np.float32) for k, v in dict(ox=org[:, 0], oy=org[:, 1], oz=org[:, 2], dx=dd[:, 0],
+                                                     dy=dd[:, 1], dz=dd[:, 2]).items()}
+    r["time"] = rng.uniform(0, 1, n).astype(np.float32)
+    r["max_t"] = np.full(n, np.inf, np.float32)
+    want = o.trace(r, threads=8)
+    assert (want["kind"] == 2).sum() > 10_000
+    for kernel in (_lib.PSY_KERNEL_PERSISTENT, _lib.PSY_KERNEL_SIMPLE):
+        tracer.set_tuning(kernel=kernel)
+        got = tracer.trace(RayBatch(r["ox"], r["oy"], r["oz"], r["dx"], r["dy"], r["dz"], r["time"], r["max_t"]))
+        assert np.array_equal(got.kind, want["kind"]), kernel
+        s = want["kind"] == 2
+        for k in ("t", "b0", "b1", "b2", "prim_id"):
+            assert np.array_equal(getattr(got, k)[s], want[k][s]), (kernel, k)
+        # Hit records number the flattened instances depth first (the root's
+        # mesh, then each lamp's two); the oracle names an instance by its
+        # index in its own assembly.
+        local = np.array([0, 0, 1, 0, 1, 0, 1], np.uint32)
+        assert np.array_equal(local[got.inst_id[s]], want["inst_id"][s]), kernel
