@@ -23,13 +23,15 @@
  * lists) come back as PSY_E_PARSE with psy_last_error() giving
  * "<kind> at byte <offset> (line <n>): <message>".
  *
- * Nested assemblies are flattened into the one-level scene: every object
- * instance inside them gets the time-sampled transforms the reference's
- * transform stack would compose for it (transform_stack.rs:33-60), and their
- * shaders and meshes are appended after the root's, depth first in
- * definition order.  Not supported (PSY_E_PARSE with a message saying so):
- * lights inside nested assemblies (the flattened scene has one light tree),
- * more than 64 time samples of a camera lens or light parameter, and light
+ * Nested assemblies stay nested: psy_file_scene.scene is the root assembly
+ * with its sub-assemblies in `assemblies`, each holding its own meshes,
+ * lights and instances (instance indices are per assembly), as
+ * AssemblyBuilder::add_assembly keeps them (scene/assembly.rs:197-282); the
+ * world's shader list holds every assembly's shaders, the root's first, then
+ * depth first in definition order.  A light instance of the root assembly
+ * without a transform gets the identity.  Not supported (PSY_E_PARSE with a
+ * message saying so): more than 64 time samples of a camera lens or light
+ * parameter, and light
  * colour samples that mix representations (the reference panics when it
  * lerps them).  Every other time sample is kept.  Files carry no
  * bounce limit: settings.max_bounces is the reference renderer's fixed 2

@@ -772,8 +772,6 @@ def to_scene(parsed: dict):
                 where.append(("assembly", len(ns.sub_assemblies)))
                 ns.sub_assemblies.append(build(o, False))
             else:
-                if not root:
-                    raise PsyParseError("UnknownVariant", 0, "lights inside nested assemblies are not supported")
                 if kind == "rect":
                     where.append(("rect", len(ns.rect_lights)))
                     ns.rect_lights.append((_one([d[0] for d in o["dims"]]), _one([d[1] for d in o["dims"]]),
@@ -784,7 +782,10 @@ def to_scene(parsed: dict):
         for i in asm["instances"]:
             kind, idx = where[i["object"]]
             xf = i["xforms"]
-            if not xf and kind in ("rect", "sphere"):
+            # A root light instance without a transform takes the identity
+            # (the reference would lerp an empty transform list); inside a
+            # sub-assembly it keeps the enclosing assembly's transforms.
+            if not xf and root and kind in ("rect", "sphere"):
                 xf = [[1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]]
             shader = None if (i["shader"] is None or kind != "mesh") else base + i["shader"]
             ns.instances.append(dict(kind=kind, index=idx, shader=shader,

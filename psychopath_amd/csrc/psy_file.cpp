@@ -413,14 +413,44 @@ struct SamplePool {
     std::deque<std::vector<psy_color>> colours;
 };
 
-struct psy_file_scene_data {
-    std::string name, output;
-    std::vector<MeshData> mesh_data;
-    std::vector<psy_mesh_desc> meshes;
+// One assembly as the reference's AssemblyBuilder holds it (scene/
+// assembly.rs:150-286): its own shaders and objects, sub-assemblies sharing
+// the object namespace, and instances of objects or sub-assemblies.
+constexpr uint32_t KIND_ASSEMBLY = 3;
+struct AsmDef {
+    std::vector<psy_shader_desc> shaders;
+    std::vector<MeshData> meshes;
     std::vector<psy_rect_light_desc> rects;
     std::vector<psy_sphere_light_desc> spheres;
+    SamplePool pool;  // light time samples
+    std::vector<std::unique_ptr<AsmDef>> subs;
+    struct Inst {
+        uint32_t kind;    // PSY_OBJECT_* or KIND_ASSEMBLY
+        uint32_t index;   // into meshes / rects / spheres / subs
+        uint32_t shader;  // into shaders, PSY_NONE if unbound
+        std::vector<psy::Xf> xf;
+    };
+    std::vector<Inst> instances;
+    uint32_t shader_base = 0, mesh_base = 0;  // positions in the scene's shader and mesh lists
+};
+
+// An assembly as descriptors (psy_scene_desc) over the arrays it owns; its
+// sub-assemblies' descriptors are contiguous, as psy_scene_desc.assemblies
+// requires.
+struct AsmOut {
+    std::vector<psy_mesh_desc> meshes;
     std::vector<psy_instance_desc> instances;
     std::vector<float> xforms;
+    std::vector<std::unique_ptr<AsmOut>> subs;
+    std::vector<psy_scene_desc> sub_descs;
+    psy_scene_desc desc{};
+};
+
+struct psy_file_scene_data {
+    std::string name, output;
+    std::vector<MeshData> mesh_data;   // every assembly's meshes, depth first in definition order
+    std::unique_ptr<AsmDef> root_def;  // the parsed assemblies (their lights and light samples)
+    std::unique_ptr<AsmOut> root_out;  // and their descriptors, nested as in the file
     std::vector<psy_shader_desc> shaders;
     std::vector<psy_distant_light_desc> distant;
     psy_color background{};
@@ -712,30 +742,7 @@ MeshData parse_mesh_surface(const Node& n) {
     return m;
 }
 
-// One assembly as the reference's AssemblyBuilder holds it (scene/
-// assembly.rs:150-286): its own shaders and objects, sub-assemblies sharing
-// the object namespace, and instances of objects or sub-assemblies.
-constexpr uint32_t KIND_ASSEMBLY = 3;
-struct AsmDef {
-    std::vector<psy_shader_desc> shaders;
-    std::vector<MeshData> meshes;
-    std::vector<psy_rect_light_desc> rects;
-    std::vector<psy_sphere_light_desc> spheres;
-    SamplePool pool;  // light time samples
-    std::vector<std::unique_ptr<AsmDef>> subs;
-    struct Inst {
-        uint32_t kind;    // PSY_OBJECT_* or KIND_ASSEMBLY
-        uint32_t index;   // into meshes / rects / spheres / subs
-        uint32_t shader;  // into shaders, PSY_NONE if unbound
-        std::vector<psy::Xf> xf;
-    };
-    std::vector<Inst> instances;
-    uint32_t shader_base = 0, mesh_base = 0;  // positions in the flattened scene
-};
-
-// parse_assembly (psy_assembly.rs:18-188).  Lights are read in the root
-// assembly only: the flattened scene samples lights from one light tree,
-// where the reference keeps one per assembly.
+// parse_assembly (psy_assembly.rs:18-188).
 std::unique_ptr<AsmDef> parse_assembly(const Node& n, bool root) {
     if (!n.internal) raise("UnknownError", n.offset, "Assembly should be an internal node");
     auto def = std::make_unique<AsmDef>();
@@ -786,8 +793,6 @@ std::unique_ptr<AsmDef> parse_assembly(const Node& n, bool root) {
             if (!c.internal || !c.has_ident) raise("UnknownError", c.offset, c.type + " should be an internal node with a name");
             if (object_map.count(c.ident))
                 raise("UnknownError", c.offset, "Attempted to add object to assembly with a name that already exists.");
-            if (c.type != "MeshSurface" && !root)
-                raise("UnknownVariant", c.offset, "lights inside nested assemblies are not supported");
             ObjectRef ref{};
             if (c.type == "MeshSurface") {
                 ref = ObjectRef{PSY_OBJECT_MESH, (uint32_t)def->meshes.size()};
@@ -817,8 +822,9 @@ std::unique_ptr<AsmDef> parse_assembly(const Node& n, bool root) {
     return def;
 }
 
-// Shaders and meshes of every assembly go to the flattened scene depth
-// first in definition order (the root's first).
+// Shaders and meshes of every assembly go to the scene's lists depth first
+// in definition order (the root's first); shader indices are global, as the
+// renderer takes one shader list.
 void place_definitions(AsmDef& d, psy_file_scene_data& sd) {
     d.shader_base = (uint32_t)sd.shaders.size();
     sd.shaders.insert(sd.shaders.end(), d.shaders.begin(), d.shaders.end());
@@ -827,40 +833,57 @@ void place_definitions(AsmDef& d, psy_file_scene_data& sd) {
     for (auto& sub : d.subs) place_definitions(*sub, sd);
 }
 
-using psy::push_xforms;
-
-// The instances of an assembly under the transform stack `top`, sub-
-// assemblies expanded in place (tracer.rs:87-150 walks them the same way).
-void flatten(const AsmDef& d, const std::vector<psy::Xf>& top, psy_file_scene_data& sd) {
-    for (const AsmDef::Inst& inst : d.instances) {
-        const std::vector<psy::Xf> xf = push_xforms(top, inst.xf);
-        if (inst.kind == KIND_ASSEMBLY) {
-            flatten(*d.subs[inst.index], xf, sd);
-            continue;
-        }
-        psy_instance_desc out{};
-        out.kind = inst.kind;
-        out.mesh = inst.kind == PSY_OBJECT_MESH ? d.mesh_base + inst.index : inst.index;
-        out.shader = inst.shader == PSY_NONE ? PSY_NONE : d.shader_base + inst.shader;
-        std::vector<psy::Xf> placed = xf;
-        if (placed.empty() && inst.kind != PSY_OBJECT_MESH) placed.push_back(psy::xf_identity());
-        out.xform_start = (uint32_t)(sd.xforms.size() / 12);
-        out.xform_count = (uint32_t)placed.size();
-        for (const psy::Xf& x : placed)
-            for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 4; c++) sd.xforms.push_back(x.m[r][c]);
-        sd.instances.push_back(out);
+// The descriptors of an assembly and its sub-assemblies, nested as in the
+// file (AssemblyBuilder::add_assembly, assembly.rs:197-282): meshes, lights
+// and sub-assemblies by their index in this assembly, each instance with its
+// own transforms.  A light instance of the root assembly without a
+// transform gets the identity (the reference would lerp an empty transform
+// list); elsewhere it keeps the enclosing assembly's transforms.
+std::unique_ptr<AsmOut> build_out(AsmDef& d, bool root, const psy_file_scene_data& sd) {
+    auto o = std::make_unique<AsmOut>();
+    for (uint32_t k = 0; k < (uint32_t)d.meshes.size(); k++) {
+        const MeshData& m = sd.mesh_data[d.mesh_base + k];
+        o->meshes.push_back(psy_mesh_desc{m.vert_count, m.time_samples, m.verts.data(),
+                                          m.normals.empty() ? nullptr : m.normals.data(),
+                                          (uint32_t)(m.indices.size() / 3), m.indices.data()});
     }
+    for (auto& sub : d.subs) o->subs.push_back(build_out(*sub, false, sd));
+    for (auto& sub : o->subs) o->sub_descs.push_back(sub->desc);
+    for (const AsmDef::Inst& inst : d.instances) {
+        psy_instance_desc out{};
+        out.kind = inst.kind;  // KIND_ASSEMBLY is PSY_OBJECT_ASSEMBLY
+        out.mesh = inst.index;
+        out.shader = inst.shader == PSY_NONE ? PSY_NONE : d.shader_base + inst.shader;
+        std::vector<psy::Xf> xf = inst.xf;
+        if (xf.empty() && root && inst.kind != PSY_OBJECT_MESH && inst.kind != KIND_ASSEMBLY)
+            xf.push_back(psy::xf_identity());
+        out.xform_start = (uint32_t)(o->xforms.size() / 12);
+        out.xform_count = (uint32_t)xf.size();
+        for (const psy::Xf& x : xf)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 4; c++) o->xforms.push_back(x.m[r][c]);
+        o->instances.push_back(out);
+    }
+    psy_scene_desc& ds = o->desc;
+    ds.mesh_count = (uint32_t)o->meshes.size();
+    ds.meshes = o->meshes.data();
+    ds.instance_count = (uint32_t)o->instances.size();
+    ds.instances = o->instances.data();
+    ds.xform_count = (uint32_t)(o->xforms.size() / 12);
+    ds.xforms = o->xforms.data();
+    ds.rect_light_count = (uint32_t)d.rects.size();
+    ds.rect_lights = d.rects.data();
+    ds.sphere_light_count = (uint32_t)d.spheres.size();
+    ds.sphere_lights = d.spheres.data();
+    ds.assembly_count = (uint32_t)o->sub_descs.size();
+    ds.assemblies = o->sub_descs.data();
+    return o;
 }
 
 void read_assembly(const Node& n, psy_file_scene_data& sd) {
-    std::unique_ptr<AsmDef> root = parse_assembly(n, true);
-    sd.rects = root->rects;
-    sd.spheres = root->spheres;
-    for (auto& v : root->pool.floats) sd.pool.floats.push_back(std::move(v));
-    for (auto& v : root->pool.colours) sd.pool.colours.push_back(std::move(v));
-    place_definitions(*root, sd);
-    flatten(*root, {}, sd);
+    sd.root_def = parse_assembly(n, true);
+    place_definitions(*sd.root_def, sd);
+    sd.root_out = build_out(*sd.root_def, true, sd);
 }
 
 // parse_scene (psy.rs:99-200).
@@ -886,11 +909,6 @@ std::unique_ptr<psy_file_scene_data> parse_scene(const Node& n) {
     parse_world(*children_of(n, "World", 0)[0], *sd);
     read_assembly(*children_of(n, "Assembly", 0)[0], *sd);
     if (n.has_ident) sd->name = n.ident.substr(1);
-    // Descriptors over the owned arrays.
-    for (const MeshData& m : sd->mesh_data)
-        sd->meshes.push_back(psy_mesh_desc{m.vert_count, m.time_samples, m.verts.data(),
-                                           m.normals.empty() ? nullptr : m.normals.data(),
-                                           (uint32_t)(m.indices.size() / 3), m.indices.data()});
     return sd;
 }
 
@@ -941,16 +959,7 @@ int psy_file_get_scene(const psy_file* f, uint32_t index, psy_file_scene* out) {
     psy_file_scene s{};
     s.name = sd.name.c_str();
     s.output_path = sd.output.c_str();
-    s.scene.mesh_count = (uint32_t)sd.meshes.size();
-    s.scene.meshes = sd.meshes.data();
-    s.scene.instance_count = (uint32_t)sd.instances.size();
-    s.scene.instances = sd.instances.data();
-    s.scene.xform_count = (uint32_t)(sd.xforms.size() / 12);
-    s.scene.xforms = sd.xforms.data();
-    s.scene.rect_light_count = (uint32_t)sd.rects.size();
-    s.scene.rect_lights = sd.rects.data();
-    s.scene.sphere_light_count = (uint32_t)sd.spheres.size();
-    s.scene.sphere_lights = sd.spheres.data();
+    s.scene = sd.root_out->desc;
     s.world.shader_count = (uint32_t)sd.shaders.size();
     s.world.shaders = sd.shaders.data();
     s.world.background = sd.background;

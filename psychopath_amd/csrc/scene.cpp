@@ -574,7 +574,14 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
             const psy_instance_desc& id = ad->instances[i];
             std::vector<Xf> own;
             for (uint32_t k = 0; k < id.xform_count; k++) own.push_back(xf_at(ad, id.xform_start + k));
-            const std::vector<Box> wb = transform_bounds(object_bounds(ad, id, info, s, f), own.data(), id.xform_count);
+            const std::vector<Box> ob = object_bounds(ad, id, info, s, f);
+            if (ob.empty()) {
+                // The reference's builder lerps the instance's empty bounds
+                // list and panics.
+                err = "an instance of an assembly without instances";
+                return PSY_E_INVALID;
+            }
+            const std::vector<Box> wb = transform_bounds(ob, own.data(), id.xform_count);
             info.inst_bounds.insert(info.inst_bounds.end(), wb.begin(), wb.end());
             info.inst_offs.push_back((uint32_t)info.inst_bounds.size());
         }

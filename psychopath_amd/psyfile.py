@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib
-from .scenes import COLOR_BLACKBODY, COLOR_TEMPERATURE, COLOR_XYZ, RenderScene
+from .scenes import COLOR_BLACKBODY, COLOR_TEMPERATURE, COLOR_XYZ, RenderScene, SubAssembly
 
 
 class PsyFileError(ValueError):
@@ -92,20 +92,19 @@ def read(text) -> list[RenderScene]:
         lib().psy_file_destroy(h)
 
 
-def _scene_from(fs: _lib.FileScene) -> RenderScene:
-    sc = RenderScene()
-    sc.name = fs.name.decode("utf-8")
-    sc.output_path = fs.output_path.decode("utf-8")
-    d = fs.scene
+def _assembly_from(d, asm) -> None:
+    """The meshes, lights, instances and (nested, as in the file)
+    sub-assemblies of the assembly descriptor `d` into `asm` (a RenderScene
+    or a SubAssembly)."""
     for i in range(d.mesh_count):
         m = d.meshes[i]
         v = _array(m.verts, 3 * m.vert_count * m.time_samples, np.float32).reshape(m.time_samples, m.vert_count, 3)
         t = _array(m.indices, 3 * m.tri_count, np.uint32).reshape(m.tri_count, 3)
         if m.normals:
             nrm = _array(m.normals, 3 * m.vert_count * m.time_samples, np.float32).reshape(v.shape)
-            sc.meshes.append((v, t, nrm))
+            asm.meshes.append((v, t, nrm))
         else:
-            sc.meshes.append((v, t))
+            asm.meshes.append((v, t))
     for i in range(d.rect_light_count):
         r = d.rect_lights[i]
         if r.dims_count:
@@ -113,19 +112,32 @@ def _scene_from(fs: _lib.FileScene) -> RenderScene:
             w, h = [float(x) for x in wh[:, 0]], [float(x) for x in wh[:, 1]]
         else:
             w, h = float(r.width), float(r.height)
-        sc.rect_lights.append((w, h, _colours(r.color_count, r.colors, r.color)))
+        asm.rect_lights.append((w, h, _colours(r.color_count, r.colors, r.color)))
     for i in range(d.sphere_light_count):
         s = d.sphere_lights[i]
         rad = [float(x) for x in _array(s.radii, s.radius_count, np.float32)] if s.radius_count else float(s.radius)
-        sc.sphere_lights.append((rad, _colours(s.color_count, s.colors, s.color)))
+        asm.sphere_lights.append((rad, _colours(s.color_count, s.colors, s.color)))
     xf = _array(d.xforms, 12 * d.xform_count, np.float32).reshape(d.xform_count, 12)
-    kinds = {_lib.PSY_OBJECT_MESH: "mesh", _lib.PSY_OBJECT_RECT_LIGHT: "rect", _lib.PSY_OBJECT_SPHERE_LIGHT: "sphere"}
+    kinds = {_lib.PSY_OBJECT_MESH: "mesh", _lib.PSY_OBJECT_RECT_LIGHT: "rect", _lib.PSY_OBJECT_SPHERE_LIGHT: "sphere",
+             _lib.PSY_OBJECT_ASSEMBLY: "assembly"}
     for i in range(d.instance_count):
         inst = d.instances[i]
-        sc.instances.append(dict(
+        asm.instances.append(dict(
             kind=kinds[inst.kind], index=int(inst.mesh),
             shader=None if inst.shader == _lib.PSY_NONE else int(inst.shader),
             xforms=xf[inst.xform_start:inst.xform_start + inst.xform_count].copy() if inst.xform_count else None))
+    asm.sub_assemblies = []
+    for i in range(d.assembly_count):
+        sub = SubAssembly()
+        _assembly_from(d.assemblies[i], sub)
+        asm.sub_assemblies.append(sub)
+
+
+def _scene_from(fs: _lib.FileScene) -> RenderScene:
+    sc = RenderScene()
+    sc.name = fs.name.decode("utf-8")
+    sc.output_path = fs.output_path.decode("utf-8")
+    _assembly_from(fs.scene, sc)
     w = fs.world
     sc.shaders = [(int(w.shaders[i].kind), _color(w.shaders[i].color), float(w.shaders[i].roughness),
                    float(w.shaders[i].fresnel)) for i in range(w.shader_count)]

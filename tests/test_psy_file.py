@@ -510,108 +510,57 @@ def nested_cornell_text(w=32, h=24, spp=2, seed=0):
     return text[:at] + sub + text[at:]
 
 
-def _oracle_math():
-    from oracle.psyref import lib
-    import ctypes as C
-    L = lib()
-    fp = C.POINTER(C.c_float)
-
-    def mul(a, b):
-        a = np.asarray(a, np.float32)
-        b = np.asarray(b, np.float32)
-        out = np.zeros(12, np.float32)
-        L.psyref_xform_mul(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp))
-        return out
-
-    def lerp(a, b, t):
-        a = np.asarray(a, np.float32)
-        b = np.asarray(b, np.float32)
-        out = np.zeros(12, np.float32)
-        L.psyref_xform_lerp(a.ctypes.data_as(fp), b.ctypes.data_as(fp), C.c_float(t), out.ctypes.data_as(fp))
-        return out
-    return mul, lerp
-
-
-def flatten_oracle(sc):
-    """The oracle's nested scene flattened with the oracle's transform
-    arithmetic, in the reader's order: instances depth first, transforms
-    composed as TransformStack::push does (transform_stack.rs:33-60,
-    algorithm.rs:266-305)."""
-    mul, lerp = _oracle_math()
-
-    def lerp_slice(s, alpha):
-        n = len(s)
-        if n == 1:
-            return s[0]
-        if alpha == 1.0:
-            return s[-1]
-        tmp = np.float32(alpha) * np.float32(n - 1)
-        i1 = int(tmp)
-        a2 = float(np.float32(tmp) - np.float32(i1))
-        return lerp(s[i1], s[i1 + 1], a2)
-
-    def push(top, own):
-        if not own:
-            return top
-        if not top:
-            return own
-        if len(top) == len(own):
-            return [mul(a, b) for a, b in zip(top, own)]
-        if len(top) > len(own):
-            s = np.float32(len(top) - 1)
-            return [mul(a, lerp_slice(own, float(np.float32(i) / s))) for i, a in enumerate(top)]
-        s = np.float32(len(own) - 1)
-        return [mul(lerp_slice(top, float(np.float32(i) / s)), b) for i, b in enumerate(own)]
-
-    meshes, insts = [], []
-
-    def place(ns):
-        ns.mesh_base = len(meshes)
-        meshes.extend(ns.meshes)
-        for sub in ns.sub_assemblies:
-            place(sub)
-
-    def walk(ns, top):
-        for i in ns.instances:
-            own = [] if i["xforms"] is None else [np.asarray(x, np.float32) for x in i["xforms"]]
-            xf = push(top, own)
-            if i["kind"] == "assembly":
-                walk(ns.sub_assemblies[i["index"]], xf)
-            else:
-                insts.append((i["kind"], ns.mesh_base + i["index"] if i["kind"] == "mesh" else i["index"],
-                              i["shader"], np.asarray(xf, np.float32).reshape(-1, 12)))
-    place(sc)
-    walk(sc, [])
-    return meshes, insts
+def same_assembly(a, b):
+    """The reader's assembly `a` (a RenderScene or SubAssembly) equals the
+    oracle reader's `b`: meshes, lights, instances with their own transforms
+    bit for bit, and the sub-assemblies, nested as in the file."""
+    assert len(a.meshes) == len(b.meshes)
+    for x, y in zip(a.meshes, b.meshes):
+        assert f32eq(x[0], np.asarray(y[0], np.float32).reshape(x[0].shape)) and np.array_equal(x[1], y[1])
+    assert len(a.rect_lights) == len(b.rect_lights) and len(a.sphere_lights) == len(b.sphere_lights)
+    for x, y in zip(a.rect_lights, b.rect_lights):
+        assert f32eq(np.atleast_1d(x[0]), np.atleast_1d(y[0])) and f32eq(np.atleast_1d(x[1]), np.atleast_1d(y[1]))
+    for x, y in zip(a.sphere_lights, b.sphere_lights):
+        assert f32eq(np.atleast_1d(x[0]), np.atleast_1d(y[0]))
+    assert len(a.instances) == len(b.instances)
+    for x, y in zip(a.instances, b.instances):
+        assert (x["kind"], x["index"], x["shader"]) == (y["kind"], y["index"], y["shader"])
+        if y["xforms"] is None:
+            assert x["xforms"] is None
+        else:
+            assert f32eq(x["xforms"], y["xforms"]), (x["xforms"], y["xforms"])
+    assert len(a.sub_assemblies) == len(b.sub_assemblies)
+    for x, y in zip(a.sub_assemblies, b.sub_assemblies):
+        same_assembly(x, y)
 
 
-def test_nested_assemblies_flatten_like_the_transform_stack():
+def test_nested_assemblies_read_like_the_oracle():
+    """Sub-assemblies stay nested (AssemblyBuilder::add_assembly,
+    assembly.rs:197-282): the traversal walks them nested, as
+    trace_assembly does (tracer.rs:87-150), with the transforms the
+    transform stack composes."""
+    from psychopath_amd import render
     text = nested_cornell_text()
     got = psyfile.read(text)[0]
     sc = opf.to_scene(opf.parse_file(text))
-    meshes, insts = flatten_oracle(sc)
-    assert len(got.meshes) == len(meshes) == 8
-    for a, b in zip(got.meshes, meshes):
-        assert f32eq(a[0], np.asarray(b[0], np.float32).reshape(a[0].shape)) and np.array_equal(a[1], b[1])
-    assert len(got.instances) == len(insts)
-    for a, (kind, idx, shader, xf) in zip(got.instances, insts):
-        assert (a["kind"], a["index"], a["shader"]) == (kind, idx, shader)
-        if xf.size == 0:
-            assert a["xforms"] is None
-        else:
-            assert f32eq(a["xforms"], xf), (a["xforms"], xf)
+    same_assembly(got, sc)
     assert len(got.shaders) == len(sc.shaders) == 3 + 2 + 1
-    # the moved copy composes two samples; its inner box a rotation and two moves
+    assert len(got.sub_assemblies) == 1 and len(got.sub_assemblies[0].sub_assemblies) == 1
+    # the moved copy of the sub-assembly has two transform samples
     assert max(0 if i["xforms"] is None else len(i["xforms"]) for i in got.instances) == 2
+    render.assembly_of(got).build()  # the packer accepts it
 
 
-def test_lights_inside_nested_assemblies_are_refused():
+def test_lights_inside_nested_assemblies_are_read_like_the_oracle():
+    from psychopath_amd import render
     text = nested_cornell_text().replace(
         "        Assembly $boxes {\n",
         "        Assembly $boxes {\n            SphereLight $bulb {\n                Radius [0.1]\n"
-        "                Color [rec709, 1 1 1]\n            }\n", 1)
-    with pytest.raises(psyfile.PsyFileError) as ce:
-        psyfile.read(text)
-    assert ce.value.kind == "UnknownVariant" and "nested" in str(ce.value)
-    with pytest.raises(opf.PsyParseError):
-        opf.to_scene(opf.parse_file(text))
+        "                Color [rec709, 1 1 1]\n            }\n"
+        "            Instance {\n                Data [$bulb]\n"
+        f"                Transform [{_xf_text([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, -1])}]\n            }}\n", 1)
+    got = psyfile.read(text)[0]
+    sc = opf.to_scene(opf.parse_file(text))
+    same_assembly(got, sc)
+    assert len(got.sub_assemblies[0].sphere_lights) == 1
+    render.assembly_of(got).build()
