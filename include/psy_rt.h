@@ -227,7 +227,10 @@ typedef struct {
     uint32_t motion_bounds_bytes; /* bytes of one time-sample BBox4 record */
     uint32_t tri_bytes;         /* bytes of one static triangle record */
     uint32_t max_depth;         /* deepest BVH4 (src/accel/bvh4.rs:87) */
-    double build_seconds;       /* host BVH build time */
+    double build_seconds;       /* BVH build time (host and device builds) */
+    uint32_t device_built_meshes; /* mesh BVHs the device builder built */
+    double device_build_ms;     /* their device build time, transfers included */
+    uint32_t device_light_trees;  /* light trees partitioned on the device */
 } psy_scene_info;
 
 /* Traversal counters of one psy_trace_counted() call. */
@@ -278,6 +281,17 @@ const char* psy_version(void);
  * repacked scene is uploaded to the current HIP device when the first tracer
  * is created, so scenes can be built and inspected without a GPU. */
 int psy_scene_create(const psy_scene_desc* desc, psy_scene** out);
+/* psy_scene_create with the mesh BVH4s of at least `min_tris` triangles (0:
+ * every mesh) built on the current HIP device, level by level (binned SAH,
+ * the Hoare partition's swaps from prefix counts, median splits near the
+ * depth limit, bounds bottom-up, the BVH4 collapse): the host builder's
+ * trees, bit for bit; light trees of at least `min_tris` lights (and two)
+ * are partitioned on the device the same way (LightTree::from_objects,
+ * light_tree.rs:211-290, SAH splits only).  A mesh or light list whose boxes
+ * hold NaN or negative zero (their unions depend on order) or more than 64
+ * time samples, or a HIP error, falls back to the host builder;
+ * psy_scene_get_info counts what the device built. */
+int psy_scene_create_on_device(const psy_scene_desc* desc, uint32_t min_tris, psy_scene** out);
 int psy_scene_destroy(psy_scene* scene);
 int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
 

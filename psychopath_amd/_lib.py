@@ -177,6 +177,9 @@ class SceneInfo(C.Structure):
         ("tri_bytes", C.c_uint32),
         ("max_depth", C.c_uint32),
         ("build_seconds", C.c_double),
+        ("device_built_meshes", C.c_uint32),
+        ("device_build_ms", C.c_double),
+        ("device_light_trees", C.c_uint32),
     ]
 
 
@@ -213,6 +216,7 @@ _SIGNATURES = {
     "psy_last_error": (C.c_char_p, []),
     "psy_version": (C.c_char_p, []),
     "psy_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    "psy_scene_create_on_device": (C.c_int, [C.POINTER(SceneDesc), C.c_uint32, C.POINTER(C.c_void_p)]),
     "psy_scene_destroy": (C.c_int, [C.c_void_p]),
     "psy_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
     "psy_scene_dump_bvh": (C.c_int64, [C.c_void_p, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_uint64]),

@@ -210,7 +210,11 @@ extern "C" {
 const char* psy_last_error(void) { return g_error.c_str(); }
 const char* psy_version(void) { return "psy_rt 0.1 gfx950"; }
 
-int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
+}  // extern "C"
+
+namespace {
+
+int scene_create(const psy_scene_desc* desc, const BuildOptions& opt, psy_scene** out) {
     if (!out) return fail(PSY_E_INVALID, "out is NULL");
     *out = nullptr;
     psy_scene* s = new (std::nothrow) psy_scene();
@@ -218,7 +222,7 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
     std::string err;
     int rc;
     try {
-        rc = pack_scene(desc, s->host, err);
+        rc = pack_scene(desc, s->host, err, opt);
     } catch (const std::bad_alloc&) {
         rc = PSY_E_NOMEM;
         err = "out of host memory while building the scene";
@@ -242,6 +246,19 @@ int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) {
     return PSY_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int psy_scene_create(const psy_scene_desc* desc, psy_scene** out) { return scene_create(desc, BuildOptions{}, out); }
+
+int psy_scene_create_on_device(const psy_scene_desc* desc, uint32_t min_tris, psy_scene** out) {
+    BuildOptions opt;
+    opt.device = true;
+    opt.device_min_tris = min_tris;
+    return scene_create(desc, opt, out);
+}
+
 int psy_scene_destroy(psy_scene* scene) {
     delete scene;
     return PSY_OK;
@@ -260,6 +277,9 @@ int psy_scene_get_info(const psy_scene* s, psy_scene_info* info) {
     info->tri_bytes = sizeof(GpuTriStatic);
     info->max_depth = s->host.max_depth;
     info->build_seconds = s->host.build_seconds;
+    info->device_built_meshes = s->host.device_built;
+    info->device_build_ms = s->host.device_build_ms;
+    info->device_light_trees = s->host.device_light_trees;
     return PSY_OK;
 }
 

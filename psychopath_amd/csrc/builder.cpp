@@ -12,6 +12,8 @@
 // emitted straight into 128-B GPU records.
 #include "builder.h"
 
+#include "build_common.h"
+
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -23,10 +25,6 @@
 namespace psy {
 
 namespace {
-
-constexpr uint32_t SAH_BINS = 13;          // objects_split.rs:13
-constexpr uint32_t MAX_BASE_DEPTH = 42;    // bvh_base.rs:7
-constexpr float UNION_FACTOR = 1.4f;       // bvh_base.rs:12
 
 // Child visit order for one node code and ray octant, as the bvh_order table
 // defines it (sub_crates/bvh_order/build.rs:5-59): the children of a split
@@ -75,13 +73,6 @@ struct OrderTable {
     }
 };
 
-// Per-object data derived once from the input bounds.
-struct ObjInfo {
-    Box mid;        // lerp_slice(bounds, 0.5)
-    V3 mid_center;  // mid.center()  (BBox::center, a lerp)
-    V3 centroid;    // (mid.min + mid.max) * 0.5
-};
-
 // BVH2 node of the intermediate tree (BVHBaseNode, bvh_base.rs:23-35).
 struct BaseNode {
     bool leaf = false;
@@ -91,24 +82,10 @@ struct BaseNode {
     uint32_t obj_start = 0, obj_end = 0;
 };
 
-Box lerp_slice_box(const Box* s, uint32_t n, float alpha) {
-    uint32_t i1;
-    float a2;
-    if (!lerp_slice_pos(n, alpha, i1, a2)) return s[n - 1];
-    return box_lerp(s[i1], s[i1 + 1], a2);
-}
-
-// The union rule: keep one union box unless it is more than 1.4x the mean
-// time-sample area (bvh_base.rs:116-132, 194-203).
+// The union rule (bvh_base.rs:116-132, 194-203) over a sample list.
 void union_or_samples(const std::vector<Box>& samples, std::vector<Box>& out) {
-    Box u = box_empty();
-    for (const Box& b : samples) u = box_union(u, b);
-    float area = 0.0f;
-    for (const Box& b : samples) area = area + box_area(b);
-    float avg = area / (float)samples.size();
-    out.clear();
-    if (box_area(u) <= (avg * UNION_FACTOR)) out.push_back(u);
-    else out = samples;
+    out.resize(samples.size());
+    out.resize(union_or_samples(samples.data(), (uint32_t)samples.size(), out.data()));
 }
 
 class Builder {
@@ -119,15 +96,8 @@ class Builder {
         for (uint32_t k = 0; k < in.count; k++) {
             const Box* s = in.bounds + in.offsets[k];
             uint32_t n = in.offsets[k + 1] - in.offsets[k];
-            ObjInfo& o = info_[k];
-            o.mid = lerp_slice_box(s, n, 0.5f);
-            o.mid_center = box_center(o.mid);
-            o.centroid = mul(add(o.mid.lo, o.mid.hi), 0.5f);
-            const float v[12] = {o.mid.lo.x, o.mid.lo.y, o.mid.lo.z, o.mid.hi.x, o.mid.hi.y, o.mid.hi.z,
-                                 o.mid_center.x, o.mid_center.y, o.mid_center.z, o.centroid.x, o.centroid.y,
-                                 o.centroid.z};
-            for (float f : v)
-                if (f != f || (f == 0.0f && std::signbit(f))) order_free_ = false;
+            info_[k] = obj_info(s, n);
+            if (!obj_order_free(info_[k])) order_free_ = false;
         }
     }
 
@@ -154,7 +124,7 @@ class Builder {
         }
         uint32_t split;
         uint8_t axis;
-        if ((uint32_t)(63 - __builtin_clzll((unsigned long long)n)) < (MAX_BASE_DEPTH - depth))
+        if (use_sah(n, depth))
             sah_split(keys, recs, n, split, axis);
         else
             median_split(keys, recs, n, split, axis);
@@ -233,28 +203,6 @@ class Builder {
         }
     }
 
-    // Hoare-style partition of algorithm.rs:44-81 over the keys and their
-    // records together; `pred` looks at a record.
-    template <typename Pred>
-    static uint32_t hoare_partition(uint32_t* s, ObjInfo* r, uint32_t n, Pred&& pred) {
-        uint32_t a = 0, b = n;
-        for (;;) {
-            while (true) {
-                if (a == b) return a;
-                if (!pred(r[a])) break;
-                a++;
-            }
-            while (true) {
-                b--;
-                if (a == b) return a;
-                if (pred(r[b])) break;
-            }
-            std::swap(s[a], s[b]);
-            std::swap(r[a], r[b]);
-            a++;
-        }
-    }
-
     // Per-bin boxes and counts of one axis: an object goes to the first bin
     // k whose divider it does not exceed (c <= divs[k]), or to the last.
     struct Bins {
@@ -273,10 +221,7 @@ class Builder {
         for (uint32_t i = lo; i < hi; i++) {
             const ObjInfo& o = recs[i];
             for (int d = 0; d < 3; d++) {
-                const float c = comp(o.centroid, d);
-                uint32_t k = SAH_BINS - 1;
-                for (int j = (int)SAH_BINS - 2; j >= 0; j--)
-                    if (c <= divs[d][j]) k = (uint32_t)j;
+                const uint32_t k = sah_bin(comp(o.centroid, d), divs[d]);
                 b.box[d][k] = box_union(b.box[d][k], o.mid);
                 b.count[d][k]++;
             }
@@ -289,15 +234,10 @@ class Builder {
         for (uint32_t i = 0; i < n; i++) cb = box_union_pt(cb, recs[i].mid_center);
 
         float divs[3][SAH_BINS - 1];
-        for (int d = 0; d < 3; d++) {
-            float lo = comp(cb.lo, d), extent = comp(cb.hi, d) - lo;
-            for (uint32_t k = 0; k < SAH_BINS - 1; k++) divs[d][k] = lo + extent * ((float)(k + 1) / (float)SAH_BINS);
-        }
+        sah_divs(cb, divs);
 
-        Box left[3][SAH_BINS - 1], right[3][SAH_BINS - 1];
-        uint32_t nl[3][SAH_BINS - 1] = {}, nr[3][SAH_BINS - 1] = {};
-        for (int d = 0; d < 3; d++)
-            for (uint32_t k = 0; k < SAH_BINS - 1; k++) left[d][k] = right[d][k] = box_empty();
+        int best_d = 0;
+        float best_div = 0.0f;
         if (order_free_) {
             // Bin once per object, split large nodes over threads, then
             // combine the bins into the candidates' left and right sides.
@@ -321,25 +261,14 @@ class Builder {
             } else {
                 bin_range(recs, 0, n, divs, bins);
             }
-            for (int d = 0; d < 3; d++) {
-                Box acc = box_empty();
-                uint32_t cnt = 0;
-                for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
-                    acc = box_union(acc, bins.box[d][k]);
-                    cnt += bins.count[d][k];
-                    left[d][k] = acc;
-                    nl[d][k] = cnt;
-                }
-                acc = box_empty();
-                cnt = 0;
-                for (int k = (int)SAH_BINS - 2; k >= 0; k--) {
-                    acc = box_union(acc, bins.box[d][k + 1]);
-                    cnt += bins.count[d][k + 1];
-                    right[d][k] = acc;
-                    nr[d][k] = cnt;
-                }
-            }
+            sah_choose(bins.box, bins.count, divs, best_d, best_div);
         } else {
+            // The reference's accumulation, object by object into every
+            // candidate's side, for boxes whose unions depend on order.
+            Box left[3][SAH_BINS - 1], right[3][SAH_BINS - 1];
+            uint32_t nl[3][SAH_BINS - 1] = {}, nr[3][SAH_BINS - 1] = {};
+            for (int d = 0; d < 3; d++)
+                for (uint32_t k = 0; k < SAH_BINS - 1; k++) left[d][k] = right[d][k] = box_empty();
             for (uint32_t i = 0; i < n; i++) {
                 const ObjInfo& o = recs[i];
                 for (int d = 0; d < 3; d++) {
@@ -355,122 +284,101 @@ class Builder {
                     }
                 }
             }
+            float best = __builtin_huge_valf();
+            for (int d = 0; d < 3; d++)
+                for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
+                    float lc = box_area(left[d][k]) * (float)nl[d][k];
+                    float rc = box_area(right[d][k]) * (float)nr[d][k];
+                    float cost = (lc * box_diag(left[d][k])) + (rc * box_diag(right[d][k]));
+                    if (cost < best) {
+                        best_d = d;
+                        best_div = divs[d][k];
+                        best = cost;
+                    }
+                }
         }
 
-        int best_d = 0;
-        float best_div = 0.0f;
-        float best = __builtin_huge_valf();
-        for (int d = 0; d < 3; d++)
-            for (uint32_t k = 0; k < SAH_BINS - 1; k++) {
-                float lc = box_area(left[d][k]) * (float)nl[d][k];
-                float rc = box_area(right[d][k]) * (float)nr[d][k];
-                float cost = (lc * box_diag(left[d][k])) + (rc * box_diag(right[d][k]));
-                if (cost < best) {
-                    best_d = d;
-                    best_div = divs[d][k];
-                    best = cost;
-                }
-            }
-
-        uint32_t s = hoare_partition(keys, recs, n, [&](const ObjInfo& o) { return comp(o.centroid, best_d) < best_div; });
+        uint32_t s = hoare_partition(keys, recs, n, [&](const ObjInfo& o) { return sah_left(o, best_d, best_div); });
         if (s < 1) s = 1;
         else if (s >= n) s = n - 1;
         split = s;
         axis = (uint8_t)best_d;
     }
+};
 
-    static uint64_t hash64(uint64_t n, uint64_t seed) {  // src/hash.rs:12-21
-        uint64_t h = n;
-        for (int i = 0; i < 4; i++) {
-            h *= 0x8d879e7eb3c956e9ull;  // 32_416_190_071 * 314_604_959
-            h ^= h >> 32;
-            h ^= seed;
-        }
-        return h;
-    }
+// Node access of the collapse for the host builder's BVH2 (BaseNode tree).
+struct BaseTreeView {
+    using H = const BaseNode*;
+    bool leaf(H n) const { return n->leaf; }
+    H kid(H n, int i) const { return n->kid[i].get(); }
+    uint32_t axis(H n) const { return n->axis; }
+    uint32_t obj_start(H n) const { return n->obj_start; }
+    uint32_t obj_end(H n) const { return n->obj_end; }
+    uint32_t blen(H n) const { return (uint32_t)n->bounds.size(); }
+    const Box* bounds(H n) const { return n->bounds.data(); }
+};
 
-    // median_split (objects_split.rs:291-341) with quick_select
-    // (algorithm.rs:203-231).
-    void median_split(uint32_t* keys, ObjInfo* recs, uint32_t n, uint32_t& split, uint8_t& axis_out) {
-        Box b = box_empty();
-        for (uint32_t i = 0; i < n; i++) b = box_union(b, recs[i].mid);
-        int axis = 0;
-        float largest = -__builtin_huge_valf();
-        for (int d = 0; d < 3; d++) {
-            float e = comp(b.hi, d) - comp(b.lo, d);
-            if (e > largest) { largest = e; axis = d; }
-        }
-        uint32_t place = n / 2;
-        if (place == 0) place = 1;
-
-        auto key_of = [&](const ObjInfo& o) { return comp(o.centroid, axis); };
-        uint32_t left = 0, right = n;
-        uint64_t seed = place;
-        for (;;) {
-            uint32_t i = left + (uint32_t)(hash64(right, seed) % (uint64_t)(right - left));
-            std::swap(keys[i], keys[right - 1]);
-            std::swap(recs[i], recs[right - 1]);
-            float pivot = key_of(recs[right - 1]);
-            uint32_t ii = left + hoare_partition(keys + left, recs + left, right - 1 - left,
-                                                 [&](const ObjInfo& o) { return key_of(o) < pivot; });
-            std::swap(keys[ii], keys[right - 1]);
-            std::swap(recs[ii], recs[right - 1]);
-            if (ii == place) break;
-            if (ii > place) right = ii; else left = ii + 1;
-            seed += 1;
-        }
-        split = place;
-        axis_out = (uint8_t)axis;
-    }
+// Node access of the collapse for a BVH2 in array form.
+struct ArrayTreeView {
+    using H = uint32_t;
+    const ArrayBvh2& t;
+    bool leaf(H n) const { return (t.meta[n] & ArrayBvh2::LEAF) != 0; }
+    H kid(H n, int i) const { return i == 0 ? t.a[n] : t.b[n]; }
+    uint32_t axis(H n) const { return t.meta[n] & 3u; }
+    uint32_t obj_start(H n) const { return t.a[n]; }
+    uint32_t obj_end(H n) const { return t.b[n]; }
+    uint32_t blen(H n) const { return t.blen[n]; }
+    const Box* bounds(H n) const { return t.bounds.data() + (uint64_t)n * t.stride; }
 };
 
 // BVH4 collapse (bvh4.rs:186-358) into GPU records.
+template <typename View>
 class Collapser {
    public:
-    explicit Collapser(HostBvh4& out) : out_(out), table_(traversal_order_table()) {}
+    using H = typename View::H;
+    Collapser(HostBvh4& out, const View& v) : out_(out), v_(v), table_(traversal_order_table()) {}
 
     // Returns the reference for `n`: a leaf reference or the node index.
-    uint32_t emit(const BaseNode* n, uint32_t slot) {
-        if (n->leaf) {
+    uint32_t emit(H n, uint32_t slot) {
+        if (v_.leaf(n)) {
             out_.node_count += 1;
             out_.leaf_count += 1;
-            return make_leaf_ref(n->obj_start, n->obj_end - n->obj_start);
+            return make_leaf_ref(v_.obj_start(n), v_.obj_end(n) - v_.obj_start(n));
         }
-        const BaseNode* l = n->kid[0].get();
-        const BaseNode* r = n->kid[1].get();
-        const BaseNode* ch[4] = {nullptr, nullptr, nullptr, nullptr};
+        const H l = v_.kid(n, 0);
+        const H r = v_.kid(n, 1);
+        H ch[4] = {};
         uint32_t cnt;
         uint32_t code;
-        if (!l->leaf && !r->leaf) {
-            ch[0] = l->kid[0].get(); ch[1] = l->kid[1].get(); ch[2] = r->kid[0].get(); ch[3] = r->kid[1].get();
+        if (!v_.leaf(l) && !v_.leaf(r)) {
+            ch[0] = v_.kid(l, 0); ch[1] = v_.kid(l, 1); ch[2] = v_.kid(r, 0); ch[3] = v_.kid(r, 1);
             cnt = 4;
-            code = n->axis + l->axis * 3u + r->axis * 9u;
-        } else if (!l->leaf) {
-            ch[0] = l->kid[0].get(); ch[1] = l->kid[1].get(); ch[2] = r;
+            code = v_.axis(n) + v_.axis(l) * 3u + v_.axis(r) * 9u;
+        } else if (!v_.leaf(l)) {
+            ch[0] = v_.kid(l, 0); ch[1] = v_.kid(l, 1); ch[2] = r;
             cnt = 3;
-            code = n->axis + l->axis * 3u + 27u;
-        } else if (!r->leaf) {
-            ch[0] = l; ch[1] = r->kid[0].get(); ch[2] = r->kid[1].get();
+            code = v_.axis(n) + v_.axis(l) * 3u + 27u;
+        } else if (!v_.leaf(r)) {
+            ch[0] = l; ch[1] = v_.kid(r, 0); ch[2] = v_.kid(r, 1);
             cnt = 3;
-            code = n->axis + r->axis * 3u + 36u;
+            code = v_.axis(n) + v_.axis(r) * 3u + 36u;
         } else {
             ch[0] = l; ch[1] = r;
             cnt = 2;
-            code = n->axis + 45u;
+            code = v_.axis(n) + 45u;
         }
         out_.node_count += cnt;
 
         uint32_t blen = 0;
-        for (int i = 0; i < 4; i++)
-            if (ch[i]) blen = std::max(blen, (uint32_t)ch[i]->bounds.size());
-        std::vector<GpuBox4> samples(blen);
+        for (uint32_t i = 0; i < cnt; i++) blen = std::max(blen, v_.blen(ch[i]));
+        GpuBox4 samples[64];  // bounds keep at most 64 time samples
         for (uint32_t s = 0; s < blen; s++) {
             float time = blen < 2 ? 0.0f : (float)s / (float)(blen - 1);
-            for (int i = 0; i < 4; i++) {
+            for (uint32_t i = 0; i < 4; i++) {
                 Box b = box_empty();
-                if (ch[i]) {
-                    if (blen < 2) b = ch[i]->bounds[0];
-                    else b = lerp_slice_box(ch[i]->bounds.data(), (uint32_t)ch[i]->bounds.size(), time);
+                if (i < cnt) {
+                    if (blen < 2) b = v_.bounds(ch[i])[0];
+                    else b = lerp_slice_box(v_.bounds(ch[i]), v_.blen(ch[i]), time);
                 }
                 samples[s].lo_x[i] = b.lo.x; samples[s].hi_x[i] = b.hi.x;
                 samples[s].lo_y[i] = b.lo.y; samples[s].hi_y[i] = b.hi.y;
@@ -481,13 +389,13 @@ class Collapser {
         // Children get a contiguous block of slots, like the reference's
         // per-node child array; leaves occupy no slot.
         uint32_t internal = 0;
-        for (uint32_t i = 0; i < cnt; i++) internal += ch[i]->leaf ? 0 : 1;
+        for (uint32_t i = 0; i < cnt; i++) internal += v_.leaf(ch[i]) ? 0 : 1;
         uint32_t base = (uint32_t)out_.nodes.size();
         out_.nodes.resize(out_.nodes.size() + internal);
         uint32_t refs[4] = {REF_EMPTY, REF_EMPTY, REF_EMPTY, REF_EMPTY};
         uint32_t next = base;
         for (uint32_t i = 0; i < cnt; i++) {
-            if (ch[i]->leaf) refs[i] = emit(ch[i], 0);
+            if (v_.leaf(ch[i])) refs[i] = emit(ch[i], 0);
             else { refs[i] = emit(ch[i], next); next++; }
         }
 
@@ -500,15 +408,27 @@ class Collapser {
         g.motion = REF_EMPTY;
         if (blen > 1) {
             g.motion = (uint32_t)out_.motion.size();
-            out_.motion.insert(out_.motion.end(), samples.begin(), samples.end());
+            out_.motion.insert(out_.motion.end(), samples, samples + blen);
         }
         return slot;
     }
 
    private:
     HostBvh4& out_;
+    const View& v_;
     const uint8_t* table_;
 };
+
+template <typename View>
+void collapse_into(HostBvh4& out, const View& v, typename View::H root) {
+    Collapser<View> c(out, v);
+    if (v.leaf(root)) {
+        out.root = c.emit(root, 0);
+    } else {
+        out.nodes.resize(1);
+        out.root = c.emit(root, 0);
+    }
+}
 
 void dump_ref(const HostBvh4& bvh, uint32_t ref, std::vector<uint32_t>& out) {
     if (ref & REF_LEAF) {
@@ -556,13 +476,18 @@ HostBvh4 build_bvh4(const BuildInput& in, uint32_t per_leaf, unsigned threads) {
     out.depth = base_depth / 2 + 1;
     out.root_bounds = root->bounds;
 
-    Collapser c(out);
-    if (root->leaf) {
-        out.root = c.emit(root.get(), 0);
-    } else {
-        out.nodes.resize(1);
-        out.root = c.emit(root.get(), 0);
-    }
+    collapse_into(out, BaseTreeView{}, root.get());
+    return out;
+}
+
+HostBvh4 collapse_bvh2(const ArrayBvh2& t, std::vector<uint32_t>&& order, uint32_t base_depth) {
+    HostBvh4 out;
+    out.order = std::move(order);
+    if (out.order.empty()) return out;
+    out.depth = base_depth / 2 + 1;
+    const Box* rb = t.bounds.data() + (uint64_t)t.root * t.stride;
+    out.root_bounds.assign(rb, rb + t.blen[t.root]);
+    collapse_into(out, ArrayTreeView{t}, t.root);
     return out;
 }
 

@@ -213,6 +213,26 @@ std::unique_ptr<LtNode> lt_build(const BuildInput& in, const std::vector<float>&
     return node;
 }
 
+// The same tree from a partition the device made (build_light_bvh2_device):
+// node `id` of `t`, whose leaves hold the lights keys[a] at positions a.
+std::unique_ptr<LtNode> lt_from_topology(const ArrayBvh2& t, uint32_t id, const BuildInput& in,
+                                         const std::vector<float>& energy, const uint32_t* keys) {
+    auto node = std::make_unique<LtNode>();
+    if (t.meta[id] & ArrayBvh2::LEAF) {
+        const uint32_t a = t.a[id];
+        node->leaf = true;
+        node->bounds.assign(in.bounds + in.offsets[keys[a]], in.bounds + in.offsets[keys[a] + 1]);
+        node->energy = energy[keys[a]];
+        node->light = a;
+        return node;
+    }
+    node->kid[0] = lt_from_topology(t, t.a[id], in, energy, keys);
+    node->kid[1] = lt_from_topology(t, t.b[id], in, energy, keys);
+    node->bounds = merge_union(node->kid[0]->bounds, node->kid[1]->bounds);
+    node->energy = node->kid[0]->energy + node->kid[1]->energy;
+    return node;
+}
+
 // The collapse to arity 8 (light_tree.rs:292-372): a node's children are its
 // descendants three levels down, or the leaves met before that, left to
 // right.
@@ -411,7 +431,7 @@ float object_energy(const psy_scene_desc* d, const psy_instance_desc& id) {
 }  // namespace
 
 // Packs a single-level description (the flattened scene `f`).
-static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::string& err) {
+static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::string& err, const BuildOptions& opt) {
     auto t0 = std::chrono::steady_clock::now();
     if (!d) { err = "scene description is NULL"; return PSY_E_INVALID; }
     if (d->mesh_count > 0 && !d->meshes) { err = "meshes is NULL"; return PSY_E_INVALID; }
@@ -505,7 +525,17 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
         }
         offs[T] = T * S;
         BuildInput bi{T, bounds.data(), offs.data()};
-        s.bvhs[m] = build_bvh4(bi, MESH_LEAF_TRIS, 0);
+        bool built = false;
+        if (opt.device && T >= opt.device_min_tris) {
+            std::string why;
+            double ms = 0;
+            built = build_bvh4_device(bi, MESH_LEAF_TRIS, s.bvhs[m], why, &ms);
+            if (built) {
+                s.device_built++;
+                s.device_build_ms += ms;
+            }
+        }
+        if (!built) s.bvhs[m] = build_bvh4(bi, MESH_LEAF_TRIS, 0);
         const HostBvh4& bvh = s.bvhs[m];
 
         GpuMesh gm{};
@@ -771,7 +801,19 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
         std::vector<uint32_t> keys(ents.size());
         for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) keys[i] = i;
         BuildInput lin{(uint32_t)keys.size(), lb.data(), lo.data()};
-        gtree[g] = lt_build(lin, le, keys.data(), (uint32_t)keys.size(), 0);
+        bool built = false;
+        if (opt.device && keys.size() >= std::max(2u, opt.device_min_tris)) {
+            ArrayBvh2 topo;
+            std::vector<uint32_t> order;
+            std::string why;
+            if (build_light_bvh2_device(lin, topo, order, why)) {
+                keys = order;
+                gtree[g] = lt_from_topology(topo, 0, lin, le, keys.data());
+                s.device_light_trees++;
+                built = true;
+            }
+        }
+        if (!built) gtree[g] = lt_build(lin, le, keys.data(), (uint32_t)keys.size(), 0);
         genergy[g] = gtree[g]->energy;
         gents[g].resize(keys.size());
         for (uint32_t i = 0; i < (uint32_t)keys.size(); i++) gents[g][i] = ents[keys[i]];
@@ -863,7 +905,7 @@ static int pack_flat(const psy_scene_desc* d, Flat& f, PackedScene& s, std::stri
     return PSY_OK;
 }
 
-int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
+int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err, const BuildOptions& opt) {
     if (!d) { err = "scene description is NULL"; return PSY_E_INVALID; }
     Flat f;
     int rc = place_defs(d, f, 0, err);
@@ -881,7 +923,7 @@ int pack_scene(const psy_scene_desc* d, PackedScene& s, std::string& err) {
     f.desc.rect_lights = f.rects.data();
     f.desc.sphere_light_count = (uint32_t)f.spheres.size();
     f.desc.sphere_lights = f.spheres.data();
-    return pack_flat(&f.desc, f, s, err);
+    return pack_flat(&f.desc, f, s, err, opt);
 }
 
 }  // namespace psy

@@ -51,11 +51,23 @@ struct PackedScene {
     uint64_t leaf_count = 0;
     uint64_t tri_count = 0;
     double build_seconds = 0;
+    uint32_t device_built = 0;      // mesh BVHs built by the device builder
+    uint32_t device_light_trees = 0;  // light trees partitioned on the device
+    double device_build_ms = 0;     // their build time (uploads and downloads included)
+};
+
+// Where the mesh BVHs are built: on the host, or on the current HIP device
+// for meshes of at least `device_min_tris` triangles (the device builder
+// builds the host builder's trees bit for bit; a mesh it cannot take, or a
+// HIP error, falls back to the host builder).
+struct BuildOptions {
+    bool device = false;
+    uint32_t device_min_tris = 0;
 };
 
 // Validates `desc` and builds every BVH.  Returns PSY_OK or an error code
 // with a message in `err`.
-int pack_scene(const psy_scene_desc* desc, PackedScene& out, std::string& err);
+int pack_scene(const psy_scene_desc* desc, PackedScene& out, std::string& err, const BuildOptions& opt = {});
 
 // TransformStack::push (src/transform_stack.rs:33-60): the parent's and an
 // instance's time samples combined by merge_slices_to (src/algorithm.rs:

@@ -152,8 +152,11 @@ class Assembly:
         self.instances.append(Instance(mesh, xforms, shader, kind))
         return len(self.instances) - 1
 
-    def build(self) -> "Scene":
-        return Scene(self)
+    def build(self, device: bool = False, device_min_tris: int = 0) -> "Scene":
+        """The scene with its BVHs built on the host, or (device=True) its
+        mesh BVHs of at least `device_min_tris` triangles built on the
+        current GPU (psy_scene_create_on_device): the same trees."""
+        return Scene(self, device=device, device_min_tris=device_min_tris)
 
 
 def _scene_desc(asm: Assembly, keep: list) -> _lib.SceneDesc:
@@ -214,11 +217,15 @@ def _scene_desc(asm: Assembly, keep: list) -> _lib.SceneDesc:
 class Scene:
     """The assembly uploaded to the GPU (psy_scene_create)."""
 
-    def __init__(self, asm: Assembly):
+    def __init__(self, asm: Assembly, device: bool = False, device_min_tris: int = 0):
         self._keep = []
         desc = _scene_desc(asm, self._keep)
         h = C.c_void_p()
-        check("psy_scene_create", lib().psy_scene_create(C.byref(desc), C.byref(h)))
+        if device:
+            check("psy_scene_create_on_device",
+                  lib().psy_scene_create_on_device(C.byref(desc), device_min_tris, C.byref(h)))
+        else:
+            check("psy_scene_create", lib().psy_scene_create(C.byref(desc), C.byref(h)))
         self.handle = h
         self._keep = None
         self.mesh_count = len(asm.meshes)

@@ -30,6 +30,17 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 }  // namespace api
+
+// This build holds the library's host code only: the device builder is not
+// linked, and the scenes built here never ask for it.
+bool build_bvh4_device(const BuildInput&, uint32_t, HostBvh4&, std::string& why, double*) {
+    why = "host-only build";
+    return false;
+}
+bool build_light_bvh2_device(const BuildInput&, ArrayBvh2&, std::vector<uint32_t>&, std::string& why) {
+    why = "host-only build";
+    return false;
+}
 }  // namespace psy
 
 namespace {
