@@ -153,21 +153,45 @@ int psy_renderer_set_tuning(psy_renderer* renderer, const psy_trace_tuning* tuni
 #define PSY_RENDER_NO_GRAPHS 1u
 #define PSY_RENDER_NO_SORT 2u
 #define PSY_RENDER_PLAIN 4u
+/* Measurement options of the wavefront form (no counterpart in the
+ * reference): PSY_RENDER_PROFILE launches every wave directly with HIP
+ * events around each kernel and tallies per-stage device time and ray
+ * counts (psy_renderer_get_profile); PSY_RENDER_COUNT traces with the
+ * instrumented traversal kernel, tallying node visits and triangle tests
+ * (slower; images are unchanged). */
+#define PSY_RENDER_PROFILE 16u
+#define PSY_RENDER_COUNT 32u
 #define PSY_RENDER_SHADE_WAVES_SHIFT 8u
 #define PSY_RENDER_SHADE_WAVES_MASK (7u << PSY_RENDER_SHADE_WAVES_SHIFT)
 #define PSY_RENDER_SHADE_WAVES(n) ((uint32_t)(n) << PSY_RENDER_SHADE_WAVES_SHIFT)
 int psy_renderer_set_options(psy_renderer* renderer, uint32_t flags);
 
+/* Tallies of the wavefront renders since the last call, with
+ * PSY_RENDER_PROFILE (stage times, paths, rays) and PSY_RENDER_COUNT
+ * (traversal counters of closest-hit and shadow rays); the call resets
+ * them.  Stage times are device milliseconds between HIP events around each
+ * kernel: camera_kernel, the closest-hit and shadow traversals, shadow
+ * results, the material sort, shade_kernel, and the film kernels. */
+typedef struct {
+    double camera_ms, trace_ms, shadow_trace_ms, shadow_ms, sort_ms, shade_ms, film_ms;
+    uint64_t paths, closest_rays, shadow_rays;
+    uint64_t closest_node_visits, closest_tri_tests, shadow_node_visits, shadow_tri_tests;
+    uint64_t motion_node_visits, motion_tri_tests, instance_visits;
+} psy_render_profile;
+int psy_renderer_get_profile(psy_renderer* renderer, psy_render_profile* profile);
+
 /* Autotuner: renders `trial` (a reduced version of the intended renders)
  * with every candidate launch shape of the traversal kernel (waves per CU,
  * refill threshold, LDS stack depth, register budget), then its descent
  * threshold (leaf_min), then every shading register budget,
- * then with and without material sorting, then the one-launch form, keeps
- * the fastest on the renderer and returns it (either output may be NULL;
- * `best_flags` holds PSY_RENDER_NO_SORT if sorting lost, PSY_RENDER_PLAIN if
- * the one-launch form won, and PSY_RENDER_SHADE_WAVES(n) of the chosen
- * budget).  Images do not depend on the choice; the request server always
- * batches in waves. */
+ * then with and without material sorting, then the one-launch form with each
+ * of its register budgets, keeps the fastest on the renderer and returns it
+ * (either output may be NULL; `best_flags` holds PSY_RENDER_NO_SORT if
+ * sorting lost, PSY_RENDER_PLAIN if the one-launch form won, and
+ * PSY_RENDER_SHADE_WAVES(n) of the chosen form's budget).  The renderer keeps
+ * the best budget of each form, so a later set_options switching form without
+ * a budget uses it.  Images do not depend on the choice; the request server
+ * always batches in waves. */
 int psy_renderer_autotune(psy_renderer* renderer, const psy_camera_desc* camera, const psy_render_settings* trial,
                           psy_trace_tuning* best_tuning, uint32_t* best_flags);
 

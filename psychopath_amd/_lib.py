@@ -183,6 +183,14 @@ class SceneInfo(C.Structure):
     ]
 
 
+class RenderProfile(C.Structure):
+    _fields_ = [(f, C.c_double) for f in ("camera_ms", "trace_ms", "shadow_trace_ms", "shadow_ms", "sort_ms",
+                                           "shade_ms", "film_ms")] + \
+               [(f, C.c_uint64) for f in ("paths", "closest_rays", "shadow_rays", "closest_node_visits",
+                                           "closest_tri_tests", "shadow_node_visits", "shadow_tri_tests",
+                                           "motion_node_visits", "motion_tri_tests", "instance_visits")]
+
+
 class TraceCounters(C.Structure):
     _fields_ = [
         ("node_visits", C.c_uint64),
@@ -250,6 +258,7 @@ _SIGNATURES = {
     "psy_renderer_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
     "psy_renderer_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(TraceTuning)]),
     "psy_renderer_set_options": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "psy_renderer_get_profile": (C.c_int, [C.c_void_p, C.POINTER(RenderProfile)]),
     "psy_renderer_autotune": (C.c_int, [C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderSettings),
                                         C.POINTER(TraceTuning), C.POINTER(C.c_uint32)]),
 }
@@ -257,6 +266,8 @@ _SIGNATURES = {
 PSY_RENDER_NO_GRAPHS = 1
 PSY_RENDER_NO_SORT = 2
 PSY_RENDER_PLAIN = 4
+PSY_RENDER_PROFILE = 16
+PSY_RENDER_COUNT = 32
 PSY_RENDER_SHADE_WAVES_SHIFT = 8
 
 

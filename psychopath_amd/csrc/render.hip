@@ -281,8 +281,11 @@ struct Isect {
     uint32_t shader;
 };
 
-__device__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float t, float b0, float b1, float b2,
-                            uint32_t prim, V3 orig, V3 dir, float time) {
+// The shading helpers below are inlined into every kernel build: a helper
+// compiled as a call keeps one register allocation for all callers, so the
+// kernels' waves-per-SIMD budgets would not apply to it.
+__device__ __forceinline__ Isect make_isect(const ShadeScene& sc, const GpuInstance& inst, float t, float b0,
+                                            float b1, float b2, uint32_t prim, V3 orig, V3 dir, float time) {
     Isect is;
     is.light = inst.kind != PSY_OBJECT_MESH;
     is.shader = inst.shader;
@@ -381,15 +384,19 @@ __device__ __forceinline__ Xf light_xf_at(const GpuXform* l, uint32_t n, float t
 // LightTree::select (src/accel/light_tree.rs:135-201) in the tree rooted at
 // `node`, for shading data already in that tree's space; returns the leaf
 // node, with the selection probability and the whittled number.
-__device__ uint32_t light_tree_descend(const ShadeScene& sc, uint32_t node, V3 inc, V3 pos, V3 nor, V3 nor_g,
-                                       const GpuClosure& clo, float time, float& n, float& prob) {
+__device__ __forceinline__ uint32_t light_tree_descend(const ShadeScene& sc, uint32_t node, V3 inc, V3 pos, V3 nor,
+                                                       V3 nor_g, const GpuClosure& clo, float time, float& n,
+                                                       float& prob) {
     float tot_prob = 1.0f;
     while (sc.lnodes[node].child_count > 0) {
         const GpuLightNode nd = sc.lnodes[node];
         const uint32_t cnt = nd.child_count;
+        // The children's probabilities, indexed only by constants (unrolled
+        // loops) so they stay in registers.
         float ps[8];
         float total = 0.0f;
-        for (uint32_t i = 0; i < 8; i++) ps[i] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) ps[k] = 0.0f;
         for (uint32_t i = 0; i < cnt; i++) {
             const GpuLightNode ch = sc.lnodes[nd.child_first + i];
             // node_prob: energy / bounding-sphere area * closure estimate
@@ -407,26 +414,36 @@ __device__ uint32_t light_tree_descend(const ShadeScene& sc, uint32_t node, V3 i
             const float inv_surface_area = 1.0f / r2;
             const float approx = closure_estimate_sphere_light(clo, inc, d, r2, nor, nor_g);
             const float p = (ch.energy * inv_surface_area) * approx;
-            ps[i] = p;
+#pragma unroll
+            for (int k = 0; k < 8; k++) ps[k] = (uint32_t)k == i ? p : ps[k];
             total += p;
         }
         if (total <= 0.0f) {
             const float p = 1.0f / (float)cnt;
-            for (uint32_t i = 0; i < 8; i++) ps[i] = p;
+#pragma unroll
+            for (int k = 0; k < 8; k++) ps[k] = p;
         } else {
-            for (uint32_t i = 0; i < 8; i++) ps[i] /= total;
+#pragma unroll
+            for (int k = 0; k < 8; k++) ps[k] /= total;
         }
         float base = 0.0f;
-        for (uint32_t i = 0; i < 8; i++) {
-            const float p = ps[i];
-            if ((n <= base + p) || (i == cnt - 1)) {
-                tot_prob *= p;
-                node = nd.child_first + i;
-                n = (n - base) / p;
-                break;
+        bool picked = false;
+        uint32_t next = node;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (!picked) {
+                const float p = ps[k];
+                if ((n <= base + p) || ((uint32_t)k == cnt - 1)) {
+                    tot_prob *= p;
+                    next = nd.child_first + (uint32_t)k;
+                    n = (n - base) / p;
+                    picked = true;
+                } else {
+                    base += p;
+                }
             }
-            base += p;
         }
+        node = next;
     }
     prob = tot_prob;
     return node;
@@ -439,23 +456,32 @@ __device__ uint32_t light_tree_descend(const ShadeScene& sc, uint32_t node, V3 i
 // tree with the whittled number.  The reference multiplies the levels'
 // probabilities as its recursion returns, innermost first.  Returns the
 // light leaf.
-__device__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, V3 nor_g, const GpuClosure& clo,
-                                  float time, float n, uint32_t& leaf, float& prob) {
+__device__ __forceinline__ bool light_tree_select(const ShadeScene& sc, V3 inc, V3 pos, V3 nor, V3 nor_g,
+                                                  const GpuClosure& clo, float time, float n, uint32_t& leaf,
+                                                  float& prob) {
     if (!sc.has_ltree) return false;
     Xf sel = xf_identity();
     uint32_t node = 0;
+    // Each level's probability, indexed only by constants (registers).
     float pl[MAX_LIGHT_LEVELS];
+#pragma unroll
+    for (int k = 0; k < (int)MAX_LIGHT_LEVELS; k++) pl[k] = 0.0f;
     for (uint32_t level = 0; level < MAX_LIGHT_LEVELS; level++) {
+        float pcur;
         node = light_tree_descend(sc, node, xf_vector(sel, inc), xf_point(sel, pos), xf_normal(sel, nor),
-                                  xf_normal(sel, nor_g), clo, time, n, pl[level]);
+                                  xf_normal(sel, nor_g), clo, time, n, pcur);
         const GpuLightLeaf lf = sc.lleaves[sc.lnodes[node].light];
         if (lf.kind == LIGHT_LEAF_LIGHT) {
-            float p = pl[level];
-            for (uint32_t k = level; k-- > 0;) p = p * pl[k];
+            float p = pcur;
+#pragma unroll
+            for (int k = (int)MAX_LIGHT_LEVELS - 1; k >= 0; k--)
+                if ((uint32_t)k < level) p = p * pl[k];
             leaf = sc.lnodes[node].light;
             prob = p;
             return true;
         }
+#pragma unroll
+        for (int k = 0; k < (int)MAX_LIGHT_LEVELS; k++) pl[k] = (uint32_t)k == level ? pcur : pl[k];
         sel = lf.top_count ? light_xf_at(sc.lxforms + lf.top_start, lf.top_count, time) : xf_identity();
         node = lf.target;
     }
@@ -474,8 +500,8 @@ struct LightPick {
     float pdf, selection_pdf;
 };
 
-__device__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float v, float wl, float time,
-                                   const Isect& is, V3 incoming, const GpuClosure& clo) {
+__device__ __forceinline__ LightPick sample_lights(const ShadeScene& sc, float n, float u, float v, float wl,
+                                                   float time, const Isect& is, V3 incoming, const GpuClosure& clo) {
     LightPick out;
     out.found = false;
     float wl_sum = 0.0f;
@@ -553,8 +579,8 @@ struct ShadeOut {
 // LightPath::next for a camera or bounce ray (renderer.rs:459-667): adds
 // emission or background, or prepares the light ray (with its pending
 // contribution in ps.pending) and the next bounce ray.
-__device__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& ps, const HitRec& h, V3 orig, V3 dir,
-                           ShadeOut& out) {
+__device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& ps, const HitRec& h,
+                                           V3 orig, V3 dir, ShadeOut& out) {
     out.shadow = false;
     out.bounce = false;
     out.smax = 1.0f;
@@ -651,11 +677,12 @@ __device__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& 
 // The shading step over the closest-hit queue (in material order when
 // `order` is given); appends the light and bounce rays to the next queues.
 // MINW: minimum waves per SIMD the build is compiled for (its register
-// budget); every build computes the same thing.
+// budget: 1 and 2 allow 256 VGPRs, 3 about 168, 4 128); every build computes
+// the same thing.
 template <int MINW>
-__global__ __launch_bounds__(KBLOCK, MINW) void shade_kernel(ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa,
-                                                             Hits h, const uint32_t* order, Queue qa_out,
-                                                             Queue qs_out) {
+__global__ __launch_bounds__(KBLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void shade_kernel(
+    ShadeScene sc, const ReqParams* reqs, Paths P, Queue qa, Hits h, const uint32_t* order, Queue qa_out,
+    Queue qs_out) {
     const uint32_t n = *qa.count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(sc.rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -729,7 +756,8 @@ __host__ __device__ inline uint32_t mega_pixels_per_block(uint32_t spp) {
 }
 
 template <int MINW>
-__global__ __launch_bounds__(MEGA_BLOCK, MINW) void megakernel(TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
+__global__ __launch_bounds__(MEGA_BLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) void megakernel(
+    TraceArgs ta, ShadeScene sc, const ReqParams* reqs) {
     __shared__ uint32_t lds_stack[MEGA_LDS * MEGA_BLOCK];
     __shared__ float lds_xyz[3 * MEGA_BLOCK];
     const ReqParams rp = reqs[0];
@@ -984,55 +1012,121 @@ TraceArgs queue_args(const psy_renderer* r, const Queue& q, const Hits& h, bool 
     return a;
 }
 
+// Stages of the renderer's profile (psy_render_profile).
+enum { ST_CAMERA, ST_TRACE, ST_SHADOW_TRACE, ST_SHADOW, ST_SORT, ST_SHADE, ST_FILM };
+
+// Adds a device queue count to a 64-bit tally (profile mode).
+__global__ void accum_count_kernel(const uint32_t* count, unsigned long long* acc) { *acc += *count; }
+
+int pool_event(psy_renderer* r, hipEvent_t& e) {
+    if (r->ev_used == r->ev_pool.size()) {
+        hipEvent_t n = nullptr;
+        PSY_HIP(hipEventCreate(&n));
+        r->ev_pool.push_back(n);
+    }
+    e = r->ev_pool[r->ev_used++];
+    return PSY_OK;
+}
+
+// Runs `launch` (which enqueues one stage's kernels on the renderer's
+// stream); in profile mode between two HIP events kept for the stage.
+template <typename F>
+int staged(psy_renderer* r, int stage, F&& launch) {
+    if (!r->profile) return launch();
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = pool_event(r, a);
+    if (!rc) rc = pool_event(r, b);
+    if (rc) return rc;
+    PSY_HIP(hipEventRecord(a, r->stream));
+    rc = launch();
+    if (rc) return rc;
+    PSY_HIP(hipEventRecord(b, r->stream));
+    r->marks.push_back(psy_renderer::Mark{stage, a, b});
+    return PSY_OK;
+}
+
 // Enqueues the steps of one wave: camera, `steps` closest/shadow rounds, the
 // last shadow round and the film.  The wave's plan is plans[*wave_idx].
 int enqueue_wave(psy_renderer* r, uint32_t steps) {
     const hipStream_t s = r->stream;
     const uint32_t grid = grid_for(r->capacity);
+    const bool tally = r->profile || r->counted;
     Queue qa_in = r->qa[0], qa_out = r->qa[1], qs_in = r->qs[0], qs_out = r->qs[1];
-    hipLaunchKernelGGL(camera_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->sc, r->P,
-                       qa_in, qs_in.count);
-    PSY_HIP(hipGetLastError());
+    int rc = staged(r, ST_CAMERA, [&]() -> int {
+        hipLaunchKernelGGL(camera_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->sc, r->P,
+                           qa_in, qs_in.count);
+        PSY_HIP(hipGetLastError());
+        return PSY_OK;
+    });
+    if (rc) return rc;
     psy_trace_tuning tune = r->tuning;
     if (tune.kernel == PSY_KERNEL_SIMPLE) tune.kernel = PSY_KERNEL_PERSISTENT;
+    auto trace = [&](const Queue& q, const Hits& h, bool occl, unsigned int* work, int stage) -> int {
+        TraceArgs a = queue_args(r, q, h, occl);
+        a.counters = r->prof_dev + (occl ? 7 : 2);
+        int e2 = staged(r, stage, [&]() -> int {
+            hipError_t e = launch_tuned(r->scene, tune, r->cu_count, work, a, r->counted, s);
+            if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+            return PSY_OK;
+        });
+        if (e2 || !tally) return e2;
+        hipLaunchKernelGGL(accum_count_kernel, dim3(1), dim3(1), 0, s, q.count, r->prof_dev + (occl ? 1 : 0));
+        PSY_HIP(hipGetLastError());
+        return PSY_OK;
+    };
     for (uint32_t step = 0; step <= steps; step++) {
         const bool last = step == steps;
         if (!last) {
-            hipError_t e = launch_tuned(r->scene, tune, r->cu_count, r->work, queue_args(r, qa_in, r->ha, false), false, s);
-            if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
+            rc = trace(qa_in, r->ha, false, r->work, ST_TRACE);
+            if (rc) return rc;
         }
-        hipError_t e = launch_tuned(r->scene, tune, r->cu_count, r->work + 16, queue_args(r, qs_in, r->hs, true), false, s);
-        if (e != hipSuccess) return fail(PSY_E_DEVICE, std::string("trace launch: ") + hipGetErrorString(e));
-        hipLaunchKernelGGL(shadow_kernel, dim3(grid), dim3(KBLOCK), 0, s, qs_in, r->hs.kind, r->P, r->sc.rays);
-        PSY_HIP(hipGetLastError());
+        rc = trace(qs_in, r->hs, true, r->work + 16, ST_SHADOW_TRACE);
+        if (!rc) rc = staged(r, ST_SHADOW, [&]() -> int {
+            hipLaunchKernelGGL(shadow_kernel, dim3(grid), dim3(KBLOCK), 0, s, qs_in, r->hs.kind, r->P, r->sc.rays);
+            PSY_HIP(hipGetLastError());
+            return PSY_OK;
+        });
+        if (rc) return rc;
         if (last) break;
         hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, qa_out.count, qs_out.count, r->counts + 4);
         PSY_HIP(hipGetLastError());
         const uint32_t* order = nullptr;
         if (r->sort_materials) {
-            const uint32_t sgrid = std::min<uint32_t>(grid, 1024);
-            hipLaunchKernelGGL(sort_count_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
-                               r->counts + 4);
-            PSY_HIP(hipGetLastError());
-            hipLaunchKernelGGL(sort_scatter_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
-                               r->counts + 4, r->order);
-            PSY_HIP(hipGetLastError());
+            rc = staged(r, ST_SORT, [&]() -> int {
+                const uint32_t sgrid = std::min<uint32_t>(grid, 1024);
+                hipLaunchKernelGGL(sort_count_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+                                   r->counts + 4);
+                PSY_HIP(hipGetLastError());
+                hipLaunchKernelGGL(sort_scatter_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
+                                   r->counts + 4, r->order);
+                PSY_HIP(hipGetLastError());
+                return PSY_OK;
+            });
+            if (rc) return rc;
             order = r->order;
         }
         auto shade = r->shade_waves >= 4 ? shade_kernel<4>
                      : r->shade_waves == 3 ? shade_kernel<3>
                      : r->shade_waves == 2 ? shade_kernel<2>
                                            : shade_kernel<1>;
-        hipLaunchKernelGGL(shade, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->reqs, r->P, qa_in, r->ha, order, qa_out,
-                           qs_out);
-        PSY_HIP(hipGetLastError());
+        rc = staged(r, ST_SHADE, [&]() -> int {
+            hipLaunchKernelGGL(shade, dim3(grid), dim3(KBLOCK), 0, s, r->sc, r->reqs, r->P, qa_in, r->ha, order, qa_out,
+                               qs_out);
+            PSY_HIP(hipGetLastError());
+            return PSY_OK;
+        });
+        if (rc) return rc;
         std::swap(qa_in, qa_out);
         std::swap(qs_in, qs_out);
     }
-    hipLaunchKernelGGL(xyz_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->P);
-    PSY_HIP(hipGetLastError());
-    hipLaunchKernelGGL(film_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->P);
-    PSY_HIP(hipGetLastError());
+    rc = staged(r, ST_FILM, [&]() -> int {
+        hipLaunchKernelGGL(xyz_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->P);
+        PSY_HIP(hipGetLastError());
+        hipLaunchKernelGGL(film_kernel, dim3(grid), dim3(KBLOCK), 0, s, r->plans, r->wave_idx, r->reqs, r->P);
+        PSY_HIP(hipGetLastError());
+        return PSY_OK;
+    });
+    if (rc) return rc;
     hipLaunchKernelGGL(wave_advance_kernel, dim3(1), dim3(1), 0, s, r->wave_idx);
     PSY_HIP(hipGetLastError());
     return PSY_OK;
@@ -1138,7 +1232,23 @@ int make_params(const psy_camera_desc* cam, const psy_render_settings* st, ReqPa
 }
 
 int run_wave(psy_renderer* r, uint32_t steps) {
-    if (!r->use_graphs) return enqueue_wave(r, steps);
+    if (r->profile) {
+        // Direct launches, then the wave's stage times.
+        r->marks.clear();
+        r->ev_used = 0;
+        int rc = enqueue_wave(r, steps);
+        if (rc) return rc;
+        PSY_HIP(hipStreamSynchronize(r->stream));
+        for (const psy_renderer::Mark& m : r->marks) {
+            float ms = 0.0f;
+            PSY_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+            r->stage_ms[m.stage] += ms;
+        }
+        r->marks.clear();
+        r->ev_used = 0;
+        return PSY_OK;
+    }
+    if (!r->use_graphs || r->counted) return enqueue_wave(r, steps);
     auto it = r->graphs.find(steps);
     if (it == r->graphs.end()) {
         hipGraph_t g = nullptr;
@@ -1275,6 +1385,9 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!rc) rc = r->alloc(&r->counts, 16);
     if (!rc) rc = r->alloc(&r->wave_idx, 1);
     if (!rc) rc = r->alloc(&r->work, 64);
+    if (!rc) rc = r->alloc(&r->prof_dev, 16);
+    if (!rc && hipMemset(r->prof_dev, 0, 16 * sizeof(unsigned long long)) != hipSuccess)
+        rc = fail(PSY_E_DEVICE, "profile counters");
     if (!rc) rc = r->alloc(&r->reqs, MAX_SEGS);
     if (!rc) rc = r->alloc(&r->plans, 2);
     if (!rc) rc = r->alloc(&r->cam, (uint64_t)MAX_SEGS * CAM_SLOTS);
@@ -1417,6 +1530,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
         PSY_HIP(hipGetLastError());
     } else {
         for (uint64_t w = 0; w < waves && !rc; w++) rc = run_wave(r, st->max_bounces + 1);
+        if (r->profile || r->counted) r->prof_paths += total;
     }
     PSY_HIP(hipEventRecord(r->ev1, s));
     if (rc) {
@@ -1443,7 +1557,8 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
 
 int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     if (!r) return fail(PSY_E_INVALID, "renderer is NULL");
-    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT | PSY_RENDER_PLAIN | PSY_RENDER_SHADE_WAVES_MASK))
+    if (flags & ~(PSY_RENDER_NO_GRAPHS | PSY_RENDER_NO_SORT | PSY_RENDER_PLAIN | PSY_RENDER_SHADE_WAVES_MASK |
+                  PSY_RENDER_PROFILE | PSY_RENDER_COUNT))
         return fail(PSY_E_INVALID, "unknown option flag");
     const uint32_t sw = (flags & PSY_RENDER_SHADE_WAVES_MASK) >> PSY_RENDER_SHADE_WAVES_SHIFT;
     if (sw > 4) return fail(PSY_E_INVALID, "shading waves per SIMD must be 1 to 4");
@@ -1463,6 +1578,35 @@ int psy_renderer_set_options(psy_renderer* r, uint32_t flags) {
     }
     r->use_graphs = graphs;
     r->sort_materials = sort;
+    r->profile = (flags & PSY_RENDER_PROFILE) != 0;
+    r->counted = (flags & PSY_RENDER_COUNT) != 0;
+    return PSY_OK;
+}
+
+int psy_renderer_get_profile(psy_renderer* r, psy_render_profile* p) {
+    if (!r || !p) return fail(PSY_E_INVALID, "NULL argument");
+    unsigned long long c[16] = {};
+    PSY_HIP(hipStreamSynchronize(r->stream));
+    PSY_HIP(hipMemcpy(c, r->prof_dev, sizeof(c), hipMemcpyDeviceToHost));
+    PSY_HIP(hipMemset(r->prof_dev, 0, sizeof(c)));
+    std::memset(p, 0, sizeof(*p));
+    double* ms[7] = {&p->camera_ms, &p->trace_ms, &p->shadow_trace_ms, &p->shadow_ms, &p->sort_ms, &p->shade_ms,
+                     &p->film_ms};
+    for (int i = 0; i < 7; i++) {
+        *ms[i] = r->stage_ms[i];
+        r->stage_ms[i] = 0.0;
+    }
+    p->paths = r->prof_paths;
+    r->prof_paths = 0;
+    p->closest_rays = c[0];
+    p->shadow_rays = c[1];
+    p->closest_node_visits = c[2];
+    p->closest_tri_tests = c[4];
+    p->shadow_node_visits = c[7];
+    p->shadow_tri_tests = c[9];
+    p->motion_node_visits = c[3] + c[8];
+    p->motion_tri_tests = c[5] + c[10];
+    p->instance_visits = c[6] + c[11];
     return PSY_OK;
 }
 
@@ -1547,33 +1691,39 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
         rc = time_trial(ms);
         if (!rc) consider(ms);
     }
-    // Stage 4: the one-launch form, which wins on small closed scenes where
-    // path state never needs to leave registers.
-    bool best_plain = false;
-    if (!rc) {
+    // Stage 4: the one-launch form with each of its register budgets, which
+    // wins on small closed scenes where path state never needs to leave
+    // registers.
+    const uint32_t saved_pw = r->plain_waves;
+    uint32_t best_pw = saved_pw;
+    float best_plain_ms = 1e30f;
+    for (uint32_t pw = 1; pw <= 4 && !rc; pw++) {
         r->tuning = best_t;
         r->shade_waves = best_sw;
         r->sort_materials = best_sort;
         r->plain = true;
+        r->plain_waves = pw;
         float ms;
         rc = time_trial(ms);
         r->plain = false;
-        if (!rc && ms < best_ms) {
-            best_ms = ms;
-            best_plain = true;
+        if (!rc && ms < best_plain_ms) {
+            best_plain_ms = ms;
+            best_pw = pw;
         }
     }
+    const bool best_plain = best_plain_ms < best_ms;
     r->release(film);
     r->tuning = rc ? saved_t : best_t;
     r->sort_materials = rc ? saved_sort : best_sort;
     r->shade_waves = rc ? saved_sw : best_sw;
     r->plain = rc ? saved_plain : best_plain;
+    r->plain_waves = rc ? saved_pw : best_pw;
     drop_graphs(r);
     if (rc) return rc;
     if (best_tuning) *best_tuning = best_t;
     if (best_flags)
         *best_flags = (best_sort ? 0u : PSY_RENDER_NO_SORT) | (best_plain ? PSY_RENDER_PLAIN : 0u) |
-                      PSY_RENDER_SHADE_WAVES(best_sw);
+                      PSY_RENDER_SHADE_WAVES(best_plain ? best_pw : best_sw);
     return PSY_OK;
 }
 

@@ -421,12 +421,18 @@ RD S4 spectrum_xyz_to_p_4(S4 lambdas, float x, float y, float z) {
         sb1[k] = sb0[k] + 1 < NUM_SAMPLES - 1 ? sb0[k] + 1 : NUM_SAMPLES - 1;
     }
     S4 sbf = sb - s4((float)sb0[0], (float)sb0[1], (float)sb0[2], (float)sb0[3]);
+    // A cell has at most six points; the loops are unrolled so the points
+    // are indexed by constants and stay in registers.
     S4 p[6];
-    for (int i = 0; i < num; i++) {
-        const float* spectrum = POINT_SPECTRUM[idx[i]];
-        S4 p0 = s4(spectrum[sb0[0]], spectrum[sb0[1]], spectrum[sb0[2]], spectrum[sb0[3]]);
-        S4 p1 = s4(spectrum[sb1[0]], spectrum[sb1[1]], spectrum[sb1[2]], spectrum[sb1[3]]);
-        p[i] = (p0 * (s4_splat(1.0f) - sbf)) + (p1 * sbf);
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        p[i] = s4_zero();
+        if (i < num) {
+            const float* spectrum = POINT_SPECTRUM[idx[i]];
+            S4 p0 = s4(spectrum[sb0[0]], spectrum[sb0[1]], spectrum[sb0[2]], spectrum[sb0[3]]);
+            S4 p1 = s4(spectrum[sb1[0]], spectrum[sb1[1]], spectrum[sb1[2]], spectrum[sb1[3]]);
+            p[i] = (p0 * (s4_splat(1.0f) - sbf)) + (p1 * sbf);
+        }
     }
     S4 interp = s4_zero();
     if (inside) {
@@ -439,7 +445,10 @@ RD S4 spectrum_xyz_to_p_4(S4 lambdas, float x, float y, float z) {
         float e0x = POINT_UV[idx[1]][0] - POINT_UV[idx[0]][0];
         float e0y = POINT_UV[idx[1]][1] - POINT_UV[idx[0]][1];
         float uu = (e0x * ey) - (ex * e0y);
-        for (int i = 0; i < num - 1; i++) {
+        bool found = false;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            if (found || i >= num - 1) continue;
             float e1x, e1y;
             if (i == num - 2) {
                 e1x = POINT_UV[idx[1]][0] - POINT_UV[idx[0]][0];
@@ -459,8 +468,10 @@ RD S4 spectrum_xyz_to_p_4(S4 lambdas, float x, float y, float z) {
                 e0y = e1y;
                 continue;
             }
-            interp = ((p[0] * wf) + (p[i + 1] * vf)) + (p[i == num - 2 ? 1 : i + 2] * uf);
-            break;
+            // i + 2 < num here unless i == num - 2, so the clamp never binds.
+            const S4 pc = i == num - 2 ? p[1] : p[i + 2 < 6 ? i + 2 : 5];
+            interp = ((p[0] * wf) + (p[i + 1] * vf)) + (pc * uf);
+            found = true;
         }
     }
     return interp * inv_norm;

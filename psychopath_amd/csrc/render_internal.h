@@ -161,6 +161,23 @@ struct psy_renderer {
     bool plain = false;  // one megakernel launch per render
     uint32_t shade_waves = 2;  // minimum waves per SIMD of shade_kernel's build (1-4)
     uint32_t plain_waves = 2;  // the same for the one-launch megakernel
+    // Measurement (PSY_RENDER_PROFILE / PSY_RENDER_COUNT): events around
+    // each kernel of the current wave (stage, start, end), a pool reused
+    // wave after wave, and the tallies.  prof_dev: [0] closest rays,
+    // [1] shadow rays, [2..7) closest-ray and [7..12) shadow-ray traversal
+    // counters (nodes, motion nodes, triangles, motion triangles, instances).
+    bool profile = false;
+    bool counted = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Mark {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    double stage_ms[7] = {};
+    uint64_t prof_paths = 0;
+    unsigned long long* prof_dev = nullptr;
 
     std::vector<DevBuf*> bufs;
     ShadeScene sc{};
@@ -183,6 +200,7 @@ struct psy_renderer {
 
     ~psy_renderer() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
         for (DevBuf* b : bufs) delete b;
         if (ev0) (void)hipEventDestroy(ev0);

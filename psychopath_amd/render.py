@@ -56,14 +56,25 @@ class Renderer:
             _lib._lib.psy_renderer_destroy(h)
             self.handle = None
 
-    def set_options(self, graphs: bool = True, sort: bool = True, plain: bool = False, shade_waves: int = 0) -> None:
+    def set_options(self, graphs: bool = True, sort: bool = True, plain: bool = False, shade_waves: int = 0,
+                    profile: bool = False, count: bool = False) -> None:
         """graphs/sort: wavefront execution choices; plain: one launch per
         render over the whole frame; shade_waves (1-4, 0 keeps the current):
-        the shading kernels' build by minimum waves per SIMD.  None of them
-        changes the image."""
+        the shading kernels' build by minimum waves per SIMD; profile:
+        per-stage HIP-event times and ray counts; count: instrumented
+        traversal counters (see profile()).  None of them changes the
+        image."""
         flags = ((0 if graphs else _lib.PSY_RENDER_NO_GRAPHS) | (0 if sort else _lib.PSY_RENDER_NO_SORT) |
-                 (_lib.PSY_RENDER_PLAIN if plain else 0) | (int(shade_waves) << _lib.PSY_RENDER_SHADE_WAVES_SHIFT))
+                 (_lib.PSY_RENDER_PLAIN if plain else 0) | (int(shade_waves) << _lib.PSY_RENDER_SHADE_WAVES_SHIFT) |
+                 (_lib.PSY_RENDER_PROFILE if profile else 0) | (_lib.PSY_RENDER_COUNT if count else 0))
         check("psy_renderer_set_options", lib().psy_renderer_set_options(self.handle, flags))
+
+    def profile(self) -> dict:
+        """psy_renderer_get_profile: stage times (ms), paths, rays and
+        traversal counters tallied since the last call."""
+        p = _lib.RenderProfile()
+        check("psy_renderer_get_profile", lib().psy_renderer_get_profile(self.handle, C.byref(p)))
+        return {f: getattr(p, f) for f, _ in _lib.RenderProfile._fields_}
 
     def autotune(self, camera: dict, trial: dict) -> dict:
         """Picks the traversal launch shape and material sorting by timing
