@@ -783,14 +783,14 @@ __global__ __launch_bounds__(MEGA_BLOCK) __attribute__((amdgpu_waves_per_eu(MINW
             V3 o, d;
             camera_path(sc, rp, (uint64_t)pixel * spp + si, ps, o, d);
             for (;;) {
-                const TraceHit th = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, o, d, ps.time, __builtin_huge_valf(),
+                const TraceHit th = trace_one<false, MEGA_LDS, MEGA_SPILL, true>(ta, o, d, ps.time, __builtin_huge_valf(),
                                                                            false, lds_col, MEGA_BLOCK, tc);
                 const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
                 ShadeOut so;
                 shade_path(sc, rp, ps, hr, o, d, so);
                 traced += so.shadow ? 2u : 1u;
                 if (so.shadow) {
-                    const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL>(ta, so.so, so.sd, ps.time, so.smax, true,
+                    const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL, true>(ta, so.so, so.sd, ps.time, so.smax, true,
                                                                                lds_col, MEGA_BLOCK, tc);
                     if (sh.kind == PSY_HIT_MISS) ps.color = ps.color + ps.pending;
                 }

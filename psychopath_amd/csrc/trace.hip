@@ -24,7 +24,7 @@ constexpr int BLOCK = TRACE_BLOCK;
 constexpr int LDS_DEPTH = TRACE_LDS_DEPTH;
 constexpr int SPILL_DEPTH = TRACE_STACK_CAPACITY - TRACE_LDS_DEPTH;
 
-template <bool COUNT>
+template <bool COUNT, bool NESTED>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     __shared__ uint32_t lds_stack[LDS_DEPTH * BLOCK];
     const uint64_t ray = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -33,7 +33,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(TraceArgs a) {
     const V3 d = v3(a.dx[ray], a.dy[ray], a.dz[ray]);
     const bool occl = a.all_occlusion != 0 || (a.flags != nullptr && (a.flags[ray] & 1u) != 0);
     TraceCount cnt;
-    const TraceHit h = trace_one<COUNT, LDS_DEPTH, SPILL_DEPTH>(a, o, d, a.time[ray], a.max_t[ray], occl,
+    const TraceHit h = trace_one<COUNT, LDS_DEPTH, SPILL_DEPTH, NESTED>(a, o, d, a.time[ray], a.max_t[ray], occl,
                                                                  lds_stack + threadIdx.x, BLOCK, cnt);
     if (a.rec != nullptr) {
         const bool s = h.kind == PSY_HIT_SURFACE;
@@ -67,10 +67,12 @@ hipError_t launch_trace(const TraceArgs& args, bool counted, hipStream_t stream)
     if (args.n == 0) return hipSuccess;
     uint64_t blocks = (args.n + BLOCK - 1) / BLOCK;
     if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    if (counted)
-        hipLaunchKernelGGL(trace_kernel<true>, dim3((uint32_t)blocks), dim3(BLOCK), 0, stream, args);
-    else
-        hipLaunchKernelGGL(trace_kernel<false>, dim3((uint32_t)blocks), dim3(BLOCK), 0, stream, args);
+    // Scenes without sub-assemblies run the build with the nested walk
+    // compiled out.
+    const bool nested = args.groups != nullptr;
+    auto k = counted ? (nested ? trace_kernel<true, true> : trace_kernel<true, false>)
+                     : (nested ? trace_kernel<false, true> : trace_kernel<false, false>);
+    hipLaunchKernelGGL(k, dim3((uint32_t)blocks), dim3(BLOCK), 0, stream, args);
     return hipGetLastError();
 }
 

@@ -30,7 +30,9 @@ struct TraceCount {
     uint64_t nodes = 0, mnodes = 0, tris = 0, mtris = 0, inst = 0;
 };
 
-template <bool COUNT, int LDS_DEPTH, int SPILL_DEPTH>
+// NESTED: false compiles the nested-assembly walk out (the caller
+// guarantees a.groups is NULL); true follows a.groups at run time.
+template <bool COUNT, int LDS_DEPTH, int SPILL_DEPTH, bool NESTED>
 __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, float time, float tmax, bool occl,
                                                uint32_t* lds_col, int stride, TraceCount& ctr) {
 #define T1_PUSH(v)                                                         \
@@ -89,7 +91,7 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
                 // Leave the instance: restore the local ray of its assembly
                 // level (tracer.rs:130-148).
                 in_inst = false;
-                if (a.groups) {
+                if (NESTED && a.groups) {
                     level_ray(a.groups[grp]);
                 } else {
                     lo = top_lo;
@@ -158,7 +160,7 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
         const uint32_t first = (cur & ~REF_LEAF) >> 2;
         const uint32_t count = cur & 3u;
         const uint32_t ii = gbase + first;
-        if (a.groups && !in_inst && count == 0u) {
+        if (NESTED && a.groups && !in_inst && count == 0u) {
             // Level marker: the sub-assembly is done; back to the enclosing
             // assembly's path and local ray (tracer.rs:131-148).
             grp = first;
@@ -168,7 +170,7 @@ __device__ __forceinline__ TraceHit trace_one(const TraceArgs& a, V3 o, V3 d, fl
             cur = REF_EMPTY;
             continue;
         }
-        if (a.groups && !in_inst && a.instances[ii].kind == PSY_OBJECT_ASSEMBLY) {
+        if (NESTED && a.groups && !in_inst && a.instances[ii].kind == PSY_OBJECT_ASSEMBLY) {
             // Sub-assembly instance (tracer.rs:95-128): remember the
             // enclosing path on the stack and walk the sub-assembly's
             // instance BVH in its space.

@@ -124,6 +124,9 @@ def test_device_light_tree_matches_host():
         assert np.array_equal(d1, d2) and np.array_equal(o1, o2), m
     r1 = render.Renderer(host, sc.shaders, sc.background, sc.distant_lights)
     r2 = render.Renderer(dev, sc.shaders, sc.background, sc.distant_lights)
-    a, _ = r1.render(sc.camera, sc.settings)
+    a, sa = r1.render(sc.camera, sc.settings)
     b, _ = r2.render(sc.camera, sc.settings)
+    # A lit image with bounces, so that equality says something.
+    assert np.all(np.isfinite(a)) and a.mean() > 0
+    assert sa["rays"] > sc.settings["width"] * sc.settings["height"] * sc.settings["spp"]
     assert np.array_equal(a, b)

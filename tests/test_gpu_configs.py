@@ -86,8 +86,9 @@ def test_served_four_scene_mix_matches_alone(motion_full):
     four device-resident scenes, arriving while others render: each equals
     the same request rendered alone, bit for bit."""
     defs = [scenes.cornell_box(), scenes.sphere_scene(9), motion_full, scenes.terrain_scene()]
-    # Path pools smaller than the larger requests, so those span waves.
-    rens = [render.build(s, max_paths=1 << 16) for s in defs]
+    # Path pools smaller than the larger requests (up to 65,536 paths), so
+    # those span waves whatever the arrival timing.
+    rens = [render.build(s, max_paths=1 << 15) for s in defs]
     rng = np.random.default_rng(3)
     sizes = [(64, 64), (96, 54), (128, 128), (160, 90)]
     reqs = []
