@@ -667,7 +667,7 @@ def large_bench(args) -> dict:
     paths = st["paths"]
     out = {
         "workload": f"large instanced scene 3840x2160, {args.large_spp} spp, depth 8: 4096 instances of 8 meshes "
-                    "(2,099,040 unique, 1.07G instanced triangles), Lambertian/GGX metal/dielectric coat/emissive, "
+                    "(2,099,040 unique, 1.07G instanced triangles), Lambertian/GGX metal/rough dielectric IOR 1.5/emissive, "
                     "1024 sphere lights + sun",
         "paths": int(paths),
         "wavefront_samples_per_s": round(paths / wave_s, 1),

@@ -31,15 +31,25 @@ extern "C" {
 
 typedef struct psy_renderer psy_renderer;
 
-/* Surface shaders (SimpleSurfaceShader, src/shading/mod.rs:28-40). */
+/* Surface shaders (SimpleSurfaceShader, src/shading/mod.rs:28-40).
+ * PSY_SHADER_DIELECTRIC has no counterpart in the reference, whose closures
+ * are Lambert, GGX reflection and emission (surface_closure.rs:19-30): a
+ * rough dielectric interface, GGX microfacet reflection and refraction with
+ * exact Fresnel reflectance (Walter et al., "Microfacet Models for
+ * Refraction through Rough Surfaces", EGSR 2007), in throughput form (no
+ * (eta_i/eta_o)^2 radiance factor).  `roughness` is its GGX alpha in
+ * [1e-4, 1], `ior` in (1.0001, 10] the index of refraction on the side the
+ * geometric normal points away from, and `color` tints refracted light. */
 #define PSY_SHADER_LAMBERT 0u
 #define PSY_SHADER_GGX 1u
 #define PSY_SHADER_EMIT 2u
+#define PSY_SHADER_DIELECTRIC 3u
 typedef struct {
     uint32_t kind;
     psy_color color;
-    float roughness;  /* GGX only */
+    float roughness;  /* GGX and dielectric */
     float fresnel;    /* GGX only */
+    float ior;        /* dielectric only */
 } psy_shader_desc;
 
 /* Distant disk light (DistantDiskLight::new,

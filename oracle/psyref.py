@@ -189,7 +189,8 @@ class _Color(C.Structure):
 
 
 class _Shader(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("color", _Color), ("roughness", C.c_float), ("fresnel", C.c_float)]
+    _fields_ = [("kind", C.c_uint32), ("color", _Color), ("roughness", C.c_float), ("fresnel", C.c_float),
+                ("ior", C.c_float)]
 
 
 class _Distant(C.Structure):
@@ -224,8 +225,8 @@ _SIGS.update({
     "psyref_planck": (C.c_float, [C.c_float, C.c_float]),
     "psyref_meng_sha256": (C.c_char_p, []),
     "psyref_closure_evaluate_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, C.c_float, _fp, _fp]),
-    "psyref_closure_sample_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, C.c_float, _fp, _fp,
-                                       _fp]),
+    "psyref_closure_sample_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, _fp, C.c_float, _fp,
+                                       _fp, _fp]),
     "psyref_closure_estimate_n": (None, [C.POINTER(_Shader), C.c_uint32, _fp, _fp, _fp, _fp, _fp, _fp]),
     "psyref_sphere_light_sample_n": (None, [C.c_float, C.POINTER(_Color), _fp, C.c_uint32, _fp, _fp, C.c_float,
                                             _fp, _fp, _fp, _fp, _fp]),
@@ -246,33 +247,48 @@ def _v3(a, n):
     return a
 
 
-def _shader(kind, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
-    return _Shader(int(kind), _color(color), float(roughness), float(fresnel))
+SHADER_DIELECTRIC = 3
 
 
-def closure_evaluate(kind, inc, out, nor, nor_g, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
+def _shader(kind, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0, ior=0.0):
+    return _Shader(int(kind), _color(color), float(roughness), float(fresnel), float(ior))
+
+
+def _scene_shader(kind, color, roughness, fresnel_or_ior):
+    """A shader of a RenderScene's list: (kind, colour, roughness, fresnel),
+    the last being the index of refraction for a dielectric."""
+    if int(kind) == SHADER_DIELECTRIC:
+        return _shader(kind, color, roughness, 0.0, fresnel_or_ior)
+    return _shader(kind, color, roughness, fresnel_or_ior)
+
+
+def closure_evaluate(kind, inc, out, nor, nor_g, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0,
+                     ior=0.0):
     """SurfaceClosure::evaluate over rows: returns (filter [n,4], pdf [n])."""
     n = max(np.asarray(a).reshape(-1, 3).shape[0] for a in (inc, out, nor, nor_g))
     inc, out, nor, nor_g = (_v3(a, n) for a in (inc, out, nor, nor_g))
     filt = np.zeros((n, 4), np.float32)
     pdf = np.zeros(n, np.float32)
-    sh = _shader(kind, color, roughness, fresnel)
+    sh = _shader(kind, color, roughness, fresnel, ior)
     lib().psyref_closure_evaluate_n(C.byref(sh), n, fptr(inc), fptr(out), fptr(nor), fptr(nor_g), wl, fptr(filt),
                                     fptr(pdf))
     return filt, pdf
 
 
-def closure_sample(kind, inc, nor, nor_g, uv, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0):
-    """SurfaceClosure::sample over rows of (u, v): returns (out [n,3], filter [n,4], pdf [n])."""
+def closure_sample(kind, inc, nor, nor_g, uv, wl=550.0, color=(0, 1.0, 1.0, 1.0), roughness=0.0, fresnel=0.0,
+                   ior=0.0, w=None):
+    """SurfaceClosure::sample over rows of (u, v), with the dielectric's third
+    number `w` per row (default 0): returns (out [n,3], filter [n,4], pdf [n])."""
     uv = f32(uv).reshape(-1, 2)
     n = uv.shape[0]
     inc, nor, nor_g = (_v3(a, n) for a in (inc, nor, nor_g))
+    ww = f32(np.zeros(n) if w is None else np.broadcast_to(np.asarray(w, np.float32), (n,)))
     out = np.zeros((n, 3), np.float32)
     filt = np.zeros((n, 4), np.float32)
     pdf = np.zeros(n, np.float32)
-    sh = _shader(kind, color, roughness, fresnel)
-    lib().psyref_closure_sample_n(C.byref(sh), n, fptr(inc), fptr(nor), fptr(nor_g), fptr(uv), wl, fptr(out),
-                                  fptr(filt), fptr(pdf))
+    sh = _shader(kind, color, roughness, fresnel, ior)
+    lib().psyref_closure_sample_n(C.byref(sh), n, fptr(inc), fptr(nor), fptr(nor_g), fptr(uv), fptr(ww), wl,
+                                  fptr(out), fptr(filt), fptr(pdf))
     return out, filt, pdf
 
 
@@ -434,7 +450,7 @@ def render(scene, threads: int = 0) -> tuple[np.ndarray, dict]:
     L = lib()
     sh = (_Shader * max(1, len(scene.shaders)))()
     for i, (k, col, rough, fres) in enumerate(scene.shaders):
-        sh[i] = _Shader(k, _color(col), rough, fres)
+        sh[i] = _scene_shader(k, col, rough, fres)
     dl = (_Distant * max(1, len(scene.distant_lights)))()
     dls = (_DistantSamples * max(1, len(scene.distant_lights)))()
     keep = []

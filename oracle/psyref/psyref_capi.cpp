@@ -261,7 +261,7 @@ int psyref_trace(void* s, const float* ox, const float* oy, const float* oz, con
 
 namespace {
 struct PsyrefColor { uint32_t kind; float a, b, c; };
-struct PsyrefShader { uint32_t kind; PsyrefColor color; float roughness, fresnel; };
+struct PsyrefShader { uint32_t kind; PsyrefColor color; float roughness, fresnel, ior; };
 struct PsyrefDistant { float radius; float dir[3]; PsyrefColor color; };
 // Time samples of a distant light (counts 0: the PsyrefDistant values).
 struct PsyrefDistantSamples {
@@ -356,6 +356,7 @@ int psyref_render(void* s, const PsyrefShader* shaders, uint32_t n_shaders, cons
         c.color = to_color(shaders[i].color);
         c.roughness = shaders[i].roughness;
         c.fresnel = shaders[i].fresnel;
+        c.ior = shaders[i].ior;
         rsc.shaders.push_back(c);
     }
     for (const Instance& inst : sc->root.instances)
@@ -435,6 +436,7 @@ Closure to_closure(const PsyrefShader* sh) {
     c.color = to_color(sh->color);
     c.roughness = sh->roughness;
     c.fresnel = sh->fresnel;
+    c.ior = sh->ior;
     return c;
 }
 Vec3 v3at(const float* p, uint32_t i) { return Vec3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
@@ -458,15 +460,18 @@ void psyref_closure_evaluate_n(const PsyrefShader* sh, uint32_t n, const float* 
     }
 }
 
-// SurfaceClosure::sample for n (inc, nor, nor_g, (u, v)) items.
+// SurfaceClosure::sample for n (inc, nor, nor_g, (u, v)) items; `w` (NULL:
+// zeros) is the dielectric's third number.
 void psyref_closure_sample_n(const PsyrefShader* sh, uint32_t n, const float* inc, const float* nor,
-                             const float* nor_g, const float* uv, float wl, float* out, float* filter, float* pdf) {
+                             const float* nor_g, const float* uv, const float* w, float wl, float* out, float* filter,
+                             float* pdf) {
     const Closure c = to_closure(sh);
     for (uint32_t i = 0; i < n; i++) {
         Vec3 o;
         Vec4 f;
         float p;
-        closure_sample(c, v3at(inc, i), v3at(nor, i), v3at(nor_g, i), uv[2 * i], uv[2 * i + 1], wl, o, f, p);
+        closure_sample(c, v3at(inc, i), v3at(nor, i), v3at(nor_g, i), uv[2 * i], uv[2 * i + 1], w ? w[i] : 0.0f, wl,
+                       o, f, p);
         put3(out, i, o);
         put4(filter, i, f);
         pdf[i] = p;

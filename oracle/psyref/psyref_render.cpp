@@ -495,7 +495,8 @@ bool path_next(LightPath& p, const RenderScene& sc, const Isect& is, RayBatch& r
                 Vec3 dir;
                 Vec4 filter;
                 float pdf;
-                closure_sample(closure, is.incoming, is.nor, is.nor_g, b4[0], b4[1], p.wavelength, dir, filter, pdf);
+                closure_sample(closure, is.incoming, is.nor, is.nor_g, b4[0], b4[1], b4[2], p.wavelength, dir, filter,
+                               pdf);
                 if ((pdf > 0.0f) && (max_element4(filter) > 0.0f)) {
                     p.next_attenuation_fac = filter;
                     p.closure_sample_pdf = pdf;

@@ -1,7 +1,8 @@
 // psyref — TEST INFRASTRUCTURE ONLY (parity oracle / CPU baseline).
 //
 // Surface closures: restates src/shading/surface_closure.rs (Lambert, GGX,
-// Emit: sample, evaluate, estimate_eval_over_sphere_light) and
+// Emit: sample, evaluate, estimate_eval_over_sphere_light), this build's
+// rough dielectric (no reference counterpart), and
 // src/shading/mod.rs (SimpleSurfaceShader), and the MIS weight of
 // src/mis.rs.
 #pragma once
@@ -15,12 +16,16 @@ namespace psyref {
 static const float INV_PI = 1.0f / PI_32;
 static const float H_PI = PI_32 / 2.0f;
 
-enum ClosureKind : uint32_t { CLOSURE_LAMBERT = 0, CLOSURE_GGX = 1, CLOSURE_EMIT = 2 };
+// CLOSURE_DIELECTRIC has no counterpart in the reference (its closures are
+// Lambert, GGX and Emit, surface_closure.rs:19-30); it is this build's rough
+// dielectric, restated here from its specification below.
+enum ClosureKind : uint32_t { CLOSURE_LAMBERT = 0, CLOSURE_GGX = 1, CLOSURE_EMIT = 2, CLOSURE_DIELECTRIC = 3 };
 
 struct Closure {
     uint32_t kind = CLOSURE_LAMBERT;
     Color color;
     float roughness = 0, fresnel = 0;
+    float ior = 0;  // dielectric only
 };
 
 // mis.rs:12-21
@@ -72,6 +77,105 @@ static inline float half_theta_sample(float u, float rough) {  // :598-606
     return std::sqrt(top / bottom);
 }
 
+// ---- rough dielectric (no reference counterpart) ---------------------------
+// Walter, Marschner, Li and Torrance, "Microfacet Models for Refraction
+// through Rough Surfaces" (EGSR 2007): GGX microfacets with the GGX
+// closure's D, G1 and half-vector sampling, exact unpolarised Fresnel
+// reflectance choosing reflection or refraction, the BSDF of eqs. 20-21 and
+// the densities of eqs. 38-41, in throughput form (no (eta_i/eta_o)^2
+// radiance factor).  `ior` is the index on the side the geometric normal
+// points away from; `color` tints refracted light.
+static inline float fresnel_dielectric(float cos_i, float eta) {
+    const float sin2_t = (eta * eta) * (1.0f - (cos_i * cos_i));
+    if (sin2_t >= 1.0f) return 1.0f;  // total internal reflection
+    const float cos_t = std::sqrt(1.0f - sin2_t);
+    const float rs = ((eta * cos_i) - cos_t) / ((eta * cos_i) + cos_t);
+    const float rp = (cos_i - (eta * cos_t)) / (cos_i + (eta * cos_t));
+    return 0.5f * ((rs * rs) + (rp * rp));
+}
+
+static inline void dielectric_evaluate(const Closure& c, Vec3 inc, Vec3 out, Vec3 nor, Vec3 nor_g, float wl,
+                                       Vec4& filter, float& pdf) {
+    filter = Vec4();
+    pdf = 0.0f;
+    const Vec3 aa = neg(normalized(inc));
+    const Vec3 bb = normalized(out);
+    Vec3 nn, fng;
+    shading_frame(inc, nor, nor_g, nn, fng);
+    const bool entering = dot(nor_g, inc) <= 0.0f;
+    const float eta_i = entering ? 1.0f : c.ior;
+    const float eta_o = entering ? c.ior : 1.0f;
+    const float na = dot(nn, aa);
+    const float nb = dot(nn, bb);
+    if (na <= 0.0f) return;
+    if (dot(fng, bb) >= 0.0f) {
+        // reflection: f = F D G / (4 |i.n| |o.n|), density F D (m.n) / (4 |o.m|)
+        if (nb <= 0.0f) return;
+        const Vec3 hh = normalized(aa + bb);
+        const float ha = dot(hh, aa);
+        const float hb = dot(hh, bb);
+        const float nh = dot(nn, hh);
+        if (hb <= 0.0f) return;
+        const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta_i / eta_o);
+        const float d = ggx_d(nh, c.roughness);
+        const float g = ggx_g(ha, na, c.roughness) * ggx_g(hb, nb, c.roughness);
+        const float v = ((f * d) * g) / (4.0f * na);
+        filter = Vec4(v, v, v, v);
+        pdf = ((f * d) * nh) / (4.0f * hb);
+        return;
+    }
+    // refraction: f = |i.m||o.m| eta_o^2 (1-F) D G / (|i.n||o.n| (eta_i i.m + eta_o o.m)^2),
+    // density (1-F) D (m.n) eta_o^2 |o.m| / (eta_i i.m + eta_o o.m)^2
+    if (nb >= 0.0f) return;
+    Vec3 hh = normalized(neg((aa * eta_i) + (bb * eta_o)));
+    if (dot(hh, nn) < 0.0f) hh = neg(hh);
+    const float ha = dot(hh, aa);
+    const float hb = dot(hh, bb);
+    if (ha <= 0.0f || hb >= 0.0f) return;
+    const float nh = dot(nn, hh);
+    const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta_i / eta_o);
+    const float d = ggx_d(nh, c.roughness);
+    const float g = ggx_g(ha, na, c.roughness) * ggx_g(hb, nb, c.roughness);
+    const float den = (eta_i * ha) + (eta_o * hb);
+    const float den2 = den * den;
+    const float t = (1.0f - f) * d;
+    const float eo2 = eta_o * eta_o;
+    filter = c.color.to_spectral_sample(wl) * (((((t * g) * eo2) * ha) * -hb) / (na * den2));
+    pdf = (((t * nh) * eo2) * -hb) / den2;
+}
+
+static inline void dielectric_sample(const Closure& c, Vec3 inc, Vec3 nor, Vec3 nor_g, float u, float v, float w,
+                                     float wl, Vec3& out, Vec4& filter, float& pdf) {
+    out = Vec3(0, 0, 0);
+    filter = Vec4();
+    pdf = 0.0f;
+    Vec3 nn, fng;
+    shading_frame(inc, nor, nor_g, nn, fng);
+    const bool entering = dot(nor_g, inc) <= 0.0f;
+    const float eta = entering ? (1.0f / c.ior) : c.ior;
+    const float theta_cos = half_theta_sample(u, c.roughness);
+    const float theta_sin = std::sqrt(1.0f - (theta_cos * theta_cos));
+    const float angle = (v * PI_32) * 2.0f;
+    const Vec3 m = normalized(zup_to_vec(Vec3(std::cos(angle) * theta_sin, std::sin(angle) * theta_sin, theta_cos), nn));
+    const Vec3 aa = neg(normalized(inc));
+    const float ha = dot(m, aa);
+    if (ha <= 0.0f) return;
+    const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta);
+    // reflections through the surface and refractions back out are
+    // rejected: evaluation attributes each direction to its side's lobe
+    if (w < f) {
+        out = (m * (2.0f * ha)) - aa;
+        if (dot(fng, out) < 0.0f) return;
+    } else {
+        const float sin2_t = (eta * eta) * (1.0f - (ha * ha));
+        if (sin2_t >= 1.0f) return;
+        const float cos_t = std::sqrt(1.0f - sin2_t);
+        out = (m * ((eta * ha) - cos_t)) - (aa * eta);
+        if (dot(fng, out) >= 0.0f) return;
+    }
+    dielectric_evaluate(c, inc, out, nor, nor_g, wl, filter, pdf);
+}
+
 // ---- evaluate -------------------------------------------------------------
 static inline void closure_evaluate(const Closure& c, Vec3 inc, Vec3 out, Vec3 nor, Vec3 nor_g, float wl, Vec4& filter,
                                     float& pdf) {
@@ -94,6 +198,10 @@ static inline void closure_evaluate(const Closure& c, Vec3 inc, Vec3 out, Vec3 n
         return;
     }
     // GGX (:450-521)
+    if (c.kind == CLOSURE_DIELECTRIC) {
+        dielectric_evaluate(c, inc, out, nor, nor_g, wl, filter, pdf);
+        return;
+    }
     Vec3 aa = neg(normalized(inc));
     Vec3 bb = normalized(out);
     Vec3 hh = normalized(aa + bb);
@@ -126,12 +234,17 @@ static inline void closure_evaluate(const Closure& c, Vec3 inc, Vec3 out, Vec3 n
 }
 
 // ---- sample ----------------------------------------------------------------
-static inline void closure_sample(const Closure& c, Vec3 inc, Vec3 nor, Vec3 nor_g, float u, float v, float wl,
-                                  Vec3& out, Vec4& filter, float& pdf) {
+// `w`, a third sample number, is read by the dielectric only.
+static inline void closure_sample(const Closure& c, Vec3 inc, Vec3 nor, Vec3 nor_g, float u, float v, float w,
+                                  float wl, Vec3& out, Vec4& filter, float& pdf) {
     if (c.kind == CLOSURE_EMIT) {  // :641-655
         out = Vec3(0, 0, 0);
         filter = c.color.to_spectral_sample(wl);
         pdf = 1.0f;
+        return;
+    }
+    if (c.kind == CLOSURE_DIELECTRIC) {
+        dielectric_sample(c, inc, nor, nor_g, u, v, w, wl, out, filter, pdf);
         return;
     }
     Vec3 nn, fng;
@@ -206,6 +319,13 @@ static inline float closure_estimate_sphere_light(const Closure& c, Vec3 inc, Ve
         float nh = clampf(dot(nn, hh), -1.0f, 1.0f);
         float fac = ggx_d(nh, fminn(1.0f, std::sqrt(c.roughness) + ((2.0f * theta) / PI_32)));
         return (fac * fminn(1.0f, 1.0f - cos_theta_max)) * INV_PI;
+    }
+    if (c.kind == CLOSURE_DIELECTRIC) {
+        // lights on either side: the light's solid angle over 2 pi
+        float dist2 = length2(to_light_center);
+        if (dist2 <= r2) return 1.0f;
+        float sin_theta_max2 = fminn(r2 / dist2, 1.0f);
+        return 1.0f - std::sqrt(1.0f - sin_theta_max2);
     }
     return 0.0f;  // Emit: unimplemented!() in the reference; never reached (emitters end paths)
 }

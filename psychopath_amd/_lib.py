@@ -41,7 +41,7 @@ class MeshDesc(C.Structure):
 
 PSY_COLOR_XYZ, PSY_COLOR_BLACKBODY, PSY_COLOR_TEMPERATURE = 0, 1, 2
 PSY_OBJECT_MESH, PSY_OBJECT_RECT_LIGHT, PSY_OBJECT_SPHERE_LIGHT, PSY_OBJECT_ASSEMBLY = 0, 1, 2, 3
-PSY_SHADER_LAMBERT, PSY_SHADER_GGX, PSY_SHADER_EMIT = 0, 1, 2
+PSY_SHADER_LAMBERT, PSY_SHADER_GGX, PSY_SHADER_EMIT, PSY_SHADER_DIELECTRIC = 0, 1, 2, 3
 
 
 class Color(C.Structure):
@@ -89,7 +89,8 @@ SceneDesc._fields_ = [
 
 
 class ShaderDesc(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("color", Color), ("roughness", C.c_float), ("fresnel", C.c_float)]
+    _fields_ = [("kind", C.c_uint32), ("color", Color), ("roughness", C.c_float), ("fresnel", C.c_float),
+                ("ior", C.c_float)]
 
 
 class DistantLightDesc(C.Structure):

@@ -162,7 +162,7 @@ struct GpuDistantLight {
 struct GpuClosure {
     uint32_t kind;
     float roughness, fresnel;
-    uint32_t pad;
+    float ior;  // PSY_SHADER_DIELECTRIC only
     GpuColor color;
 };
 

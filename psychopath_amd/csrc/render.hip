@@ -214,7 +214,8 @@ __global__ __launch_bounds__(KBLOCK) void shadow_kernel(Queue qs, const uint8_t*
     }
 }
 
-// Sort key of a closest hit: 0 miss, 1 light surface, 2 + closure kind.
+// Sort key of a closest hit: 0 miss, 1 light surface, 2 + closure kind
+// (below NUM_BINS for every PSY_SHADER_* kind).
 __device__ __forceinline__ uint32_t hit_key(const ShadeScene& sc, const Hits& h, uint32_t q) {
     if (h.kind[q] != PSY_HIT_SURFACE) return 0u;
     const GpuInstance inst = sc.instances[sc.inst_pos[h.inst[q]]];
@@ -224,9 +225,10 @@ __device__ __forceinline__ uint32_t hit_key(const ShadeScene& sc, const Hits& h,
 
 // Counting sort of the closest hits by key, in two kernels over the same
 // block partition: block b takes queue entries [b*chunk, (b+1)*chunk).
-// sort_count adds each block's per-key counts to bins[0..5); sort_scatter
-// reserves each block's range in every bin with one atomic on bins[5..10)
-// and places the block's entries with LDS counters.
+// sort_count adds each block's per-key counts to bins[0, NUM_BINS);
+// sort_scatter reserves each block's range in every bin with one atomic on
+// bins[NUM_BINS, 2 * NUM_BINS) and places the block's entries with LDS
+// counters.
 __device__ __forceinline__ uint32_t sort_chunk(uint32_t n) {
     return (n + gridDim.x - 1) / gridDim.x;
 }
@@ -600,6 +602,7 @@ __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams
         clo.color = is.emit;
         clo.roughness = 0.0f;
         clo.fresnel = 0.0f;
+        clo.ior = 0.0f;
     } else {
         clo = is.shader == PSY_NONE ? sc.unassigned : sc.shaders[is.shader];
     }
@@ -650,7 +653,7 @@ __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams
         V3 ndir;
         S4 filter;
         float pdf;
-        closure_sample(clo, dir, is.nor, is.nor_g, bs.v[0], bs.v[1], wl, ndir, filter, pdf);
+        closure_sample(clo, dir, is.nor, is.nor_g, bs.v[0], bs.v[1], bs.v[2], wl, ndir, filter, pdf);
         if ((pdf > 0.0f) && (max_element4(filter) > 0.0f)) {
             ps.next_fac = filter;
             ps.csp = pdf;
@@ -1088,17 +1091,17 @@ int enqueue_wave(psy_renderer* r, uint32_t steps) {
         });
         if (rc) return rc;
         if (last) break;
-        hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, qa_out.count, qs_out.count, r->counts + 4);
+        hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, qa_out.count, qs_out.count, r->counts + COUNTS_BINS);
         PSY_HIP(hipGetLastError());
         const uint32_t* order = nullptr;
         if (r->sort_materials) {
             rc = staged(r, ST_SORT, [&]() -> int {
                 const uint32_t sgrid = std::min<uint32_t>(grid, 1024);
                 hipLaunchKernelGGL(sort_count_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
-                                   r->counts + 4);
+                                   r->counts + COUNTS_BINS);
                 PSY_HIP(hipGetLastError());
                 hipLaunchKernelGGL(sort_scatter_kernel, dim3(sgrid), dim3(KBLOCK), 0, s, r->sc, r->ha, qa_in.count,
-                                   r->counts + 4, r->order);
+                                   r->counts + COUNTS_BINS, r->order);
                 PSY_HIP(hipGetLastError());
                 return PSY_OK;
             });
@@ -1300,10 +1303,15 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     std::vector<GpuClosure> shaders;
     for (uint32_t i = 0; i < world->shader_count; i++) {
         const psy_shader_desc& d = world->shaders[i];
-        if (d.kind > PSY_SHADER_EMIT) return fail(PSY_E_INVALID, "unknown shader kind");
+        if (d.kind > PSY_SHADER_DIELECTRIC) return fail(PSY_E_INVALID, "unknown shader kind");
+        if (d.kind == PSY_SHADER_DIELECTRIC &&
+            !(d.roughness >= 1e-4f && d.roughness <= 1.0f && d.ior > 1.0001f && d.ior <= 10.0f))
+            return fail(PSY_E_INVALID, "a dielectric needs roughness in [1e-4, 1] and ior in (1.0001, 10]");
         int rc = color_check(d.color);
         if (rc) return rc;
-        shaders.push_back(GpuClosure{d.kind, d.roughness, d.fresnel, 0, GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
+        shaders.push_back(GpuClosure{d.kind, d.roughness, d.fresnel,
+                                     d.kind == PSY_SHADER_DIELECTRIC ? d.ior : 0.0f,
+                                     GpuColor{d.color.kind, d.color.a, d.color.b, d.color.c}});
     }
     // Distant lights (DistantDiskLight::new, distant_disk_light.rs:24-36),
     // with their time samples after the records and the mean colour energy
@@ -1382,7 +1390,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     if (!rc) rc = r->alloc(&r->hs.prim, C);
     if (!rc) rc = r->alloc(&r->hs.inst, C);
     if (!rc) rc = r->alloc(&r->order, C);
-    if (!rc) rc = r->alloc(&r->counts, 16);
+    if (!rc) rc = r->alloc(&r->counts, COUNTS_WORDS);
     if (!rc) rc = r->alloc(&r->wave_idx, 1);
     if (!rc) rc = r->alloc(&r->work, 64);
     if (!rc) rc = r->alloc(&r->prof_dev, 16);
@@ -1409,7 +1417,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
     sc.tri_rec = scene->tri_rec;
     sc.normals = scene->normals;
     sc.nidx = scene->nidx;
-    sc.rays = reinterpret_cast<unsigned long long*>(r->counts + 14);
+    sc.rays = reinterpret_cast<unsigned long long*>(r->counts + COUNTS_RAYS);
     sc.xforms = scene->xforms;
     sc.inv_static = scene->inv_static;
     sc.rects = scene->rect_lights;
@@ -1426,7 +1434,7 @@ int psy_renderer_create(const psy_scene* scene_c, const psy_world_desc* world, u
         // surfaces without a shader (tracer.rs:161-164).
         float m[3];
         psy::color::apply(psy::color::matrices().rec709_to_xyz, 1.0f, 0.0f, 1.0f, m);
-        sc.unassigned = GpuClosure{PSY_SHADER_EMIT, 0.0f, 0.0f, 0, GpuColor{PSY_COLOR_XYZ, m[0], m[1], m[2]}};
+        sc.unassigned = GpuClosure{PSY_SHADER_EMIT, 0.0f, 0.0f, 0.0f, GpuColor{PSY_COLOR_XYZ, m[0], m[1], m[2]}};
     }
     sc.distant_count = (uint32_t)distant.size();
     sc.background = GpuColor{world->background.kind, world->background.a, world->background.b, world->background.c};
@@ -1513,7 +1521,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
     PSY_HIP(hipMemcpyAsync(r->reqs, &rp, sizeof(rp), hipMemcpyHostToDevice, s));
     PSY_HIP(hipMemcpyAsync(r->plans, plans.data(), waves * sizeof(WavePlan), hipMemcpyHostToDevice, s));
     rc = launch_zero(film, pixels * 3, s);
-    if (!rc) rc = launch_zero(reinterpret_cast<float*>(r->counts + 14), 2, s);  // ray counter
+    if (!rc) rc = launch_zero(reinterpret_cast<float*>(r->counts + COUNTS_RAYS), 2, s);  // ray counter
     if (!rc) rc = launch_set_wave(r, 0u);
     if (rc) return rc;
     PSY_HIP(hipEventRecord(r->ev0, s));
@@ -1545,7 +1553,7 @@ int psy_render(psy_renderer* r, const psy_camera_desc* cam, const psy_render_set
         float ms = 0.0f;
         PSY_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
         unsigned long long rays = 0;
-        PSY_HIP(hipMemcpy(&rays, r->counts + 14, sizeof(rays), hipMemcpyDeviceToHost));
+        PSY_HIP(hipMemcpy(&rays, r->counts + COUNTS_RAYS, sizeof(rays), hipMemcpyDeviceToHost));
         stats->paths = total;
         stats->rays = rays;
         stats->waves = r->plain ? 0 : waves;

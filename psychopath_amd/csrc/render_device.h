@@ -768,6 +768,113 @@ RD float half_theta_sample(float u, float rough) {  // :598-606
     return sqrtf(top / bottom);
 }
 
+// ---- rough dielectric (PSY_SHADER_DIELECTRIC) ------------------------------
+// The reference has no transmissive closure; this one follows Walter,
+// Marschner, Li and Torrance, "Microfacet Models for Refraction through
+// Rough Surfaces" (EGSR 2007): GGX microfacets (the D, G1 and half-vector
+// sampling of the GGX closure above), exact Fresnel reflectance choosing
+// between reflection and refraction, eqs. 20-21 for the BSDF and 38-41 for
+// the sampling densities.  It is kept in throughput form: the refraction
+// term carries no (eta_i / eta_o)^2 radiance factor, which cancels between
+// entering and leaving for paths that start and end outside.  `ior` is the
+// index on the side the geometric normal points away from, `color` tints
+// transmitted light, `roughness` is GGX alpha.
+
+// Unpolarised Fresnel reflectance for the cosine on the incident side and
+// eta = eta_incident / eta_transmitted; 1 under total internal reflection.
+RD float fresnel_dielectric(float cos_i, float eta) {
+    const float sin2_t = (eta * eta) * (1.0f - (cos_i * cos_i));
+    if (sin2_t >= 1.0f) return 1.0f;
+    const float cos_t = sqrtf(1.0f - sin2_t);
+    const float rs = ((eta * cos_i) - cos_t) / ((eta * cos_i) + cos_t);
+    const float rp = (cos_i - (eta * cos_t)) / (cos_i + (eta * cos_t));
+    return 0.5f * ((rs * rs) + (rp * rp));
+}
+
+// f(inc -> out) |cos(out)| and the sampling density of `out`.
+RD void dielectric_evaluate(const GpuClosure& c, V3 inc, V3 out, V3 nor, V3 nor_g, float wl, S4& filter,
+                            float& pdf) {
+    filter = s4_zero();
+    pdf = 0.0f;
+    const V3 aa = neg3(normalize(inc));
+    const V3 bb = normalize(out);
+    V3 nn, fng;
+    shading_frame(inc, nor, nor_g, nn, fng);
+    const bool entering = dot(nor_g, inc) <= 0.0f;
+    const float eta_i = entering ? 1.0f : c.ior;
+    const float eta_o = entering ? c.ior : 1.0f;
+    const float na = dot(nn, aa);
+    const float nb = dot(nn, bb);
+    if (na <= 0.0f) return;
+    if (dot(fng, bb) >= 0.0f) {
+        // Reflection (eq. 20, density eq. 38 times F).
+        if (nb <= 0.0f) return;
+        const V3 hh = normalize(add(aa, bb));
+        const float ha = dot(hh, aa);
+        const float hb = dot(hh, bb);
+        const float nh = dot(nn, hh);
+        if (hb <= 0.0f) return;
+        const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta_i / eta_o);
+        const float d = ggx_d(nh, c.roughness);
+        const float g = ggx_g(ha, na, c.roughness) * ggx_g(hb, nb, c.roughness);
+        filter = s4_splat(((f * d) * g) / (4.0f * na));
+        pdf = ((f * d) * nh) / (4.0f * hb);
+        return;
+    }
+    // Refraction (eq. 21, density eqs. 39-41 times 1 - F).
+    if (nb >= 0.0f) return;
+    V3 hh = normalize(neg3(add(mul(aa, eta_i), mul(bb, eta_o))));
+    if (dot(hh, nn) < 0.0f) hh = neg3(hh);
+    const float ha = dot(hh, aa);
+    const float hb = dot(hh, bb);
+    if (ha <= 0.0f || hb >= 0.0f) return;
+    const float nh = dot(nn, hh);
+    const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta_i / eta_o);
+    const float d = ggx_d(nh, c.roughness);
+    const float g = ggx_g(ha, na, c.roughness) * ggx_g(hb, nb, c.roughness);
+    const float den = (eta_i * ha) + (eta_o * hb);
+    const float den2 = den * den;
+    const float t = (1.0f - f) * d;
+    const float eo2 = eta_o * eta_o;
+    filter = color_spectral(c.color, wl) * (((((t * g) * eo2) * ha) * -hb) / (na * den2));
+    pdf = (((t * nh) * eo2) * -hb) / den2;
+}
+
+// A microfacet normal drawn from D(m) cos(m) as for GGX, then reflection
+// with probability F (the third number, w) or refraction.
+RD void dielectric_sample(const GpuClosure& c, V3 inc, V3 nor, V3 nor_g, float u, float v, float w, float wl,
+                          V3& out, S4& filter, float& pdf) {
+    out = v3(0.0f, 0.0f, 0.0f);
+    filter = s4_zero();
+    pdf = 0.0f;
+    V3 nn, fng;
+    shading_frame(inc, nor, nor_g, nn, fng);
+    const bool entering = dot(nor_g, inc) <= 0.0f;
+    const float eta = entering ? (1.0f / c.ior) : c.ior;
+    const float theta_cos = half_theta_sample(u, c.roughness);
+    const float theta_sin = sqrtf(1.0f - (theta_cos * theta_cos));
+    const float angle = (v * PI_F) * 2.0f;
+    const V3 m = normalize(zup_to_vec(v3(psy_cosf(angle) * theta_sin, psy_sinf(angle) * theta_sin, theta_cos), nn));
+    const V3 aa = neg3(normalize(inc));
+    const float ha = dot(m, aa);
+    if (ha <= 0.0f) return;
+    const float f = fresnel_dielectric(clampf(ha, 0.0f, 1.0f), eta);
+    // A rough surface can send a reflection through the surface or a
+    // refraction back out; such samples are rejected, since evaluation
+    // attributes every direction to the lobe of its side.
+    if (w < f) {
+        out = sub(mul(m, 2.0f * ha), aa);
+        if (dot(fng, out) < 0.0f) return;
+    } else {
+        const float sin2_t = (eta * eta) * (1.0f - (ha * ha));
+        if (sin2_t >= 1.0f) return;
+        const float cos_t = sqrtf(1.0f - sin2_t);
+        out = sub(mul(m, (eta * ha) - cos_t), mul(aa, eta));
+        if (dot(fng, out) >= 0.0f) return;
+    }
+    dielectric_evaluate(c, inc, out, nor, nor_g, wl, filter, pdf);
+}
+
 // SurfaceClosure::evaluate (Lambert :313-331, GGX :450-521, Emit :657-669)
 RD void closure_evaluate(const GpuClosure& c, V3 inc, V3 out, V3 nor, V3 nor_g, float wl, S4& filter, float& pdf) {
     if (c.kind == PSY_SHADER_EMIT) {
@@ -786,6 +893,10 @@ RD void closure_evaluate(const GpuClosure& c, V3 inc, V3 out, V3 nor, V3 nor_g, 
             filter = s4_zero();
             pdf = 0.0f;
         }
+        return;
+    }
+    if (c.kind == PSY_SHADER_DIELECTRIC) {
+        dielectric_evaluate(c, inc, out, nor, nor_g, wl, filter, pdf);
         return;
     }
     V3 aa = neg3(normalize(inc));
@@ -819,13 +930,18 @@ RD void closure_evaluate(const GpuClosure& c, V3 inc, V3 out, V3 nor, V3 nor_g, 
     pdf = dist * INV_PI_F;
 }
 
-// SurfaceClosure::sample (Lambert :284-311, GGX :418-448, Emit :641-655)
-RD void closure_sample(const GpuClosure& c, V3 inc, V3 nor, V3 nor_g, float u, float v, float wl, V3& out, S4& filter,
-                       float& pdf) {
+// SurfaceClosure::sample (Lambert :284-311, GGX :418-448, Emit :641-655);
+// `w`, a third sample number, is read by the dielectric only.
+RD void closure_sample(const GpuClosure& c, V3 inc, V3 nor, V3 nor_g, float u, float v, float w, float wl, V3& out,
+                       S4& filter, float& pdf) {
     if (c.kind == PSY_SHADER_EMIT) {
         out = v3(0.0f, 0.0f, 0.0f);
         filter = color_spectral(c.color, wl);
         pdf = 1.0f;
+        return;
+    }
+    if (c.kind == PSY_SHADER_DIELECTRIC) {
+        dielectric_sample(c, inc, nor, nor_g, u, v, w, wl, out, filter, pdf);
         return;
     }
     V3 nn, fng;
@@ -876,7 +992,8 @@ RD float sphere_lambert(float nlcos, float rcos) {
 }
 
 // SurfaceClosure::estimate_eval_over_sphere_light (Lambert :333-407, GGX
-// :523-592); the reference leaves it unimplemented for Emit.
+// :523-592); the reference leaves it unimplemented for Emit and has no
+// dielectric.
 RD float closure_estimate_sphere_light(const GpuClosure& c, V3 inc, V3 to_light_center, float r2, V3 nor, V3 nor_g) {
     if (c.kind == PSY_SHADER_LAMBERT) {
         float dist2 = length2(to_light_center);
@@ -900,6 +1017,13 @@ RD float closure_estimate_sphere_light(const GpuClosure& c, V3 inc, V3 to_light_
         float nh = clampf(dot(nn, hh), -1.0f, 1.0f);
         float fac = ggx_d(nh, fminn(1.0f, sqrtf(c.roughness) + ((2.0f * theta) / PI_F)));
         return (fac * fminn(1.0f, 1.0f - cos_theta_max)) * INV_PI_F;
+    }
+    if (c.kind == PSY_SHADER_DIELECTRIC) {
+        // Lights on either side count: the light's solid angle over 2 pi.
+        float dist2 = length2(to_light_center);
+        if (dist2 <= r2) return 1.0f;
+        float sin_theta_max2 = fminn(r2 / dist2, 1.0f);
+        return 1.0f - sqrtf(1.0f - sin_theta_max2);
     }
     return 0.0f;
 }

@@ -22,7 +22,13 @@ using rd::HaltonDim;
 
 constexpr int KBLOCK = 256;
 constexpr uint32_t MAX_SEGS = 64;  // requests interleaved in one wave
-constexpr uint32_t NUM_BINS = 5;   // miss, light, Lambert, GGX, emit
+constexpr uint32_t NUM_BINS = 6;   // miss, light, Lambert, GGX, emit, dielectric
+// Layout of the renderer's counter words: [0, 4) queue counts, then the sort
+// bins (2 * NUM_BINS), then the rays-traced u64 at an even word.
+constexpr uint32_t COUNTS_BINS = 4;
+constexpr uint32_t COUNTS_RAYS = 16;
+constexpr uint32_t COUNTS_WORDS = COUNTS_RAYS + 2;
+static_assert(COUNTS_BINS + 2 * NUM_BINS <= COUNTS_RAYS && COUNTS_RAYS % 2 == 0, "counter words");
 
 // A request's camera on the device: its transforms, then (when any lens
 // parameter has several time samples) the lens lists as floats, linear
@@ -185,7 +191,7 @@ struct psy_renderer {
     Queue qa[2]{}, qs[2]{};
     Hits ha{}, hs{};
     uint32_t* order = nullptr;
-    uint32_t* counts = nullptr;  // [0..4) queue counts, [4..14) sort bins, [14..16) rays traced (u64)
+    uint32_t* counts = nullptr;  // COUNTS_WORDS words: queue counts, sort bins, rays traced (u64)
     uint32_t* wave_idx = nullptr;
     unsigned int* work = nullptr;
     WavePlan* plans = nullptr;   // device plan array

@@ -239,7 +239,10 @@ def write(sc: RenderScene, name: str = "Scene", output: str = "render.png") -> s
         out.append("        }\n")
     out.append("    }\n    Assembly {\n")
     for i, (kind, col, rough, fres) in enumerate(sc.shaders):
-        t = {SHADER_LAMBERT: "Lambert", SHADER_GGX: "GGX", SHADER_EMIT: "Emit"}[kind]
+        names = {SHADER_LAMBERT: "Lambert", SHADER_GGX: "GGX", SHADER_EMIT: "Emit"}
+        if kind not in names:
+            raise ValueError(f"shader {i}: the .psy format has only Lambert, GGX and Emit shaders")
+        t = names[kind]
         out.append(f"        SurfaceShader $shader{i} {{\n            Type [{t}]\n"
                    f"            Color [{_color_text(col)}]\n")
         if kind == SHADER_GGX:

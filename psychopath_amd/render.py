@@ -33,7 +33,12 @@ class Renderer:
         self.scene = scene  # keeps the scene alive
         sh = (_lib.ShaderDesc * max(1, len(shaders)))()
         for i, (kind, col, rough, fres) in enumerate(shaders):
-            sh[i] = _lib.ShaderDesc(int(kind), _color(col), float(rough), float(fres))
+            # (kind, colour, roughness, fresnel): the last is the index of
+            # refraction for a dielectric.
+            if int(kind) == _lib.PSY_SHADER_DIELECTRIC:
+                sh[i] = _lib.ShaderDesc(int(kind), _color(col), float(rough), 0.0, float(fres))
+            else:
+                sh[i] = _lib.ShaderDesc(int(kind), _color(col), float(rough), float(fres), 0.0)
         dl = (_lib.DistantLightDesc * max(1, len(distant_lights)))()
         keep = []
         for i, (radius, direction, col) in enumerate(distant_lights):

@@ -207,7 +207,7 @@ def raycast_workload(levels: int = 9, n_rays: int = 1 << 24):
 # src/shading/mod.rs:27-40).
 
 COLOR_XYZ, COLOR_BLACKBODY, COLOR_TEMPERATURE = 0, 1, 2
-SHADER_LAMBERT, SHADER_GGX, SHADER_EMIT = 0, 1, 2
+SHADER_LAMBERT, SHADER_GGX, SHADER_EMIT, SHADER_DIELECTRIC = 0, 1, 2, 3
 
 
 def rec709_e_to_xyz_matrix() -> np.ndarray:
@@ -242,7 +242,7 @@ class RenderScene:
         self.rect_lights = []     # (width, height, colour)
         self.sphere_lights = []   # (radius, colour)
         self.instances = []       # dict(kind="mesh"|"rect"|"sphere", index, shader, xforms)
-        self.shaders = []         # (kind, colour, roughness, fresnel)
+        self.shaders = []         # (kind, colour, roughness, fresnel; the IOR for a dielectric)
         self.background = (COLOR_XYZ, 0.0, 0.0, 0.0)
         self.distant_lights = []  # (radius, direction (3), colour)
         self.camera = dict(xforms=np.zeros((1, 12), np.float32), fov=1.0, aperture=0.0, focus=1.0)
@@ -602,9 +602,9 @@ def large_scene(grid: int = 64, lights: int = 1024, width: int = 3840, height: i
     meshes of 10K-1M triangles (four displaced spheres, four heightfield
     patches; 2,099,040 unique triangles) with a random orientation and
     scale, and one of four material families: Lambertian, GGX metal of
-    roughness 0.05-0.8, a smooth dielectric coat (GGX, normal-incidence
-    reflectance 0.04 = IOR 1.5, the reference having no transmissive
-    closure) and emissive.  Lit by `lights` blackbody sphere lights and one
+    roughness 0.05-0.8, a rough dielectric of IOR 1.5 (GGX alpha 0.05; the
+    reference has no transmissive closure, so this is the build's own
+    PSY_SHADER_DIELECTRIC) and emissive.  Lit by `lights` blackbody sphere lights and one
     distant disk light (the sun).  `mesh_levels_cap` lowers the sphere
     meshes for small test renders."""
     sc = RenderScene()
@@ -618,7 +618,7 @@ def large_scene(grid: int = 64, lights: int = 1024, width: int = 3840, height: i
                 nx, ny = max(8, nx >> (9 - mesh_levels_cap)), max(8, ny >> (9 - mesh_levels_cap))
             v, f = _rock_patch(nx, ny, mseed)
         sc.meshes.append((v, f))
-    # Shaders: 0-5 Lambertian, 6-13 GGX metal, 14 dielectric coat, 15-16 emissive.
+    # Shaders: 0-5 Lambertian, 6-13 GGX metal, 14 dielectric, 15-16 emissive.
     lam = [(0.8, 0.8, 0.8), (0.7, 0.25, 0.2), (0.25, 0.6, 0.3), (0.2, 0.3, 0.7), (0.75, 0.65, 0.3),
            (0.45, 0.4, 0.35)]
     sc.shaders = [(SHADER_LAMBERT, rec709(*c), 0.0, 0.0) for c in lam]
@@ -626,7 +626,7 @@ def large_scene(grid: int = 64, lights: int = 1024, width: int = 3840, height: i
     for k in range(8):
         rough = 0.05 + 0.75 * k / 7.0
         sc.shaders.append((SHADER_GGX, rec709(*metals[k % 4]), float(np.float32(rough)), 1.0))
-    sc.shaders.append((SHADER_GGX, rec709(0.9, 0.9, 0.9), 0.05, 0.04))
+    sc.shaders.append((SHADER_DIELECTRIC, rec709(0.95, 0.97, 0.98), 0.05, 1.5))
     sc.shaders.append((SHADER_EMIT, (COLOR_TEMPERATURE, 3200.0, 4.0, 0.0), 0.0, 0.0))
     sc.shaders.append((SHADER_EMIT, (COLOR_TEMPERATURE, 7500.0, 4.0, 0.0), 0.0, 0.0))
     n = grid * grid

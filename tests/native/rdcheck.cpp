@@ -74,26 +74,30 @@ int64_t rc_run(int which, int64_t n, uint64_t seed, int64_t* first) {
     for (int64_t i = 0; i < n; i++) {
         bool ok = true;
         const float wl = uf(380.0f, 700.0f);
-        const int kind = (int)(urand() * 2.0);  // Lambert or GGX
+        const int pick = (int)(urand() * 3.0);  // Lambert, GGX or the dielectric
+        const int kind = pick == 2 ? 3 : pick;
         const float ca = uf(0.01f, 1.0f), cb = uf(0.01f, 1.0f), cc = uf(0.01f, 1.0f);
-        const float rough = (urand() < 0.1) ? 0.0f : uf(0.02f, 1.0f), fres = uf(0.0f, 1.0f);
+        float rough = (urand() < 0.1) ? 0.0f : uf(0.02f, 1.0f);
+        const float fres = uf(0.0f, 1.0f), ior = uf(1.05f, 2.5f);
+        if (kind == 3 && rough == 0.0f) rough = 0.02f;  // the renderer refuses a smooth dielectric
         psyref::Closure co;
         co.kind = (uint32_t)kind;
         co.color = rcolor_o(0, ca, cb, cc);
         co.roughness = rough;
         co.fresnel = fres;
-        psy::GpuClosure cp{(uint32_t)kind, rough, fres, 0, rcolor_p(0, ca, cb, cc)};
+        co.ior = ior;
+        psy::GpuClosure cp{(uint32_t)kind, rough, fres, ior, rcolor_p(0, ca, cb, cc)};
         const psyref::Vec3 inc = rvec(1.0f), nor = rvec(1.0f), nor_g = rvec(1.0f), out = rvec(1.0f);
         if (which == 0) {
-            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f);
+            const float u = uf(0.0f, 1.0f), v = uf(0.0f, 1.0f), w = uf(0.0f, 1.0f);
             psyref::Vec3 od;
             psyref::Vec4 of;
             float op;
-            psyref::closure_sample(co, inc, nor, nor_g, u, v, wl, od, of, op);
+            psyref::closure_sample(co, inc, nor, nor_g, u, v, w, wl, od, of, op);
             psy::V3 pd;
             psy::rd::S4 pf;
             float pp;
-            psy::rd::closure_sample(cp, cv(inc), cv(nor), cv(nor_g), u, v, wl, pd, pf, pp);
+            psy::rd::closure_sample(cp, cv(inc), cv(nor), cv(nor_g), u, v, w, wl, pd, pf, pp);
             ok = same3(od, pd) && same4(of, pf) && same(op, pp);
         } else if (which == 1) {
             psyref::Vec4 of;

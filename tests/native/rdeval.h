@@ -5,6 +5,7 @@
 //   0  closure sample: kind rough fres rgb inc nor nor_g u v wl   dir filter pdf
 //   1  closure evaluate: kind rough fres rgb inc out nor nor_g wl  filter pdf
 //   2  sphere estimate: kind rough fres rgb inc to r2 nor nor_g    value
+//      (closures: the dielectric's ior at [28], its sample number w at [29])
 //   3  rect light sample: w h rgb xf[12] arr u v wl               color point normal err pdf, sample_pdf
 //   4  sphere light sample: r temp xf[12] arr u v wl              color point normal err pdf, sample_pdf
 //   5  colour: kind a b c wl                                      spectrum xyz
@@ -35,7 +36,7 @@ RD GpuClosure clo(const float* p) {
     c.kind = (uint32_t)p[0];
     c.roughness = p[1];
     c.fresnel = p[2];
-    c.pad = 0;
+    c.ior = p[28];
     c.color = GpuColor{PSY_COLOR_XYZ, p[3], p[4], p[5]};
     return c;
 }
@@ -46,7 +47,7 @@ RD void eval(int which, const float* in, float* o) {
         V3 d;
         S4 f;
         float pdf;
-        closure_sample(clo(in), in3(in + 6), in3(in + 9), in3(in + 12), in[15], in[16], in[17], d, f, pdf);
+        closure_sample(clo(in), in3(in + 6), in3(in + 9), in3(in + 12), in[15], in[16], in[29], in[17], d, f, pdf);
         out3(o, d);
         out4(o + 3, f);
         o[7] = pdf;

@@ -50,6 +50,28 @@ def test_cornell_box_matches_oracle():
     check_parity(scenes.cornell_box(96, 96, 8, 4, 0), "cornell 96x96x8")
 
 
+def test_dielectric_boxes_match_oracle():
+    """The rough dielectric (PSY_SHADER_DIELECTRIC, no reference counterpart;
+    the oracle's restatement is pinned in tests/test_oracle_dielectric.py)
+    on the Cornell box's two boxes, one clear and nearly smooth, one tinted,
+    rough and denser: the GPU render matches the oracle's, and light now
+    passes through the boxes.  Bad parameters are refused."""
+    sc = scenes.cornell_box(96, 96, 8, 6, 0)
+    sc.shaders.append((scenes.SHADER_DIELECTRIC, scenes.rec709(0.95, 0.95, 0.95), 0.02, 1.5))
+    sc.shaders.append((scenes.SHADER_DIELECTRIC, scenes.rec709(0.9, 0.6, 0.4), 0.3, 1.7))
+    assert sc.instances[5]["kind"] == "mesh" and sc.instances[6]["kind"] == "mesh"
+    sc.instances[5]["shader"] = 3
+    sc.instances[6]["shader"] = 4
+    img, _ = check_parity(sc, "cornell, dielectric boxes, 96x96x8 depth 6")
+    lam, _, _ = gpu_render(scenes.cornell_box(96, 96, 8, 6, 0))
+    assert not np.allclose(img, lam)
+    for bad in ((scenes.SHADER_DIELECTRIC, scenes.rec709(1, 1, 1), 0.0, 1.5),
+                (scenes.SHADER_DIELECTRIC, scenes.rec709(1, 1, 1), 0.1, 1.0)):
+        sc.shaders[3] = bad
+        with pytest.raises(Exception):
+            render.build(sc)
+
+
 def test_cornell_box_seed_and_depth():
     check_parity(scenes.cornell_box(64, 48, 4, 2, 7), "cornell 64x48x4 depth 2 seed 7")
 
@@ -239,7 +261,7 @@ def test_autotune_keeps_images():
 
 def test_large_instanced_scene_reduced():
     """The large-scene workload's structure (eight meshes, Lambertian, GGX
-    metal, dielectric coat and emissive instances, sphere lights and a sun)
+    metal, rough dielectric and emissive instances, sphere lights and a sun)
     at reduced size: 256 instances of meshes capped at 65,536 triangles, 64
     lights, depth 8."""
     sc = scenes.large_scene(grid=16, lights=64, width=96, height=54, spp=8, mesh_levels_cap=7)

@@ -263,13 +263,27 @@ def assert_same_scene(sc, o):
         assert f32eq(np.atleast_1d(a[0]), b[0]) and f32eq(dirs, b[1]) and same_colours(a[2], b[2])
 
 
+def psy_expressible(sc):
+    """The scene with its dielectric shaders (which the .psy format cannot
+    hold) replaced by GGX ones."""
+    sc.shaders = [(scenes.SHADER_GGX, c, r, 0.04) if k == scenes.SHADER_DIELECTRIC else (k, c, r, f)
+                  for k, c, r, f in sc.shaders]
+    return sc
+
+
 def generated_scenes():
     return [
         ("cornell", scenes.cornell_box()),
         ("motion", scenes.motion_instances(grid=2, levels=2, lights=3, width=64, height=36, spp=2)),
         ("terrain", scenes.terrain_scene(nx=16, ny=12, width=32, height=32, spp=2)),
-        ("large", scenes.large_scene(grid=3, lights=5, width=32, height=18, spp=1, mesh_levels_cap=2)),
+        ("large", psy_expressible(scenes.large_scene(grid=3, lights=5, width=32, height=18, spp=1,
+                                                     mesh_levels_cap=2))),
     ]
+
+
+def test_dielectric_shaders_are_refused_by_the_writer():
+    with pytest.raises(ValueError, match="only Lambert, GGX and Emit"):
+        psyfile.write(scenes.large_scene(grid=3, lights=5, width=32, height=18, spp=1, mesh_levels_cap=2))
 
 
 @pytest.mark.parametrize("label,sc", generated_scenes())

@@ -55,8 +55,11 @@ def random_inputs(which, n, seed):
     rng = np.random.default_rng(seed)
     x = np.zeros((n, IN), np.float32)
     if which in (0, 1, 2):
-        x[:, 0] = rng.integers(0, 2, n)                      # Lambert or GGX
+        x[:, 0] = rng.choice([0, 1, 3], n)                   # Lambert, GGX or the dielectric
         x[:, 1] = np.where(rng.random(n) < 0.1, 0.0, rng.uniform(0.02, 1.0, n))
+        x[:, 1] = np.where((x[:, 0] == 3) & (x[:, 1] == 0), 0.02, x[:, 1])
+        x[:, 28] = rng.uniform(1.05, 2.5, n)                 # dielectric ior
+        x[:, 29] = rng.uniform(0, 1, n)                      # dielectric sample number
         x[:, 2] = rng.uniform(0, 1, n)
         x[:, 3:6] = rng.uniform(0.01, 1.0, (n, 3))
         x[:, 6:18] = rng.uniform(-1, 1, (n, 12))
