@@ -572,26 +572,47 @@ struct HitRec {
     uint32_t prim, inst;
 };
 // What the shading step asks to trace next.
+// What one shading step leaves for its caller, so that the path's colour
+// and its kept bounce ray need not stay in registers through the step (the
+// wavefront kernel reads them from path memory only when used):
+//   * `added`: the path's colour gets `add` (emission or background);
+//   * `kept`: the path continues along the bounce ray kept from an earlier
+//     step (its next_o, next_d), its attenuation times next_fac;
+//   * `fresh`: a new bounce ray was drawn into ps.next_o/next_d/next_fac.
 struct ShadeOut {
-    bool shadow, bounce;
+    bool shadow, bounce, added, kept, fresh;
     V3 so, sd, bo, bd;
     float smax;
+    S4 add;
 };
 
 // LightPath::next for a camera or bounce ray (renderer.rs:459-667): adds
 // emission or background, or prepares the light ray (with its pending
-// contribution in ps.pending) and the next bounce ray.
+// contribution in ps.pending) and the next bounce ray.  DEFER (the wavefront
+// kernel) leaves the colour addition and the kept bounce ray to the caller
+// (out.added, out.kept), so ps.color and the kept next_* are never read;
+// without it (the one-launch kernel, whose path state lives in registers)
+// they are applied here.
+template <bool DEFER>
 __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams& rp, PathRegs& ps, const HitRec& h,
                                            V3 orig, V3 dir, ShadeOut& out) {
     out.shadow = false;
     out.bounce = false;
+    out.added = false;
+    out.kept = false;
+    out.fresh = false;
     out.smax = 1.0f;
     const float wl = ps.wl;
     const float time = ps.time;
     if (h.kind != PSY_HIT_SURFACE) {
         // Background (renderer.rs:656-666)
         const S4 bg = color_spectral(sc.background, wl);
-        ps.color = ps.color + ((bg * ps.atten) / ps.csp);
+        if (DEFER) {
+            out.add = (bg * ps.atten) / ps.csp;
+            out.added = true;
+        } else {
+            ps.color = ps.color + ((bg * ps.atten) / ps.csp);
+        }
         return;
     }
     const GpuInstance inst = sc.instances[sc.inst_pos[h.inst]];
@@ -608,11 +629,18 @@ __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams
     }
     if (clo.kind == PSY_SHADER_EMIT) {
         const S4 c = color_spectral(clo.color, wl);
+        S4 add;
         if (ps.bounces == 0u) {
-            ps.color = ps.color + c;
+            add = c;
         } else {
             const float mis_pdf = power_heuristic(ps.csp, is.sample_pdf);
-            ps.color = ps.color + ((c * ps.atten) / mis_pdf);
+            add = (c * ps.atten) / mis_pdf;
+        }
+        if (DEFER) {
+            out.add = add;
+            out.added = true;
+        } else {
+            ps.color = ps.color + add;
         }
         return;
     }
@@ -661,6 +689,7 @@ __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams
             ps.next_d = ndir;
             ps.has_next = true;
             do_bounce = true;
+            out.fresh = true;
         }
     } else {
         ps.has_next = false;
@@ -669,9 +698,15 @@ __device__ __forceinline__ void shade_path(const ShadeScene& sc, const ReqParams
     // has one, even one kept from an earlier bounce; without a light ray only
     // a fresh bounce continues it.
     if (out.shadow ? ps.has_next : do_bounce) {
-        out.bo = ps.next_o;
-        out.bd = ps.next_d;
-        la = la * ps.next_fac;
+        if (!DEFER || do_bounce) {
+            out.bo = ps.next_o;
+            out.bd = ps.next_d;
+            la = la * ps.next_fac;
+        } else {
+            // along the bounce ray kept from an earlier step: the caller
+            // reads it and multiplies the attenuation by its next_fac
+            out.kept = true;
+        }
         out.bounce = true;
     }
     ps.atten = la;
@@ -703,9 +738,9 @@ __global__ __launch_bounds__(KBLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) 
             p = qa.path[q];
             const ReqParams rp = reqs[P.req[p]];
             PathRegs ps;
-            ps.color = s4_load(P.color, p);
+            ps.color = s4_zero();    // not read by shade_path (out.add)
             ps.atten = s4_load(P.atten, p);
-            ps.pending = s4_load(P.pending, p);
+            ps.pending = s4_zero();  // only written
             ps.csp = P.csp[p];
             ps.wl = P.wl[p];
             ps.time = P.time[p];
@@ -715,23 +750,26 @@ __global__ __launch_bounds__(KBLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) 
             ps.y = P.pix[p] / rp.width;
             ps.sample = P.sample[p];
             ps.has_next = P.has_next[p] != 0u;
-            if (ps.has_next) {
-                const float4 a = P.next_o[p], b = P.next_d[p];
-                ps.next_o = v3(a.x, a.y, a.z);
-                ps.next_d = v3(b.x, b.y, b.z);
-                ps.next_fac = s4_load(P.next_fac, p);
-            }
+            ps.next_o = v3(0.0f, 0.0f, 0.0f);  // the kept ray stays in path memory (so.kept)
+            ps.next_d = v3(0.0f, 0.0f, 0.0f);
+            ps.next_fac = s4_zero();
             ptime = ps.time;
             const HitRec hr{h.kind[q], h.t[q], h.b0[q], h.b1[q], h.b2[q], h.prim[q], h.inst[q]};
-            shade_path(sc, rp, ps, hr, v3(qa.ox[q], qa.oy[q], qa.oz[q]), v3(qa.dx[q], qa.dy[q], qa.dz[q]), so);
-            s4_store(P.color, p, ps.color);
+            shade_path<true>(sc, rp, ps, hr, v3(qa.ox[q], qa.oy[q], qa.oz[q]), v3(qa.dx[q], qa.dy[q], qa.dz[q]), so);
+            if (so.added) s4_store(P.color, p, s4_load(P.color, p) + so.add);
+            if (so.kept) {
+                const float4 a = P.next_o[p], b = P.next_d[p];
+                so.bo = v3(a.x, a.y, a.z);
+                so.bd = v3(b.x, b.y, b.z);
+                ps.atten = ps.atten * s4_load(P.next_fac, p);
+            }
             s4_store(P.atten, p, ps.atten);
             if (so.shadow) s4_store(P.pending, p, ps.pending);
             P.csp[p] = ps.csp;
             P.seed[p] = ps.seed;
             P.bounces[p] = ps.bounces;
             P.has_next[p] = ps.has_next ? 1u : 0u;
-            if (ps.has_next) {
+            if (so.fresh) {
                 P.next_o[p] = make_float4(ps.next_o.x, ps.next_o.y, ps.next_o.z, 0.0f);
                 P.next_d[p] = make_float4(ps.next_d.x, ps.next_d.y, ps.next_d.z, 0.0f);
                 s4_store(P.next_fac, p, ps.next_fac);
@@ -790,7 +828,7 @@ __global__ __launch_bounds__(MEGA_BLOCK) __attribute__((amdgpu_waves_per_eu(MINW
                                                                            false, lds_col, MEGA_BLOCK, tc);
                 const HitRec hr{th.kind, th.t, th.b0, th.b1, th.b2, th.prim, th.inst};
                 ShadeOut so;
-                shade_path(sc, rp, ps, hr, o, d, so);
+                shade_path<false>(sc, rp, ps, hr, o, d, so);
                 traced += so.shadow ? 2u : 1u;
                 if (so.shadow) {
                     const TraceHit sh = trace_one<false, MEGA_LDS, MEGA_SPILL, true>(ta, so.so, so.sd, ps.time, so.smax, true,
