@@ -4,10 +4,11 @@
 // order, src/accel/bvh4.rs:101-184, with the same box and triangle tests),
 // reorganised for throughput on incoherent rays:
 //   * one wave64 per workgroup, launched only as many waves as the chip holds
-//     at once; each wave keeps taking rays from a global counter;
+//     at once; each wave keeps taking chunks of rays from a global counter;
 //   * lanes whose ray finished are refilled in bulk once at least
-//     `refill_min` lanes of the wave are idle (one atomic per refill, the
-//     new rays spread over the idle lanes by a ballot prefix count);
+//     `refill_min` lanes of the wave are idle, from the wave's chunk (an
+//     atomic only when the chunk runs short), the new rays spread over the
+//     idle lanes by a ballot prefix count;
 //   * the traversal is "while-while": lanes descend internal nodes (popping
 //     past dead ends) until the whole wave stands on leaves, then run the
 //     leaf tests together;
@@ -28,6 +29,8 @@ namespace psy {
 namespace {
 
 constexpr int WAVE = 64;
+// Most rays a wave takes from the global counter at once.
+constexpr uint32_t CHUNK_MAX = 512;
 
 // Short stack: the newest LDS_DEPTH entries of each lane live in an LDS ring
 // (one column per lane, stride WAVE); when the ring is full the oldest entry
@@ -128,20 +131,39 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
     uint32_t hprim = PSY_NONE, hinst = PSY_NONE;
     uint64_t c_nodes = 0, c_mnodes = 0, c_tris = 0, c_mtris = 0, c_inst = 0;
     bool exhausted = false;
+    // The wave's chunk: rays [next, end) taken from the counter and not yet
+    // handed to a lane; `seen` is where the counter stood after the wave's
+    // last fetch, and `drained` that it had passed n then.
+    uint32_t next = 0, end = 0, seen = 0;
+    bool drained = false;
 
     for (;;) {
         // ---- refill idle lanes ------------------------------------------
+        // Idle lanes take the rest of the wave's chunk first; one atomic
+        // fetches a new chunk only when the rest is short, sized by guided
+        // self-scheduling (the remaining rays over 2x the waves, at most
+        // CHUNK_MAX), so the counter sees few atomics while the last rays
+        // still spread over all waves.
         const uint64_t idle = __ballot(!has);
         if (!exhausted && (uint32_t)__popcll(idle) >= refill_min) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
             const uint32_t want = (uint32_t)__popcll(idle);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(work, want);
-            base = __shfl(base, leader);
-            if (base + want >= n) exhausted = true;
+            const uint32_t avail = end - next;
+            uint32_t fresh = 0, fresh_end = 0;
+            if (avail < want && !drained) {
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                const uint32_t left = n > seen ? n - seen : 0u;
+                const uint32_t take =
+                    max(want - avail, min(left / (2u * (uint32_t)gridDim.x), (uint32_t)CHUNK_MAX));
+                if (lane == leader) fresh = atomicAdd(work, take);
+                fresh = __shfl(fresh, leader);
+                seen = fresh + take;
+                drained = seen >= n;
+                fresh_end = fresh < n ? min(seen, n) : fresh;
+            }
             if (!has) {
-                const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (mine < n) {
+                const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                const uint32_t mine = rank < avail ? next + rank : fresh + (rank - avail);
+                if (rank < avail || mine < fresh_end) {
                     has = true;
                     ray = mine;
                     o = v3(a.ox[ray], a.oy[ray], a.oz[ray]);
@@ -165,6 +187,15 @@ void trace_persistent_kernel(TraceArgs a, unsigned int* work, uint32_t refill_mi
                     kind = PSY_HIT_MISS;
                 }
             }
+            if (avail >= want) {
+                next += want;
+            } else {
+                // The lanes took the whole rest and the head of the new chunk
+                // (none when the counter had already passed n).
+                next = min(fresh + (want - avail), fresh_end);
+                end = fresh_end;
+            }
+            exhausted = drained && next == end;
         }
         if (__ballot(has) == 0) break;
         if (!has) continue;
