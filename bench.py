@@ -474,13 +474,14 @@ def render_bench(args) -> dict:
     rens = [render.build(s) for _, s in scene_defs]
     log(f"serving scenes built in {time.time() - t0:.1f}s")
     # Each scene's launch shapes and register budgets, wavefront and
-    # one-launch forms alike, from the autotuner on a 480x270x4 trial: the
+    # one-launch forms alike, from the autotuner on a 1024x1024x4 trial (one
+    # full wave of paths, the size the server's waves run at): the
     # server batches with the tuned wavefront settings and the plain
     # baseline renders with its own tuned budget.
     t0 = time.time()
     tuned = []
     for k, (_, sd) in enumerate(scene_defs):
-        trial = dict(width=480, height=270, spp=4, max_bounces=sd.settings["max_bounces"], seed=0)
+        trial = dict(width=1024, height=1024, spp=4, max_bounces=sd.settings["max_bounces"], seed=0)
         c = rens[k].autotune(sd.camera, trial)
         rens[k].set_options(sort=c["sort"])
         tuned.append(c)
@@ -644,7 +645,9 @@ def large_bench(args) -> dict:
     sc = scenes.large_scene(spp=args.large_spp)
     r = render.build(sc)
     log(f"large scene built in {time.time() - t0:.1f}s")
-    trial = dict(width=960, height=540, spp=4, max_bounces=8, seed=0)
+    # One full wave of paths (1920x1080x2 = 4,147,200), the size the
+    # render's own waves run at.
+    trial = dict(width=1920, height=1080, spp=2, max_bounces=8, seed=0)
     t1 = time.time()
     choice = r.autotune(sc.camera, trial)
     log(f"large scene autotune {time.time() - t1:.1f}s -> {choice}")

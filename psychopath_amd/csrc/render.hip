@@ -1699,9 +1699,9 @@ int psy_renderer_autotune(psy_renderer* r, const psy_camera_desc* cam, const psy
     // Stage 1: the traversal launch shape, sorting on, current shading build.
     r->sort_materials = true;
     for (uint32_t sw : {4u, 8u}) {
-        for (uint32_t w : {8u, 16u, 32u}) {
+        for (uint32_t w : {16u, 24u, 32u}) {
             for (uint32_t refill : {8u, 16u, 32u}) {
-                for (uint32_t lds : {16u, 24u}) {
+                for (uint32_t lds : {8u, 16u, 24u}) {
                     if (rc) break;
                     r->tuning = psy_trace_tuning{PSY_KERNEL_PERSISTENT, w, refill, lds, sw, 0};
                     float ms;

@@ -253,7 +253,7 @@ def test_autotune_keeps_images():
     r = render.build(sc)
     before, _ = r.render(sc.camera, sc.settings)
     choice = r.autotune(sc.camera, dict(width=32, height=32, spp=2, max_bounces=3, seed=0))
-    assert choice["waves_per_cu"] in (8, 16, 32) and choice["stack_lds"] in (16, 24)
+    assert choice["waves_per_cu"] in (16, 24, 32) and choice["stack_lds"] in (8, 16, 24)
     assert choice["shade_waves"] in (1, 2, 3, 4) and choice["plain"] in (True, False)
     after, _ = r.render(sc.camera, sc.settings)
     assert np.array_equal(before, after)
