@@ -50,16 +50,33 @@ __device__ __forceinline__ const WaveSeg& find_seg(const WavePlan& p, uint32_t i
     return p.seg[s];
 }
 
-// Wave-aggregated append: one atomic per wave64 for the lanes with `want`.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return 0xffffffffu;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+// Block-aggregated appends to two queues: one atomic per block and queue,
+// shared by the block's KBLOCK / 64 waves (`cnt` and `base` are the block's
+// LDS, 2 x KBLOCK / 64 words each).  Every thread of the block calls it.
+__device__ __forceinline__ void block_append2(uint32_t* cs, bool ws, uint32_t* cb, bool wb, uint32_t* cnt,
+                                              uint32_t* base, uint32_t& at_s, uint32_t& at_b) {
+    constexpr int NW = KBLOCK / 64;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t ms = __ballot(ws), mb = __ballot(wb);
+    if (lane == 0) {
+        cnt[w] = (uint32_t)__popcll(ms);
+        cnt[NW + w] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        uint32_t tot = 0;
+        for (int j = 0; j < NW; j++) tot += cnt[k * NW + j];
+        uint32_t b0 = tot ? atomicAdd(k == 0 ? cs : cb, tot) : 0u;
+        for (int j = 0; j < NW; j++) {
+            base[k * NW + j] = b0;
+            b0 += cnt[k * NW + j];
+        }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    at_s = base[w] + (uint32_t)__popcll(ms & below);
+    at_b = base[NW + w] + (uint32_t)__popcll(mb & below);
 }
 
 __device__ __forceinline__ void queue_put(const Queue& q, uint32_t at, uint32_t path, V3 o, V3 d, float time,
@@ -724,9 +741,12 @@ __global__ __launch_bounds__(KBLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) 
     const uint32_t n = *qa.count;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(sc.rays, (unsigned long long)n);
     const uint32_t stride = gridDim.x * blockDim.x;
-    // Every lane of a wave runs the loop the same number of times so the
-    // wave-wide appends see all lanes.
-    const uint32_t n_round = (n + 63u) & ~63u;
+    // Every thread of a block runs the loop the same number of times, so the
+    // block-wide appends see all of them.  One atomic per block and queue:
+    // per wave, the appends of a launch over a few million rays contend on
+    // the two queue counters.
+    __shared__ uint32_t app_cnt[2 * (KBLOCK / 64)], app_base[2 * (KBLOCK / 64)];
+    const uint32_t n_round = (n + (uint32_t)KBLOCK - 1u) / (uint32_t)KBLOCK * (uint32_t)KBLOCK;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
         ShadeOut so;
         so.shadow = false;
@@ -775,9 +795,9 @@ __global__ __launch_bounds__(KBLOCK) __attribute__((amdgpu_waves_per_eu(MINW))) 
                 s4_store(P.next_fac, p, ps.next_fac);
             }
         }
-        const uint32_t at_s = wave_append(qs_out.count, so.shadow);
+        uint32_t at_s, at_b;
+        block_append2(qs_out.count, so.shadow, qa_out.count, so.bounce, app_cnt, app_base, at_s, at_b);
         if (so.shadow) queue_put(qs_out, at_s, p, so.so, so.sd, ptime, so.smax);
-        const uint32_t at_b = wave_append(qa_out.count, so.bounce);
         if (so.bounce) queue_put(qa_out, at_b, p, so.bo, so.bd, ptime, __builtin_huge_valf());
     }
 }
