@@ -433,6 +433,34 @@ def test_launch_shapes_give_identical_hits(small, instanced):
         tracer.set_tuning(0, 16, 16, 16, 4, 0)
 
 
+def test_every_ray_written_once_at_ragged_sizes(small):
+    """The persistent kernel hands each ray of a launch to exactly one lane
+    (waves take chunks of rays from the counter and refill lanes from them):
+    with the device records pre-filled with a sentinel, every record of
+    ragged batch sizes is written and equals the one-lane grid kernel's, for
+    launch shapes from one wave per CU to 32, refilling at 1 to 64 idle
+    lanes."""
+    v, f, s, o, tr = small
+    r = scenes.incoherent_rays(v.min(0), v.max(0), 40_001, seeds=(151, 152, 153))
+    shapes = ((1, 1, 1, 8, 4), (0, 1, 1, 8, 4, 0), (0, 1, 64, 16, 4, 0), (0, 24, 16, 8, 4, 32),
+              (0, 32, 8, 24, 8, 0), (2, 16, 16, 8, 4))
+    for n in (1, 63, 65, 4097, 40_001):
+        dr = DeviceRays(ray_batch({k: a[:n] for k, a in r.items()}))
+        ref = None
+        for shape in shapes:
+            tr.set_tuning(*shape)
+            hits = DeviceHitRecords(n)
+            hits.buf.upload(np.full(n * 32, 0xFF, np.uint8))
+            tr.trace_device(dr, hits)
+            rec = hits.records()
+            assert (rec["kind"] <= 2).all(), (n, shape)
+            if ref is None:
+                ref = rec
+            else:
+                assert rec.tobytes() == ref.tobytes(), (n, shape)
+    tr.set_tuning(0, 16, 16, 16, 4, 0)
+
+
 def test_deep_stack_spills_to_private_memory():
     # 4000 copies of one triangle build a deep tree (median splits): many
     # pending entries overflow the 8-entry LDS ring into private memory and
