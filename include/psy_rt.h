@@ -250,8 +250,8 @@ typedef struct {
  * scenes without time samples (elsewhere it runs as the persistent kernel).
  * The persistent kernel keeps
  * `waves_per_cu` wave64 workgroups per compute unit resident, each taking
- * rays from a global counter; lanes whose ray finished are refilled once
- * `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
+ * chunks of rays from a global counter; lanes whose ray finished are
+ * refilled from the wave's chunk once `refill_min` lanes of a wave are idle; `stack_lds` stack entries per lane
  * live in LDS (8, 16 or 24) and the rest in private memory; `simd_waves`
  * is the register budget, as resident waves per SIMD (4: up to 128 VGPRs,
  * 8: at most 64); `leaf_min` 0 lets each lane descend until it reaches a
